@@ -1,0 +1,168 @@
+"""Benchmark: Newton iterations x Mcells/s of the pseudo-transient Newton-Krylov
+step on one MI355X, on the 2D SA RANS case (Karman-Trefftz 10 deg section,
+C-grid 512 x 128 cells, M 0.73, alpha 2.5 deg, Re 6.5e6, Roe + smooth van
+Albada, GMRES(30) x 2 with the line preconditioner, FP64).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  A step is one full Newton step of the case (residual,
+wave speeds, line factor, GMRES, update) on the device; the state starts at
+the freestream and W steps run untimed first.  The timed region is bracketed
+by device synchronisation.  The roofline entry times the dominant kernel with
+HIP events on the library's stream.  The CPU baseline times the FP64 oracle
+(NumPy, one thread) on one Newton step of the same case in a child process.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "jax-rans_amd")]
+
+import numpy as np  # noqa: E402
+
+CASE = ROOT / "cases" / "kt10_c_sa"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(name: str, ctx, m: int) -> float:
+    """Compulsory bytes of one launch (each array touched once), FP64 unless
+    the line factor is FP32 (``mixed``)."""
+    at = ctx.atlas
+    nv, nc = ctx.nvar, at.n_canvas
+    chain_cells = at.lines.cells.size
+    free = nc - chain_cells
+    fb = 4 if ctx.config.precision == "mixed" else 8
+    if name == "precond_apply":
+        # factor L, P^-1, C (3 m^2) + SA l, p^-1, c (3), r read + z write on every cell
+        return ctx.n_cases * (chain_cells * (3 * 16 * fb + 3 * fb) + (chain_cells + free) * 2 * nv * 8)
+    if name == "tangent_faces":
+        # Q, dQ read once per cell; two face vectors; tangent face fluxes written (2 faces x nv)
+        return ctx.n_cases * nc * (2 * nv * 8 + 4 * 8 + 2 * nv * 8)
+    if name == "cgs2_update_dots":
+        j = m // 2
+        return ctx.n_cases * nc * nv * 8 * (j + 1 + 2)
+    if name == "residual_faces":
+        return ctx.n_cases * nc * (nv * 8 + 4 * 8 + 2 * nv * 8)
+    if name == "tangent_cells":
+        return ctx.n_cases * nc * (2 * nv * 8 + 2 * nv * 8 + 8 + nv * 8)
+    raise KeyError(name)
+
+
+def cpu_baseline() -> dict:
+    """One oracle Newton step of the same case, timed in a one-thread child process."""
+    code = r"""
+import os, sys, time, json
+sys.path[:0] = [sys.argv[1] + '/jax-rans_amd', sys.argv[1]]
+import numpy as np
+from jaxrans_amd import load_case
+from oracle import residual as OR, solver as OS
+case = load_case(sys.argv[1] + '/cases/kt10_c_sa')
+at = case.grid()
+st = OR.setup_for(at, case.flow, case.numerics)
+s = OS.NewtonState(OR.U0_of(st), case.implicit.cfl)
+np.seterr(all='ignore')
+t = time.perf_counter()
+s, n, info = OS.newton_step(s, st, case.implicit)
+dt = time.perf_counter() - t
+print(json.dumps({"seconds": dt, "cells": at.cells}))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, "-c", code, str(ROOT)], env=env, capture_output=True, text=True,
+                         timeout=900)
+    if out.returncode != 0:
+        return {"value": None, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+                "sample": "oracle Newton step failed: " + out.stderr[-300:]}
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["cells"] / r["seconds"] / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one FP64 Newton step (GMRES(30) x 2, line preconditioner) of the NumPy oracle on the "
+                      f"same {r['cells']}-cell case from the freestream, {r['seconds']:.1f} s, one thread"}
+
+
+def run_steps(case, atlas, precision: str, steps: int, warmup: int):
+    from dataclasses import replace
+
+    from jaxrans_amd.core.state import q_infinity, uniform_canvas
+    from jaxrans_amd.device import context_for
+    cfg = replace(case.implicit, precision=precision)
+    ctx = context_for(atlas, case.flow, case.numerics, cfg)
+    q = q_infinity(case.flow)
+    ctx.set_freestream(q, case.flow.mu_inf)
+    ctx.set_state(uniform_canvas(q, atlas.shape, case.flow.gamma))
+    ctx.newton_init(cfg.cfl)
+    if warmup:
+        ctx.newton_block(warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    hist = ctx.newton_block(steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return ctx, dt, hist
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
+              f"a batch of cases would shard over GPUs with no communication", file=sys.stderr)
+        return 2
+    from jaxrans_amd import load_case
+    case = load_case(CASE)
+    atlas = case.grid()
+    cells = atlas.cells
+    ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup)
+    ms_per_step = 1e3 * dt / args.steps
+    value = cells * args.steps / dt / 1e6
+    m = case.implicit.krylov_subspace
+    # dominant kernel: the largest device time per Newton step among the hot kernels
+    calls = {"precond_apply": 2 * (m + 1), "tangent_faces": 2 * m, "cgs2_update_dots": 2 * m,
+             "residual_faces": 1, "tangent_cells": 2 * m}
+    times = {k: ctx.profile(k, reps=20) for k in calls}
+    share = {k: times[k] * calls[k] for k in calls}
+    dom = max(share, key=share.get)
+    alg = algorithmic_bytes(dom, ctx, m)
+    achieved = alg / (times[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "kernel_ms": round(times[dom], 5), "algorithmic_bytes": int(alg),
+                "kernel_ms_all": {k: round(v, 5) for k, v in times.items()}}
+    scal = ctx.scalars()[0]
+    ctx.close()
+    extra = {}
+    if not args.no_mixed:
+        cm, dtm, histm = run_steps(case, atlas, "mixed", args.steps, args.warmup)
+        extra = {"mixed_value": round(cells * args.steps / dtm / 1e6, 4), "mixed_ms_per_step": round(1e3 * dtm / args.steps, 3),
+                 "mixed_last_residual": float(histm[-1, 0])}
+        cm.close()
+    line = {
+        "metric": "Newton iters x Mcells/s", "value": round(value, 4), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2D SA RANS, Karman-Trefftz 10 deg, C-grid 512x128 (65536 cells), M 0.73, "
+                               "alpha 2.5, Re 6.5e6, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64",
+                   "cells": cells, "krylov_subspace": m, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "residual_first": float(hist[0, 0]), "residual_last": float(hist[-1, 0]), "cfl_last": float(scal[0]),
+        **extra,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,157 @@
+/* jrhip -- C ABI of the MI355X-native Newton-Krylov hot path.
+ *
+ * One context holds one canvas (a grid's plans, uploaded once) and
+ * n_cases cases on it (a single run, or a batched polar).  The library owns
+ * every device buffer; the state U lives on the device between calls.
+ * Return codes: 0 OK, < 0 error (text in jr_last_error()).  A non-physical
+ * or non-finite state is not an error: it is a rejected step, visible in the
+ * residual history and the per-case scalars.
+ *
+ * Layout of every canvas field: (case, variable, i, j), j fastest, FP64 --
+ * the reference's (nvar, NI, NJ) canvas layout with a case axis in front.
+ *
+ * Each entry point names the reference interface it replaces (paths
+ * relative to the reference repository).  Threading: one context is bound to
+ * one HIP stream and must not be used from two host threads at once.
+ */
+#ifndef JRHIP_H
+#define JRHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jr_ctx jr_ctx;
+
+/* The atlas plans (src/jaxrans/mesh/atlas.py:97-228), host arrays. */
+typedef struct {
+  int32_t NI, NJ;
+  const double* area;          /* [NI*NJ] */
+  const double* face_i;        /* [2][(NI+1)*NJ] */
+  const double* face_j;        /* [2][NI*(NJ+1)] */
+  const double* centres;       /* [2][(NI+2)*(NJ+2)] one-ghost edge-padded cell centres */
+  const double* distance;      /* [NI*NJ] wall distance (SA) */
+  const uint8_t* interior;     /* [NI*NJ] */
+  const int64_t* neighbours;   /* [NI*NJ] */
+  int32_t n_task[2];           /* ghost-fill stages */
+  const int64_t* task_dest[2];
+  const int64_t* task_src[2];
+  const int32_t* task_kind[2]; /* 0 copy, 1 slip, 2 no-slip, 3 symmetry, 4 far field, 5 inflow, 6 outflow */
+  const double* task_normal[2];/* [2][n] */
+  int32_t n_lines, n_slots;    /* line chains */
+  const int64_t* line_start;   /* [n_lines+1] */
+  const int64_t* line_cells;   /* [n_slots] */
+  const int64_t* line_up_face;
+  const int64_t* line_up_sign;
+  const int64_t* line_down_face;
+  const int64_t* line_down_sign;
+  int32_t n_walls;
+  const int64_t* wall_cell;
+  const int64_t* wall_family;
+  const int64_t* wall_face;
+  const double* wall_into;
+  const double* wall_mid;      /* [2][n_walls] */
+} jr_grid_desc;
+
+/* Discretization and solver settings (src/jaxrans/core/config.py:112-212). */
+typedef struct {
+  double gamma;
+  double kappa;                /* MUSCL kappa, 0: limited */
+  int32_t flux;                /* 0 hllc, 1 roe */
+  int32_t limiter;             /* 0 minmod, 1 none, 2 van_albada, 3 van_albada_smooth */
+  int32_t viscous;
+  int32_t sa;                  /* Spalart-Allmaras row */
+  double sutherland;           /* S/T_inf, <= 0: constant viscosity */
+  double c_cr1;                /* QCR2000, 0: off */
+  int32_t jacobian_exact;      /* 0: first-order Jacobian (defect correction) */
+  int32_t mixed;               /* 1: FP32 line-preconditioner blocks and block-Thomas */
+  int32_t krylov_subspace;
+  int32_t krylov_restarts;
+  double krylov_tol;
+  double krylov_success;
+  double cfl_growth, cfl_cutback, cfl_min, cfl_max;
+  double max_relative_update;  /* <= 0: none */
+} jr_numerics;
+
+/* Context (replaces make_residual + make_newton_step set-up,
+ * src/jaxrans/numerics/residual.py:273-334, solvers/implicit.py:84-98). */
+int jr_ctx_create(const jr_grid_desc* grid, const jr_numerics* num, int n_cases, jr_ctx** out);
+int jr_ctx_destroy(jr_ctx* ctx);
+
+/* Per-case freestream primitives q_inf[n_cases][5] (rho, u, v, p, nu_hat ratio)
+ * and mu_inf[n_cases] (NULL: inviscid) (src/jaxrans/core/state.py:176-192). */
+int jr_set_freestream(jr_ctx* ctx, const double* q_inf, const double* mu_inf);
+
+/* State on the device, host copies in/out: [n_cases][nvar][NI][NJ]. */
+int jr_set_state(jr_ctx* ctx, const double* U_host);
+int jr_get_state(jr_ctx* ctx, double* U_host);
+
+/* R(U) (ResidualOperator.residual, src/jaxrans/numerics/residual.py:324-327);
+ * first_order selects first_order_faces (reconstruction.py:71-80). */
+int jr_residual(jr_ctx* ctx, const double* U_host, double* R_host, int first_order);
+int jr_residual_dev(jr_ctx* ctx, const double* U_dev, double* R_dev, int first_order);
+
+/* J dU (+ V/dt dU if with_vdt, at the context's current CFL): the Krylov
+ * operator of src/jaxrans/solvers/implicit.py:105-116. */
+int jr_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, double* out_host, int first_order,
+               int with_vdt);
+int jr_tangent_dev(jr_ctx* ctx, const double* U_dev, const double* dU_dev, double* out_dev, int first_order,
+                   int with_vdt);
+
+/* Face wave speeds lam[n_cases][(NI+1)*NJ + NI*(NJ+1)] and their cell sums
+ * lsum[n_cases][NI*NJ] (face_wave_speeds + cell_wave_speed_sum,
+ * src/jaxrans/numerics/residual.py:169-203). */
+int jr_wave_speeds(jr_ctx* ctx, const double* U_host, double* lam_host, double* lsum_host);
+
+/* Line preconditioner: factor at U and the current CFL, then z = M^-1 r
+ * (line_implicit_j, src/jaxrans/solvers/preconditioners.py:290-306). */
+int jr_precond_factor(jr_ctx* ctx, const double* U_host);
+int jr_precond_apply(jr_ctx* ctx, const double* r_host, double* z_host);
+
+/* Newton control: start every case at cfl0[n_cases] with |R|_prev = inf
+ * (ImplicitState, src/jaxrans/solvers/implicit.py:35-51). */
+int jr_newton_init(jr_ctx* ctx, const double* cfl0);
+
+/* n_steps pseudo-transient Newton-Krylov steps on the device state
+ * (one block of run_to_steady, src/jaxrans/solvers/driver.py:87-104);
+ * hist_host[n_steps][n_cases] receives |R(U)| before each update. */
+int jr_newton_block(jr_ctx* ctx, int n_steps, double* hist_host);
+
+/* Per-case scalars [n_cases][8]: cfl, |R|_prev, |R|, linear reduction,
+ * omega, accepted(last step), steps, rejected. */
+int jr_get_scalars(jr_ctx* ctx, double* out_host);
+int jr_set_cfl(jr_ctx* ctx, const double* cfl);
+
+/* Forces of the device state (force_coefficients, src/jaxrans/post/forces.py:108-161):
+ * out[n_cases][8] = cl, cd, cm, cd_friction, fx, fy, 0, 0. */
+int jr_forces(jr_ctx* ctx, double length, double centre_x, double centre_y, double* out_host);
+
+/* Average device time of one launch of a hot kernel on the device state,
+ * timed with HIP events on the context stream: 0 residual (all launches),
+ * 1 tangent (all launches), 2 line-preconditioner apply, 3 residual face
+ * kernel, 4 tangent face kernel, 5 CGS2 update+dots at j = m/2, 6 line
+ * factor, 7 tangent cell kernel.  No reference counterpart: measurement. */
+int jr_profile(jr_ctx* ctx, int kernel, int reps, float* ms_per_launch);
+
+/* Device timing of the context's stream (HIP events): start, then stop
+ * returns the elapsed milliseconds.  No reference counterpart: measurement. */
+int jr_timer_start(jr_ctx* ctx);
+int jr_timer_stop(jr_ctx* ctx, float* ms);
+int jr_synchronize(jr_ctx* ctx);
+
+/* Device buffers for the *_dev entry points. */
+int jr_device_alloc(size_t bytes, void** out);
+int jr_device_free(void* p);
+int jr_copy_to_device(void* dst, const void* src, size_t bytes);
+int jr_copy_to_host(void* dst, const void* src, size_t bytes);
+int jr_device_count(int* n);
+
+const char* jr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

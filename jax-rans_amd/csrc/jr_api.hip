@@ -1,0 +1,788 @@
+// C-ABI of the Newton-Krylov hot path: context set-up, plan upload, and the
+// launch sequences of the residual, its tangent, the line preconditioner and
+// whole Newton steps.  Declarations and the reference interfaces each entry
+// point replaces: include/jrhip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jrhip.h"
+#include "jr_krylov.h"
+
+using namespace jr;
+
+static thread_local std::string g_error;
+
+static int fail(const std::string& msg) {
+  g_error = msg;
+  return -1;
+}
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+struct jr_ctx {
+  Plan P{};
+  Physics ph{};
+  int nv = 4, n_cases = 1;
+  int m = 10, restarts = 2, exact = 0, mixed = 0;
+  double ktol = 1e-2, max_rel = -1.0;
+  Ramp ramp{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t N = 0;      // entries of one case's field (nv * nc)
+  int nblk = 1;      // workgroups per case of the vector kernels
+  std::vector<void*> allocs;
+  double *U = nullptr, *R = nullptr, *Q = nullptr, *dQ = nullptr, *Fi = nullptr, *Fj = nullptr;
+  double *X = nullptr, *Act = nullptr, *Rv = nullptr, *Z = nullptr, *W = nullptr, *T = nullptr, *V = nullptr;
+  double *tmpA = nullptr, *tmpB = nullptr;
+  double *lam = nullptr, *sam = nullptr, *sad = nullptr, *lsum = nullptr, *vdt = nullptr, *sa_diag = nullptr;
+  void *fac = nullptr, *fac_sa = nullptr, *scr = nullptr;
+  Ctl* ctl = nullptr;
+  double *hv = nullptr, *h2v = nullptr, *H = nullptr, *y = nullptr, *hy = nullptr, *hist = nullptr, *coef = nullptr;
+  int hist_cap = 0;
+  Red red{};
+  bool has_q = false;
+};
+
+template <class T>
+static int dalloc(jr_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(&q, count * sizeof(T)));
+  HIPCHK(hipMemset(q, 0, count * sizeof(T)));
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return 0;
+}
+
+template <class D, class S>
+static int upload(jr_ctx* c, const D** dst, const S* src, size_t count) {
+  if (!src) return fail("a required plan array is null");
+  std::vector<D> h(count);
+  for (size_t i = 0; i < count; ++i) h[i] = static_cast<D>(src[i]);
+  D* d = nullptr;
+  if (dalloc(c, &d, count)) return -1;
+  HIPCHK(hipMemcpy(d, h.data(), count * sizeof(D), hipMemcpyHostToDevice));
+  *dst = d;
+  return 0;
+}
+
+#define TRY(x)          \
+  do {                  \
+    if ((x) != 0) {     \
+      return -1;        \
+    }                   \
+  } while (0)
+
+static dim3 grid1(size_t n, int per, int cases) { return dim3((unsigned)((n + per - 1) / per), (unsigned)cases); }
+
+// --- residual and tangent launch sequences -----------------------------------
+
+template <int NV, bool VISC>
+static void launch_residual(jr_ctx* c, const double* U, double* out, int first_order) {
+  const Plan& P = c->P;
+  hipLaunchKernelGGL((k_prim<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P.nc,
+                     c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
+  for (int s = 0; s < 2; ++s)
+    if (P.n_task[s] > 0)
+      hipLaunchKernelGGL((k_ghost<double, NV>), grid1(P.n_task[s], BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
+                         s, c->ph.gamma, c->ctl, c->Q, nullptr, 0);
+  hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Fi, c->Fj, 0);
+  hipLaunchKernelGGL((k_cells<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, nullptr, c->Q, nullptr, c->Fi, c->Fj, nullptr, out, 0);
+}
+
+// Requires c->Q (primitives with ghosts) for the same U.
+template <int NV, bool VISC>
+static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double* out, int first_order,
+                           const double* vdt, int check_active) {
+  const Plan& P = c->P;
+  hipLaunchKernelGGL((k_prim<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P.nc,
+                     c->ph.gamma, c->ctl, U, dU, nullptr, c->dQ, check_active);
+  for (int s = 0; s < 2; ++s)
+    if (P.n_task[s] > 0)
+      hipLaunchKernelGGL((k_ghost<Dual, NV>), grid1(P.n_task[s], BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, s,
+                         c->ph.gamma, c->ctl, c->Q, c->dQ, check_active);
+  hipLaunchKernelGGL((k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream,
+                     P, c->ph, first_order, c->ctl, c->Q, c->dQ, c->Fi, c->Fj, check_active);
+  hipLaunchKernelGGL((k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, dU, c->Q, c->dQ, c->Fi, c->Fj, vdt, out, check_active);
+}
+
+template <int NV>
+static void launch_speeds(jr_ctx* c, const double* U) {
+  const Plan& P = c->P;
+  hipLaunchKernelGGL((k_face_speeds<NV>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, c->lam, c->sam, c->sad);
+  hipLaunchKernelGGL((k_cell_speeds<NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl,
+                     U, c->lam, c->sam, c->sad, c->lsum, c->vdt, c->sa_diag);
+}
+
+template <class R, int NV>
+static void launch_factor(jr_ctx* c, const double* U) {
+  const Plan& P = c->P;
+  hipLaunchKernelGGL((k_factor<R, NV>), grid1(P.n_lines, 64, c->n_cases), dim3(64), 0, c->stream, P, c->ph, c->ctl, U,
+                     c->lam, c->lsum, c->sam, c->sad, c->sa_diag, (R*)c->fac, (R*)c->fac_sa);
+}
+
+template <class R, int NV>
+static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, int check_active) {
+  const Plan& P = c->P;
+  hipLaunchKernelGGL((k_apply<R, NV>), grid1((size_t)P.n_lines + P.n_free, 64, c->n_cases), dim3(64), 0, c->stream, P,
+                     c->ctl, (const R*)c->fac, (const R*)c->fac_sa, (R*)c->scr, r, z, accumulate, check_active);
+}
+
+// --- one Newton step ----------------------------------------------------------
+// Krylov basis layout: V[k][case][N], so V_k of every case is one contiguous
+// field batch (case stride N) that the apply and tangent kernels take as is.
+
+template <int NV, bool VISC, int MM, class R>
+static void gmres_and_update(jr_ctx* c) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  const size_t N = c->N, vstride = (size_t)c->n_cases * N;
+  const int fo = c->exact ? 0 : 1;
+  for (int cyc = 0; cyc < c->restarts; ++cyc) {
+    hipLaunchKernelGGL((k_cycle_begin<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->Rv, c->V, N, c->restarts,
+                       c->ktol);
+    for (int j = 0; j < c->m; ++j) {
+      const double* Vj = c->V + (size_t)j * vstride;
+      launch_apply<R, NV>(c, Vj, c->Z, 0, 1);
+      launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1);
+      hipLaunchKernelGGL((k_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W, j,
+                         c->red, c->hv);
+      hipLaunchKernelGGL((k_update_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
+                         j, c->hv, c->red, c->h2v);
+      hipLaunchKernelGGL((k_update_norm<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
+                         j, c->hv, c->h2v, c->red, c->H, c->m);
+      hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
+                         c->V + (size_t)(j + 1) * vstride, N);
+    }
+    hipLaunchKernelGGL(k_lstsq, dim3(c->n_cases), dim3(64), 0, c->stream, c->n_cases, c->ctl, c->H, c->m, c->y, c->hy);
+    hipLaunchKernelGGL((k_combine<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->y, c->hy,
+                       c->m, c->T, c->Act, c->Rv, c->R, c->red);
+    launch_apply<R, NV>(c, c->T, c->X, 1, 1);
+  }
+  const dim3 cg(c->nblk, c->n_cases);
+  if (c->max_rel > 0.0)
+    hipLaunchKernelGGL((k_relax<NV>), cg, dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, c->U, c->X, c->red,
+                       c->max_rel);
+  Ramp ramp = c->ramp;
+  ramp.relaxed = c->max_rel > 0.0;
+  hipLaunchKernelGGL((k_check<NV>), cg, dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, c->U, c->X, c->Act, c->R,
+                     c->red, ramp);
+  hipLaunchKernelGGL((k_update<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->U, c->X, ramp.relaxed);
+}
+
+template <int NV, bool VISC, int MM, class R>
+static void newton_step(jr_ctx* c, double* hist_row) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  launch_residual<NV, VISC>(c, c->U, c->R, 0);
+  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
+                     hist_row, c->n_cases);
+  launch_speeds<NV>(c, c->U);
+  launch_factor<R, NV>(c, c->U);
+  gmres_and_update<NV, VISC, MM, R>(c);
+}
+
+// --- dispatch over the compile-time variants ------------------------------------
+
+template <int NV, bool VISC, int MM>
+static void step_dispatch_r(jr_ctx* c, double* hist_row) {
+  if (c->mixed) newton_step<NV, VISC, MM, float>(c, hist_row);
+  else newton_step<NV, VISC, MM, double>(c, hist_row);
+}
+
+template <int NV, bool VISC>
+static void step_dispatch_m(jr_ctx* c, double* hist_row) {
+  if (c->m <= 16) step_dispatch_r<NV, VISC, 16>(c, hist_row);
+  else if (c->m <= 32) step_dispatch_r<NV, VISC, 32>(c, hist_row);
+  else step_dispatch_r<NV, VISC, 64>(c, hist_row);
+}
+
+static void step_dispatch(jr_ctx* c, double* hist_row) {
+  if (c->nv == 5) step_dispatch_m<5, true>(c, hist_row);
+  else if (c->ph.viscous) step_dispatch_m<4, true>(c, hist_row);
+  else step_dispatch_m<4, false>(c, hist_row);
+}
+
+static void residual_dispatch(jr_ctx* c, const double* U, double* out, int fo) {
+  if (c->nv == 5) launch_residual<5, true>(c, U, out, fo);
+  else if (c->ph.viscous) launch_residual<4, true>(c, U, out, fo);
+  else launch_residual<4, false>(c, U, out, fo);
+}
+
+static void tangent_dispatch(jr_ctx* c, const double* U, const double* dU, double* out, int fo, const double* vdt) {
+  if (c->nv == 5) launch_tangent<5, true>(c, U, dU, out, fo, vdt, 0);
+  else if (c->ph.viscous) launch_tangent<4, true>(c, U, dU, out, fo, vdt, 0);
+  else launch_tangent<4, false>(c, U, dU, out, fo, vdt, 0);
+}
+
+static void speeds_dispatch(jr_ctx* c, const double* U) {
+  if (c->nv == 5) launch_speeds<5>(c, U);
+  else launch_speeds<4>(c, U);
+}
+
+static void factor_dispatch(jr_ctx* c, const double* U) {
+  if (c->nv == 5) { if (c->mixed) launch_factor<float, 5>(c, U); else launch_factor<double, 5>(c, U); }
+  else { if (c->mixed) launch_factor<float, 4>(c, U); else launch_factor<double, 4>(c, U); }
+}
+
+static void apply_dispatch(jr_ctx* c, const double* r, double* z) {
+  if (c->nv == 5) { if (c->mixed) launch_apply<float, 5>(c, r, z, 0, 0); else launch_apply<double, 5>(c, r, z, 0, 0); }
+  else { if (c->mixed) launch_apply<float, 4>(c, r, z, 0, 0); else launch_apply<double, 4>(c, r, z, 0, 0); }
+}
+
+static int forces_dispatch(jr_ctx* c, double length, double cx, double cy) {
+  const Plan& P = c->P;
+  if (c->nv == 5) {
+    launch_residual<5, true>(c, c->U, c->tmpA, 0);
+    hipLaunchKernelGGL((k_forces<5>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, length, cx,
+                       cy, c->coef);
+  } else {
+    if (c->ph.viscous) launch_residual<4, true>(c, c->U, c->tmpA, 0);
+    else launch_residual<4, false>(c, c->U, c->tmpA, 0);
+    hipLaunchKernelGGL((k_forces<4>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, length, cx,
+                       cy, c->coef);
+  }
+  return 0;
+}
+
+// --- host helpers -----------------------------------------------------------------
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+static size_t field_bytes(const jr_ctx* c) { return (size_t)c->n_cases * c->N * sizeof(double); }
+
+extern "C" {
+
+const char* jr_last_error(void) { return g_error.c_str(); }
+
+int jr_device_count(int* n) {
+  HIPCHK(hipGetDeviceCount(n));
+  return 0;
+}
+
+int jr_device_alloc(size_t bytes, void** out) {
+  HIPCHK(hipMalloc(out, bytes ? bytes : 1));
+  return 0;
+}
+
+int jr_device_free(void* p) {
+  HIPCHK(hipFree(p));
+  return 0;
+}
+
+int jr_copy_to_device(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int jr_copy_to_host(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr_ctx** out) {
+  if (!g || !num || !out) return fail("jr_ctx_create: null argument");
+  if (n_cases < 1) return fail("jr_ctx_create: n_cases must be >= 1");
+  if (g->NI < 5 || g->NJ < 5) return fail("jr_ctx_create: canvas too small");
+  if (num->krylov_subspace < 1 || num->krylov_subspace > MMAX)
+    return fail("jr_ctx_create: krylov_subspace must lie in [1, " + std::to_string(MMAX) + "]");
+  if (num->krylov_restarts < 1) return fail("jr_ctx_create: krylov_restarts must be >= 1");
+  if (num->flux < 0 || num->flux > 1) return fail("jr_ctx_create: unknown flux");
+  if (num->limiter < 0 || num->limiter > 3) return fail("jr_ctx_create: unknown limiter");
+  if (num->sa && !num->viscous) return fail("jr_ctx_create: SA needs a viscous flow");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("jr_ctx_create: no HIP device");
+  jr_ctx* c = new jr_ctx();
+  auto bail = [&](int) {
+    jr_ctx_destroy(c);
+    return -1;
+  };
+  Plan& P = c->P;
+  P.NI = g->NI;
+  P.NJ = g->NJ;
+  P.nc = g->NI * g->NJ;
+  P.nfi = (g->NI + 1) * g->NJ;
+  P.nfj = g->NI * (g->NJ + 1);
+  const size_t nc = P.nc;
+  if (upload(c, &P.area, g->area, nc) || upload(c, &P.fi, g->face_i, 2 * (size_t)P.nfi) ||
+      upload(c, &P.fj, g->face_j, 2 * (size_t)P.nfj) ||
+      upload(c, &P.cen, g->centres, 2 * (size_t)(g->NI + 2) * (g->NJ + 2)) ||
+      upload(c, &P.dist, g->distance, nc) || upload(c, &P.interior, g->interior, nc) ||
+      upload(c, &P.nbr, g->neighbours, nc))
+    return bail(0);
+  // validate indices on the host before any kernel sees them
+  for (size_t i = 0; i < nc; ++i)
+    if (g->neighbours[i] < 0 || g->neighbours[i] >= (int64_t)nc) {
+      fail("jr_ctx_create: neighbour index out of range");
+      return bail(0);
+    }
+  for (int s = 0; s < 2; ++s) {
+    P.n_task[s] = g->n_task[s];
+    for (int t = 0; t < g->n_task[s]; ++t) {
+      if (g->task_dest[s][t] < 0 || g->task_dest[s][t] >= (int64_t)nc || g->task_src[s][t] < 0 ||
+          g->task_src[s][t] >= (int64_t)nc || g->task_kind[s][t] < 0 || g->task_kind[s][t] > 6) {
+        fail("jr_ctx_create: ghost-fill task out of range");
+        return bail(0);
+      }
+    }
+    if (upload(c, &P.t_dest[s], g->task_dest[s], g->n_task[s]) ||
+        upload(c, &P.t_src[s], g->task_src[s], g->n_task[s]) ||
+        upload(c, &P.t_kind[s], g->task_kind[s], g->n_task[s]) ||
+        upload(c, &P.t_norm[s], g->task_normal[s], 2 * (size_t)g->n_task[s]))
+      return bail(0);
+  }
+  P.n_lines = g->n_lines;
+  P.n_slots = g->n_slots;
+  int max_len = 0;
+  std::vector<uint8_t> on_chain(nc, 0);
+  for (int l = 0; l < g->n_lines; ++l) {
+    int64_t a = g->line_start[l], b = g->line_start[l + 1];
+    if (a < 0 || b < a || b > g->n_slots) {
+      fail("jr_ctx_create: line starts out of range");
+      return bail(0);
+    }
+    max_len = std::max<int>(max_len, (int)(b - a));
+    for (int64_t s = a; s < b; ++s) {
+      int64_t cell = g->line_cells[s];
+      if (cell < 0 || cell >= (int64_t)nc || on_chain[cell]) {
+        fail("jr_ctx_create: chain cell out of range or repeated");
+        return bail(0);
+      }
+      on_chain[cell] = 1;
+      bool up = g->line_up_sign[s] != 0, dn = g->line_down_sign[s] != 0;
+      if (up != dn || (up && (s + 1 >= b || g->line_up_face[s] < 0 || g->line_up_face[s] >= P.nfj ||
+                              g->line_down_face[s] < 0 || g->line_down_face[s] >= P.nfj))) {
+        fail("jr_ctx_create: chain link out of range");
+        return bail(0);
+      }
+    }
+    if (b > a && g->line_up_sign[b - 1] != 0) {
+      fail("jr_ctx_create: the last slot of a chain must have no link");
+      return bail(0);
+    }
+  }
+  P.max_len = max_len;
+  std::vector<int> freec;
+  for (size_t i = 0; i < nc; ++i)
+    if (!on_chain[i]) freec.push_back((int)i);
+  P.n_free = (int)freec.size();
+  if (upload(c, &P.l_start, g->line_start, (size_t)g->n_lines + 1) ||
+      upload(c, &P.l_cell, g->line_cells, g->n_slots) || upload(c, &P.l_up, g->line_up_face, g->n_slots) ||
+      upload(c, &P.l_ups, g->line_up_sign, g->n_slots) || upload(c, &P.l_dn, g->line_down_face, g->n_slots) ||
+      upload(c, &P.l_dns, g->line_down_sign, g->n_slots) || upload(c, &P.free_cell, freec.data(), freec.size()))
+    return bail(0);
+  P.n_walls = g->n_walls;
+  for (int w = 0; w < g->n_walls; ++w) {
+    int64_t fam = g->wall_family[w], f = g->wall_face[w], cell = g->wall_cell[w];
+    if ((fam != 0 && fam != 1) || f < 0 || f >= (fam == 0 ? P.nfi : P.nfj) || cell < 0 || cell >= (int64_t)nc) {
+      fail("jr_ctx_create: wall face out of range");
+      return bail(0);
+    }
+  }
+  if (upload(c, &P.w_cell, g->wall_cell, g->n_walls) || upload(c, &P.w_fam, g->wall_family, g->n_walls) ||
+      upload(c, &P.w_face, g->wall_face, g->n_walls) || upload(c, &P.w_into, g->wall_into, g->n_walls) ||
+      upload(c, &P.w_mid, g->wall_mid, 2 * (size_t)g->n_walls))
+    return bail(0);
+
+  Physics& ph = c->ph;
+  ph.gamma = num->gamma;
+  ph.kappa = num->kappa;
+  ph.flux = num->flux;
+  ph.limiter = num->limiter;
+  ph.nv = num->sa ? 5 : 4;
+  ph.viscous = num->viscous;
+  ph.sa = num->sa;
+  ph.sutherland = num->sutherland;
+  ph.c_cr1 = num->c_cr1;
+  c->nv = ph.nv;
+  c->n_cases = n_cases;
+  c->m = num->krylov_subspace;
+  c->restarts = num->krylov_restarts;
+  c->exact = num->jacobian_exact;
+  c->mixed = num->mixed;
+  c->ktol = num->krylov_tol;
+  c->max_rel = num->max_relative_update;
+  c->ramp.cfl_growth = num->cfl_growth;
+  c->ramp.cfl_cutback = num->cfl_cutback;
+  c->ramp.cfl_min = num->cfl_min;
+  c->ramp.cfl_max = num->cfl_max;
+  c->ramp.krylov_success = num->krylov_success;
+  c->N = (size_t)c->nv * nc;
+  c->nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (c->N + BLOCK * 4 - 1) / (BLOCK * 4)));
+
+  const size_t F = (size_t)n_cases * c->N;
+  const size_t nf = (size_t)P.nfi + P.nfj;
+  if (dalloc(c, &c->U, F) || dalloc(c, &c->R, F) || dalloc(c, &c->Q, F) || dalloc(c, &c->dQ, F) ||
+      dalloc(c, &c->Fi, (size_t)n_cases * c->nv * P.nfi) || dalloc(c, &c->Fj, (size_t)n_cases * c->nv * P.nfj) ||
+      dalloc(c, &c->X, F) || dalloc(c, &c->Act, F) || dalloc(c, &c->Rv, F) || dalloc(c, &c->Z, F) ||
+      dalloc(c, &c->W, F) || dalloc(c, &c->T, F) || dalloc(c, &c->V, F * (c->m + 1)) || dalloc(c, &c->tmpA, F) ||
+      dalloc(c, &c->tmpB, F) || dalloc(c, &c->lam, (size_t)n_cases * nf) || dalloc(c, &c->sam, (size_t)n_cases * nf) ||
+      dalloc(c, &c->sad, (size_t)n_cases * nf) || dalloc(c, &c->lsum, (size_t)n_cases * nc) ||
+      dalloc(c, &c->vdt, (size_t)n_cases * nc) || dalloc(c, &c->sa_diag, (size_t)n_cases * nc))
+    return bail(0);
+  const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
+  {
+    void* p;
+    if (dalloc(c, (char**)&p, (size_t)n_cases * P.n_slots * 48 * rsz)) return bail(0);
+    c->fac = p;
+    if (dalloc(c, (char**)&p, (size_t)n_cases * P.n_slots * 3 * rsz)) return bail(0);
+    c->fac_sa = p;
+    if (dalloc(c, (char**)&p, (size_t)n_cases * P.n_slots * c->nv * rsz)) return bail(0);
+    c->scr = p;
+  }
+  if (dalloc(c, &c->ctl, (size_t)n_cases) || dalloc(c, &c->hv, (size_t)n_cases * (MMAX + 1)) ||
+      dalloc(c, &c->h2v, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->H, (size_t)n_cases * (MMAX + 1) * MMAX) ||
+      dalloc(c, &c->y, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->hy, (size_t)n_cases * (MMAX + 1)) ||
+      dalloc(c, &c->coef, (size_t)n_cases * 8))
+    return bail(0);
+  c->red.stride = 1024 * (MMAX + 2);
+  if (dalloc(c, &c->red.partials, (size_t)n_cases * c->red.stride) ||
+      dalloc(c, &c->red.counters, (size_t)n_cases * 4))
+    return bail(0);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail("jr_ctx_create: stream");
+    return bail(0);
+  }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    fail("jr_ctx_create: events");
+    return bail(0);
+  }
+  // Control blocks: freestream unset, CFL 1.
+  std::vector<Ctl> h(n_cases);
+  for (auto& x : h) {
+    std::memset(&x, 0, sizeof(Ctl));
+    x.q_inf[0] = 1.0;
+    x.q_inf[3] = 1.0 / ph.gamma;
+    x.cfl = 1.0;
+    x.res_prev = INFINITY;
+  }
+  if (hipMemcpy(c->ctl, h.data(), sizeof(Ctl) * n_cases, hipMemcpyHostToDevice) != hipSuccess) {
+    fail("jr_ctx_create: control upload");
+    return bail(0);
+  }
+  *out = c;
+  return 0;
+}
+
+int jr_ctx_destroy(jr_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->hist) (void)hipFree(c->hist);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+static int get_ctl(jr_ctx* c, std::vector<Ctl>& h) {
+  h.resize(c->n_cases);
+  HIPCHK(hipMemcpyAsync(h.data(), c->ctl, sizeof(Ctl) * c->n_cases, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int put_ctl(jr_ctx* c, const std::vector<Ctl>& h) {
+  HIPCHK(hipMemcpyAsync(c->ctl, h.data(), sizeof(Ctl) * c->n_cases, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_set_freestream(jr_ctx* c, const double* q_inf, const double* mu_inf) {
+  if (!c || !q_inf) return fail("jr_set_freestream: null argument");
+  if (c->ph.viscous && !mu_inf) return fail("jr_set_freestream: a viscous context needs mu_inf");
+  std::vector<Ctl> h;
+  TRY(get_ctl(c, h));
+  for (int k = 0; k < c->n_cases; ++k) {
+    for (int v = 0; v < 5; ++v) h[k].q_inf[v] = q_inf[k * 5 + v];
+    h[k].mu_inf = mu_inf ? mu_inf[k] : 0.0;
+    if (!(h[k].q_inf[0] > 0.0) || !(h[k].q_inf[3] > 0.0)) return fail("jr_set_freestream: non-physical freestream");
+  }
+  c->has_q = true;
+  return put_ctl(c, h);
+}
+
+int jr_set_state(jr_ctx* c, const double* U) {
+  if (!c || !U) return fail("jr_set_state: null argument");
+  HIPCHK(hipMemcpyAsync(c->U, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_get_state(jr_ctx* c, double* U) {
+  if (!c || !U) return fail("jr_get_state: null argument");
+  HIPCHK(hipMemcpyAsync(U, c->U, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_residual_dev(jr_ctx* c, const double* U, double* R, int first_order) {
+  if (!c || !U || !R) return fail("jr_residual_dev: null argument");
+  if (!c->has_q) return fail("jr_residual_dev: set the freestream first");
+  residual_dispatch(c, U, R, first_order);
+  return check_launch();
+}
+
+int jr_residual(jr_ctx* c, const double* U, double* R, int first_order) {
+  if (!c || !U || !R) return fail("jr_residual: null argument");
+  HIPCHK(hipMemcpyAsync(c->tmpA, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  TRY(jr_residual_dev(c, c->tmpA, c->tmpB, first_order));
+  HIPCHK(hipMemcpyAsync(R, c->tmpB, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_tangent_dev(jr_ctx* c, const double* U, const double* dU, double* out, int first_order, int with_vdt) {
+  if (!c || !U || !dU || !out) return fail("jr_tangent_dev: null argument");
+  if (!c->has_q) return fail("jr_tangent_dev: set the freestream first");
+  // primal primitives with ghosts at U, then the tangent
+  residual_dispatch(c, U, c->R, first_order);
+  if (with_vdt) speeds_dispatch(c, U);
+  tangent_dispatch(c, U, dU, out, first_order, with_vdt ? c->vdt : nullptr);
+  return check_launch();
+}
+
+int jr_tangent(jr_ctx* c, const double* U, const double* dU, double* out, int first_order, int with_vdt) {
+  if (!c || !U || !dU || !out) return fail("jr_tangent: null argument");
+  HIPCHK(hipMemcpyAsync(c->tmpA, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Z, dU, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  TRY(jr_tangent_dev(c, c->tmpA, c->Z, c->tmpB, first_order, with_vdt));
+  HIPCHK(hipMemcpyAsync(out, c->tmpB, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_wave_speeds(jr_ctx* c, const double* U, double* lam, double* lsum) {
+  if (!c || !U) return fail("jr_wave_speeds: null argument");
+  if (!c->has_q) return fail("jr_wave_speeds: set the freestream first");
+  HIPCHK(hipMemcpyAsync(c->tmpA, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  speeds_dispatch(c, c->tmpA);
+  TRY(check_launch());
+  const size_t nf = (size_t)c->P.nfi + c->P.nfj;
+  if (lam) HIPCHK(hipMemcpyAsync(lam, c->lam, (size_t)c->n_cases * nf * 8, hipMemcpyDeviceToHost, c->stream));
+  if (lsum) HIPCHK(hipMemcpyAsync(lsum, c->lsum, (size_t)c->n_cases * c->P.nc * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_precond_factor(jr_ctx* c, const double* U) {
+  if (!c || !U) return fail("jr_precond_factor: null argument");
+  if (!c->has_q) return fail("jr_precond_factor: set the freestream first");
+  HIPCHK(hipMemcpyAsync(c->tmpA, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  speeds_dispatch(c, c->tmpA);
+  factor_dispatch(c, c->tmpA);
+  TRY(check_launch());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_precond_apply(jr_ctx* c, const double* r, double* z) {
+  if (!c || !r || !z) return fail("jr_precond_apply: null argument");
+  HIPCHK(hipMemcpyAsync(c->tmpA, r, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  apply_dispatch(c, c->tmpA, c->tmpB);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(z, c->tmpB, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_newton_init(jr_ctx* c, const double* cfl0) {
+  if (!c || !cfl0) return fail("jr_newton_init: null argument");
+  std::vector<Ctl> h;
+  TRY(get_ctl(c, h));
+  for (int k = 0; k < c->n_cases; ++k) {
+    if (!(cfl0[k] > 0.0)) return fail("jr_newton_init: cfl must be positive");
+    h[k].cfl = cfl0[k];
+    h[k].res_prev = INFINITY;
+    h[k].steps = 0;
+    h[k].rejected = 0;
+    h[k].accept = 0;
+    h[k].omega = 1.0;
+  }
+  return put_ctl(c, h);
+}
+
+int jr_set_cfl(jr_ctx* c, const double* cfl) {
+  if (!c || !cfl) return fail("jr_set_cfl: null argument");
+  std::vector<Ctl> h;
+  TRY(get_ctl(c, h));
+  for (int k = 0; k < c->n_cases; ++k) h[k].cfl = cfl[k];
+  return put_ctl(c, h);
+}
+
+int jr_newton_block(jr_ctx* c, int n_steps, double* hist_host) {
+  if (!c) return fail("jr_newton_block: null context");
+  if (n_steps < 1) return fail("jr_newton_block: n_steps must be >= 1");
+  if (!c->has_q) return fail("jr_newton_block: set the freestream first");
+  if (n_steps > c->hist_cap) {
+    if (c->hist) (void)hipFree(c->hist);
+    c->hist = nullptr;
+    HIPCHK(hipMalloc(&c->hist, (size_t)n_steps * c->n_cases * sizeof(double)));
+    c->hist_cap = n_steps;
+  }
+  for (int s = 0; s < n_steps; ++s) {
+    step_dispatch(c, c->hist + (size_t)s * c->n_cases);
+    TRY(check_launch());
+  }
+  if (hist_host) {
+    HIPCHK(hipMemcpyAsync(hist_host, c->hist, (size_t)n_steps * c->n_cases * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_get_scalars(jr_ctx* c, double* out) {
+  if (!c || !out) return fail("jr_get_scalars: null argument");
+  std::vector<Ctl> h;
+  TRY(get_ctl(c, h));
+  for (int k = 0; k < c->n_cases; ++k) {
+    double* o = out + k * 8;
+    o[0] = h[k].cfl;
+    o[1] = h[k].res_prev;
+    o[2] = h[k].norm;
+    o[3] = h[k].lin_red;
+    o[4] = h[k].omega;
+    o[5] = h[k].accept;
+    o[6] = h[k].steps;
+    o[7] = h[k].rejected;
+  }
+  return 0;
+}
+
+int jr_forces(jr_ctx* c, double length, double cx, double cy, double* out) {
+  if (!c || !out) return fail("jr_forces: null argument");
+  if (!c->has_q) return fail("jr_forces: set the freestream first");
+  forces_dispatch(c, length, cx, cy);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"
+
+template <int NV, bool VISC, int MM, class R>
+static void profile_launch(jr_ctx* c, int which) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  const size_t N = c->N, vstride = (size_t)c->n_cases * N;
+  switch (which) {
+    case 0: launch_residual<NV, VISC>(c, c->U, c->R, 0); break;
+    case 1: launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, c->exact ? 0 : 1, c->vdt, 0); break;
+    case 2: launch_apply<R, NV>(c, c->Rv, c->Z, 0, 0); break;
+    case 3:
+      hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                         c->stream, P, c->ph, 0, c->ctl, c->Q, nullptr, c->Fi, c->Fj, 0);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                         c->stream, P, c->ph, c->exact ? 0 : 1, c->ctl, c->Q, c->dQ, c->Fi, c->Fj, 0);
+      break;
+    case 5: {
+      int j = c->m / 2;
+      hipLaunchKernelGGL((k_update_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
+                         j, c->hv, c->red, c->h2v);
+      break;
+    }
+    case 6: launch_factor<R, NV>(c, c->U); break;
+    case 7:
+      hipLaunchKernelGGL((k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                         c->ctl, c->U, c->Z, c->Q, c->dQ, c->Fi, c->Fj, c->vdt, c->W, 0);
+      break;
+    default: break;
+  }
+}
+
+template <int NV, bool VISC, int MM>
+static void profile_r(jr_ctx* c, int which) {
+  if (c->mixed) profile_launch<NV, VISC, MM, float>(c, which);
+  else profile_launch<NV, VISC, MM, double>(c, which);
+}
+template <int NV, bool VISC>
+static void profile_m(jr_ctx* c, int which) {
+  if (c->m <= 16) profile_r<NV, VISC, 16>(c, which);
+  else if (c->m <= 32) profile_r<NV, VISC, 32>(c, which);
+  else profile_r<NV, VISC, 64>(c, which);
+}
+static void profile_dispatch(jr_ctx* c, int which) {
+  if (c->nv == 5) profile_m<5, true>(c, which);
+  else if (c->ph.viscous) profile_m<4, true>(c, which);
+  else profile_m<4, false>(c, which);
+}
+
+extern "C" int jr_profile(jr_ctx* c, int which, int reps, float* ms) {
+  if (!c || !ms) return fail("jr_profile: null argument");
+  if (which < 0 || which > 7 || reps < 1) return fail("jr_profile: bad kernel id or reps");
+  if (!c->has_q) return fail("jr_profile: set the freestream first");
+  // inputs: the residual at the device state (fills Q), wave speeds, factor,
+  // a Krylov-like vector in Z and Rv, V filled with it
+  residual_dispatch(c, c->U, c->R, 0);
+  speeds_dispatch(c, c->U);
+  factor_dispatch(c, c->U);
+  HIPCHK(hipMemcpyAsync(c->Z, c->R, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Rv, c->R, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->W, c->R, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  for (int k = 0; k <= c->m; ++k)
+    HIPCHK(hipMemcpyAsync(c->V + (size_t)k * c->n_cases * c->N, c->R, field_bytes(c), hipMemcpyDeviceToDevice,
+                          c->stream));
+  tangent_dispatch(c, c->U, c->Z, c->W, c->exact ? 0 : 1, c->vdt);   // fills dQ for id 4, 7
+  {
+    std::vector<Ctl> h;
+    TRY(get_ctl(c, h));
+    for (auto& x : h) x.active = 1;
+    TRY(put_ctl(c, h));
+  }
+  profile_dispatch(c, which);   // warm
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; ++r) profile_dispatch(c, which);
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  TRY(check_launch());
+  float t = 0.0f;
+  HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t / reps;
+  return 0;
+}
+
+extern "C" {
+
+int jr_timer_start(jr_ctx* c) {
+  if (!c) return fail("jr_timer_start: null context");
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  return 0;
+}
+
+int jr_timer_stop(jr_ctx* c, float* ms) {
+  if (!c || !ms) return fail("jr_timer_stop: null argument");
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return 0;
+}
+
+int jr_synchronize(jr_ctx* c) {
+  if (!c) return fail("jr_synchronize: null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,526 @@
+// GMRES(m) with CGS2 Arnoldi, the Newton update with accept/reject and the
+// SER CFL ramp, and the wall force integration -- all device-resident, one
+// control block per case, no host synchronisation inside a Newton step.
+//
+// Restates src/jaxrans/solvers/krylov.py:34-137, solvers/implicit.py:54-166,
+// post/forces.py:48-161.
+#pragma once
+#include "jr_solver.h"
+
+namespace jr {
+
+// Vector kernels run a grid (nblk, n_cases) over the N = NV*nc entries of each
+// case, grid-strided.  Reduction scratch per case: partials [nblk][MM+2], one counter.
+struct Red {
+  double* partials;     // [n_cases][maxblk][MMAX+2]
+  unsigned* counters;   // [n_cases][4]
+  int stride;           // doubles per case in partials
+};
+
+#define JR_VEC_LOOP(N) for (size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x; e < (N); e += (size_t)gridDim.x * BLOCK)
+
+// After the residual: the reported norm (density row, volume weighted,
+// residual.py:237-240), ||R||, and the GMRES start x = 0, action = 0, r = -R.
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, const double* __restrict__ R,
+                                                 double* __restrict__ X, double* __restrict__ Act,
+                                                 double* __restrict__ Rv, Red red, double* __restrict__ hist,
+                                                 int n_cases) {
+  __shared__ double sh[3][4];
+  const int cs = blockIdx.y;
+  const int nc = P.nc;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  double v[3] = {0.0, 0.0, 0.0};
+  JR_VEC_LOOP(N) {
+    double r = R[base + e];
+    X[base + e] = 0.0;
+    Act[base + e] = 0.0;
+    Rv[base + e] = -r;
+    v[2] += r * r;
+    if (e < (size_t)nc && P.interior[e]) {
+      double w = P.area[e];
+      v[0] += w * (r * r);
+      v[1] += w;
+    }
+  }
+  double tot[3];
+  if (finish_sum<3>(v, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
+    if (threadIdx.x == 0) {
+      Ctl& c = ctl[cs];
+      c.norm = sqrt(tot[0] / tot[1]);
+      c.bnorm = sqrt(tot[2]);
+      c.rnorm = c.bnorm;
+      c.cycles = 0;
+      if (hist) hist[cs] = c.norm;
+    }
+  }
+}
+
+// Cycle start: active = cycles < maxiter and |r| > tol |b| (krylov.py:85-87);
+// V0 = r / max(beta, tiny).
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_cycle_begin(int nc, Ctl* __restrict__ ctl, const double* __restrict__ Rv,
+                                                       double* __restrict__ V0, size_t vstride, int maxiter,
+                                                       double tol) {
+  const int cs = blockIdx.y;
+  Ctl& c = ctl[cs];
+  const bool act = c.cycles < maxiter && c.rnorm > tol * c.bnorm;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c.active = act;
+    c.beta = c.rnorm;
+  }
+  if (!act) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double den = fmax(c.rnorm, 2.2250738585072014e-308);
+  double* v0 = V0 + (size_t)cs * vstride;
+  JR_VEC_LOOP(N) v0[e] = Rv[base + e] / den;
+}
+
+// CGS2 pass 1: h_k = V_k . w, k <= j.
+template <int NV, int MM>
+__global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ V,
+                                                size_t vstride, size_t vcase, const double* __restrict__ W, int j,
+                                                Red red, double* __restrict__ hv) {
+  __shared__ double sh[MM + 1][4];
+  const int cs = blockIdx.y;
+  if (!ctl[cs].active) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double* Vc = V + (size_t)cs * vcase;
+  double acc[MM + 1];
+#pragma unroll
+  for (int k = 0; k <= MM; ++k) acc[k] = 0.0;
+  JR_VEC_LOOP(N) {
+    double w = W[base + e];
+#pragma unroll
+    for (int k = 0; k <= MM; ++k)
+      if (k <= j) acc[k] += Vc[(size_t)k * vstride + e] * w;
+  }
+  double tot[MM + 1];
+  if (finish_sum<MM + 1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot, j + 1)) {
+    if (threadIdx.x <= j) hv[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
+  }
+}
+
+// CGS2 pass 2: w <- w - sum h_k V_k, then h2_k = V_k . w.
+template <int NV, int MM>
+__global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
+                                                       const double* __restrict__ V, size_t vstride, size_t vcase,
+                                                       double* __restrict__ W, int j, const double* __restrict__ hv,
+                                                       Red red, double* __restrict__ h2v) {
+  __shared__ double sh[MM + 1][4];
+  const int cs = blockIdx.y;
+  if (!ctl[cs].active) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double* Vc = V + (size_t)cs * vcase;
+  double h[MM + 1];
+#pragma unroll
+  for (int k = 0; k <= MM; ++k) h[k] = k <= j ? hv[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  double acc[MM + 1];
+#pragma unroll
+  for (int k = 0; k <= MM; ++k) acc[k] = 0.0;
+  JR_VEC_LOOP(N) {
+    double vk[MM + 1];
+    double w = W[base + e];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k <= MM; ++k)
+      if (k <= j) {
+        vk[k] = Vc[(size_t)k * vstride + e];
+        s += h[k] * vk[k];
+      }
+    w = w - s;
+    W[base + e] = w;
+#pragma unroll
+    for (int k = 0; k <= MM; ++k)
+      if (k <= j) acc[k] += vk[k] * w;
+  }
+  double tot[MM + 1];
+  if (finish_sum<MM + 1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot, j + 1)) {
+    if (threadIdx.x <= j) h2v[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
+  }
+}
+
+// CGS2 pass 3: w <- w - sum h2_k V_k, |w|; H[:, j] = h + h2, H[j+1, j] = |w|.
+template <int NV, int MM>
+__global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __restrict__ ctl,
+                                                       const double* __restrict__ V, size_t vstride, size_t vcase,
+                                                       double* __restrict__ W, int j, const double* __restrict__ hv,
+                                                       const double* __restrict__ h2v, Red red, double* __restrict__ H,
+                                                       int m) {
+  __shared__ double sh[1][4];
+  const int cs = blockIdx.y;
+  if (!ctl[cs].active) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double* Vc = V + (size_t)cs * vcase;
+  double h2[MM + 1];
+#pragma unroll
+  for (int k = 0; k <= MM; ++k) h2[k] = k <= j ? h2v[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  double acc[1] = {0.0};
+  JR_VEC_LOOP(N) {
+    double w = W[base + e];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k <= MM; ++k)
+      if (k <= j) s += h2[k] * Vc[(size_t)k * vstride + e];
+    w = w - s;
+    W[base + e] = w;
+    acc[0] += w * w;
+  }
+  double tot[1];
+  if (finish_sum<1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
+    double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;   // H[row][col], row-major (MMAX+1) x MMAX
+    for (int k = threadIdx.x; k <= j; k += BLOCK)
+      Hc[k * MMAX + j] = hv[(size_t)cs * (MMAX + 1) + k] + h2v[(size_t)cs * (MMAX + 1) + k];
+    if (threadIdx.x == 0) {
+      double nw = sqrt(tot[0]);
+      Hc[(j + 1) * MMAX + j] = nw;
+      for (int k = j + 2; k <= m; ++k) Hc[k * MMAX + j] = 0.0;
+      const_cast<Ctl*>(ctl)[cs].wnorm = nw;
+    }
+  }
+}
+
+// V_{j+1} = w / max(|w|, tiny)
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_normalize(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ W,
+                                                     double* __restrict__ Vj1, size_t vcase) {
+  const int cs = blockIdx.y;
+  if (!ctl[cs].active) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double den = fmax(ctl[cs].wnorm, 2.2250738585072014e-308);
+  double* v = Vj1 + (size_t)cs * vcase;
+  JR_VEC_LOOP(N) v[e] = W[base + e] / den;
+}
+
+// Givens least squares of the (m+1) x m Hessenberg with the pivot drop
+// (krylov.py:100-137); y and Hy per case.  One thread per case.
+__global__ void k_lstsq(int n_cases, const Ctl* __restrict__ ctl, const double* __restrict__ H, int m,
+                        double* __restrict__ y, double* __restrict__ hy) {
+  __shared__ double h[MMAX + 1][MMAX];
+  const int cs = blockIdx.x;
+  if (!ctl[cs].active) return;
+  const double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;
+  for (int e = threadIdx.x; e < (m + 1) * MMAX; e += blockDim.x) h[e / MMAX][e % MMAX] = Hc[e];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double g[MMAX + 1], cs_[MMAX], sn[MMAX];
+  g[0] = ctl[cs].beta;
+  for (int i = 1; i <= m; ++i) g[i] = 0.0;
+  for (int jj = 0; jj < m; ++jj) {
+    for (int i = 0; i < jj; ++i) {
+      double a = h[i][jj], c = h[i + 1][jj];
+      h[i][jj] = cs_[i] * a + sn[i] * c;
+      h[i + 1][jj] = cs_[i] * c - sn[i] * a;
+    }
+    double a = h[jj][jj], c = h[jj + 1][jj];
+    double r = sqrt(a * a + c * c);
+    double safe = r > 0 ? r : 1.0;
+    cs_[jj] = r > 0 ? a / safe : 1.0;
+    sn[jj] = r > 0 ? c / safe : 0.0;
+    h[jj][jj] = r;
+    h[jj + 1][jj] = 0.0;
+    double gj = g[jj];
+    g[jj] = cs_[jj] * gj;
+    g[jj + 1] = -sn[jj] * gj;
+  }
+  double largest = fabs(h[0][0]);
+  for (int jj = 1; jj < m; ++jj) largest = fmax(largest, fabs(h[jj][jj]));
+  const double rcond = 2.220446049250313e-16 * (m + 1);
+  double yy[MMAX];
+  for (int jj = m - 1; jj >= 0; --jj) {
+    double rest = g[jj];
+    for (int k = jj + 1; k < m; ++k) rest = rest - h[jj][k] * yy[k];
+    yy[jj] = fabs(h[jj][jj]) > rcond * largest ? rest / h[jj][jj] : 0.0;
+  }
+  for (int jj = 0; jj < m; ++jj) y[(size_t)cs * (MMAX + 1) + jj] = yy[jj];
+  // H y with the original H
+  for (int i = 0; i <= m; ++i) {
+    double s = 0.0;
+    for (int jj = 0; jj < m; ++jj) s += Hc[i * MMAX + jj] * yy[jj];
+    hy[(size_t)cs * (MMAX + 1) + i] = s;
+  }
+}
+
+// End of a cycle: T = sum_{k<m} y_k V_k; action += sum_{k<=m} (Hy)_k V_k;
+// r = b - action = -R - action; |r|.
+template <int NV, int MM>
+__global__ void __launch_bounds__(BLOCK) k_combine(int nc, Ctl* __restrict__ ctl, const double* __restrict__ V,
+                                                   size_t vstride, size_t vcase, const double* __restrict__ y,
+                                                   const double* __restrict__ hy, int m, double* __restrict__ T,
+                                                   double* __restrict__ Act, double* __restrict__ Rv,
+                                                   const double* __restrict__ R, Red red) {
+  __shared__ double sh[1][4];
+  const int cs = blockIdx.y;
+  if (!ctl[cs].active) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double* Vc = V + (size_t)cs * vcase;
+  double yc[MM + 1], hc[MM + 1];
+#pragma unroll
+  for (int k = 0; k <= MM; ++k) {
+    yc[k] = k < m ? y[(size_t)cs * (MMAX + 1) + k] : 0.0;
+    hc[k] = k <= m ? hy[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  }
+  double acc[1] = {0.0};
+  JR_VEC_LOOP(N) {
+    double t = 0.0, a = 0.0;
+#pragma unroll
+    for (int k = 0; k <= MM; ++k)
+      if (k <= m) {
+        double v = Vc[(size_t)k * vstride + e];
+        if (k < m) t += yc[k] * v;
+        a += hc[k] * v;
+      }
+    T[base + e] = t;
+    double act = Act[base + e] + a;
+    Act[base + e] = act;
+    double r = -R[base + e] - act;
+    Rv[base + e] = r;
+    acc[0] += r * r;
+  }
+  double tot[1];
+  if (finish_sum<1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
+    if (threadIdx.x == 0) {
+      ctl[cs].rnorm = sqrt(tot[0]);
+      ctl[cs].cycles += 1;
+    }
+  }
+}
+
+// Relaxation factor (implicit.py:54-81): the largest relative change of
+// density, linearized pressure and nu_hat; omega = min(1, fraction / worst).
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_relax(Plan P, double gamma, Ctl* __restrict__ ctl,
+                                                 const double* __restrict__ U, const double* __restrict__ X,
+                                                 Red red, double fraction) {
+  __shared__ double sh[1][4];
+  __shared__ int is_last;
+  const int cs = blockIdx.y;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  double worst = 0.0;
+  for (int c = blockIdx.x * BLOCK + threadIdx.x; c < nc; c += gridDim.x * BLOCK) {
+    double u[NV], q[NV], d[NV];
+    for (int k = 0; k < NV; ++k) { u[k] = U[base + (size_t)k * nc + c]; d[k] = X[base + (size_t)k * nc + c]; }
+    primitive<double, NV>(u, gamma, q);
+    double de = d[3] - q[1] * d[1] - q[2] * d[2];
+    double dp = (gamma - 1.0) * (de + 0.5 * (q[1] * q[1] + q[2] * q[2]) * d[0]);
+    double r1 = fabs(d[0]) / q[0], r2 = fabs(dp) / q[3];
+    double w = fmax(r1, r2);
+    if constexpr (NV > 4) {
+      double nh = u[4] / q[0];
+      w = fmax(w, fabs((d[4] - nh * d[0]) / q[0]) / fmax(fabs(nh), 1.0));
+    }
+    worst = fmax(worst, w);
+  }
+  // max-reduction: wave, block, then the last block over the partials
+  double wv = wave_max(worst);
+  if ((threadIdx.x & 63) == 0) sh[0][threadIdx.x >> 6] = wv;
+  __syncthreads();
+  double* part = red.partials + (size_t)cs * red.stride;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+    __threadfence();
+    unsigned prev = atomicAdd(red.counters + cs * 4, 1u);
+    is_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int b = 0; b < (int)gridDim.x; ++b)
+      m = fmax(m, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    ctl[cs].omega = fmin(1.0, fraction / fmax(m, 1e-300));
+    red.counters[cs * 4] = 0u;
+  }
+}
+
+// Accept/reject and the SER CFL ramp (implicit.py:139-166).
+struct Ramp {
+  double cfl_growth, cfl_cutback, cfl_min, cfl_max, krylov_success;
+  int relaxed;   // max_relative_update given
+};
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_check(Plan P, double gamma, Ctl* __restrict__ ctl,
+                                                 const double* __restrict__ U, const double* __restrict__ X,
+                                                 const double* __restrict__ Act, const double* __restrict__ R,
+                                                 Red red, Ramp ramp) {
+  __shared__ double sh[2][4];
+  const int cs = blockIdx.y;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  const double omega = ramp.relaxed ? ctl[cs].omega : 1.0;
+  double v[2] = {0.0, 0.0};   // bad cells, |omega action + R|^2
+  for (int c = blockIdx.x * BLOCK + threadIdx.x; c < nc; c += gridDim.x * BLOCK) {
+    double cand[NV];
+    bool finite = true;
+    for (int k = 0; k < NV; ++k) {
+      size_t o = base + (size_t)k * nc + c;
+      double dx = omega * X[o];
+      finite = finite && isfinite(dx);
+      cand[k] = U[o] + dx;
+      double lr = omega * Act[o] + R[o];
+      v[1] += lr * lr;
+    }
+    double q[NV];
+    primitive<double, NV>(cand, gamma, q);
+    bool phys = q[0] > 1e-10 && q[3] > 1e-10 && isfinite(q[0]) && isfinite(q[3]);
+    if (!(phys && finite)) v[0] += 1.0;
+  }
+  double tot[2];
+  if (finish_sum<2>(v, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
+    if (threadIdx.x == 0) {
+      Ctl& c = ctl[cs];
+      double lin_red = sqrt(tot[1]) / fmax(c.bnorm, 1e-300);
+      bool accept = tot[0] == 0.0 && lin_red < 1.0;
+      double reduction = c.res_prev / fmax(c.norm, 1e-300);
+      double nxt;
+      if (accept && lin_red < ramp.krylov_success && omega >= 1.0)
+        nxt = c.cfl * fmin(fmax(reduction, 1.0), ramp.cfl_growth);
+      else if (accept)
+        nxt = c.cfl;
+      else
+        nxt = c.cfl / ramp.cfl_cutback;
+      c.cfl = fmin(fmax(nxt, ramp.cfl_min), ramp.cfl_max);
+      if (accept) c.res_prev = c.norm;
+      c.lin_red = lin_red;
+      c.accept = accept;
+      c.steps += 1;
+      if (!accept) c.rejected += 1;
+    }
+  }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_update(int nc, const Ctl* __restrict__ ctl, double* __restrict__ U,
+                                                  const double* __restrict__ X, int relaxed) {
+  const int cs = blockIdx.y;
+  if (!ctl[cs].accept) return;
+  const double omega = relaxed ? ctl[cs].omega : 1.0;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  JR_VEC_LOOP(N) U[base + e] = U[base + e] + omega * X[base + e];
+}
+
+// ---------------------------------------------------------------------------
+// Forces (forces.py:48-161): gauge wall pressure and laminar wall shear from
+// the ghost-filled primitives Q; CL, CD, CM (pressure only, 2D), CD friction,
+// Fx, Fy per case.  One workgroup per case, a fixed-order block sum.
+
+__device__ void wall_shear(const Plan& P, const Physics& ph, const double* Q, size_t base, double mu_inf, int fam,
+                           int ff, double* fx, double* fy) {
+  const int NJ = P.NJ, nc = P.nc, NJ2 = NJ + 2;
+  const double gamma = ph.gamma;
+  int cL, cR;
+  double sx, sy;
+  if (fam == 0) {
+    int a = ff / NJ, b = ff - a * NJ;
+    cR = a * NJ + b; cL = cR - NJ;
+    sx = P.fi[ff]; sy = P.fi[P.nfi + ff];
+  } else {
+    int a = ff / (NJ + 1), b = ff - a * (NJ + 1);
+    cR = a * NJ + b; cL = cR - 1;
+    sx = P.fj[ff]; sy = P.fj[P.nfj + ff];
+  }
+  int iL = cL / NJ, jL = cL - iL * NJ, iR = cR / NJ, jR = cR - iR * NJ;
+  const size_t ycen = (size_t)(P.NI + 2) * NJ2;
+  double d1x = P.cen[(iR + 1) * NJ2 + jR + 1] - P.cen[(iL + 1) * NJ2 + jL + 1];
+  double d1y = P.cen[ycen + (iR + 1) * NJ2 + jR + 1] - P.cen[ycen + (iL + 1) * NJ2 + jL + 1];
+  int n0i = iR, n0j = jR, n1i = fam == 0 ? iR : iR + 1, n1j = fam == 0 ? jR + 1 : jR;
+  auto node_c = [&](int ni, int nj, int row) -> double {
+    const double* c = P.cen + (size_t)row * ycen;
+    return 0.25 * (((c[ni * NJ2 + nj] + c[(ni + 1) * NJ2 + nj]) + c[ni * NJ2 + nj + 1]) + c[(ni + 1) * NJ2 + nj + 1]);
+  };
+  double d2x = node_c(n1i, n1j, 0) - node_c(n0i, n0j, 0);
+  double d2y = node_c(n1i, n1j, 1) - node_c(n0i, n0j, 1);
+  double det = d1x * d2y - d2x * d1y;
+  auto phi_at = [&](int i, int j, double* out) {
+    double q[4];
+    for (int k = 0; k < 4; ++k) q[k] = Q[base + (size_t)k * nc + i * NJ + j];
+    out[0] = q[1]; out[1] = q[2]; out[2] = gamma * q[3] / q[0];
+  };
+  auto node_phi = [&](int ni, int nj, double* out) {
+    double p00[3], p10[3], p01[3], p11[3];
+    phi_at(ni - 1, nj - 1, p00); phi_at(ni, nj - 1, p10); phi_at(ni - 1, nj, p01); phi_at(ni, nj, p11);
+    for (int k = 0; k < 3; ++k) out[k] = 0.25 * (((p00[k] + p10[k]) + p01[k]) + p11[k]);
+  };
+  double pn0[3], pn1[3], pl[3], pr[3];
+  node_phi(n0i, n0j, pn0);
+  node_phi(n1i, n1j, pn1);
+  phi_at(iL, jL, pl);
+  phi_at(iR, jR, pr);
+  double gx[3], gy[3];
+  for (int k = 0; k < 3; ++k) {
+    double dp1 = pr[k] - pl[k], dp2 = pn1[k] - pn0[k];
+    gx[k] = (dp1 * d2y - dp2 * d1y) / det;
+    gy[k] = (d1x * dp2 - d2x * dp1) / det;
+  }
+  double gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
+  double gT[2] = {gx[2], gy[2]};
+  double vel[2] = {0.5 * (pl[0] + pr[0]), 0.5 * (pl[1] + pr[1])};
+  double Tf = 0.5 * (pl[2] + pr[2]);
+  double mu = viscosity<double>(Tf, mu_inf, ph.sutherland);
+  double kc = mu / (PR * (gamma - 1.0));
+  double f[4];
+  viscous_flux<double>(vel, gv, gT, sx, sy, mu, kc, nullptr, f);
+  *fx = f[1];
+  *fy = f[2];
+}
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_forces(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                  const double* __restrict__ Q, double length, double cx, double cy,
+                                                  double* __restrict__ out) {
+  __shared__ double sh[6][4];
+  const int cs = blockIdx.x;
+  const size_t base = (size_t)cs * NV * P.nc;
+  const double gamma = ph.gamma;
+  const double* qi = ctl[cs].q_inf;
+  const double speed = hypot_j(qi[1], qi[2]);
+  const double dx = qi[1] / speed, dy = qi[2] / speed;   // drag direction
+  const double lx = -dy, ly = dx;                        // lift direction
+  const double q_area = 0.5 * qi[0] * speed * speed * length;
+  double v[6] = {0, 0, 0, 0, 0, 0};   // lift, drag, moment, shear x, shear y, (spare)
+  double fxs = 0.0, fys = 0.0;
+  for (int w = threadIdx.x; w < P.n_walls; w += BLOCK) {
+    int cell = P.w_cell[w], fam = P.w_fam[w], ff = P.w_face[w];
+    double into = P.w_into[w];
+    double sx = into * (fam == 0 ? P.fi[ff] : P.fj[ff]);
+    double sy = into * (fam == 0 ? P.fi[P.nfi + ff] : P.fj[P.nfj + ff]);
+    double p = Q[base + (size_t)3 * P.nc + cell] - 1.0 / gamma;
+    double px = -p * sx, py = -p * sy;
+    double tx = 0.0, ty = 0.0;
+    if (ph.viscous) {
+      wall_shear(P, ph, Q, base, ctl[cs].mu_inf, fam, ff, &tx, &ty);
+      tx *= into;
+      ty *= into;
+    }
+    double ldx = px + tx, ldy = py + ty;
+    double rx = P.w_mid[w] - cx, ry = P.w_mid[P.n_walls + w] - cy;
+    double moment = ry * px - rx * py;
+    v[0] += (lx * ldx + ly * ldy) / q_area;
+    v[1] += (dx * ldx + dy * ldy) / q_area;
+    v[2] += (moment / length) / q_area;
+    v[3] += tx;
+    v[4] += ty;
+    fxs += ldx;
+    fys += ldy;
+  }
+  v[5] = 0.0;
+  block_sum<6>(v, sh);
+  double f2[6] = {fxs, fys, 0, 0, 0, 0};
+  block_sum<6>(f2, sh);
+  if (threadIdx.x == 0) {
+    double* o = out + (size_t)cs * 8;
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
+    o[3] = (dx * v[3] + dy * v[4]) / q_area;
+    o[4] = f2[0];
+    o[5] = f2[1];
+    o[6] = 0.0;
+    o[7] = 0.0;
+  }
+}
+
+}  // namespace jr

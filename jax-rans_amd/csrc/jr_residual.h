@@ -1,0 +1,301 @@
+// Residual assembly and its tangent on the canvas: conservative -> primitive,
+// staged ghost fill, face fluxes (MUSCL + HLLC/Roe, diamond-gradient viscous
+// flux, SA face flux), and the per-cell divergence with the SA source.
+//
+// Restates src/jaxrans/numerics/residual.py:54-124, 312-327,
+// numerics/boundary.py:257-267, reconstruction.py:29-80, gradients.py:31-95,
+// turbulence.py:56-151.  Every kernel is templated on the scalar type:
+// double gives R(U), Dual gives (R, dR/dU . dU) in one pass.
+#pragma once
+#include "jr_physics.h"
+#include "jr_types.h"
+
+namespace jr {
+
+template <class T> struct Io;
+template <> struct Io<double> {
+  static __device__ __forceinline__ double ld(const double* a, const double*, size_t i) { return a[i]; }
+  static __device__ __forceinline__ void st(double* a, double*, size_t i, double x) { a[i] = x; }
+};
+template <> struct Io<Dual> {
+  static __device__ __forceinline__ Dual ld(const double* a, const double* d, size_t i) { return mkd(a[i], d[i]); }
+  static __device__ __forceinline__ void st(double*, double* d, size_t i, Dual x) { d[i] = x.d; }
+};
+
+// Q (and dQ) <- primitive of U (and its tangent) on every canvas cell.
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_prim(int nc, double gamma, const Ctl* __restrict__ ctl,
+                                                const double* __restrict__ U, const double* __restrict__ dU,
+                                                double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
+  int c = blockIdx.x * BLOCK + threadIdx.x;
+  int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (c >= nc) return;
+  size_t base = (size_t)cs * NV * nc + c;
+  T u[NV], q[NV];
+  for (int k = 0; k < NV; ++k) u[k] = Io<T>::ld(U, dU, base + (size_t)k * nc);
+  primitive<T, NV>(u, gamma, q);
+  for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc, q[k]);
+}
+
+// One ghost-fill stage: dest <- bc(src) or a copy of the partner cell.
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_ghost(Plan P, int stage, double gamma, const Ctl* __restrict__ ctl,
+                                                 double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
+  int t = blockIdx.x * BLOCK + threadIdx.x;
+  int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (t >= P.n_task[stage]) return;
+  int dst = P.t_dest[stage][t], src = P.t_src[stage][t], kind = P.t_kind[stage][t];
+  size_t base = (size_t)cs * NV * P.nc;
+  T q[NV], g[NV];
+  for (int k = 0; k < NV; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * P.nc + src);
+  if (kind == K_COPY) {
+    for (int k = 0; k < NV; ++k) g[k] = q[k];
+  } else {
+    int n = P.n_task[stage];
+    bc_state<T, NV>(kind, q, P.t_norm[stage][t], P.t_norm[stage][n + t], ctl[cs].q_inf, gamma, g);
+  }
+  for (int k = 0; k < NV; ++k) {
+    if (sizeof(T) == sizeof(double)) Q[base + (size_t)k * P.nc + dst] = val(g[k]);
+    else dQ[base + (size_t)k * P.nc + dst] = tan_of(g[k]);
+  }
+}
+
+template <class T, int NV>
+__device__ __forceinline__ void load_q(const double* Q, const double* dQ, size_t base, int nc, int c, T* q) {
+  for (int k = 0; k < NV; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + c);
+}
+
+// Face fluxes of both families.  Face index f < nfi: i-face (fi, j) between
+// cells (fi-1, j) and (fi, j); else j-face (i, fj) between (i, fj-1) and (i, fj).
+// Faces not adjacent to an interior cell are zero.  Rows 0..3 hold the
+// inviscid minus the viscous flux, row 4 (SA) the nu_hat face flux.
+template <class T, int NV, bool VISC>
+__global__ void __launch_bounds__(BLOCK) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
+                                                 const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                 double* __restrict__ Fi, double* __restrict__ Fj, int check_active) {
+  int f = blockIdx.x * BLOCK + threadIdx.x;
+  int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (f >= P.nfi + P.nfj) return;
+  const int NJ = P.NJ, nc = P.nc;
+  const bool ifam = f < P.nfi;
+  int a, b, dL, ff;   // (a, b): the face's own index pair; dL: canvas step across the face
+  double sx, sy;
+  if (ifam) {
+    ff = f;
+    a = f / NJ; b = f - a * NJ;           // face a between cells a-1, a (row), column b
+    dL = NJ;
+    sx = P.fi[ff]; sy = P.fi[P.nfi + ff];
+  } else {
+    ff = f - P.nfi;
+    a = ff / (NJ + 1); b = ff - a * (NJ + 1);   // row a, face b between columns b-1, b
+    dL = 1;
+    sx = P.fj[ff]; sy = P.fj[P.nfj + ff];
+  }
+  double* F = ifam ? Fi : Fj;
+  const int nfam = ifam ? P.nfi : P.nfj;
+  const size_t fbase = (size_t)cs * NV * nfam + ff;
+  // Cells either side.
+  int cL, cR;
+  bool inside;
+  if (ifam) {
+    inside = a >= 1 && a <= P.NI - 1;
+    cR = a * NJ + b; cL = cR - NJ;
+  } else {
+    inside = b >= 1 && b <= NJ - 1;
+    cR = a * NJ + b; cL = cR - 1;
+  }
+  if (!inside || !(P.interior[cL] || P.interior[cR])) {
+    for (int k = 0; k < NV; ++k) F[fbase + (size_t)k * nfam] = 0.0;
+    return;
+  }
+  const size_t base = (size_t)cs * NV * nc;
+  const double gamma = ph.gamma;
+  T qm[NV], qp[NV];
+  load_q<T, NV>(Q, dQ, base, nc, cL, qm);
+  load_q<T, NV>(Q, dQ, base, nc, cR, qp);
+  T qL[4], qR[4];
+  if (first_order) {
+    for (int k = 0; k < 4; ++k) { qL[k] = qm[k]; qR[k] = qp[k]; }
+  } else {
+    T qmm[4], qpp[4];
+    for (int k = 0; k < 4; ++k) {
+      qmm[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + cL - dL);
+      qpp[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + cR + dL);
+    }
+    if (ph.kappa != 0.0) {
+      double lo = 0.25 * (1.0 - ph.kappa), hi = 0.25 * (1.0 + ph.kappa);
+      for (int k = 0; k < 4; ++k) {
+        qL[k] = qm[k] + lo * (qm[k] - qmm[k]) + hi * (qp[k] - qm[k]);
+        qR[k] = qp[k] - lo * (qpp[k] - qp[k]) - hi * (qp[k] - qm[k]);
+      }
+    } else {
+      for (int k = 0; k < 4; ++k) {
+        qL[k] = qm[k] + 0.5 * limit<T>(ph.limiter, qm[k] - qmm[k], qp[k] - qm[k]);
+        qR[k] = qp[k] - 0.5 * limit<T>(ph.limiter, qp[k] - qm[k], qpp[k] - qp[k]);
+      }
+    }
+  }
+  T Fl[NV];
+  if (ph.flux == FLUX_ROE) roe<T>(qL, qR, sx, sy, gamma, Fl);
+  else hllc<T>(qL, qR, sx, sy, gamma, Fl);
+  if constexpr (NV > 4) Fl[4] = lift<T>(0.0);
+
+  if (VISC) {
+    const double mu_inf = ctl[cs].mu_inf;
+    // Diamond gradient: the cells' centre difference and the face's node difference.
+    const int NJ2 = NJ + 2;
+    int iL = cL / NJ, jL = cL - iL * NJ, iR = cR / NJ, jR = cR - iR * NJ;
+    const size_t ycen = (size_t)(P.NI + 2) * NJ2;
+    double d1x = P.cen[(iR + 1) * NJ2 + jR + 1] - P.cen[(iL + 1) * NJ2 + jL + 1];
+    double d1y = P.cen[ycen + (iR + 1) * NJ2 + jR + 1] - P.cen[ycen + (iL + 1) * NJ2 + jL + 1];
+    // Nodes n0, n1 of the face: the lower-left corners (p1 node index = canvas cell
+    // index) of cell R and of cell R shifted one step along the face.
+    int n0i = iR, n0j = jR;
+    int n1i = ifam ? iR : iR + 1, n1j = ifam ? jR + 1 : jR;
+    auto node_c = [&](int ni, int nj, int row) -> double {
+      // average of padded centres (ni, nj), (ni+1, nj), (ni, nj+1), (ni+1, nj+1)
+      const double* c = P.cen + (size_t)row * ycen;
+      return 0.25 * (((c[ni * NJ2 + nj] + c[(ni + 1) * NJ2 + nj]) + c[ni * NJ2 + nj + 1]) + c[(ni + 1) * NJ2 + nj + 1]);
+    };
+    double d2x = node_c(n1i, n1j, 0) - node_c(n0i, n0j, 0);
+    double d2y = node_c(n1i, n1j, 1) - node_c(n0i, n0j, 1);
+    double det = d1x * d2y - d2x * d1y;
+    // phi = (u, v, T[, nu_hat]) at the four cells round each node: p1 node (ni, nj)
+    // averages canvas cells (ni-1, nj-1), (ni, nj-1), (ni-1, nj), (ni, nj).
+    const int NPHI = NV > 4 ? 4 : 3;
+    auto phi_at = [&](int i, int j, T* out) {
+      T q[NV];
+      load_q<T, NV>(Q, dQ, base, nc, i * NJ + j, q);
+      out[0] = q[1]; out[1] = q[2]; out[2] = temperature(q, gamma);
+      if (NPHI > 3) out[3] = q[NV - 1];
+    };
+    auto node_phi = [&](int ni, int nj, T* out) {
+      T p00[4], p10[4], p01[4], p11[4];
+      phi_at(ni - 1, nj - 1, p00); phi_at(ni, nj - 1, p10); phi_at(ni - 1, nj, p01); phi_at(ni, nj, p11);
+      for (int k = 0; k < NPHI; ++k) out[k] = 0.25 * (((p00[k] + p10[k]) + p01[k]) + p11[k]);
+    };
+    T pn0[4], pn1[4], pl[4], pr[4];
+    node_phi(n0i, n0j, pn0);
+    node_phi(n1i, n1j, pn1);
+    pl[0] = qm[1]; pl[1] = qm[2]; pl[2] = temperature(qm, gamma);
+    pr[0] = qp[1]; pr[1] = qp[2]; pr[2] = temperature(qp, gamma);
+    if (NPHI > 3) { pl[3] = qm[NV - 1]; pr[3] = qp[NV - 1]; }
+    T gx[4], gy[4];
+    for (int k = 0; k < NPHI; ++k) {
+      T dp1 = pr[k] - pl[k];
+      T dp2 = pn1[k] - pn0[k];
+      gx[k] = (dp1 * d2y - dp2 * d1y) / det;
+      gy[k] = (d1x * dp2 - d2x * dp1) / det;
+    }
+    T gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
+    T gT[2] = {gx[2], gy[2]};
+    T vel[2] = {0.5 * (qm[1] + qp[1]), 0.5 * (qm[2] + qp[2])};
+    T Tf = 0.5 * (temperature(qm, gamma) + temperature(qp, gamma));
+    T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
+    T kc = mu / (PR * (gamma - 1.0));
+    T extra[2][2];
+    bool has_extra = false;
+    if constexpr (NV > 4) {
+      T face[5];
+      for (int k = 0; k < 5; ++k) face[k] = 0.5 * (qm[k] + qp[k]);
+      T muf = viscosity<T>(temperature(face, gamma), mu_inf, ph.sutherland);
+      T mut = eddy_viscosity<T>(face[0], face[4] * mu_inf, muf / face[0]);
+      mu = mu + mut;
+      kc = kc + mut / (PR_T * (gamma - 1.0));
+      if (ph.c_cr1 != 0.0) { qcr_stress<T>(gv, mut, ph.c_cr1, extra); has_extra = true; }
+      // SA face flux: upwind convection on the mass flux, diffusion through the face gradient.
+      T up = val(Fl[0]) > 0.0 ? qm[4] : qp[4];
+      T coef = diffusion_coefficient<T>(face[0], face[4] * mu_inf, muf);
+      Fl[4] = Fl[0] * up - coef * (gx[3] * sx + gy[3] * sy);
+    }
+    T fv[4];
+    viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, has_extra ? extra : nullptr, fv);
+    for (int k = 0; k < 4; ++k) Fl[k] = Fl[k] - fv[k];
+  }
+  for (int k = 0; k < NV; ++k) {
+    if (sizeof(T) == sizeof(double)) F[fbase + (size_t)k * nfam] = val(Fl[k]);
+    else F[fbase + (size_t)k * nfam] = tan_of(Fl[k]);
+  }
+}
+
+// Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere.
+// Tangent mode writes  out = vdt * dU + dR  (vdt may be null: no pseudo-time term).
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_cells(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                 const double* __restrict__ U, const double* __restrict__ dU,
+                                                 const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                 const double* __restrict__ Fi, const double* __restrict__ Fj,
+                                                 const double* __restrict__ vdt, double* __restrict__ out,
+                                                 int check_active) {
+  int c = blockIdx.x * BLOCK + threadIdx.x;
+  int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ, nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  const bool tangent = sizeof(T) != sizeof(double);
+  double vd = vdt ? vdt[(size_t)cs * nc + c] : 0.0;
+  if (!P.interior[c]) {
+    if (!tangent) {
+      double u0[5];
+      conservative<double, NV>(ctl[cs].q_inf, ph.gamma, u0);
+      for (int k = 0; k < NV; ++k) out[base + (size_t)k * nc + c] = U[base + (size_t)k * nc + c] - u0[k];
+    } else {
+      for (int k = 0; k < NV; ++k) {
+        double d = dU[base + (size_t)k * nc + c];
+        out[base + (size_t)k * nc + c] = vdt ? vd * d + d : d;
+      }
+    }
+    return;
+  }
+  int i = c / NJ, j = c - i * NJ;
+  const size_t bi = (size_t)cs * NV * P.nfi, bj = (size_t)cs * NV * P.nfj;
+  const int fi0 = i * NJ + j, fi1 = fi0 + NJ;
+  const int fj0 = i * (NJ + 1) + j, fj1 = fj0 + 1;
+  double r[NV];
+  for (int k = 0; k < NV; ++k) {
+    r[k] = (Fi[bi + (size_t)k * P.nfi + fi1] - Fi[bi + (size_t)k * P.nfi + fi0]) +
+           (Fj[bj + (size_t)k * P.nfj + fj1] - Fj[bj + (size_t)k * P.nfj + fj0]);
+  }
+  if constexpr (NV > 4) {
+    // SA source at the cell (turbulence.py:140-151): vorticity and grad nu_hat by
+    // Green-Gauss over the face-averaged neighbours.
+    const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
+    T qc[NV], qw[NV], qe[NV], qs[NV], qn[NV];
+    load_q<T, NV>(Q, dQ, base, nc, c, qc);
+    load_q<T, NV>(Q, dQ, base, nc, c - NJ, qw);
+    load_q<T, NV>(Q, dQ, base, nc, c + NJ, qe);
+    load_q<T, NV>(Q, dQ, base, nc, c - 1, qs);
+    load_q<T, NV>(Q, dQ, base, nc, c + 1, qn);
+    const double sxi0 = P.fi[fi0], syi0 = P.fi[P.nfi + fi0], sxi1 = P.fi[fi1], syi1 = P.fi[P.nfi + fi1];
+    const double sxj0 = P.fj[fj0], syj0 = P.fj[P.nfj + fj0], sxj1 = P.fj[fj1], syj1 = P.fj[P.nfj + fj1];
+    const double V = P.area[c];
+    auto gg = [&](int k, T& gxo, T& gyo) {
+      T fw = 0.5 * (qw[k] + qc[k]), fe = 0.5 * (qc[k] + qe[k]);
+      T fs = 0.5 * (qs[k] + qc[k]), fn = 0.5 * (qc[k] + qn[k]);
+      gxo = ((fe * sxi1 - fw * sxi0) + (fn * sxj1 - fs * sxj0)) / V;
+      gyo = ((fe * syi1 - fw * syi0) + (fn * syj1 - fs * syj0)) / V;
+    };
+    T gux, guy, gvx, gvy, gnx, gny;
+    gg(1, gux, guy);
+    gg(2, gvx, gvy);
+    gg(4, gnx, gny);
+    T omega = jabs(gvx - guy);
+    T rho = qc[0];
+    T mu = viscosity<T>(temperature(qc, gamma), nu_inf, ph.sutherland);
+    T prod, destr;
+    sa_source<T>(qc[4] * nu_inf, mu / rho, omega, P.dist[c], prod, destr);
+    T src = rho * ((prod - destr) / nu_inf + CB2 / SIGMA * nu_inf * (gnx * gnx + gny * gny));
+    T sv = src * V;
+    r[4] = r[4] - (tangent ? tan_of(sv) : val(sv));
+  }
+  for (int k = 0; k < NV; ++k) {
+    if (tangent && vdt) r[k] = vd * dU[base + (size_t)k * nc + c] + r[k];
+    out[base + (size_t)k * nc + c] = r[k];
+  }
+}
+
+}  // namespace jr

@@ -1,0 +1,79 @@
+// Device-side plan and per-case control structures shared by the kernels and
+// the C-ABI layer.  Plain structs of device pointers: a plan is uploaded once
+// per grid, a case block per solve.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+namespace jr {
+
+constexpr int BLOCK = 256;     // threads per workgroup of the streaming kernels
+constexpr int MMAX = 64;       // largest Krylov subspace supported
+
+// The canvas and its static plans (one grid, shared by every case of a context).
+struct Plan {
+  int NI, NJ, nc;                  // canvas shape, cells
+  int nfi, nfj;                    // i-faces (NI+1)*NJ, j-faces NI*(NJ+1)
+  const double* area;              // [nc]
+  const double* fi;                // [2][nfi] i-face area vectors
+  const double* fj;                // [2][nfj] j-face area vectors
+  const double* cen;               // [2][(NI+2)*(NJ+2)] centres, edge-padded by one
+  const double* dist;              // [nc] wall distance
+  const uint8_t* interior;         // [nc]
+  const int* nbr;                  // [nc] face-neighbour map
+  int n_task[2];                   // ghost-fill stages
+  const int* t_dest[2];
+  const int* t_src[2];
+  const int* t_kind[2];
+  const double* t_norm[2];         // [2][n]
+  int n_lines, n_slots, max_len;   // line chains
+  const int* l_start;              // [n_lines+1]
+  const int* l_cell;               // [n_slots]
+  const int* l_up;                 // j-face of the link to the next slot
+  const int8_t* l_ups;
+  const int* l_dn;                 // j-face the next slot couples across
+  const int8_t* l_dns;
+  int n_free;                      // canvas cells on no chain
+  const int* free_cell;
+  int n_walls;
+  const int* w_cell;
+  const int* w_fam;
+  const int* w_face;
+  const double* w_into;
+  const double* w_mid;             // [2][n_walls]
+};
+
+struct Physics {
+  double gamma;
+  double kappa;        // MUSCL kappa (0: limited)
+  int flux;            // Flux
+  int limiter;         // Limiter
+  int nv;              // 4 Euler/laminar, 5 SA
+  int viscous;
+  int sa;
+  double sutherland;   // <= 0: constant viscosity
+  double c_cr1;        // QCR2000 coefficient (0: off)
+};
+
+// Per-case solver control, resident on the device.
+struct Ctl {
+  double q_inf[5];
+  double mu_inf;
+  double cfl;
+  double res_prev;     // |R| of the last accepted step (inf at start)
+  double norm;         // |R(U)| reported for this step (density row, volume weighted)
+  double bnorm;        // ||R||_2 over the whole canvas vector
+  double rnorm;        // ||b - A x|| of the current GMRES iterate
+  double omega;        // relaxation factor
+  double lin_red;
+  double beta;         // ||r|| at the start of the current GMRES cycle
+  double wnorm;        // |w| of the latest Arnoldi vector before normalisation
+  int cycles;          // GMRES cycles done this step
+  int active;          // GMRES cycle running
+  int accept;          // last step accepted
+  int steps;           // Newton steps taken
+  int rejected;
+  int pad_;
+};
+
+}  // namespace jr

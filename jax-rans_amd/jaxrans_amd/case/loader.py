@@ -1,0 +1,337 @@
+"""A case directory (``case.json`` + ``numerical.json``) -> a resolved ``Case``.
+
+Keys, defaults, cross-key rules and error messages follow the reference
+loader (``src/jaxrans/case/loader.py:31-373``, ``case/domain.py:404-538``), so
+``load_case`` is a drop-in.  Grids are generated here (``o_grid``,
+``c_grid``); the ``file`` and ``multi_element`` sources are refused.
+"""
+
+from __future__ import annotations
+
+import copy
+import json
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+
+from ..core import constants
+from ..core.config import FlowConditions, ImplicitConfig, NumericsConfig
+from ..geometry.airfoil import Airfoil, airfoil_source
+from ..mesh.assemble import assemble
+from ..mesh.atlas import Atlas, assign_conditions, build_atlas, patches
+from ..mesh.block import folded
+from ..mesh.generate import c_grid, o_grid
+from . import schema
+
+CASE_FILE, NUMERICAL_FILE = "case.json", "numerical.json"
+
+
+class CaseError(ValueError):
+    """A case that cannot be run as written."""
+
+
+@dataclass(frozen=True)
+class GridSpec:
+    grid: str = "c_grid"
+    airfoil: str | None = None
+    cells_airfoil: int | None = None
+    cells_wake: int | None = None
+    cells_normal: int | None = None
+    wall_spacing: float | None = None
+    far_field: float = 100.0
+    trailing_edge: str = "as_given"
+    leading_edge_spacing: float = 2e-3
+    trailing_edge_spacing: float = 2e-3
+    file: str | None = None
+    format: str = "auto"
+    coordinate_scale: float = 1.0
+    patches: tuple = ()
+    elements: tuple = ()
+    cells_per_chord: float | None = None
+    layer_cells: int = 32
+    layer_thickness: float = 0.02
+    gap_cells: int = 16
+    even_cells: int = 4
+    reference_length: float = 1.0
+    reference_area: float | None = None
+    moment_centre: tuple = (0.25, 0.0, 0.0)
+    section: str = "streamwise"
+    base_blocks: bool = False
+
+    def airfoil_shape(self, base: Path) -> Airfoil:
+        foil = airfoil_source(self.airfoil, base).unit_chord()
+        if self.trailing_edge == "close":
+            foil = foil.closed()
+        return foil
+
+    def build(self, base: Path):
+        if self.grid not in ("o_grid", "c_grid"):
+            raise CaseError(f"domain.grid {self.grid!r} is not built by this implementation "
+                            f"(generated o_grid and c_grid only)")
+        foil = self.airfoil_shape(base)
+        common = dict(cells_normal=self.cells_normal, wall_spacing=self.wall_spacing, far_field=self.far_field,
+                      le=self.leading_edge_spacing, te=self.trailing_edge_spacing)
+        if self.grid == "o_grid":
+            raw = [o_grid(foil, self.cells_airfoil, **common)]
+        else:
+            if not foil.sharp:
+                raise CaseError(f"a blunt trailing edge (gap {foil.trailing_edge_gap:.3g}) needs the base and wake "
+                                f"blocks of a blunt C-grid, not built here; set domain.trailing_edge = \"close\" "
+                                f"or use an o_grid")
+            raw = [c_grid(foil, self.cells_airfoil, self.cells_wake, **common)]
+        return assemble(raw)
+
+    def describe(self) -> str:
+        wake = f" + 2x{self.cells_wake} wake" if self.grid == "c_grid" else ""
+        return (f"{self.grid} round {self.airfoil}: {self.cells_airfoil}{wake} x {self.cells_normal} cells, "
+                f"wall spacing {self.wall_spacing:g}, far field {self.far_field:g}")
+
+
+@dataclass(frozen=True)
+class Case:
+    name: str
+    description: str
+    output: Path
+    grid_spec: GridSpec
+    conditions: dict
+    flow: FlowConditions
+    numerics: NumericsConfig
+    implicit: ImplicitConfig
+    rk_scheme: str
+    block_iters: int
+    directory: Path
+    raw: dict = field(compare=False, repr=False, default_factory=dict)
+    sequencing: int = 0
+
+    def grid(self, coarsen: int = 0) -> Atlas:
+        if coarsen:
+            raise CaseError("grid sequencing is not built by this implementation")
+        if self.flow.swept:
+            raise CaseError("'freestream.sweep_deg' (2.5D) is not built by this implementation")
+        mesh = self.grid_spec.build(self.directory)
+        kinds = resolve_conditions(self.conditions, patches(mesh), self.flow)
+        mesh = assign_conditions(mesh, kinds)
+        if any(folded(b) for b in mesh):
+            raise CaseError("the grid has folded cells")
+        return build_atlas(mesh, self.numerics.n_ghost)
+
+
+def _section(doc: dict, file: str, name: str, required: bool = False) -> dict:
+    if name not in doc:
+        if required:
+            raise CaseError(f"missing section '{name}'")
+        return {}
+    section = doc[name]
+    if not isinstance(section, dict):
+        raise CaseError(f"'{name}' must be an object, got {type(section).__name__}")
+    allowed = schema.keys(file, name)
+    unknown = set(section) - set(allowed) - {"notes"}
+    if unknown:
+        raise CaseError(f"unknown key(s) {sorted(f'{name}.{k}' for k in unknown)}; allowed: {sorted(allowed)}")
+    return {k: v for k, v in section.items() if k != "notes"}
+
+
+def _values(section: dict, file: str, name: str) -> dict:
+    out = {}
+    for key in schema.FILES[file][name]:
+        path = f"{name}.{key.name}"
+        if key.name in section:
+            try:
+                key.check(path, section[key.name])
+            except (TypeError, ValueError) as error:
+                raise CaseError(str(error)) from None
+            out[key.name] = section[key.name]
+        elif key.default is schema.REQUIRED:
+            raise CaseError(f"missing '{path}'")
+        else:
+            out[key.name] = key.default
+    return out
+
+
+def _top_level(doc: dict, file: str) -> None:
+    unknown = set(doc) - set(schema.FILES[file]) - ({"boundary_conditions"} if file == CASE_FILE else set())
+    if unknown:
+        raise CaseError(f"{file}: unknown section(s) {sorted(unknown)}; allowed: {sorted(schema.FILES[file])}")
+
+
+def _grid_spec(case_doc: dict) -> GridSpec:
+    raw = _section(case_doc, CASE_FILE, "domain", required=True)
+    if "grid" not in raw:
+        raise CaseError(f"missing 'domain.grid'; choose from {sorted(schema.GRIDS)}")
+    domain = _values(raw, CASE_FILE, "domain")
+    kind = domain["grid"]
+    ks = schema.keys(CASE_FILE, "domain")
+    foreign = sorted(k for k in raw if ks[k].grids is not None and kind not in ks[k].grids)
+    if foreign:
+        takes = sorted(k for k, key in ks.items() if key.grids and kind in key.grids)
+        raise CaseError(f"'domain.{foreign[0]}' does not apply to grid {kind!r}; it takes {takes}")
+    for name, key in ks.items():
+        if kind in key.required_for and domain[name] is None:
+            raise CaseError(f"'domain.{name}' is required for grid {kind!r}")
+    values = {k: v for k, v in domain.items() if v is not None}
+    values["patches"] = tuple(domain["patches"])
+    values["elements"] = tuple(domain["elements"])
+    values["moment_centre"] = tuple(domain["moment_centre"])
+    return GridSpec(**values)
+
+
+def _flow(case_doc: dict) -> FlowConditions:
+    given = {n: _section(case_doc, CASE_FILE, n, required=n == "freestream")
+             for n in ("freestream", "material_properties", "turbulence")}
+    freestream, material, turbulence = (_values(given[n], CASE_FILE, n) for n in given)
+    inviscid = freestream["reynolds"] is None
+    ratio = None
+    if inviscid:
+        for key in ("dynamic_viscosity", "prandtl_number", "sutherland_temperature"):
+            if key in given["material_properties"]:
+                raise CaseError(f"'material_properties.{key}' given for an inviscid case "
+                                f"(freestream.reynolds is null)")
+    else:
+        if material["dynamic_viscosity"] == "sutherland":
+            if freestream["temperature"] is None or material["sutherland_temperature"] is None:
+                raise CaseError("Sutherland's law needs 'freestream.temperature' and "
+                                "'material_properties.sutherland_temperature', in one absolute unit")
+            ratio = material["sutherland_temperature"] / freestream["temperature"]
+        elif material["sutherland_temperature"] is not None:
+            raise CaseError("'material_properties.sutherland_temperature' given with a constant viscosity")
+        if material["prandtl_number"] != constants.PRANDTL_LAMINAR:
+            raise CaseError(f"'material_properties.prandtl_number' = {material['prandtl_number']} is not "
+                            f"implemented; only {constants.PRANDTL_LAMINAR}")
+    model = turbulence["model"]
+    if model != "none" and inviscid:
+        raise CaseError("'turbulence.model' needs a viscous case (freestream.reynolds)")
+    if model == "none" and "freestream_nu_tilde_ratio" in given["turbulence"]:
+        raise CaseError("'turbulence.freestream_nu_tilde_ratio' given without a model")
+    try:
+        return FlowConditions(mach=freestream["mach"], alpha_deg=freestream["alpha_deg"],
+                              reynolds=freestream["reynolds"], gamma=material["specific_heat_ratio"],
+                              sutherland_ratio=ratio, turbulence=None if model == "none" else model,
+                              nu_tilde_ratio=turbulence["freestream_nu_tilde_ratio"],
+                              sweep_deg=freestream["sweep_deg"])
+    except ValueError as error:
+        raise CaseError(str(error)) from None
+
+
+def default_conditions(flow: FlowConditions) -> dict[str, str]:
+    return {"wall": "slip_wall" if flow.inviscid else "no_slip_wall", "farfield": "far_field",
+            "symmetry": "symmetry", "inflow": "inflow_total", "outflow": "outflow_static"}
+
+
+def _boundaries(case_doc: dict, flow: FlowConditions) -> tuple[dict, str]:
+    section = case_doc.get("boundary_conditions", {})
+    if not isinstance(section, dict):
+        raise CaseError("'boundary_conditions' must be an object")
+    kinds, models = {}, set()
+    allowed = {k.name: k for k in schema.BOUNDARY_KEYS}
+    for patch, entry in section.items():
+        if patch == "notes":
+            continue
+        where = f"boundary_conditions.{patch}"
+        if not isinstance(entry, dict):
+            raise CaseError(f"'{where}' must be an object with a 'type', got {entry!r}")
+        unknown = set(entry) - set(allowed) - {"notes"}
+        if unknown:
+            raise CaseError(f"unknown key(s) {sorted(f'{where}.{k}' for k in unknown)}; allowed: {sorted(allowed)}")
+        if "type" not in entry:
+            raise CaseError(f"missing '{where}.type'")
+        for name in set(entry) - {"notes"}:
+            try:
+                allowed[name].check(f"{where}.{name}", entry[name])
+            except (TypeError, ValueError) as error:
+                raise CaseError(str(error)) from None
+        if "model" in entry and entry["type"] != "far_field":
+            raise CaseError(f"'{where}.model' applies to a far_field patch only")
+        if entry["type"] == "far_field":
+            models.add(entry.get("model", "freestream"))
+        if entry["type"] == "no_slip_wall" and flow.inviscid:
+            raise CaseError(f"'{where}.type' = 'no_slip_wall' in an inviscid case (freestream.reynolds is null)")
+        kinds[patch] = entry["type"]
+    if len(models) > 1:
+        raise CaseError(f"far-field patches name different models {sorted(models)}; one model per case")
+    return kinds, schema.FAR_FIELD_MODELS[models.pop() if models else "freestream"]
+
+
+def resolve_conditions(named: dict, grid_patches: list, flow: FlowConditions) -> dict:
+    unknown = [p for p in named if p not in grid_patches]
+    if unknown:
+        raise CaseError(f"'boundary_conditions' names patch(es) {unknown} the grid does not have; "
+                        f"its patches are {grid_patches}")
+    defaults = default_conditions(flow)
+    missing = [p for p in grid_patches if p not in named and p not in defaults]
+    if missing:
+        raise CaseError(f"no boundary condition for patch(es) {missing}: give each a 'type' in "
+                        f"'boundary_conditions' (the grid's patches are {grid_patches})")
+    return {p: named.get(p, defaults.get(p)) for p in grid_patches}
+
+
+def parse_case(case_doc: dict, numerical_doc: dict, directory: Path = Path(".")) -> Case:
+    _top_level(case_doc, CASE_FILE)
+    _top_level(numerical_doc, NUMERICAL_FILE)
+    general = _values(_section(case_doc, CASE_FILE, "general", required=True), CASE_FILE, "general")
+    spec = _grid_spec(case_doc)
+    flow = _flow(case_doc)
+    if "section" in case_doc["domain"] and not flow.swept:
+        raise CaseError("'domain.section' applies to a swept case only ('freestream.sweep_deg')")
+    conditions, far_field = _boundaries(case_doc, flow)
+    fluxes, stepping, implicit, convergence = (
+        _values(_section(numerical_doc, NUMERICAL_FILE, n), NUMERICAL_FILE, n)
+        for n in ("convective_fluxes", "time_integration", "implicit", "convergence"))
+    try:
+        implicit_cfg = ImplicitConfig(**implicit)
+        numerics = NumericsConfig(flux=fluxes["riemann_solver"], limiter=fluxes["limiter"],
+                                  kappa=fluxes["muscl_kappa"], cfl=stepping["cfl"],
+                                  max_iters=convergence["max_iters"], tol=convergence["tol"],
+                                  forces_tol=convergence["forces_tol"], solver=stepping["solver"],
+                                  far_field=far_field)
+    except ValueError as error:
+        raise CaseError(str(error)) from None
+    block_iters = convergence["block_iters"] or (200 if numerics.solver == "explicit" else 20)
+    if block_iters < 1:
+        raise CaseError(f"'convergence.block_iters' must be >= 1, got {block_iters}")
+    if convergence["grid_sequencing"] < 0:
+        raise CaseError(f"'convergence.grid_sequencing' must be >= 0, got {convergence['grid_sequencing']}")
+    directory = Path(directory)
+    output = Path(general["save_path"])
+    return Case(name=general["case_name"], description=general["description"] or general["case_name"],
+                output=output if output.is_absolute() else directory / output, grid_spec=spec,
+                conditions=conditions, flow=flow, numerics=numerics, implicit=implicit_cfg,
+                rk_scheme=stepping["rk_scheme"], block_iters=block_iters, directory=directory,
+                raw={"case": case_doc, "numerical": numerical_doc}, sequencing=convergence["grid_sequencing"])
+
+
+def apply_override(docs: dict, assignment: str) -> None:
+    """``file.section.key=value`` (value read as JSON, else a bare string)."""
+    if "=" not in assignment:
+        raise CaseError(f"override {assignment!r} is not of the form file.section.key=value")
+    path, text = assignment.split("=", 1)
+    parts = path.strip().split(".")
+    if len(parts) != 3 or parts[0] not in docs:
+        raise CaseError(f"override path {path!r} must be case|numerical.section.key, "
+                        f"e.g. numerical.convective_fluxes.limiter")
+    try:
+        value = json.loads(text)
+    except json.JSONDecodeError:
+        value = text
+    docs[parts[0]].setdefault(parts[1], {})[parts[2]] = value
+
+
+def load_case(directory: str | Path, overrides: tuple[str, ...] = ()) -> Case:
+    directory = Path(directory)
+    if not (directory / CASE_FILE).is_file():
+        raise FileNotFoundError(f"{directory} holds no {CASE_FILE}")
+    docs = {}
+    for label, filename in (("case", CASE_FILE), ("numerical", NUMERICAL_FILE)):
+        path = directory / filename
+        try:
+            docs[label] = json.loads(path.read_text()) if path.is_file() else {}
+        except json.JSONDecodeError as error:
+            raise CaseError(f"{path}: {error}") from None
+    docs = copy.deepcopy(docs)
+    for assignment in overrides:
+        apply_override(docs, assignment)
+    try:
+        return parse_case(docs["case"], docs["numerical"], directory)
+    except CaseError as error:
+        raise CaseError(f"{directory}: {error}") from None

@@ -1,0 +1,296 @@
+"""ctypes binding of the device library ``libjrhip.so`` (declared in ``include/jrhip.h``).
+
+The library is the only compute path: there is no CPU fallback.  A missing or
+unloadable library raises ``DeviceUnavailable`` at first use.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .mesh.atlas import Atlas
+
+LIB_PATH = Path(__file__).with_name("libjrhip.so")
+
+FLUX_CODES = {"hllc": 0, "roe": 1}
+LIMITER_CODES = {"minmod": 0, "none": 1, "van_albada": 2, "van_albada_smooth": 3}
+
+EXPORTS = (
+    "jr_ctx_create", "jr_ctx_destroy", "jr_set_freestream", "jr_set_state", "jr_get_state", "jr_residual",
+    "jr_residual_dev", "jr_tangent", "jr_tangent_dev", "jr_wave_speeds", "jr_precond_factor", "jr_precond_apply",
+    "jr_newton_init", "jr_newton_block", "jr_get_scalars", "jr_set_cfl", "jr_forces", "jr_timer_start",
+    "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
+    "jr_device_count", "jr_profile", "jr_last_error",
+)
+
+
+class DeviceUnavailable(RuntimeError):
+    """The HIP library is missing, or no GPU is visible."""
+
+
+class DeviceError(RuntimeError):
+    """A call into the HIP library failed."""
+
+
+_P = C.POINTER
+_d, _i64, _i32, _u8 = _P(C.c_double), _P(C.c_int64), _P(C.c_int32), _P(C.c_uint8)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("NI", C.c_int32), ("NJ", C.c_int32),
+        ("area", _d), ("face_i", _d), ("face_j", _d), ("centres", _d), ("distance", _d),
+        ("interior", _u8), ("neighbours", _i64),
+        ("n_task", C.c_int32 * 2), ("task_dest", _i64 * 2), ("task_src", _i64 * 2), ("task_kind", _i32 * 2),
+        ("task_normal", _d * 2),
+        ("n_lines", C.c_int32), ("n_slots", C.c_int32),
+        ("line_start", _i64), ("line_cells", _i64), ("line_up_face", _i64), ("line_up_sign", _i64),
+        ("line_down_face", _i64), ("line_down_sign", _i64),
+        ("n_walls", C.c_int32),
+        ("wall_cell", _i64), ("wall_family", _i64), ("wall_face", _i64), ("wall_into", _d), ("wall_mid", _d),
+    ]
+
+
+class Numerics(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("kappa", C.c_double), ("flux", C.c_int32), ("limiter", C.c_int32),
+        ("viscous", C.c_int32), ("sa", C.c_int32), ("sutherland", C.c_double), ("c_cr1", C.c_double),
+        ("jacobian_exact", C.c_int32), ("mixed", C.c_int32), ("krylov_subspace", C.c_int32),
+        ("krylov_restarts", C.c_int32), ("krylov_tol", C.c_double), ("krylov_success", C.c_double),
+        ("cfl_growth", C.c_double), ("cfl_cutback", C.c_double), ("cfl_min", C.c_double), ("cfl_max", C.c_double),
+        ("max_relative_update", C.c_double),
+    ]
+
+
+_LIB = None
+
+
+def library(path: str | os.PathLike | None = None):
+    """Load ``libjrhip.so`` (once)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.is_file():
+        raise DeviceUnavailable(f"{p} is not built; run __graft_entry__.build() or make -C jax-rans_amd/csrc")
+    lib = C.CDLL(str(p))
+    for name in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = C.c_char_p if name == "jr_last_error" else C.c_int
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise DeviceError(library().jr_last_error().decode())
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = library().jr_device_count(C.byref(n))
+    return int(n.value) if rc == 0 else 0
+
+
+class Context:
+    """One device context: an atlas, its numerics, ``n_cases`` cases.
+
+    Arrays handed in are copied by the library; the state stays on the device
+    between ``newton_block`` calls.
+    """
+
+    def __init__(self, atlas: Atlas, *, gamma: float, flux: str, limiter: str, kappa: float = 0.0,
+                 viscous: bool = False, sa: bool = False, sutherland: float | None = None, c_cr1: float = 0.0,
+                 implicit=None, n_cases: int = 1):
+        lib = library()
+        if device_count() < 1:
+            raise DeviceUnavailable("no HIP device is visible")
+        self.atlas, self.n_cases = atlas, int(n_cases)
+        self.nvar = 5 if sa else 4
+        self.shape = (self.n_cases, self.nvar) + tuple(atlas.shape)
+        self._keep = []
+        g = GridDesc()
+        NI, NJ = atlas.shape
+        g.NI, g.NJ = NI, NJ
+
+        def arr(a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            self._keep.append(a)
+            return a
+
+        g.area = _ptr(arr(atlas.area, np.float64), C.c_double)
+        g.face_i = _ptr(arr(atlas.face_i.reshape(2, -1), np.float64), C.c_double)
+        g.face_j = _ptr(arr(atlas.face_j.reshape(2, -1), np.float64), C.c_double)
+        g.centres = _ptr(arr(atlas.centres, np.float64), C.c_double)
+        g.distance = _ptr(arr(np.where(np.isfinite(atlas.distance), atlas.distance, 1e300), np.float64), C.c_double)
+        g.interior = _ptr(arr(atlas.interior, np.uint8), C.c_uint8)
+        g.neighbours = _ptr(arr(atlas.neighbours, np.int64), C.c_int64)
+        for s, st in enumerate(atlas.stages):
+            g.n_task[s] = st.dest.size
+            g.task_dest[s] = _ptr(arr(st.dest, np.int64), C.c_int64)
+            g.task_src[s] = _ptr(arr(st.src, np.int64), C.c_int64)
+            g.task_kind[s] = _ptr(arr(st.kind, np.int32), C.c_int32)
+            g.task_normal[s] = _ptr(arr(st.normal, np.float64), C.c_double)
+        L = atlas.lines
+        g.n_lines, g.n_slots = L.n_lines, L.cells.size
+        g.line_start = _ptr(arr(L.start, np.int64), C.c_int64)
+        g.line_cells = _ptr(arr(L.cells, np.int64), C.c_int64)
+        g.line_up_face = _ptr(arr(L.up_face, np.int64), C.c_int64)
+        g.line_up_sign = _ptr(arr(L.up_sign, np.int64), C.c_int64)
+        g.line_down_face = _ptr(arr(L.down_face, np.int64), C.c_int64)
+        g.line_down_sign = _ptr(arr(L.down_sign, np.int64), C.c_int64)
+        W = atlas.walls
+        g.n_walls = W.n
+        g.wall_cell = _ptr(arr(W.cell, np.int64), C.c_int64)
+        g.wall_family = _ptr(arr(W.family, np.int64), C.c_int64)
+        g.wall_face = _ptr(arr(W.face, np.int64), C.c_int64)
+        g.wall_into = _ptr(arr(W.into_fluid, np.float64), C.c_double)
+        g.wall_mid = _ptr(arr(W.mid.T, np.float64), C.c_double)
+
+        from .core.config import ImplicitConfig
+        cfg = implicit if implicit is not None else ImplicitConfig()
+        n = Numerics()
+        n.gamma, n.kappa = gamma, kappa
+        n.flux, n.limiter = FLUX_CODES[flux], LIMITER_CODES[limiter]
+        n.viscous, n.sa = int(viscous), int(sa)
+        n.sutherland = -1.0 if sutherland is None else float(sutherland)
+        n.c_cr1 = c_cr1
+        n.jacobian_exact = int(cfg.jacobian == "exact")
+        n.mixed = int(cfg.precision == "mixed")
+        n.krylov_subspace, n.krylov_restarts = cfg.krylov_subspace, cfg.krylov_restarts
+        n.krylov_tol, n.krylov_success = cfg.krylov_tol, cfg.krylov_success
+        n.cfl_growth, n.cfl_cutback, n.cfl_min, n.cfl_max = cfg.cfl_growth, cfg.cfl_cutback, cfg.cfl_min, cfg.cfl_max
+        n.max_relative_update = -1.0 if cfg.max_relative_update is None else cfg.max_relative_update
+        if cfg.preconditioner != "line_j":
+            raise DeviceError("only the line_j preconditioner is implemented on the device")
+        self.config = cfg
+        h = C.c_void_p()
+        _check(lib.jr_ctx_create(C.byref(g), C.byref(n), self.n_cases, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.jr_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- set-up ---------------------------------------------------------------
+    def set_freestream(self, q_inf: np.ndarray, mu_inf=None) -> None:
+        q = np.zeros((self.n_cases, 5))
+        qi = np.atleast_2d(np.asarray(q_inf, np.float64))
+        q[:, :qi.shape[1]] = qi
+        mu = None if mu_inf is None else np.ascontiguousarray(np.broadcast_to(mu_inf, (self.n_cases,)), np.float64)
+        _check(self._lib.jr_set_freestream(self._h, _ptr(q, C.c_double),
+                                           None if mu is None else _ptr(mu, C.c_double)))
+
+    def set_state(self, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        _check(self._lib.jr_set_state(self._h, _ptr(U, C.c_double)))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        _check(self._lib.jr_get_state(self._h, _ptr(out, C.c_double)))
+        return out
+
+    # --- components -------------------------------------------------------------
+    def residual(self, U: np.ndarray, first_order: bool = False) -> np.ndarray:
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        out = np.empty(self.shape)
+        _check(self._lib.jr_residual(self._h, _ptr(U, C.c_double), _ptr(out, C.c_double), int(first_order)))
+        return out
+
+    def tangent(self, U, dU, first_order: bool = True, with_vdt: bool = False) -> np.ndarray:
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        dU = np.ascontiguousarray(np.broadcast_to(dU, self.shape), np.float64)
+        out = np.empty(self.shape)
+        _check(self._lib.jr_tangent(self._h, _ptr(U, C.c_double), _ptr(dU, C.c_double), _ptr(out, C.c_double),
+                                    int(first_order), int(with_vdt)))
+        return out
+
+    def wave_speeds(self, U):
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        NI, NJ = self.atlas.shape
+        lam = np.empty((self.n_cases, (NI + 1) * NJ + NI * (NJ + 1)))
+        lsum = np.empty((self.n_cases, NI, NJ))
+        _check(self._lib.jr_wave_speeds(self._h, _ptr(U, C.c_double), _ptr(lam, C.c_double), _ptr(lsum, C.c_double)))
+        return lam, lsum
+
+    def precond_factor(self, U) -> None:
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        _check(self._lib.jr_precond_factor(self._h, _ptr(U, C.c_double)))
+
+    def precond_apply(self, r) -> np.ndarray:
+        r = np.ascontiguousarray(np.broadcast_to(r, self.shape), np.float64)
+        out = np.empty(self.shape)
+        _check(self._lib.jr_precond_apply(self._h, _ptr(r, C.c_double), _ptr(out, C.c_double)))
+        return out
+
+    # --- Newton -------------------------------------------------------------------
+    def newton_init(self, cfl0) -> None:
+        c = np.ascontiguousarray(np.broadcast_to(cfl0, (self.n_cases,)), np.float64)
+        _check(self._lib.jr_newton_init(self._h, _ptr(c, C.c_double)))
+
+    def set_cfl(self, cfl) -> None:
+        c = np.ascontiguousarray(np.broadcast_to(cfl, (self.n_cases,)), np.float64)
+        _check(self._lib.jr_set_cfl(self._h, _ptr(c, C.c_double)))
+
+    def newton_block(self, n_steps: int) -> np.ndarray:
+        hist = np.empty((n_steps, self.n_cases))
+        _check(self._lib.jr_newton_block(self._h, int(n_steps), _ptr(hist, C.c_double)))
+        return hist
+
+    def scalars(self) -> np.ndarray:
+        out = np.empty((self.n_cases, 8))
+        _check(self._lib.jr_get_scalars(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def forces(self, length: float = 1.0, centre=(0.25, 0.0)) -> np.ndarray:
+        out = np.empty((self.n_cases, 8))
+        _check(self._lib.jr_forces(self._h, C.c_double(length), C.c_double(centre[0]), C.c_double(centre[1]),
+                                   _ptr(out, C.c_double)))
+        return out
+
+    # --- timing ---------------------------------------------------------------------
+    def timer_start(self) -> None:
+        _check(self._lib.jr_timer_start(self._h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        _check(self._lib.jr_timer_stop(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    PROFILE = {"residual": 0, "tangent": 1, "precond_apply": 2, "residual_faces": 3, "tangent_faces": 4,
+               "cgs2_update_dots": 5, "precond_factor": 6, "tangent_cells": 7}
+
+    def profile(self, kernel: str, reps: int = 20) -> float:
+        """Average milliseconds of one launch of ``kernel`` (HIP events, device stream)."""
+        ms = C.c_float()
+        _check(self._lib.jr_profile(self._h, self.PROFILE[kernel], int(reps), C.byref(ms)))
+        return float(ms.value)
+
+    def synchronize(self) -> None:
+        _check(self._lib.jr_synchronize(self._h))
+
+
+def context_for(atlas: Atlas, flow, numerics, implicit=None, n_cases: int = 1) -> Context:
+    """A context for a case's flow and numerics (``FlowConditions``, ``NumericsConfig``)."""
+    if numerics.far_field != "characteristic":
+        raise DeviceError(f"far field model {numerics.far_field!r} is not implemented on the device")
+    return Context(atlas, gamma=flow.gamma, flux=numerics.flux, limiter=numerics.limiter, kappa=numerics.kappa,
+                   viscous=not flow.inviscid, sa=flow.turbulence is not None, sutherland=flow.sutherland_ratio,
+                   c_cr1=0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0, implicit=implicit, n_cases=n_cases)

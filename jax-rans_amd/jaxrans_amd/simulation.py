@@ -1,0 +1,324 @@
+"""Run a case on the device: the drop-in for the reference's ``run``.
+
+Same call, same results record as ``src/jaxrans/simulation.py:213-284`` and the
+driver ``src/jaxrans/solvers/driver.py:123-201``: blocks of ``block_iters``
+Newton steps, each block one device call with no host synchronisation inside
+it; between blocks the host reads the residual history, integrates the forces
+(on the device) and applies the convergence, stall, non-finite and settled
+tests.  Every number is computed by the HIP library; there is no CPU path.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field, replace
+from typing import Callable
+
+import numpy as np
+
+from .case.loader import Case, CaseError
+from .core.config import FlowConditions
+from .core.state import q_infinity, uniform_canvas
+from .device import Context, context_for
+from .mesh.atlas import Atlas
+
+Progress = Callable[[int, float, float, "Forces"], None]
+
+
+class ExitReason:
+    CONVERGED = "converged"
+    SETTLED = "settled"
+    MAX_ITERS = "max_iters"
+    NON_FINITE = "non_finite"
+    STALLED = "stalled"
+
+
+@dataclass
+class ConvergenceHistory:
+    """``residual``: ``(n_iters,)`` or ``(n_iters, n_cases)`` (driver.py:42-84)."""
+
+    residual: np.ndarray
+    reason: str
+    n_iters: int
+    seconds: float = float("nan")
+    first_block_seconds: float = float("nan")
+    case_reasons: tuple = ()
+
+    def case(self, k: int) -> "ConvergenceHistory":
+        return ConvergenceHistory(self.residual[:, k], self.case_reasons[k], self.n_iters, self.seconds,
+                                  self.first_block_seconds)
+
+    @property
+    def converged(self) -> bool:
+        return self.reason == ExitReason.CONVERGED
+
+    @property
+    def peak(self) -> float:
+        finite = self.residual[np.isfinite(self.residual)]
+        return float(finite.max()) if finite.size else float("nan")
+
+    @property
+    def orders_dropped(self) -> float:
+        if self.residual.size == 0:
+            return 0.0
+        peak, last = self.peak, self.residual[-1]
+        if np.ndim(last):
+            return float("nan")
+        if not (np.isfinite(peak) and np.isfinite(last)) or peak <= 0 or last <= 0:
+            return float("nan")
+        return float(np.log10(peak / last))
+
+    def summary(self) -> str:
+        return (f"{self.reason} after {self.n_iters} iterations, residual peak {self.peak:.6e} -> "
+                f"{float(np.ravel(self.residual)[-1]):.6e} ({self.orders_dropped:.2f} orders from peak)")
+
+
+@dataclass(frozen=True)
+class Forces:
+    cl: float
+    cd: float
+    cm: float
+    fx: float
+    fy: float
+    cd_friction: float = 0.0
+    bodies: tuple = ()
+    fz: float = 0.0
+    cy: float = 0.0
+
+    def summary(self) -> str:
+        friction = f" (friction {self.cd_friction:+.6f})" if self.cd_friction else ""
+        return f"CL = {self.cl:+.6f}   CD = {self.cd:+.6f}{friction}   CM = {self.cm:+.6f}"
+
+
+def _forces(row: np.ndarray) -> Forces:
+    return Forces(float(row[0]), float(row[1]), float(row[2]), float(row[4]), float(row[5]), float(row[3]))
+
+
+@dataclass(frozen=True)
+class Timing:
+    """Wall clock of a run (simulation.py:53-101): the first block carries the
+    set-up, the later ones give the steady cost."""
+
+    total: float
+    first_block: float
+    iterations: int
+    block_iters: int
+    cells: int
+    cases: int = 1
+    devices: int = 1
+
+    @property
+    def per_iteration(self) -> float:
+        later = self.iterations - self.block_iters
+        return (self.total - self.first_block) / later if later > 0 else float("nan")
+
+    @property
+    def compile(self) -> float:
+        return self.first_block - self.block_iters * self.per_iteration
+
+    @property
+    def case_iterations_per_second(self) -> float:
+        return self.cases / self.per_iteration
+
+    def record(self) -> dict:
+        return {"total_s": self.total, "compile_s": self.compile, "per_iteration_s": self.per_iteration,
+                "case_iterations_per_s": self.case_iterations_per_second,
+                "ns_per_cell_iteration": 1e9 / (self.case_iterations_per_second * self.cells),
+                "iterations": self.iterations, "cells": self.cells, "cases": self.cases, "devices": self.devices}
+
+
+SETTLED_CL_SPREAD = 1e-4
+
+
+def cl_spread(history) -> float:
+    tail = np.asarray(history)[3 * len(history) // 4:]
+    return float(tail.max() - tail.min()) if tail.size >= 4 else float("nan")
+
+
+@dataclass(frozen=True)
+class Result:
+    case: Case
+    grid: Atlas
+    U: np.ndarray
+    forces: Forces
+    history: ConvergenceHistory
+    cl_history: np.ndarray
+    timing: Timing
+    levels: tuple = ()
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def seconds(self) -> float:
+        return self.timing.total
+
+    @property
+    def cl_spread(self) -> float:
+        return cl_spread(self.cl_history)
+
+    @property
+    def cells(self) -> int:
+        return self.grid.cells
+
+
+def _check_case(case: Case) -> None:
+    if case.numerics.solver != "implicit":
+        raise CaseError("only the implicit (Newton-Krylov) solver runs on the device; set "
+                        "numerical.time_integration.solver = \"implicit\"")
+    if case.flow.swept:
+        raise CaseError("swept (2.5D) flow is not built by this implementation")
+
+
+def make_context(case: Case, grid: Atlas, n_cases: int = 1) -> Context:
+    return context_for(grid, case.flow, case.numerics, case.implicit, n_cases=n_cases)
+
+
+def freestream_of(flow: FlowConditions) -> tuple[np.ndarray, float | None]:
+    return q_infinity(flow), (None if flow.inviscid else flow.mu_inf)
+
+
+def run_to_steady(ctx: Context, *, max_iters: int, tol: float, block_iters: int,
+                  on_block: Callable[[int, np.ndarray], bool] | None = None):
+    """Blocks of Newton steps until every case converges (``last <= tol *
+    running peak``), stalls, goes non-finite, settles or hits ``max_iters``
+    (driver.py:123-201, the same tests in the same order)."""
+    n_blocks = max(1, -(-max_iters // block_iters))
+    blocks = []
+    peak = reasons = None
+    started = time.perf_counter()
+    first = float("nan")
+    for b in range(n_blocks):
+        res = ctx.newton_block(block_iters)
+        if b == 0:
+            first = time.perf_counter() - started
+        blocks.append(res)
+        stop = on_block(b, res) if on_block is not None else False
+        cols = res.reshape(res.shape[0], -1)
+        if peak is None:
+            peak = np.zeros(cols.shape[1])
+            reasons = np.full(cols.shape[1], ExitReason.MAX_ITERS, dtype=object)
+        finite = np.where(np.isfinite(cols), cols, 0.0)
+        peak = np.maximum(peak, finite.max(axis=0))
+        last = cols[-1]
+        running = reasons == ExitReason.MAX_ITERS
+        failed = ~np.all(np.isfinite(cols), axis=0)
+        reasons[running & failed] = ExitReason.NON_FINITE
+        reasons[running & ~failed & (peak > 0) & (last <= tol * peak)] = ExitReason.CONVERGED
+        frozen = (len(cols) > 1) & np.all(cols == cols[0], axis=0)
+        reasons[(reasons == ExitReason.MAX_ITERS) & frozen] = ExitReason.STALLED
+        if stop:
+            reasons[reasons == ExitReason.MAX_ITERS] = ExitReason.SETTLED
+        if not np.any(reasons == ExitReason.MAX_ITERS):
+            break
+    residual = np.concatenate(blocks) if blocks else np.zeros((0, ctx.n_cases))
+    failures = [r for r in reasons if r not in (ExitReason.CONVERGED, ExitReason.SETTLED)]
+    settled = ExitReason.SETTLED in reasons
+    if ctx.n_cases == 1:
+        residual = residual[:, 0]
+    return ConvergenceHistory(residual=residual,
+                              reason=failures[0] if failures else ExitReason.SETTLED if settled else
+                              ExitReason.CONVERGED,
+                              n_iters=int(residual.shape[0]), seconds=time.perf_counter() - started,
+                              first_block_seconds=first,
+                              case_reasons=tuple(reasons) if ctx.n_cases > 1 else ())
+
+
+def run(case: Case, grid: Atlas | None = None, progress: Progress | None = None, on_level=None, devices=None, *,
+        initial: np.ndarray | None = None) -> Result:
+    """March ``case`` to steady state on the device (simulation.py:213-240)."""
+    if devices is not None and len(devices) > 1:
+        raise CaseError("this implementation runs one case on one GPU; multi-device slabs are not built")
+    if case.sequencing:
+        raise CaseError("grid sequencing is not built by this implementation")
+    _check_case(case)
+    grid = case.grid() if grid is None else grid
+    return _march(case, grid, progress, initial=initial)
+
+
+def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) -> Result:
+    flow, numerics = case.flow, case.numerics
+    spec = case.grid_spec
+    ctx = make_context(case, grid)
+    q, mu = freestream_of(flow)
+    ctx.set_freestream(q, mu)
+    U0 = uniform_canvas(q, grid.shape, flow.gamma)
+    if initial is not None:
+        initial = np.asarray(initial, np.float64)
+        if initial.shape != U0.shape:
+            raise ValueError(f"initial field must have shape {U0.shape}, got {initial.shape}")
+        U0 = np.where(grid.interior[None], initial, U0)
+    ctx.set_state(U0)
+    ctx.newton_init(case.implicit.cfl)
+    cl_history, coefficients, peak = [], [], [0.0]
+    centre = tuple(spec.moment_centre[:2])
+
+    def on_block(index: int, residuals: np.ndarray) -> bool:
+        finite = residuals[np.isfinite(residuals)]
+        if finite.size:
+            peak[0] = max(peak[0], float(finite.max()))
+        f = _forces(ctx.forces(spec.reference_length, centre)[0])
+        cl_history.append(f.cl)
+        coefficients.append((f.cl, f.cd, f.cm))
+        if progress is not None:
+            last = float(residuals[-1, 0])
+            orders = float(np.log10(peak[0] / last)) if peak[0] > 0.0 and last > 0.0 else 0.0
+            progress((index + 1) * case.block_iters, last, orders, f)
+        return numerics.forces_tol is not None and all(
+            cl_spread(h) < numerics.forces_tol for h in np.transpose(coefficients))
+
+    history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
+                            on_block=on_block)
+    U = ctx.get_state()[0]
+    forces = _forces(ctx.forces(spec.reference_length, centre)[0])
+    timing = Timing(history.seconds, history.first_block_seconds, history.n_iters, case.block_iters, grid.cells)
+    result = Result(case=case, grid=grid, U=U, forces=forces, history=history, cl_history=np.asarray(cl_history),
+                    timing=timing, extra={"scalars": ctx.scalars()[0]})
+    ctx.close()
+    return result
+
+
+def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None = None,
+              progress: Progress | None = None):
+    """A batch of freestreams on one grid in one device context (the reference's
+    ``run_polar``, ``src/jaxrans/polar.py:80-121``, with per-case Mach and
+    Reynolds as its slot scheduler carries them, ``sweep.py:196-201``).
+    Returns one ``Result`` per case."""
+    _check_case(case)
+    grid = case.grid() if grid is None else grid
+    alphas = np.atleast_1d(np.asarray(alphas, np.float64))
+    n = alphas.size
+    machs = np.full(n, case.flow.mach) if machs is None else np.broadcast_to(machs, (n,)).astype(np.float64)
+    flows = []
+    for a, m in zip(alphas, machs):
+        re = case.flow.reynolds if reynolds is None else float(reynolds)
+        flows.append(replace(case.flow, alpha_deg=float(a), mach=float(m), reynolds=re))
+    ctx = make_context(case, grid, n_cases=n)
+    qs = np.stack([q_infinity(f) for f in flows])
+    q5 = np.zeros((n, 5))
+    q5[:, :qs.shape[1]] = qs
+    mus = None if case.flow.inviscid else np.array([f.mu_inf for f in flows])
+    ctx.set_freestream(q5, mus)
+    U0 = np.stack([uniform_canvas(q, grid.shape, f.gamma) for q, f in zip(qs, flows)])
+    ctx.set_state(U0)
+    ctx.newton_init(np.full(n, case.implicit.cfl))
+    spec, numerics = case.grid_spec, case.numerics
+    centre = tuple(spec.moment_centre[:2])
+    cl_hist = []
+
+    def on_block(index, residuals):
+        cl_hist.append(ctx.forces(spec.reference_length, centre)[:, 0].copy())
+        return False
+
+    history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
+                            on_block=on_block)
+    U = ctx.get_state()
+    F = ctx.forces(spec.reference_length, centre)
+    cl_hist = np.asarray(cl_hist)
+    timing = Timing(history.seconds, history.first_block_seconds, history.n_iters, case.block_iters, grid.cells,
+                    cases=n)
+    out = []
+    for k in range(n):
+        h = history.case(k) if n > 1 else history
+        out.append(Result(case=replace(case, flow=flows[k]), grid=grid, U=U[k], forces=_forces(F[k]), history=h,
+                          cl_history=cl_hist[:, k] if cl_hist.size else cl_hist, timing=timing))
+    ctx.close()
+    return out

@@ -1,0 +1,355 @@
+"""Pointwise physics of the oracle: limiters, Riemann fluxes, boundary states,
+viscous stress, Spalart-Allmaras.  Rows are Python lists of arrays (or dual
+numbers), 2D only.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Each function restates the reference function cited in its docstring, in the
+same arithmetic order, so FP64 results agree to a few ulps.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import dual as M
+
+# --- constants (src/jaxrans/core/constants.py:35-52) ----------------------------
+PR, PR_T = 0.72, 0.9
+MIN_DENSITY = MIN_PRESSURE = 1.0e-10
+
+# SA constants (src/jaxrans/physics/spalart_allmaras.py:65-89)
+CB1, SIGMA, CB2, KAPPA = 0.1355, 2.0 / 3.0, 0.622, 0.41
+CW2, CW3, CV1, C2, C3, CN1, RMAX = 0.3, 2.0, 7.1, 0.7, 0.9, 16.0, 10.0
+CW1 = CB1 / KAPPA**2 + (1.0 + CB2) / SIGMA
+
+
+def dot2(a, b):
+    """``a0 b0 + a1 b1`` in index order (src/jaxrans/core/state.py:120-126)."""
+    return a[0] * b[0] + a[1] * b[1]
+
+
+def hypot_jnp(x, y):
+    """``jnp.hypot``'s formula: ``max * sqrt(1 + (min/max)^2)`` (constant data only)."""
+    x, y = np.abs(x), np.abs(y)
+    hi, lo = np.maximum(x, y), np.minimum(x, y)
+    safe = np.where(hi == 0, 1.0, hi)
+    return np.where(hi == 0, hi, hi * np.sqrt(1.0 + (lo / safe) ** 2))
+
+
+def primitive(U, gamma):
+    """``(rho, u, v, p[, nu_hat])`` rows from conservative rows
+    (src/jaxrans/numerics/residual.py:46-51, core/state.py:141-146)."""
+    rho = U[0]
+    u, v = U[1] / rho, U[2] / rho
+    p = (gamma - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v))
+    out = [rho, u, v, p]
+    if len(U) > 4:
+        out.append(U[4] / rho)
+    return out
+
+
+def conservative(q, gamma):
+    rho, u, v, p = q[0], q[1], q[2], q[3]
+    e = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    out = [rho, rho * u, rho * v, e]
+    if len(q) > 4:
+        out.append(rho * q[4])
+    return out
+
+
+# --- limiters (src/jaxrans/numerics/limiters.py:234-290) --------------------------
+
+def limiter(name):
+    def none(a, b):
+        return 0.5 * (a + b)
+
+    def minmod(a, b):
+        return M.where(a * b > 0.0, _sign(a) * M.minimum(M.fabs(a), M.fabs(b)), 0.0)
+
+    def van_albada(a, b):
+        same = a * b > 0.0
+        denom = M.where(same, a * a + b * b, 1.0)
+        return M.where(same, a * b * (a + b) / denom, 0.0)
+
+    def van_albada_smooth(a, b):
+        e = 1e-12
+        return ((b * b + e) * a + (a * a + e) * b) / (a * a + b * b + 2.0 * e)
+
+    return {"none": none, "minmod": minmod, "van_albada": van_albada, "van_albada_smooth": van_albada_smooth}[name]
+
+
+def _sign(a):
+    """``jnp.sign``: piecewise constant, zero tangent."""
+    return np.sign(M.value(a))
+
+
+# --- Euler pieces (src/jaxrans/physics/euler.py:28-94) ---------------------------
+
+def normal_flux(q, n, gamma):
+    rho, u, v, p = q[0], q[1], q[2], q[3]
+    vn = u * n[0] + v * n[1]
+    e = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return [rho * vn, rho * u * vn + p * n[0], rho * v * vn + p * n[1], (e + p) * vn]
+
+
+def total_enthalpy(rho, u, v, p, gamma):
+    return gamma / (gamma - 1.0) * p / rho + 0.5 * (u * u + v * v)
+
+
+def roe_average(qL, qR, gamma):
+    rl, ul, vl, pl = qL[:4]
+    rr, ur, vr, pr = qR[:4]
+    hl = total_enthalpy(rl, ul, vl, pl, gamma)
+    hr = total_enthalpy(rr, ur, vr, pr, gamma)
+    w = M.sqrt(rr / rl)
+    den = 1.0 + w
+    rho = M.sqrt(rl * rr)
+    u = (ul + w * ur) / den
+    v = (vl + w * vr) / den
+    h = (hl + w * hr) / den
+    a2 = (gamma - 1.0) * (h - 0.5 * (u * u + v * v))
+    return rho, u, v, h, M.sqrt(M.maximum(a2, 1e-30))
+
+
+def unit_normal(s):
+    """(src/jaxrans/numerics/riemann.py:47-51)"""
+    length = hypot_jnp(s[0], s[1])
+    safe = np.maximum(length, 1e-300)
+    return length, (s[0] / safe, s[1] / safe)
+
+
+def hllc(qL, qR, s, gamma):
+    """HLLC with Einfeldt speeds (src/jaxrans/numerics/riemann.py:116-198)."""
+    length, n = unit_normal(s)
+    rl, ul, vl, pl = qL[:4]
+    rr, ur, vr, pr = qR[:4]
+    vnl = ul * n[0] + vl * n[1]
+    vnr = ur * n[0] + vr * n[1]
+    al = M.sqrt(gamma * pl / rl)
+    ar = M.sqrt(gamma * pr / rr)
+    _, uro, vro, _, aro = roe_average(qL, qR, gamma)
+    vnro = uro * n[0] + vro * n[1]
+    sl = M.minimum(vnl - al, vnro - aro)
+    sr = M.maximum(vnr + ar, vnro + aro)
+    ml = rl * (sl - vnl)
+    mr = rr * (sr - vnr)
+    sm = (pr - pl + ml * vnl - mr * vnr) / M.minimum(ml - mr, -1e-300)
+
+    def star(rho, u, v, p, vn, sk, fk, uk, widen):
+        pstar = p + rho * (vn - sk) * (vn - sm)
+        rstar = rho * (sk - vn) / widen
+        estar = ((sk - vn) * uk[3] - p * vn + pstar * sm) / widen
+        ustar = [rstar, rstar * (u + (sm - vn) * n[0]), rstar * (v + (sm - vn) * n[1]), estar]
+        return [fk[k] + sk * (ustar[k] - uk[k]) for k in range(4)]
+
+    ucl, ucr = conservative(qL[:4], gamma), conservative(qR[:4], gamma)
+    fl, fr = normal_flux(qL, n, gamma), normal_flux(qR, n, gamma)
+    stl = star(rl, ul, vl, pl, vnl, sl, fl, ucl, M.minimum(sl - sm, -1e-300))
+    strr = star(rr, ur, vr, pr, vnr, sr, fr, ucr, M.maximum(sr - sm, 1e-300))
+    out = []
+    for k in range(4):
+        f = M.where(sl >= 0.0, fl[k], M.where(sm >= 0.0, stl[k], M.where(sr >= 0.0, strr[k], fr[k])))
+        out.append(length * f)
+    return out
+
+
+def roe(qL, qR, s, gamma):
+    """Roe flux, no entropy fix (src/jaxrans/numerics/riemann.py:54-113)."""
+    length, n = unit_normal(s)
+    rho, u, v, h, a = roe_average(qL, qR, gamma)
+    vn = u * n[0] + v * n[1]
+    rl, ul, vl, pl = qL[:4]
+    rr, ur, vr, pr = qR[:4]
+    drho, dp, du, dv = rr - rl, pr - pl, ur - ul, vr - vl
+    dvn = du * n[0] + dv * n[1]
+    dvt = (du - dvn * n[0], dv - dvn * n[1])
+    a2 = a * a
+    am = (dp - rho * a * dvn) / (2.0 * a2)
+    ae = drho - dp / a2
+    ap = (dp + rho * a * dvn) / (2.0 * a2)
+    lm, lmid, lp = M.fabs(vn - a), M.fabs(vn), M.fabs(vn + a)
+    kin = 0.5 * (u * u + v * v)
+    one = 1.0
+    rm = [one, u - a * n[0], v - a * n[1], h - a * vn]
+    re = [one, u, v, kin]
+    sh = [0.0, rho * dvt[0], rho * dvt[1], rho * (u * dvt[0] + v * dvt[1])]
+    rp = [one, u + a * n[0], v + a * n[1], h + a * vn]
+    fl, fr = normal_flux(qL, n, gamma), normal_flux(qR, n, gamma)
+    out = []
+    for k in range(4):
+        diss = lm * am * rm[k] + lmid * ae * re[k] + lmid * sh[k] + lp * ap * rp[k]
+        central = 0.5 * (fl[k] + fr[k])
+        out.append(length * (central - 0.5 * diss))
+    return out
+
+
+FLUXES = {"hllc": hllc, "roe": roe}
+
+
+# --- boundary states (src/jaxrans/numerics/boundary.py:54-157) -----------------------
+
+def bc_state(kind: int, q, n, q_inf, gamma):
+    """Ghost primitives for task ``kind`` from the source cell ``q`` (rows),
+    outward unit normal ``n``; kinds as ``jaxrans_amd.mesh.atlas.KIND_CODES``."""
+    rho, u, v, p = q[:4]
+    scal = q[4:]
+    if kind in (1, 3):                     # slip wall / symmetry: mirror
+        vn = u * n[0] + v * n[1]
+        um, vm = u - 2.0 * vn * n[0], v - 2.0 * vn * n[1]
+        return [rho, um, vm, p] + ([-s for s in scal] if kind == 1 else list(scal))
+    if kind == 2:                          # no-slip wall
+        return [rho, -u, -v, p] + [-s for s in scal]
+    if kind == 4:                          # characteristic far field
+        return characteristic(q, n, q_inf, gamma)
+    if kind == 5:
+        return inflow_total(q, q_inf, gamma)
+    if kind == 6:
+        return [rho, u, v, 0.0 * p + q_inf[3]] + list(scal)
+    raise ValueError(kind)
+
+
+def characteristic(q, n, q_inf, gamma):
+    """(src/jaxrans/numerics/boundary.py:91-122)"""
+    rho_i, u_i, v_i, p_i = q[:4]
+    rho_f, u_f, v_f, p_f = q_inf[:4]
+    a_i = M.sqrt(gamma * p_i / rho_i)
+    a_f = np.sqrt(gamma * p_f / rho_f)
+    two_over = 2.0 / (gamma - 1.0)
+    vn_i = u_i * n[0] + v_i * n[1]
+    vn_f = u_f * n[0] + v_f * n[1]
+    r_plus = vn_i + two_over * a_i
+    r_minus = vn_f - two_over * a_f
+    vn_b = 0.5 * (r_plus + r_minus)
+    a_b = 0.25 * (gamma - 1.0) * (r_plus - r_minus)
+    out = vn_b > 0.0
+    entropy = M.where(out, p_i / rho_i**gamma, p_f / rho_f**gamma + 0.0 * n[0])
+    tu = M.where(out, u_i - vn_i * n[0], u_f - vn_f * n[0])
+    tv = M.where(out, v_i - vn_i * n[1], v_f - vn_f * n[1])
+    rho_b = (a_b * a_b / (gamma * entropy)) ** (1.0 / (gamma - 1.0))
+    p_b = rho_b * a_b * a_b / gamma
+    res = [rho_b, vn_b * n[0] + tu, vn_b * n[1] + tv, p_b]
+    for k in range(4, len(q)):
+        res.append(M.where(out, q[k], q_inf[k] + 0.0 * n[0]))
+    return res
+
+
+def inflow_total(q, q_inf, gamma):
+    """(src/jaxrans/numerics/boundary.py:130-147)"""
+    g1 = gamma - 1.0
+    rho_f, u_f, v_f, p_f = q_inf[:4]
+    t_f = gamma * p_f / rho_f
+    speed_f = float(hypot_jnp(np.float64(u_f), np.float64(v_f)))
+    t_tot = t_f + 0.5 * g1 * speed_f * speed_f
+    p_tot = p_f * (t_tot / t_f) ** (gamma / g1)
+    p_b = q[3]
+    mach2 = M.maximum(2.0 / g1 * ((p_tot / p_b) ** (g1 / gamma) - 1.0), 0.0)
+    t_b = t_tot / (1.0 + 0.5 * g1 * mach2)
+    speed = M.sqrt(mach2 * t_b)
+    res = [gamma * p_b / t_b, speed * u_f / speed_f, speed * v_f / speed_f, p_b]
+    for k in range(4, len(q)):
+        res.append(0.0 * p_b + q_inf[k])
+    return res
+
+
+# --- viscous (src/jaxrans/physics/navier_stokes.py:154-222) ---------------------------
+
+def viscosity(T, mu_inf, sutherland):
+    if sutherland is None:
+        return mu_inf
+    return mu_inf * T * M.sqrt(T) * (1.0 + sutherland) / (T + sutherland)
+
+
+def temperature(q, gamma):
+    return gamma * q[3] / q[0]
+
+
+def viscous_flux(vel, gv, gT, s, mu, k, extra=None):
+    """``F_v . S`` rows; ``gv[a][b] = d v_a / d x_b``."""
+    div = gv[0][0] + gv[1][1]
+    ttd = (2.0 / 3.0) * div
+
+    def tau(a, b):
+        if a == b:
+            lin = mu * (2.0 * gv[a][a] - ttd)
+        else:
+            lo, hi = min(a, b), max(a, b)
+            lin = mu * (gv[lo][hi] + gv[hi][lo])
+        return lin if extra is None else lin + extra[a][b]
+
+    force = [tau(a, 0) * s[0] + tau(a, 1) * s[1] for a in range(2)]
+    energy = (vel[0] * force[0] + vel[1] * force[1]) + k * (gT[0] * s[0] + gT[1] * s[1])
+    return [0.0 * energy, force[0], force[1], energy]
+
+
+def qcr_stress(gv, mu_t, c_cr1):
+    """QCR2000 addition (src/jaxrans/physics/spalart_allmaras.py:197-216)."""
+    square = gv[0][0] * gv[0][0] + gv[0][1] * gv[0][1] + gv[1][0] * gv[1][0] + gv[1][1] * gv[1][1]
+    sq = M.value(square)
+    positive = sq > 0.0
+    norm = M.where(positive, M.sqrt(M.where(positive, square, 1.0)), 1.0)
+    rot = [[(gv[a][b] - gv[b][a]) / norm for b in range(2)] for a in range(2)]
+    div = gv[0][0] + gv[1][1]
+    tau = [[mu_t * (gv[a][b] + gv[b][a] - (2.0 / 3.0) * div * (1.0 if a == b else 0.0)) for b in range(2)]
+           for a in range(2)]
+    prod = [[rot[i][0] * tau[j][0] + rot[i][1] * tau[j][1] for j in range(2)] for i in range(2)]
+    return [[-c_cr1 * (prod[a][b] + prod[b][a]) for b in range(2)] for a in range(2)]
+
+
+# --- Spalart-Allmaras (src/jaxrans/physics/spalart_allmaras.py:119-194) -------------------
+
+def f_v1(chi):
+    chi3 = chi**3
+    return chi3 / (chi3 + CV1**3)
+
+
+def f_v2(chi):
+    return 1.0 - chi / (1.0 + chi * f_v1(chi))
+
+
+def eddy_viscosity(rho, nu_tilde, nu):
+    pos = nu_tilde > 0.0
+    chi = M.where(pos, nu_tilde / nu, 0.0)
+    return M.where(pos, rho * nu_tilde * f_v1(chi), 0.0)
+
+
+def s_tilde(omega, nu_tilde, nu, d):
+    chi = nu_tilde / nu
+    s_bar = nu_tilde * f_v2(chi) / (KAPPA * d) ** 2
+    ordinary = s_bar >= -C2 * omega
+    den = M.where(ordinary, 1.0, (C3 - 2.0 * C2) * omega - s_bar)
+    clipped = omega + omega * (C2**2 * omega + C3 * s_bar) / den
+    return M.where(ordinary, omega + s_bar, clipped)
+
+
+def f_w(r):
+    g = r + CW2 * (r**6 - r)
+    c6 = CW3**6
+    return g * ((1.0 + c6) / (g**6 + c6)) ** (1.0 / 6.0)
+
+
+def r_of(nu_tilde, s_t, d):
+    den = s_t * (KAPPA * d) ** 2
+    pos = den > 0.0
+    safe = M.where(pos, den, 1.0)
+    return M.where(pos, M.minimum(nu_tilde / safe, RMAX), RMAX)
+
+
+def source(nu_tilde, nu, omega, d):
+    pos = nu_tilde >= 0.0
+    st = s_tilde(omega, M.where(pos, nu_tilde, 0.0), nu, d)
+    wall = (nu_tilde / d) ** 2
+    production = M.where(pos, CB1 * st, CB1 * omega) * nu_tilde
+    destruction = M.where(pos, CW1 * f_w(r_of(nu_tilde, st, d)), -CW1 + 0.0 * M.value(d)) * wall
+    return production, destruction
+
+
+def f_n(nu_tilde, nu):
+    pos = nu_tilde >= 0.0
+    chi3 = M.where(pos, 0.0, (nu_tilde / nu) ** 3)
+    return M.where(pos, 1.0, (CN1 + chi3) / (CN1 - chi3))
+
+
+def diffusion_coefficient(rho, nu_tilde, mu):
+    nu = mu / rho
+    return (mu + rho * nu_tilde * f_n(nu_tilde, nu)) / SIGMA

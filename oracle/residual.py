@@ -1,0 +1,341 @@
+"""The oracle's residual ``R(U)``, its first-order variant and tangent, the
+face wave speeds and the residual norm, on the 2D canvas.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Restates ``src/jaxrans/numerics/residual.py:54-91`` (block residual),
+``:94-124`` (viscous fluxes), ``:169-203`` (wave speeds), ``:237-240`` (norm),
+``:312-327`` (ghost fill, edge pad, interior mask); ``numerics/boundary.py:257-267``
+(staged ghost fill); ``numerics/reconstruction.py:29-80`` (MUSCL);
+``numerics/gradients.py:31-95`` (diamond face gradients);
+``numerics/turbulence.py:56-151`` (SA row).  The tangent is the same code run
+on dual numbers (``oracle/dual.py``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import dual as M
+from . import physics as P
+
+
+@dataclass(frozen=True)
+class Setup:
+    """What a residual evaluation needs besides the state."""
+
+    atlas: object
+    gamma: float
+    q_inf: np.ndarray           # primitives (+ nu_hat ratio with SA)
+    flux: str = "hllc"
+    limiter: str = "minmod"
+    kappa: float = 0.0
+    mu_inf: float | None = None
+    sutherland: float | None = None
+    sa: bool = False
+    c_cr1: float = 0.0
+
+    @property
+    def nvar(self) -> int:
+        return 5 if self.sa else 4
+
+    @property
+    def viscous(self) -> bool:
+        return self.mu_inf is not None
+
+
+def setup_for(atlas, flow, numerics, freestream=None, mu_inf=None) -> Setup:
+    """A ``Setup`` from the host configs (``FlowConditions``, ``NumericsConfig``)."""
+    from jaxrans_amd.core.state import q_infinity
+    q_inf = q_infinity(flow) if freestream is None else np.asarray(freestream, np.float64)
+    mu = None if flow.inviscid else (flow.mu_inf if mu_inf is None else float(mu_inf))
+    if flow.turbulence is not None and q_inf.size == 4:
+        q_inf = np.concatenate([q_inf, [flow.nu_tilde_ratio]])
+    return Setup(atlas, flow.gamma, q_inf, numerics.flux, numerics.limiter, numerics.kappa, mu,
+                 flow.sutherland_ratio, flow.turbulence is not None,
+                 0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0)
+
+
+def rows(U):
+    return [U[k] for k in range(len(U))]
+
+
+def U0_of(setup: Setup) -> np.ndarray:
+    NI, NJ = setup.atlas.shape
+    c = np.array([np.float64(x) for x in P.conservative(list(setup.q_inf), setup.gamma)])
+    return np.broadcast_to(c[:, None, None], (c.size, NI, NJ)).copy()
+
+
+def _scatter(row, dest, vals):
+    if isinstance(row, M.D) or isinstance(vals, M.D):
+        rv, rd = M.value(row).copy(), M.tangent(row).copy() if isinstance(row, M.D) else np.zeros(row.shape)
+        rv[dest] = M.value(vals)
+        rd[dest] = M.tangent(vals) if isinstance(vals, M.D) else 0.0
+        return M.D(rv, rd)
+    out = row.copy()
+    out[dest] = vals
+    return out
+
+
+def fill_ghosts(q, setup: Setup):
+    """Staged ghost fill on flat primitive rows ``(n_canvas,)`` each."""
+    for st in setup.atlas.stages:
+        src_rows = [M.take(r, st.src) for r in q]
+        new = [None] * len(q)
+        vals_by_row = [np.zeros(st.dest.size) for _ in q]
+        pieces = []
+        for kind in np.unique(st.kind):
+            sel = np.flatnonzero(st.kind == kind)
+            part = [M.take(r, sel) for r in src_rows]
+            if kind == 0:
+                out = part
+            else:
+                n = (st.normal[0, sel], st.normal[1, sel])
+                out = P.bc_state(int(kind), part, n, setup.q_inf, setup.gamma)
+            pieces.append((sel, out))
+        for k in range(len(q)):
+            anyd = any(isinstance(o[k], M.D) for _, o in pieces)
+            if anyd:
+                v, d = np.zeros(st.dest.size), np.zeros(st.dest.size)
+                for sel, o in pieces:
+                    v[sel] = M.value(o[k])
+                    d[sel] = M.tangent(o[k]) if isinstance(o[k], M.D) else 0.0
+                vals_by_row[k] = M.D(v, d)
+            else:
+                v = np.zeros(st.dest.size)
+                for sel, o in pieces:
+                    v[sel] = o[k]
+                vals_by_row[k] = v
+        for k in range(len(q)):
+            new[k] = _scatter(q[k], st.dest, vals_by_row[k])
+        q = new
+    return q
+
+
+def face_states(p, g, shape, axis, lim, kappa, first_order):
+    """MUSCL states of one face family from padded rows ``p`` (``(NI+2g, NJ+2g)`` each)."""
+    NI, NJ = shape
+
+    def window(off):
+        if axis == 0:
+            return [r[g - 2 + off:g - 2 + off + NI + 1, g:g + NJ] for r in p]
+        return [r[g:g + NI, g - 2 + off:g - 2 + off + NJ + 1] for r in p]
+
+    qmm, qm, qp, qpp = window(0), window(1), window(2), window(3)
+    if first_order:
+        return qm, qp
+    if kappa != 0.0:
+        lo, hi = 0.25 * (1.0 - kappa), 0.25 * (1.0 + kappa)
+        return ([qm[k] + lo * (qm[k] - qmm[k]) + hi * (qp[k] - qm[k]) for k in range(len(p))],
+                [qp[k] - lo * (qpp[k] - qp[k]) - hi * (qp[k] - qm[k]) for k in range(len(p))])
+    qL = [qm[k] + 0.5 * lim(qm[k] - qmm[k], qp[k] - qm[k]) for k in range(len(p))]
+    qR = [qp[k] - 0.5 * lim(qp[k] - qm[k], qpp[k] - qp[k]) for k in range(len(p))]
+    return qL, qR
+
+
+def _corner(a):
+    return 0.25 * (a[:-1, :-1] + a[1:, :-1] + a[:-1, 1:] + a[1:, 1:])
+
+
+def face_gradients(phi, centres):
+    """Diamond gradients per family: ``grads[a][b][c]`` = d phi_c / d x_b on
+    the faces of family ``a`` (src/jaxrans/numerics/gradients.py:59-95)."""
+    NI, NJ = centres.shape[1] - 2, centres.shape[2] - 2
+    xn = [_corner(centres[0]), _corner(centres[1])]
+    pn = [_corner(f) for f in phi]
+    out = []
+    for a in range(2):
+        if a == 0:
+            L, R = (slice(0, NI + 1), slice(1, 1 + NJ)), (slice(1, NI + 2), slice(1, 1 + NJ))
+            n0, n1 = (slice(0, NI + 1), slice(0, NJ)), (slice(0, NI + 1), slice(1, NJ + 1))
+        else:
+            L, R = (slice(1, 1 + NI), slice(0, NJ + 1)), (slice(1, 1 + NI), slice(1, NJ + 2))
+            n0, n1 = (slice(0, NI), slice(0, NJ + 1)), (slice(1, NI + 1), slice(0, NJ + 1))
+        d1x = centres[0][R] - centres[0][L]
+        d1y = centres[1][R] - centres[1][L]
+        d2x = xn[0][n1] - xn[0][n0]
+        d2y = xn[1][n1] - xn[1][n0]
+        det = d1x * d2y - d2x * d1y
+        gx, gy = [], []
+        for c in range(len(phi)):
+            dp1 = phi[c][R] - phi[c][L]
+            dp2 = pn[c][n1] - pn[c][n0]
+            gx.append((dp1 * d2y - dp2 * d1y) / det)
+            gy.append((d1x * dp2 - d2x * dp1) / det)
+        out.append((gx, gy))
+    return out
+
+
+def _lr(p1, a):
+    """Cells before/after each face of family ``a`` from one-ghost rows ``p1``."""
+    NI, NJ = p1[0].shape[0] - 2, p1[0].shape[1] - 2
+    if a == 0:
+        return [r[0:NI + 1, 1:NJ + 1] for r in p1], [r[1:NI + 2, 1:NJ + 1] for r in p1]
+    return [r[1:NI + 1, 0:NJ + 1] for r in p1], [r[1:NI + 1, 1:NJ + 2] for r in p1]
+
+
+def divergence(fl):
+    """``sum_a F[hi] - F[lo]`` for per-family face fields (turbulence.py:56-67)."""
+    f0, f1 = fl
+    return (f0[1:, :] - f0[:-1, :]) + (f1[:, 1:] - f1[:, :-1])
+
+
+def green_gauss(phi_faces, setup):
+    """``(1/V) sum_f phi_f S_f`` per direction (turbulence.py:70-75 via
+    ``green_gauss``), for one scalar per family."""
+    at = setup.atlas
+    fi, fj = at.face_i, at.face_j
+    return [divergence((phi_faces[0] * fi[c], phi_faces[1] * fj[c])) / at.area for c in range(2)]
+
+
+def net_flux(q, setup: Setup, first_order: bool):
+    """Net flux of every canvas cell from primitive rows on the canvas (ghosts filled)."""
+    at, g = setup.atlas, setup.atlas.g
+    NI, NJ = at.shape
+    nf = 4
+    pad = [M.pad_edge(r, ((g, g), (g, g))) for r in q]
+    lim = P.limiter(setup.limiter)
+    flux = P.FLUXES[setup.flux]
+    S = (at.face_i, at.face_j)
+    fluxes = []
+    for a in range(2):
+        qL, qR = face_states(pad[:nf], g, (NI, NJ), a, lim, setup.kappa, first_order)
+        fluxes.append(flux(qL, qR, S[a], setup.gamma))
+    sa_row = None
+    if setup.viscous:
+        p1 = [r[g - 1:r.shape[0] - (g - 1), g - 1:r.shape[1] - (g - 1)] for r in pad]
+        T1 = P.temperature(p1, setup.gamma)
+        phi = [p1[1], p1[2], T1] + ([p1[4]] if setup.sa else [])
+        grads = face_gradients(phi, at.centres)
+        visc = []
+        mass = []
+        for a in range(2):
+            left, right = _lr(p1, a)
+            gx, gy = grads[a]
+            gv = [[gx[0], gy[0]], [gx[1], gy[1]]]
+            gT = [gx[2], gy[2]]
+            vel = [0.5 * (left[1] + right[1]), 0.5 * (left[2] + right[2])]
+            T = 0.5 * (P.temperature(left, setup.gamma) + P.temperature(right, setup.gamma))
+            mu = P.viscosity(T, setup.mu_inf, setup.sutherland)
+            k = mu / (P.PR * (setup.gamma - 1.0))
+            extra = None
+            if setup.sa:
+                face = [0.5 * (left[c] + right[c]) for c in range(5)]
+                Tf = P.temperature(face, setup.gamma)
+                muf = P.viscosity(Tf, setup.mu_inf, setup.sutherland)
+                mut = P.eddy_viscosity(face[0], face[4] * setup.mu_inf, muf / face[0])
+                mu = mu + mut
+                k = k + mut / (P.PR_T * (setup.gamma - 1.0))
+                if setup.c_cr1:
+                    extra = P.qcr_stress(gv, mut, setup.c_cr1)
+            visc.append(P.viscous_flux(vel, gv, gT, S[a], mu, k, extra))
+            mass.append(fluxes[a][0])
+        if setup.sa:
+            sa_row = _sa_row(p1, mass, [(grads[a][0][3], grads[a][1][3]) for a in range(2)], setup)
+        fluxes = [[fluxes[a][c] - visc[a][c] for c in range(4)] for a in range(2)]
+    out = [divergence((fluxes[0][c], fluxes[1][c])) for c in range(4)]
+    if sa_row is not None:
+        out.append(sa_row)
+    return out
+
+
+def _sa_row(p1, mass, grad_nu, setup):
+    """SA residual row (src/jaxrans/numerics/turbulence.py:112-151)."""
+    at, gamma, nu_inf = setup.atlas, setup.gamma, setup.mu_inf
+    S = (at.face_i, at.face_j)
+    fl = []
+    faces = []
+    for a in range(2):
+        left, right = _lr(p1, a)
+        up = M.where(M.value(mass[a]) > 0.0, left[4], right[4])
+        face = [0.5 * (left[c] + right[c]) for c in range(5)]
+        faces.append(face)
+        mu = P.viscosity(P.temperature(face, gamma), setup.mu_inf, setup.sutherland)
+        coef = P.diffusion_coefficient(face[0], face[4] * nu_inf, mu)
+        gx, gy = grad_nu[a]
+        fl.append(mass[a] * up - coef * (gx * S[a][0] + gy * S[a][1]))
+    NI, NJ = at.shape
+    cell = [r[1:NI + 1, 1:NJ + 1] for r in p1]
+    rho = cell[0]
+    mu = P.viscosity(P.temperature(cell, gamma), setup.mu_inf, setup.sutherland)
+    # Vorticity by Green-Gauss of the face-averaged velocity.
+    gu = green_gauss((faces[0][1], faces[1][1]), setup)
+    gvv = green_gauss((faces[0][2], faces[1][2]), setup)
+    omega = M.fabs(gvv[0] - gu[1])
+    prod, dest = P.source(cell[4] * nu_inf, mu / rho, omega, at.distance)
+    gn = green_gauss((faces[0][4], faces[1][4]), setup)
+    src = rho * ((prod - dest) / nu_inf + P.CB2 / P.SIGMA * nu_inf * (gn[0] * gn[0] + gn[1] * gn[1]))
+    return divergence((fl[0], fl[1])) - src * at.area
+
+
+def residual(U, setup: Setup, first_order: bool = False):
+    """``R(U)`` on the canvas, ``(nvar, NI, NJ)`` (array or dual)."""
+    at = setup.atlas
+    NI, NJ = at.shape
+    Ur = rows(U)
+    q = P.primitive(Ur, setup.gamma)
+    flat = [r.reshape(-1) for r in q]
+    filled = fill_ghosts(flat, setup)
+    qc = [r.reshape(NI, NJ) for r in filled]
+    nf = net_flux(qc, setup, first_order)
+    U0 = U0_of(setup)
+    out = [M.where(at.interior, nf[k], Ur[k] - U0[k]) for k in range(len(Ur))]
+    return M.stack(out)
+
+
+def jvp(U, dU, setup: Setup, first_order: bool = True):
+    """Tangent ``dR/dU . dU`` (the first-order residual's by default)."""
+    return residual(M.D(U, dU), setup, first_order).d
+
+
+def norm(R, setup: Setup) -> float:
+    """Volume-weighted RMS of the density row over interior cells (residual.py:237-240)."""
+    at = setup.atlas
+    w = np.where(at.interior, at.area, 0.0)
+    return float(np.sqrt(np.sum(w * R[0] ** 2) / np.sum(w)))
+
+
+def face_averages(a, atlas):
+    """Neighbour-filled face averages per family (atlas.py:163-183)."""
+    filled = a.reshape(a.shape[:-2] + (-1,))[..., atlas.neighbours].reshape(a.shape)
+    lead = filled.ndim - 2
+    out = []
+    for axis in range(2):
+        w = [(0, 0)] * filled.ndim
+        w[lead + axis] = (1, 1)
+        f = np.pad(filled, w, mode="edge")
+        lo = [slice(None)] * f.ndim
+        hi = [slice(None)] * f.ndim
+        lo[lead + axis], hi[lead + axis] = slice(0, -1), slice(1, None)
+        out.append(0.5 * (f[tuple(lo)] + f[tuple(hi)]))
+    return out
+
+
+def wave_speeds(U, setup: Setup):
+    """Face ``lambda`` per family (residual.py:169-192)."""
+    at, gamma = setup.atlas, setup.gamma
+    q = np.stack(P.primitive(rows(U), gamma))
+    qf = face_averages(q, at)
+    S = (at.face_i, at.face_j)
+    lams = []
+    vols = face_averages(at.area, at) if setup.viscous else None
+    for a in range(2):
+        f, s = qf[a], S[a]
+        mag = P.hypot_jnp(s[0], s[1])
+        lam = np.abs(f[1] * s[0] + f[2] * s[1]) + np.sqrt(gamma * f[3] / f[0]) * mag
+        if setup.viscous:
+            mu = P.viscosity(gamma * f[3] / f[0], setup.mu_inf, setup.sutherland)
+            if setup.sa:
+                mu = mu + P.eddy_viscosity(f[0], f[4] * setup.mu_inf, mu / f[0])
+            factor = 2.0 * gamma / P.PR
+            lam = lam + factor * mu * (s[0] * s[0] + s[1] * s[1]) / (f[0] * vols[a])
+        lams.append(lam)
+    return lams
+
+
+def wave_speed_sum(lams):
+    """``sum_faces lambda`` per cell (residual.py:195-203)."""
+    l0, l1 = lams
+    return ((l0[:-1, :] + l0[1:, :]) + l1[:, :-1]) + l1[:, 1:]
