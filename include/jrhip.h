@@ -67,7 +67,8 @@ typedef struct {
   double sutherland;           /* S/T_inf, <= 0: constant viscosity */
   double c_cr1;                /* QCR2000, 0: off */
   int32_t jacobian_exact;      /* 0: first-order Jacobian (defect correction) */
-  int32_t mixed;               /* 1: FP32 line-preconditioner blocks and block-Thomas */
+  int32_t mixed;               /* 1: FP32 line-preconditioner factor and solve */
+  int32_t line_solver;         /* 0: block cyclic reduction (linalg.py:159-200), 1: block Thomas */
   int32_t krylov_subspace;
   int32_t krylov_restarts;
   double krylov_tol;

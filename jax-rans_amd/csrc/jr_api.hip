@@ -12,6 +12,7 @@
 
 #include "../../include/jrhip.h"
 #include "jr_krylov.h"
+#include "jr_cr.h"
 
 using namespace jr;
 
@@ -32,7 +33,7 @@ struct jr_ctx {
   Plan P{};
   Physics ph{};
   int nv = 4, n_cases = 1;
-  int m = 10, restarts = 2, exact = 0, mixed = 0;
+  int m = 10, restarts = 2, exact = 0, mixed = 0, thomas = 0, cr_threads = 64;
   double ktol = 1e-2, max_rel = -1.0;
   Ramp ramp{};
   hipStream_t stream = nullptr;
@@ -44,9 +45,10 @@ struct jr_ctx {
   double *X = nullptr, *Act = nullptr, *Rv = nullptr, *Z = nullptr, *W = nullptr, *T = nullptr, *V = nullptr;
   double *tmpA = nullptr, *tmpB = nullptr;
   double *lam = nullptr, *sam = nullptr, *sad = nullptr, *lsum = nullptr, *vdt = nullptr, *sa_diag = nullptr;
-  void *fac = nullptr, *fac_sa = nullptr, *scr = nullptr;
+  void *fac = nullptr, *fac_sa = nullptr, *ws = nullptr, *ws_sa = nullptr;
+  double* blk = nullptr;
   Ctl* ctl = nullptr;
-  double *hv = nullptr, *h2v = nullptr, *H = nullptr, *y = nullptr, *hy = nullptr, *hist = nullptr, *coef = nullptr;
+  double *hv = nullptr, *h2v = nullptr, *G = nullptr, *H = nullptr, *y = nullptr, *hy = nullptr, *hist = nullptr, *coef = nullptr;
   int hist_cap = 0;
   Red red{};
   bool has_q = false;
@@ -130,15 +132,30 @@ static void launch_speeds(jr_ctx* c, const double* U) {
 template <class R, int NV>
 static void launch_factor(jr_ctx* c, const double* U) {
   const Plan& P = c->P;
-  hipLaunchKernelGGL((k_factor<R, NV>), grid1(P.n_lines, 64, c->n_cases), dim3(64), 0, c->stream, P, c->ph, c->ctl, U,
-                     c->lam, c->lsum, c->sam, c->sad, c->sa_diag, (R*)c->fac, (R*)c->fac_sa);
+  hipLaunchKernelGGL((k_blocks<NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl,
+                     U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
+  if (c->thomas)
+    hipLaunchKernelGGL((k_thomas<R, NV>), dim3(P.n_lines, c->n_cases), dim3(64), 0, c->stream, P, c->blk, (R*)c->fac,
+                       (R*)c->fac_sa);
+  else
+    hipLaunchKernelGGL((k_cr_factor<R, NV>), dim3(P.n_lines, c->n_cases), dim3(c->cr_threads), 0, c->stream, P, c->blk,
+                       (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
 }
 
 template <class R, int NV>
 static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, int check_active) {
   const Plan& P = c->P;
-  hipLaunchKernelGGL((k_apply<R, NV>), grid1((size_t)P.n_lines + P.n_free, 64, c->n_cases), dim3(64), 0, c->stream, P,
-                     c->ctl, (const R*)c->fac, (const R*)c->fac_sa, (R*)c->scr, r, z, accumulate, check_active);
+  const size_t lds = (size_t)std::max(1, P.max_len) * NV * sizeof(R);
+  if (c->thomas) {
+    const unsigned groups = (unsigned)(P.n_lines + (P.n_free + 63) / 64);
+    hipLaunchKernelGGL((k_apply<R, NV>), dim3(groups, c->n_cases), dim3(64), lds, c->stream, P, c->ctl,
+                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active);
+  } else {
+    const int T = c->cr_threads;
+    const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
+    hipLaunchKernelGGL((k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, c->stream, P, c->ctl,
+                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active);
+  }
 }
 
 // --- one Newton step ----------------------------------------------------------
@@ -153,7 +170,7 @@ static void gmres_and_update(jr_ctx* c) {
   const int fo = c->exact ? 0 : 1;
   for (int cyc = 0; cyc < c->restarts; ++cyc) {
     hipLaunchKernelGGL((k_cycle_begin<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->Rv, c->V, N, c->restarts,
-                       c->ktol);
+                       c->ktol, c->G);
     for (int j = 0; j < c->m; ++j) {
       const double* Vj = c->V + (size_t)j * vstride;
       launch_apply<R, NV>(c, Vj, c->Z, 0, 1);
@@ -163,11 +180,12 @@ static void gmres_and_update(jr_ctx* c) {
       hipLaunchKernelGGL((k_update_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
                          j, c->hv, c->red, c->h2v);
       hipLaunchKernelGGL((k_update_norm<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                         j, c->hv, c->h2v, c->red, c->H, c->m);
+                         j, c->hv, c->h2v, c->red, c->H, c->G, c->m);
       hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
                          c->V + (size_t)(j + 1) * vstride, N);
     }
-    hipLaunchKernelGGL(k_lstsq, dim3(c->n_cases), dim3(64), 0, c->stream, c->n_cases, c->ctl, c->H, c->m, c->y, c->hy);
+    hipLaunchKernelGGL(k_lstsq, dim3(c->n_cases), dim3(64), 0, c->stream, c->n_cases, c->ctl, c->H, c->G, c->m, c->y,
+                       c->hy);
     hipLaunchKernelGGL((k_combine<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->y, c->hy,
                        c->m, c->T, c->Act, c->Rv, c->R, c->red);
     launch_apply<R, NV>(c, c->T, c->X, 1, 1);
@@ -379,6 +397,10 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
     }
   }
   P.max_len = max_len;
+  if ((size_t)max_len * 5 * sizeof(double) > 60 * 1024) {
+    fail("jr_ctx_create: a line chain longer than " + std::to_string(60 * 1024 / 40) + " cells is not supported");
+    return bail(0);
+  }
   std::vector<int> freec;
   for (size_t i = 0; i < nc; ++i)
     if (!on_chain[i]) freec.push_back((int)i);
@@ -417,6 +439,8 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   c->restarts = num->krylov_restarts;
   c->exact = num->jacobian_exact;
   c->mixed = num->mixed;
+  c->thomas = num->line_solver == 1;
+  c->cr_threads = std::min(512, std::max(64, (P.max_len + 63) / 64 * 64));
   c->ktol = num->krylov_tol;
   c->max_rel = num->max_relative_update;
   c->ramp.cfl_growth = num->cfl_growth;
@@ -425,7 +449,7 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   c->ramp.cfl_max = num->cfl_max;
   c->ramp.krylov_success = num->krylov_success;
   c->N = (size_t)c->nv * nc;
-  c->nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (c->N + BLOCK * 4 - 1) / (BLOCK * 4)));
+  c->nblk = (int)std::min<size_t>(512, std::max<size_t>(1, (c->N + BLOCK * 2 - 1) / (BLOCK * 2)));
 
   const size_t F = (size_t)n_cases * c->N;
   const size_t nf = (size_t)P.nfi + P.nfj;
@@ -440,22 +464,43 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
   {
     void* p;
-    if (dalloc(c, (char**)&p, (size_t)n_cases * P.n_slots * 48 * rsz)) return bail(0);
+    const size_t rows = (size_t)n_cases * 2 * P.n_slots;   // cyclic reduction: <= 2 rows per slot
+    if (dalloc(c, (char**)&p, rows * 48 * rsz)) return bail(0);
     c->fac = p;
-    if (dalloc(c, (char**)&p, (size_t)n_cases * P.n_slots * 3 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, rows * 5 * rsz)) return bail(0);
     c->fac_sa = p;
-    if (dalloc(c, (char**)&p, (size_t)n_cases * P.n_slots * c->nv * rsz)) return bail(0);
-    c->scr = p;
+    if (dalloc(c, (char**)&p, rows * 48 * rsz)) return bail(0);
+    c->ws = p;
+    if (dalloc(c, (char**)&p, rows * 3 * rsz)) return bail(0);
+    c->ws_sa = p;
+    if (dalloc(c, &c->blk, (size_t)n_cases * P.n_slots * BLKW)) return bail(0);
   }
   if (dalloc(c, &c->ctl, (size_t)n_cases) || dalloc(c, &c->hv, (size_t)n_cases * (MMAX + 1)) ||
       dalloc(c, &c->h2v, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->H, (size_t)n_cases * (MMAX + 1) * MMAX) ||
-      dalloc(c, &c->y, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->hy, (size_t)n_cases * (MMAX + 1)) ||
-      dalloc(c, &c->coef, (size_t)n_cases * 8))
+      dalloc(c, &c->G, (size_t)n_cases * GIV) || dalloc(c, &c->y, (size_t)n_cases * (MMAX + 1)) ||
+      dalloc(c, &c->hy, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->coef, (size_t)n_cases * 8))
     return bail(0);
   c->red.stride = 1024 * (MMAX + 2);
   if (dalloc(c, &c->red.partials, (size_t)n_cases * c->red.stride) ||
       dalloc(c, &c->red.counters, (size_t)n_cases * 4))
     return bail(0);
+  // Every device buffer that a launch takes exists before the first launch.
+  for (const void* p : {(const void*)c->U, (const void*)c->R, (const void*)c->Q, (const void*)c->dQ,
+                        (const void*)c->Fi, (const void*)c->Fj, (const void*)c->X, (const void*)c->Act,
+                        (const void*)c->Rv, (const void*)c->Z, (const void*)c->W, (const void*)c->T,
+                        (const void*)c->V, (const void*)c->tmpA, (const void*)c->tmpB, (const void*)c->lam,
+                        (const void*)c->sam, (const void*)c->sad, (const void*)c->lsum, (const void*)c->vdt,
+                        (const void*)c->sa_diag, (const void*)c->ctl, (const void*)c->hv, (const void*)c->h2v,
+                        (const void*)c->G, (const void*)c->H, (const void*)c->y, (const void*)c->hy,
+                        (const void*)c->coef, (const void*)c->red.partials, (const void*)c->red.counters})
+    if (!p) {
+      fail("jr_ctx_create: a device buffer was not allocated");
+      return bail(0);
+    }
+  if (!c->fac || !c->fac_sa || !c->ws || !c->ws_sa || !c->blk) {
+    fail("jr_ctx_create: a line-preconditioner buffer was not allocated");
+    return bail(0);
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     fail("jr_ctx_create: stream");
     return bail(0);

@@ -17,6 +17,10 @@ struct Red {
   int stride;           // doubles per case in partials
 };
 
+// Incremental Givens QR of the Arnoldi Hessenberg, per case: the rotated
+// columns (MMAX+1) x MMAX row-major, then cos[MMAX], sin[MMAX], g[MMAX+1].
+constexpr size_t GIV = (size_t)(MMAX + 1) * MMAX + 2 * MMAX + (MMAX + 1);
+
 #define JR_VEC_LOOP(N) for (size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x; e < (N); e += (size_t)gridDim.x * BLOCK)
 
 // After the residual: the reported norm (density row, volume weighted,
@@ -43,7 +47,7 @@ __global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, 
       v[1] += w;
     }
   }
-  double tot[3];
+  __shared__ double tot[3];
   if (finish_sum<3>(v, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
     if (threadIdx.x == 0) {
       Ctl& c = ctl[cs];
@@ -61,13 +65,17 @@ __global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, 
 template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_cycle_begin(int nc, Ctl* __restrict__ ctl, const double* __restrict__ Rv,
                                                        double* __restrict__ V0, size_t vstride, int maxiter,
-                                                       double tol) {
+                                                       double tol, double* __restrict__ G) {
   const int cs = blockIdx.y;
   Ctl& c = ctl[cs];
   const bool act = c.cycles < maxiter && c.rnorm > tol * c.bnorm;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c.active = act;
-    c.beta = c.rnorm;
+  if (blockIdx.x == 0) {
+    double* g = G + (size_t)cs * GIV + (size_t)(MMAX + 1) * MMAX + 2 * MMAX;
+    for (int k = threadIdx.x; k <= MMAX; k += BLOCK) g[k] = k == 0 ? c.rnorm : 0.0;
+    if (threadIdx.x == 0) {
+      c.active = act;
+      c.beta = c.rnorm;
+    }
   }
   if (!act) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
@@ -76,106 +84,249 @@ __global__ void __launch_bounds__(BLOCK) k_cycle_begin(int nc, Ctl* __restrict__
   JR_VEC_LOOP(N) v0[e] = Rv[base + e] / den;
 }
 
-// CGS2 pass 1: h_k = V_k . w, k <= j.
+// --- CGS2 over the basis V_0..V_j ---------------------------------------------
+// Each element's basis values are loaded KC at a time, every load of a chunk
+// issued before the first is used (indices past the last vector are clamped
+// to it; those duplicates hit the cache and are masked out of the sums), so a
+// pass over j + 1 vectors costs ceil((j + 1) / KC) memory round trips per
+// element rather than j + 1.
+constexpr int KC = 8;
+
+__device__ __forceinline__ void basis_chunk(const double* __restrict__ Vc, size_t vstride, size_t e, int k0, int kmax,
+                                            double (&v)[KC]) {
+#pragma unroll
+  for (int i = 0; i < KC; ++i) v[i] = Vc[(size_t)min(k0 + i, kmax) * vstride + e];
+  __builtin_amdgcn_sched_barrier(0);   // keep the KC loads ahead of their first use
+}
+
+// Sum of MM values (MM a power of two, at most 64) over the 64 lanes of a
+// wave by recursive halving: at the step with lane offset o each lane keeps
+// half of its values (the upper half when bit o of the lane is set) and adds
+// its partner's copy of that half.  Returns in every lane the wave total of
+// value lane / (64 / MM): MM - 1 + log2(64 / MM) exchanges instead of
+// 6 MM for MM separate butterflies.
+template <int H, int O, int MM>
+__device__ __forceinline__ void halve(double (&v)[MM], int lane) {
+  if constexpr (H >= 1) {
+    const bool up = (lane & O) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const double keep = up ? v[i + H] : v[i];
+      const double give = up ? v[i] : v[i + H];
+      v[i] = keep + __shfl_xor(give, O, 64);
+    }
+    halve<H / 2, O / 2, MM>(v, lane);
+  }
+}
+
+template <int MM>
+__device__ __forceinline__ double wave_reduce_scatter(double (&v)[MM]) {
+  static_assert(MM >= 1 && MM <= 64 && (MM & (MM - 1)) == 0, "MM must be a power of two <= 64");
+  const int lane = threadIdx.x & 63;
+  halve<MM / 2, 32, MM>(v, lane);
+  double s = v[0];
+#pragma unroll
+  for (int o = 32 / MM; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+// Workgroup sums of MM values -> per-block partials ([k][block], coalesced
+// for the finishing pass) -> the last workgroup to arrive adds them in block
+// order into the shared array tot[MM].  Returns true in every thread of that
+// workgroup.  Deterministic: the order of every sum is fixed by the launch.
+template <int MM>
+__device__ bool finish_dots(double (&v)[MM], double* partials, unsigned* counter, double* tot) {
+  __shared__ double sh[MM][4];
+  __shared__ int is_last;
+  constexpr int SPAN = 64 / MM;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nblk = gridDim.x, blk = blockIdx.x;
+  {
+    const double s = wave_reduce_scatter<MM>(v);
+    if (lane % SPAN == 0) sh[lane / SPAN][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < MM) {
+    const int k = threadIdx.x;
+    partials[(size_t)k * nblk + blk] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    __threadfence();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(counter, 1u);
+    is_last = prev == (unsigned)(nblk - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double acc[MM];
+#pragma unroll
+  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      acc[k] += __hip_atomic_load(&partials[(size_t)k * nblk + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  {
+    const double s = wave_reduce_scatter<MM>(acc);
+    if (lane % SPAN == 0) sh[lane / SPAN][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < MM) {
+    const int k = threadIdx.x;
+    tot[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  __syncthreads();
+  return true;
+}
+
+// CGS2 pass 1: h_k = V_k . w, k <= j (j < MM).
 template <int NV, int MM>
 __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ V,
                                                 size_t vstride, size_t vcase, const double* __restrict__ W, int j,
                                                 Red red, double* __restrict__ hv) {
-  __shared__ double sh[MM + 1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double acc[MM + 1];
+  double acc[MM];
 #pragma unroll
-  for (int k = 0; k <= MM; ++k) acc[k] = 0.0;
+  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
   JR_VEC_LOOP(N) {
-    double w = W[base + e];
+    const double w = W[base + e];
 #pragma unroll
-    for (int k = 0; k <= MM; ++k)
-      if (k <= j) acc[k] += Vc[(size_t)k * vstride + e] * w;
+    for (int c = 0; c < MM / KC; ++c)
+      if (c * KC <= j) {
+        double v[KC];
+        basis_chunk(Vc, vstride, e, c * KC, j, v);
+#pragma unroll
+        for (int i = 0; i < KC; ++i) acc[c * KC + i] += v[i] * w;
+      }
   }
-  double tot[MM + 1];
-  if (finish_sum<MM + 1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot, j + 1)) {
+  __shared__ double tot[MM];
+  if (finish_dots<MM>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
     if (threadIdx.x <= j) hv[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
   }
 }
 
-// CGS2 pass 2: w <- w - sum h_k V_k, then h2_k = V_k . w.
+// CGS2 pass 2: w <- w - sum_{k<=j} h_k V_k, then h2_k = V_k . w.
 template <int NV, int MM>
 __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
                                                        double* __restrict__ W, int j, const double* __restrict__ hv,
                                                        Red red, double* __restrict__ h2v) {
-  __shared__ double sh[MM + 1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double h[MM + 1];
+  double h[MM];
 #pragma unroll
-  for (int k = 0; k <= MM; ++k) h[k] = k <= j ? hv[(size_t)cs * (MMAX + 1) + k] : 0.0;
-  double acc[MM + 1];
+  for (int k = 0; k < MM; ++k) h[k] = k <= j ? hv[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  double acc[MM];
 #pragma unroll
-  for (int k = 0; k <= MM; ++k) acc[k] = 0.0;
+  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
   JR_VEC_LOOP(N) {
-    double vk[MM + 1];
+    double vk[MM];
     double w = W[base + e];
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k <= MM; ++k)
-      if (k <= j) {
-        vk[k] = Vc[(size_t)k * vstride + e];
-        s += h[k] * vk[k];
+    for (int c = 0; c < MM / KC; ++c)
+      if (c * KC <= j) {
+        double v[KC];
+        basis_chunk(Vc, vstride, e, c * KC, j, v);
+#pragma unroll
+        for (int i = 0; i < KC; ++i) {
+          vk[c * KC + i] = v[i];
+          if (c * KC + i <= j) s += h[c * KC + i] * v[i];
+        }
       }
     w = w - s;
     W[base + e] = w;
 #pragma unroll
-    for (int k = 0; k <= MM; ++k)
-      if (k <= j) acc[k] += vk[k] * w;
+    for (int c = 0; c < MM / KC; ++c)
+      if (c * KC <= j) {
+#pragma unroll
+        for (int i = 0; i < KC; ++i) acc[c * KC + i] += vk[c * KC + i] * w;
+      }
   }
-  double tot[MM + 1];
-  if (finish_sum<MM + 1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot, j + 1)) {
+  __shared__ double tot[MM];
+  if (finish_dots<MM>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
     if (threadIdx.x <= j) h2v[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
   }
 }
 
 // CGS2 pass 3: w <- w - sum h2_k V_k, |w|; H[:, j] = h + h2, H[j+1, j] = |w|.
+// The finishing workgroup also advances the Givens QR of the Hessenberg by
+// column j (krylov.py:121-131, the same operations in the same order as the
+// whole-matrix sweep).
+
 template <int NV, int MM>
-__global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __restrict__ ctl,
-                                                       const double* __restrict__ V, size_t vstride, size_t vcase,
-                                                       double* __restrict__ W, int j, const double* __restrict__ hv,
-                                                       const double* __restrict__ h2v, Red red, double* __restrict__ H,
+__global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, Ctl* __restrict__ ctl, const double* __restrict__ V,
+                                                       size_t vstride, size_t vcase, double* __restrict__ W, int j,
+                                                       const double* __restrict__ hv, const double* __restrict__ h2v,
+                                                       Red red, double* __restrict__ H, double* __restrict__ G,
                                                        int m) {
   __shared__ double sh[1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double h2[MM + 1];
+  double h2[MM];
 #pragma unroll
-  for (int k = 0; k <= MM; ++k) h2[k] = k <= j ? h2v[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  for (int k = 0; k < MM; ++k) h2[k] = k <= j ? h2v[(size_t)cs * (MMAX + 1) + k] : 0.0;
   double acc[1] = {0.0};
   JR_VEC_LOOP(N) {
     double w = W[base + e];
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k <= MM; ++k)
-      if (k <= j) s += h2[k] * Vc[(size_t)k * vstride + e];
+    for (int c = 0; c < MM / KC; ++c)
+      if (c * KC <= j) {
+        double v[KC];
+        basis_chunk(Vc, vstride, e, c * KC, j, v);
+#pragma unroll
+        for (int i = 0; i < KC; ++i)
+          if (c * KC + i <= j) s += h2[c * KC + i] * v[i];
+      }
     w = w - s;
     W[base + e] = w;
     acc[0] += w * w;
   }
-  double tot[1];
+  __shared__ double tot[1];
+  __shared__ double gcol[MMAX + 1];
   if (finish_sum<1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
     double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;   // H[row][col], row-major (MMAX+1) x MMAX
     for (int k = threadIdx.x; k <= j; k += BLOCK)
       Hc[k * MMAX + j] = hv[(size_t)cs * (MMAX + 1) + k] + h2v[(size_t)cs * (MMAX + 1) + k];
+    __syncthreads();
     if (threadIdx.x == 0) {
       double nw = sqrt(tot[0]);
       Hc[(j + 1) * MMAX + j] = nw;
       for (int k = j + 2; k <= m; ++k) Hc[k * MMAX + j] = 0.0;
-      const_cast<Ctl*>(ctl)[cs].wnorm = nw;
+      ctl[cs].wnorm = nw;
+      double* Gc = G + (size_t)cs * GIV;
+      double* Rr = Gc;                                   // rotated columns
+      double* cs_ = Gc + (size_t)(MMAX + 1) * MMAX;
+      double* sn = cs_ + MMAX;
+      double* g = sn + MMAX;
+      double* col = gcol;
+      for (int k = 0; k <= j + 1; ++k) col[k] = Hc[k * MMAX + j];
+      for (int i = 0; i < j; ++i) {
+        double a = col[i], c = col[i + 1];
+        col[i] = cs_[i] * a + sn[i] * c;
+        col[i + 1] = cs_[i] * c - sn[i] * a;
+      }
+      double a = col[j], c = col[j + 1];
+      double r = sqrt(a * a + c * c);
+      double safe = r > 0 ? r : 1.0;
+      cs_[j] = r > 0 ? a / safe : 1.0;
+      sn[j] = r > 0 ? c / safe : 0.0;
+      col[j] = r;
+      col[j + 1] = 0.0;
+      const double gj = g[j];
+      g[j] = cs_[j] * gj;
+      g[j + 1] = -sn[j] * gj;
+      for (int k = 0; k <= j + 1; ++k) Rr[k * MMAX + j] = col[k];
     }
   }
 }
@@ -192,51 +343,35 @@ __global__ void __launch_bounds__(BLOCK) k_normalize(int nc, const Ctl* __restri
   JR_VEC_LOOP(N) v[e] = W[base + e] / den;
 }
 
-// Givens least squares of the (m+1) x m Hessenberg with the pivot drop
-// (krylov.py:100-137); y and Hy per case.  One thread per case.
-__global__ void k_lstsq(int n_cases, const Ctl* __restrict__ ctl, const double* __restrict__ H, int m,
-                        double* __restrict__ y, double* __restrict__ hy) {
-  __shared__ double h[MMAX + 1][MMAX];
+// Least squares of the cycle (krylov.py:100-137): the Givens sweep was
+// advanced column by column during the Arnoldi process; here the pivot-drop
+// back substitution (a direction whose pivot is below eps (m+1) times the
+// largest is dropped) and H y.  One workgroup per case.
+__global__ void k_lstsq(int n_cases, const Ctl* __restrict__ ctl, const double* __restrict__ H,
+                        const double* __restrict__ G, int m, double* __restrict__ y, double* __restrict__ hy) {
+  __shared__ double ys[MMAX];
   const int cs = blockIdx.x;
   if (!ctl[cs].active) return;
   const double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;
-  for (int e = threadIdx.x; e < (m + 1) * MMAX; e += blockDim.x) h[e / MMAX][e % MMAX] = Hc[e];
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double g[MMAX + 1], cs_[MMAX], sn[MMAX];
-  g[0] = ctl[cs].beta;
-  for (int i = 1; i <= m; ++i) g[i] = 0.0;
-  for (int jj = 0; jj < m; ++jj) {
-    for (int i = 0; i < jj; ++i) {
-      double a = h[i][jj], c = h[i + 1][jj];
-      h[i][jj] = cs_[i] * a + sn[i] * c;
-      h[i + 1][jj] = cs_[i] * c - sn[i] * a;
+  const double* Gc = G + (size_t)cs * GIV;
+  const double* Rr = Gc;
+  const double* g = Gc + (size_t)(MMAX + 1) * MMAX + 2 * MMAX;
+  if (threadIdx.x == 0) {
+    double largest = fabs(Rr[0]);
+    for (int jj = 1; jj < m; ++jj) largest = fmax(largest, fabs(Rr[jj * MMAX + jj]));
+    const double rcond = 2.220446049250313e-16 * (m + 1);
+    for (int jj = m - 1; jj >= 0; --jj) {
+      double rest = g[jj];
+      for (int k = jj + 1; k < m; ++k) rest = rest - Rr[jj * MMAX + k] * ys[k];
+      const double piv = Rr[jj * MMAX + jj];
+      ys[jj] = fabs(piv) > rcond * largest ? rest / piv : 0.0;
     }
-    double a = h[jj][jj], c = h[jj + 1][jj];
-    double r = sqrt(a * a + c * c);
-    double safe = r > 0 ? r : 1.0;
-    cs_[jj] = r > 0 ? a / safe : 1.0;
-    sn[jj] = r > 0 ? c / safe : 0.0;
-    h[jj][jj] = r;
-    h[jj + 1][jj] = 0.0;
-    double gj = g[jj];
-    g[jj] = cs_[jj] * gj;
-    g[jj + 1] = -sn[jj] * gj;
   }
-  double largest = fabs(h[0][0]);
-  for (int jj = 1; jj < m; ++jj) largest = fmax(largest, fabs(h[jj][jj]));
-  const double rcond = 2.220446049250313e-16 * (m + 1);
-  double yy[MMAX];
-  for (int jj = m - 1; jj >= 0; --jj) {
-    double rest = g[jj];
-    for (int k = jj + 1; k < m; ++k) rest = rest - h[jj][k] * yy[k];
-    yy[jj] = fabs(h[jj][jj]) > rcond * largest ? rest / h[jj][jj] : 0.0;
-  }
-  for (int jj = 0; jj < m; ++jj) y[(size_t)cs * (MMAX + 1) + jj] = yy[jj];
-  // H y with the original H
-  for (int i = 0; i <= m; ++i) {
+  __syncthreads();
+  for (int jj = threadIdx.x; jj < m; jj += blockDim.x) y[(size_t)cs * (MMAX + 1) + jj] = ys[jj];
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) {
     double s = 0.0;
-    for (int jj = 0; jj < m; ++jj) s += Hc[i * MMAX + jj] * yy[jj];
+    for (int jj = 0; jj < m; ++jj) s += Hc[i * MMAX + jj] * ys[jj];
     hy[(size_t)cs * (MMAX + 1) + i] = s;
   }
 }
@@ -254,9 +389,10 @@ __global__ void __launch_bounds__(BLOCK) k_combine(int nc, Ctl* __restrict__ ctl
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double yc[MM + 1], hc[MM + 1];
+  constexpr int NCH = MM / KC + 1;   // k <= m <= MM
+  double yc[NCH * KC], hc[NCH * KC];
 #pragma unroll
-  for (int k = 0; k <= MM; ++k) {
+  for (int k = 0; k < NCH * KC; ++k) {
     yc[k] = k < m ? y[(size_t)cs * (MMAX + 1) + k] : 0.0;
     hc[k] = k <= m ? hy[(size_t)cs * (MMAX + 1) + k] : 0.0;
   }
@@ -264,11 +400,16 @@ __global__ void __launch_bounds__(BLOCK) k_combine(int nc, Ctl* __restrict__ ctl
   JR_VEC_LOOP(N) {
     double t = 0.0, a = 0.0;
 #pragma unroll
-    for (int k = 0; k <= MM; ++k)
-      if (k <= m) {
-        double v = Vc[(size_t)k * vstride + e];
-        if (k < m) t += yc[k] * v;
-        a += hc[k] * v;
+    for (int c = 0; c < NCH; ++c)
+      if (c * KC <= m) {
+        double v[KC];
+        basis_chunk(Vc, vstride, e, c * KC, m, v);
+#pragma unroll
+        for (int i = 0; i < KC; ++i) {
+          const int k = c * KC + i;
+          if (k < m) t += yc[k] * v[i];
+          if (k <= m) a += hc[k] * v[i];
+        }
       }
     T[base + e] = t;
     double act = Act[base + e] + a;
@@ -277,7 +418,7 @@ __global__ void __launch_bounds__(BLOCK) k_combine(int nc, Ctl* __restrict__ ctl
     Rv[base + e] = r;
     acc[0] += r * r;
   }
-  double tot[1];
+  __shared__ double tot[1];
   if (finish_sum<1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
     if (threadIdx.x == 0) {
       ctl[cs].rnorm = sqrt(tot[0]);
@@ -326,10 +467,15 @@ __global__ void __launch_bounds__(BLOCK) k_relax(Plan P, double gamma, Ctl* __re
   __syncthreads();
   if (!is_last) return;
   __threadfence();
+  double m = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK)
+    m = fmax(m, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  m = wave_max(m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[0][threadIdx.x >> 6] = m;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double m = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b)
-      m = fmax(m, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    m = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
     ctl[cs].omega = fmin(1.0, fraction / fmax(m, 1e-300));
     red.counters[cs * 4] = 0u;
   }
@@ -368,7 +514,7 @@ __global__ void __launch_bounds__(BLOCK) k_check(Plan P, double gamma, Ctl* __re
     bool phys = q[0] > 1e-10 && q[3] > 1e-10 && isfinite(q[0]) && isfinite(q[3]);
     if (!(phys && finite)) v[0] += 1.0;
   }
-  double tot[2];
+  __shared__ double tot[2];
   if (finish_sum<2>(v, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
     if (threadIdx.x == 0) {
       Ctl& c = ctl[cs];

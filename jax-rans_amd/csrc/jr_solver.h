@@ -27,30 +27,35 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
-// Sum NV values over the workgroup; result valid in thread 0.
+// Sum NV values over the workgroup; result valid in thread 0.  All loops run
+// over the compile-time NV so the values stay in registers.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[4]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = wave_sum(v[k]);
     if (lane == 0) sh[k][w] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
+#pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+  }
   __syncthreads();
 }
 
-// Partial sums -> the last workgroup gets the totals in `tot` (all its threads).
-// Returns true in every thread of the last workgroup of this case.
+// Partial sums -> the last workgroup to finish adds them in block order and
+// leaves the NV totals in the shared array `out`.  Returns true in every
+// thread of that workgroup.
 template <int NV>
-__device__ bool finish_sum(double (&v)[NV], double* partials, unsigned* counter, double (*sh)[4], double* tot,
-                           int nvals = NV) {
+__device__ bool finish_sum(double (&v)[NV], double* partials, unsigned* counter, double (*sh)[4], double* out) {
   __shared__ int is_last;
   block_sum<NV>(v, sh);
   const int nblk = gridDim.x, blk = blockIdx.x;
   if (threadIdx.x == 0) {
-    for (int k = 0; k < nvals; ++k) partials[(size_t)blk * NV + k] = v[k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)blk * NV + k] = v[k];
     __threadfence();
     unsigned prev = atomicAdd(counter, 1u);
     is_last = prev == (unsigned)(nblk - 1);
@@ -59,18 +64,20 @@ __device__ bool finish_sum(double (&v)[NV], double* partials, unsigned* counter,
   if (!is_last) return false;
   __threadfence();
   double acc[NV];
+#pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += BLOCK)
-    for (int k = 0; k < nvals; ++k) acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
+  for (int b = threadIdx.x; b < nblk; b += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   block_sum<NV>(acc, sh);
-  __shared__ double out[NV];
   if (threadIdx.x == 0) {
+#pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = acc[k];
     *counter = 0u;
   }
   __syncthreads();
-  for (int k = 0; k < NV; ++k) tot[k] = out[k];
   return true;
 }
 
@@ -251,175 +258,320 @@ __device__ __forceinline__ void link_block(const double* Ujs, size_t base, int n
     for (int c = 0; c < 4; ++c) B[r][c] = (R)(0.5 * A[r][c] - (r == c ? 0.5 * lamf : 0.0));
 }
 
-// Factor storage per slot: [L(16) | Pinv(16) | C(16)] flow, then [l, pinv, c] SA.
-template <class R, int NV>
-__global__ void __launch_bounds__(64) k_factor(Plan P, Physics ph, const Ctl* __restrict__ ctl,
-                                               const double* __restrict__ U, const double* __restrict__ lam,
-                                               const double* __restrict__ lsum, const double* __restrict__ sam,
-                                               const double* __restrict__ sad, const double* __restrict__ sa_diag,
-                                               R* __restrict__ fac, R* __restrict__ fac_sa) {
-  int ln = blockIdx.x * 64 + threadIdx.x;
-  int cs = blockIdx.y;
-  if (ln >= P.n_lines) return;
+// Chain blocks of every slot, in parallel over slots (preconditioners.py:209-287):
+// blk[slot] = [d, Lo(16), Up(16), sa_lo, sa_d, sa_up, pad].  A slot couples to
+// its predecessor when the predecessor's link is live: the last slot of every
+// chain has no link, so the first slot of the next chain never reads across.
+constexpr int BLKW = 36;
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                  const double* __restrict__ U, const double* __restrict__ lam,
+                                                  const double* __restrict__ lsum, const double* __restrict__ sam,
+                                                  const double* __restrict__ sad, const double* __restrict__ sa_diag,
+                                                  double* __restrict__ blk) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (s >= P.n_slots) return;
   const int nc = P.nc, nf = P.nfi + P.nfj;
   const size_t base = (size_t)cs * NV * nc;
   const double gamma = ph.gamma, cfl = ctl[cs].cfl;
-  const double* L = lam + (size_t)cs * nf + P.nfi;   // j-face wave speeds
-  R* F = fac + (size_t)cs * P.n_slots * 48;
-  R Cp[4][4];
+  const double* L = lam + (size_t)cs * nf + P.nfi;
+  const int cell = P.l_cell[s];
+  double* o = blk + ((size_t)cs * P.n_slots + s) * BLKW;
+  o[0] = lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5);
+  double B[4][4];
+  const bool has_lo = s > 0 && P.l_dns[s - 1] != 0;
+  if (has_lo) {
+    int fce = P.l_dn[s - 1];
+    double sg = P.l_dns[s - 1];
+    link_block<double>(U, base, nc, P.l_cell[s - 1], sg * P.fj[fce], sg * P.fj[P.nfj + fce], L[fce], gamma, B);
+  }
   for (int r = 0; r < 4; ++r)
-    for (int c = 0; c < 4; ++c) Cp[r][c] = R(0);
-  const int s0 = P.l_start[ln], s1 = P.l_start[ln + 1];
-  // SA scalar chain
-  R* Fs = fac_sa ? fac_sa + (size_t)cs * P.n_slots * 3 : nullptr;
-  const double* mj = sam ? sam + (size_t)cs * nf + P.nfi : nullptr;
-  const double* dj = sad ? sad + (size_t)cs * nf + P.nfi : nullptr;
-  R cps = R(0);
-  for (int s = s0; s < s1; ++s) {
-    const int cell = P.l_cell[s];
-    R D = (R)(lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5));
-    R Lo[4][4], Up[4][4];
-    if (s > s0 && P.l_dns[s - 1] != 0) {
+    for (int c = 0; c < 4; ++c) o[1 + r * 4 + c] = has_lo ? B[r][c] : 0.0;
+  const bool has_up = P.l_ups[s] != 0;
+  if (has_up) {
+    int fce = P.l_up[s];
+    double sg = P.l_ups[s];
+    link_block<double>(U, base, nc, P.l_cell[s + 1], sg * P.fj[fce], sg * P.fj[P.nfj + fce], L[fce], gamma, B);
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) o[17 + r * 4 + c] = has_up ? B[r][c] : 0.0;
+  if constexpr (NV > 4) {
+    const double* mj = sam + (size_t)cs * nf + P.nfi;
+    const double* dj = sad + (size_t)cs * nf + P.nfi;
+    double lo = 0.0, up = 0.0;
+    if (has_lo) {
       int fce = P.l_dn[s - 1];
       double sg = P.l_dns[s - 1];
-      link_block<R>(U, base, nc, P.l_cell[s - 1], sg * P.fj[fce], sg * P.fj[P.nfj + fce], L[fce], gamma, Lo);
-    } else {
-      for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) Lo[r][c] = R(0);
+      lo = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s - 1]];
     }
-    if (P.l_ups[s] != 0) {
+    if (has_up) {
       int fce = P.l_up[s];
       double sg = P.l_ups[s];
-      link_block<R>(U, base, nc, P.l_cell[s + 1], sg * P.fj[fce], sg * P.fj[P.nfj + fce], L[fce], gamma, Up);
-    } else {
-      for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) Up[r][c] = R(0);
+      up = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s + 1]];
     }
-    R Pm[4][4], Pi[4][4], Cn[4][4];
-    for (int r = 0; r < 4; ++r)
-      for (int c = 0; c < 4; ++c) {
-        R acc = R(0);
-        for (int k = 0; k < 4; ++k) acc += Lo[r][k] * Cp[k][c];
-        Pm[r][c] = (r == c ? D : R(0)) - acc;
-      }
-    invert4<R>(Pm, Pi);
-    for (int r = 0; r < 4; ++r)
-      for (int c = 0; c < 4; ++c) {
-        R acc = R(0);
-        for (int k = 0; k < 4; ++k) acc += Pi[r][k] * Up[k][c];
-        Cn[r][c] = acc;
-      }
-    R* o = F + (size_t)s * 48;
-    for (int r = 0; r < 4; ++r)
-      for (int c = 0; c < 4; ++c) {
-        o[r * 4 + c] = Lo[r][c];
-        o[16 + r * 4 + c] = Pi[r][c];
-        o[32 + r * 4 + c] = Cn[r][c];
-        Cp[r][c] = Cn[r][c];
-      }
-    if constexpr (NV > 4) {
-      // scalar SA chain (preconditioners.py:277-287)
-      R ds = (R)sa_diag[(size_t)cs * nc + cell];
-      R lo = R(0), up = R(0);
-      if (s > s0 && P.l_dns[s - 1] != 0) {
-        int fce = P.l_dn[s - 1];
-        double sg = P.l_dns[s - 1];
-        double rho = U[base + P.l_cell[s - 1]];
-        lo = (R)(-(fmax(-sg * mj[fce], 0.0) + dj[fce]) / rho);
-      }
-      if (P.l_ups[s] != 0) {
-        int fce = P.l_up[s];
-        double sg = P.l_ups[s];
-        double rho = U[base + P.l_cell[s + 1]];
-        up = (R)(-(fmax(-sg * mj[fce], 0.0) + dj[fce]) / rho);
-      }
-      R pv = ds - lo * cps;
-      R pinv = R(1) / pv;
-      R cn = pinv * up;
-      R* os = Fs + (size_t)s * 3;
-      os[0] = lo; os[1] = pinv; os[2] = cn;
-      cps = cn;
-    }
+    o[33] = lo;
+    o[34] = sa_diag[(size_t)cs * nc + cell];
+    o[35] = up;
   }
 }
 
-// z <- M^-1 r  (accumulate: z += M^-1 r).  Threads beyond the chains copy the
-// cells on no chain (identity there, preconditioners.py:110-115).  `scr` holds
-// the forward-sweep values, NV per slot per case.
+// Block Thomas factor (linalg.py:80-100): one wavefront per chain.  The
+// wavefront streams the chain's blocks through LDS in chunks of CH slots
+// (the next chunk's loads in flight while lane 0 runs the recurrence on the
+// current one) and writes the factor back coalesced:
+//   fac[slot][32] = L (16), P^-1 (16);  fac[n_slots*32 + slot*16] = C (16);
+//   fac_sa[slot][3] = l, 1/p, c  (the scalar SA chain).
+constexpr int CH = 32;
+
+template <class R>
+__device__ __forceinline__ void invert4_gj(R A[4][4], R X[4][4]) { invert4<R>(A, X); }
+
+template <class R, int NV>
+__global__ void __launch_bounds__(64) k_thomas(Plan P, const double* __restrict__ blk, R* __restrict__ fac,
+                                               R* __restrict__ fac_sa) {
+  __shared__ double sb[CH * BLKW];
+  __shared__ R so[CH * 51];
+  const int lane = threadIdx.x, chain = blockIdx.x, cs = blockIdx.y;
+  if (chain >= P.n_lines) return;
+  const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
+  const int n = s1 - s0;
+  const double* B = blk + (size_t)cs * P.n_slots * BLKW;
+  R* F = fac + (size_t)cs * P.n_slots * 48;
+  R* Fc = F + (size_t)P.n_slots * 32;
+  R* Fs = fac_sa + (size_t)cs * P.n_slots * 3;
+  constexpr int PER = (CH * BLKW + 63) / 64;
+  double pf[PER];
+  auto fetch = [&](int c) {
+    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
+    const double* src = B + (size_t)sc * BLKW;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      int idx = t * 64 + lane;
+      pf[t] = idx < m * BLKW ? src[idx] : 0.0;
+    }
+  };
+  R Cp[4][4];
+  for (int r = 0; r < 4; ++r)
+    for (int q = 0; q < 4; ++q) Cp[r][q] = R(0);
+  R cps = R(0);
+  const int nch = (n + CH - 1) / CH;
+  if (nch > 0) fetch(0);
+  for (int c = 0; c < nch; ++c) {
+    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      int idx = t * 64 + lane;
+      if (idx < CH * BLKW) sb[idx] = pf[t];
+    }
+    __syncthreads();
+    if (c + 1 < nch) fetch(c + 1);
+    if (lane == 0) {
+      for (int k = 0; k < m; ++k) {
+        const double* b = sb + k * BLKW;
+        const R d = (R)b[0];
+        R Pm[4][4], Pi[4][4], Cn[4][4];
+        for (int r = 0; r < 4; ++r)
+          for (int q = 0; q < 4; ++q) {
+            R acc = R(0);
+            for (int t = 0; t < 4; ++t) acc += (R)b[1 + r * 4 + t] * Cp[t][q];
+            Pm[r][q] = (r == q ? d : R(0)) - acc;
+          }
+        invert4<R>(Pm, Pi);
+        for (int r = 0; r < 4; ++r)
+          for (int q = 0; q < 4; ++q) {
+            R acc = R(0);
+            for (int t = 0; t < 4; ++t) acc += Pi[r][t] * (R)b[17 + t * 4 + q];
+            Cn[r][q] = acc;
+          }
+        R* o = so + k * 51;
+        for (int r = 0; r < 4; ++r)
+          for (int q = 0; q < 4; ++q) {
+            o[r * 4 + q] = (R)b[1 + r * 4 + q];
+            o[16 + r * 4 + q] = Pi[r][q];
+            o[32 + r * 4 + q] = Cn[r][q];
+            Cp[r][q] = Cn[r][q];
+          }
+        if constexpr (NV > 4) {
+          R lo = (R)b[33], ds = (R)b[34], up = (R)b[35];
+          R pinv = R(1) / (ds - lo * cps);
+          R cn = pinv * up;
+          o[48] = lo;
+          o[49] = pinv;
+          o[50] = cn;
+          cps = cn;
+        }
+      }
+    }
+    __syncthreads();
+    for (int idx = lane; idx < m * 32; idx += 64) F[(size_t)sc * 32 + idx] = so[(idx >> 5) * 51 + (idx & 31)];
+    for (int idx = lane; idx < m * 16; idx += 64) Fc[(size_t)sc * 16 + idx] = so[(idx >> 4) * 51 + 32 + (idx & 15)];
+    if constexpr (NV > 4)
+      for (int idx = lane; idx < m * 3; idx += 64) Fs[(size_t)sc * 3 + idx] = so[(idx / 3) * 51 + 48 + idx % 3];
+    __syncthreads();
+  }
+}
+
+// z <- M^-1 r (accumulate: z += M^-1 r): one wavefront per chain, factor and
+// right-hand side streamed through LDS, lane 0 runs the forward and backward
+// substitutions (linalg.py:103-132).  Workgroups past the chains copy the cells
+// on no chain (identity there, preconditioners.py:110-115), 64 per workgroup.
+// Dynamic LDS: max_len * NV values of R (the forward sweep's d).
 template <class R, int NV>
 __global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
-                                              const R* __restrict__ fac_sa, R* __restrict__ scr,
-                                              const double* __restrict__ r, double* __restrict__ z, int accumulate,
-                                              int check_active) {
-  int t = blockIdx.x * 64 + threadIdx.x;
-  int cs = blockIdx.y;
+                                              const R* __restrict__ fac_sa, const double* __restrict__ r,
+                                              double* __restrict__ z, int accumulate, int check_active) {
+  extern __shared__ unsigned char dyn[];
+  R* dbuf = reinterpret_cast<R*>(dyn);
+  __shared__ R fb[CH * 34];       // forward: L, P^-1, l, 1/p per slot; backward: C, c
+  __shared__ R rb[CH * NV];       // right-hand side, then the solution of the chunk
+  __shared__ int cb[CH];
+  const int lane = threadIdx.x, chain = blockIdx.x, cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  if (t >= P.n_lines) {
-    int k = t - P.n_lines;
+  if (chain >= P.n_lines) {
+    const int k = (chain - P.n_lines) * 64 + lane;
     if (k >= P.n_free) return;
-    int cell = P.free_cell[k];
+    const int cell = P.free_cell[k];
     for (int v = 0; v < NV; ++v) {
       size_t o = base + (size_t)v * nc + cell;
-      double x = r[o];
-      z[o] = accumulate ? z[o] + x : x;
+      z[o] = accumulate ? z[o] + r[o] : r[o];
     }
     return;
   }
+  const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
+  const int n = s1 - s0;
   const R* F = fac + (size_t)cs * P.n_slots * 48;
+  const R* Fc = F + (size_t)P.n_slots * 32;
   const R* Fs = NV > 4 ? fac_sa + (size_t)cs * P.n_slots * 3 : nullptr;
-  R* S = scr + (size_t)cs * P.n_slots * NV;
-  const int s0 = P.l_start[t], s1 = P.l_start[t + 1];
-  // forward: d_k = P_k^-1 (r_k - L_k d_{k-1})
+  const int nch = (n + CH - 1) / CH;
+  constexpr int PF = (CH * 34 + 63) / 64, PR = (CH * NV + 63) / 64;
+  R pf[PF];
+  R pr[PR];
+  // ---- forward sweep, chunks in order
+  auto fetch_fwd = [&](int c) {
+    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      int idx = t * 64 + lane;
+      R v = R(0);
+      if (idx < m * 32) v = F[(size_t)sc * 32 + idx];
+      else if (NV > 4 && idx >= CH * 32 && idx - CH * 32 < m * 2) {
+        int j = idx - CH * 32;
+        v = Fs[(size_t)(sc + (j >> 1)) * 3 + (j & 1)];
+      }
+      pf[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < PR; ++t) {
+      int idx = t * 64 + lane;
+      R v = R(0);
+      if (idx < m * NV) {
+        int k = idx / NV, var = idx - k * NV;
+        v = (R)r[base + (size_t)var * nc + P.l_cell[sc + k]];
+      }
+      pr[t] = v;
+    }
+  };
   R dp[4] = {R(0), R(0), R(0), R(0)};
   R dps = R(0);
-  for (int s = s0; s < s1; ++s) {
-    const int cell = P.l_cell[s];
-    const R* o = F + (size_t)s * 48;
-    R tmp[4];
-    for (int i = 0; i < 4; ++i) {
-      R acc = (R)r[base + (size_t)i * nc + cell];
-      for (int k = 0; k < 4; ++k) acc -= o[i * 4 + k] * dp[k];
-      tmp[i] = acc;
+  if (nch > 0) fetch_fwd(0);
+  for (int c = 0; c < nch; ++c) {
+    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      int idx = t * 64 + lane;
+      if (idx < CH * 34) fb[idx] = pf[t];
     }
-    for (int i = 0; i < 4; ++i) {
-      R acc = R(0);
-      for (int k = 0; k < 4; ++k) acc += o[16 + i * 4 + k] * tmp[k];
-      dp[i] = acc;
+#pragma unroll
+    for (int t = 0; t < PR; ++t) {
+      int idx = t * 64 + lane;
+      if (idx < CH * NV) rb[idx] = pr[t];
     }
-    R* sv = S + (size_t)s * NV;
-    for (int v = 0; v < 4; ++v) sv[v] = dp[v];
-    if constexpr (NV > 4) {
-      const R* os = Fs + (size_t)s * 3;
-      dps = os[1] * ((R)r[base + (size_t)4 * nc + cell] - os[0] * dps);
-      sv[4] = dps;
+    __syncthreads();
+    if (c + 1 < nch) fetch_fwd(c + 1);
+    if (lane == 0) {
+      for (int k = 0; k < m; ++k) {
+        const R* o = fb + k * 32;
+        R tmp[4];
+        for (int i = 0; i < 4; ++i) {
+          R acc = rb[k * NV + i];
+          for (int q = 0; q < 4; ++q) acc -= o[i * 4 + q] * dp[q];
+          tmp[i] = acc;
+        }
+        for (int i = 0; i < 4; ++i) {
+          R acc = R(0);
+          for (int q = 0; q < 4; ++q) acc += o[16 + i * 4 + q] * tmp[q];
+          dp[i] = acc;
+        }
+        R* dv = dbuf + (size_t)(c * CH + k) * NV;
+        for (int i = 0; i < 4; ++i) dv[i] = dp[i];
+        if constexpr (NV > 4) {
+          const R* os = fb + CH * 32 + k * 2;
+          dps = os[1] * (rb[k * NV + 4] - os[0] * dps);
+          dv[4] = dps;
+        }
+      }
     }
+    __syncthreads();
   }
-  // backward: x_k = d_k - C_k x_{k+1}
+  // ---- backward sweep, chunks in reverse
+  constexpr int PB = (CH * 17 + 63) / 64;
+  R pb[PB];
+  auto fetch_bwd = [&](int c) {   // chunk c counted from the end
+    const int hi = s1 - c * CH, m = min(CH, hi - s0), sc = hi - m;
+#pragma unroll
+    for (int t = 0; t < PB; ++t) {
+      int idx = t * 64 + lane;
+      R v = R(0);
+      if (idx < m * 16) v = Fc[(size_t)sc * 16 + idx];
+      else if (NV > 4 && idx >= CH * 16 && idx - CH * 16 < m) v = Fs[(size_t)(sc + idx - CH * 16) * 3 + 2];
+      pb[t] = v;
+    }
+  };
   R xn[4] = {R(0), R(0), R(0), R(0)};
   R xns = R(0);
-  for (int s = s1 - 1; s >= s0; --s) {
-    const int cell = P.l_cell[s];
-    const R* o = F + (size_t)s * 48;
-    const R* sv = S + (size_t)s * NV;
-    R x[4];
-    for (int i = 0; i < 4; ++i) {
-      R acc = sv[i];
-      for (int k = 0; k < 4; ++k) acc -= o[32 + i * 4 + k] * xn[k];
-      x[i] = acc;
+  if (nch > 0) fetch_bwd(0);
+  for (int c = 0; c < nch; ++c) {
+    const int hi = s1 - c * CH, m = min(CH, hi - s0), sc = hi - m;
+#pragma unroll
+    for (int t = 0; t < PB; ++t) {
+      int idx = t * 64 + lane;
+      if (idx < CH * 17) fb[idx] = pb[t];
     }
-    for (int i = 0; i < 4; ++i) {
-      xn[i] = x[i];
-      size_t oo = base + (size_t)i * nc + cell;
-      z[oo] = accumulate ? z[oo] + (double)x[i] : (double)x[i];
+    if (lane < m) cb[lane] = P.l_cell[sc + lane];
+    __syncthreads();
+    if (c + 1 < nch) fetch_bwd(c + 1);
+    if (lane == 0) {
+      for (int k = m - 1; k >= 0; --k) {
+        const R* o = fb + k * 16;
+        const R* dv = dbuf + (size_t)(sc - s0 + k) * NV;
+        R x[4];
+        for (int i = 0; i < 4; ++i) {
+          R acc = dv[i];
+          for (int q = 0; q < 4; ++q) acc -= o[i * 4 + q] * xn[q];
+          x[i] = acc;
+        }
+        for (int i = 0; i < 4; ++i) {
+          xn[i] = x[i];
+          rb[k * NV + i] = x[i];
+        }
+        if constexpr (NV > 4) {
+          xns = dv[4] - fb[CH * 16 + k] * xns;
+          rb[k * NV + 4] = xns;
+        }
+      }
     }
-    if constexpr (NV > 4) {
-      const R* os = Fs + (size_t)s * 3;
-      xns = sv[4] - os[2] * xns;
-      size_t oo = base + (size_t)4 * nc + cell;
-      z[oo] = accumulate ? z[oo] + (double)xns : (double)xns;
+    __syncthreads();
+    for (int idx = lane; idx < m * NV; idx += 64) {
+      int k = idx / NV, var = idx - k * NV;
+      size_t o = base + (size_t)var * nc + cb[k];
+      double x = (double)rb[idx];
+      z[o] = accumulate ? z[o] + x : x;
     }
+    __syncthreads();
   }
 }
 

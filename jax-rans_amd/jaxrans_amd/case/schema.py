@@ -132,6 +132,7 @@ NUMERICAL: dict[str, tuple[Key, ...]] = {
         Key("krylov_success", "number", _I.krylov_success),
         Key("max_relative_update", "number|null", _I.max_relative_update),
         Key("precision", "string", _I.precision, choices=("fp64", "mixed")),
+        Key("line_solver", "string", _I.line_solver, choices=("cyclic_reduction", "thomas")),
     ),
     "convergence": (
         Key("max_iters", "integer", _N.max_iters),

@@ -134,7 +134,9 @@ class ImplicitConfig:
     ``precision`` is this implementation's addition: ``"fp64"`` (the drop-in,
     every array double) or ``"mixed"`` (line-preconditioner blocks and their
     block-Thomas factor/solve in float32; residual, tangent, Krylov and update
-    in float64)."""
+    in float64).  ``line_solver`` is how each line chain is solved: block cyclic
+    reduction (the reference's, ``src/jaxrans/core/linalg.py:159-200``) or block
+    Thomas (``linalg.py:80-132``); the two agree to round-off."""
 
     cfl: float = 10.0
     cfl_max: float = 1.0e3
@@ -149,6 +151,7 @@ class ImplicitConfig:
     krylov_success: float = 0.5
     max_relative_update: float | None = None
     precision: str = "fp64"
+    line_solver: str = "cyclic_reduction"
 
     def __post_init__(self) -> None:
         if not self.cfl > 0.0:
@@ -175,3 +178,5 @@ class ImplicitConfig:
             raise ValueError(f"krylov_success must lie in (0, 1), got {self.krylov_success}")
         if self.precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {self.precision!r}")
+        if self.line_solver not in ("cyclic_reduction", "thomas"):
+            raise ValueError(f"line_solver must be 'cyclic_reduction' or 'thomas', got {self.line_solver!r}")

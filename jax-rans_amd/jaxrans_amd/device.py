@@ -59,7 +59,8 @@ class Numerics(C.Structure):
     _fields_ = [
         ("gamma", C.c_double), ("kappa", C.c_double), ("flux", C.c_int32), ("limiter", C.c_int32),
         ("viscous", C.c_int32), ("sa", C.c_int32), ("sutherland", C.c_double), ("c_cr1", C.c_double),
-        ("jacobian_exact", C.c_int32), ("mixed", C.c_int32), ("krylov_subspace", C.c_int32),
+        ("jacobian_exact", C.c_int32), ("mixed", C.c_int32), ("line_solver", C.c_int32),
+        ("krylov_subspace", C.c_int32),
         ("krylov_restarts", C.c_int32), ("krylov_tol", C.c_double), ("krylov_success", C.c_double),
         ("cfl_growth", C.c_double), ("cfl_cutback", C.c_double), ("cfl_min", C.c_double), ("cfl_max", C.c_double),
         ("max_relative_update", C.c_double),
@@ -166,6 +167,7 @@ class Context:
         n.c_cr1 = c_cr1
         n.jacobian_exact = int(cfg.jacobian == "exact")
         n.mixed = int(cfg.precision == "mixed")
+        n.line_solver = {"cyclic_reduction": 0, "thomas": 1}[cfg.line_solver]
         n.krylov_subspace, n.krylov_restarts = cfg.krylov_subspace, cfg.krylov_restarts
         n.krylov_tol, n.krylov_success = cfg.krylov_tol, cfg.krylov_success
         n.cfl_growth, n.cfl_cutback, n.cfl_min, n.cfl_max = cfg.cfl_growth, cfg.cfl_cutback, cfg.cfl_min, cfg.cfl_max

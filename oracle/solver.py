@@ -343,9 +343,12 @@ class NewtonState:
         self.U, self.cfl, self.residual_norm = U, cfl, residual_norm
 
 
-def newton_step(state: NewtonState, setup: Setup, cfg, method: str = "thomas", exact: bool = False):
+def newton_step(state: NewtonState, setup: Setup, cfg, method: str | None = None, exact: bool = False):
     """One pseudo-transient Newton-Krylov step (implicit.py:99-166); returns
-    ``(new state, |R| before the update, info)``."""
+    ``(new state, |R| before the update, info)``.  ``method`` (``"cr"`` or
+    ``"thomas"``) defaults to the line solver ``cfg`` names."""
+    if method is None:
+        method = {"cyclic_reduction": "cr", "thomas": "thomas"}[getattr(cfg, "line_solver", "thomas")]
     U, cfl = state.U, state.cfl
     R = residual(U, setup)
     nrm = norm(R, setup)
@@ -427,7 +430,7 @@ def forces(U, setup: Setup, length=1.0, centre=(0.25, 0.0)):
 # --- driver ----------------------------------------------------------------------------
 
 
-def run_oracle(setup: Setup, cfg, *, tol: float, max_iters: int, block_iters: int = 20, U=None, method="thomas",
+def run_oracle(setup: Setup, cfg, *, tol: float, max_iters: int, block_iters: int = 20, U=None, method=None,
                length=1.0, centre=(0.25, 0.0)):
     """Newton steps in blocks until ``last <= tol * peak`` at a block end
     (driver.py:123-201 without the settled test); returns the final state,

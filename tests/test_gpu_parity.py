@@ -2,8 +2,9 @@
 
 Tolerances: residual and tangent 1e-12 normwise relative (the FP64 path
 restates the oracle's arithmetic; ulp-level differences only), wave speeds
-1e-13, the FP64 line solve 1e-10 (Thomas on the device, Thomas in the oracle,
-different summation order), the FP32 line solve 2e-5.
+1e-13, the FP64 line solve 1e-10 (the same elimination on both sides --
+block Thomas or block cyclic reduction -- in a different summation order),
+the FP32 line solve 2e-5.
 """
 
 from __future__ import annotations
@@ -93,11 +94,12 @@ def test_wave_speeds_match_oracle(name, kind, flowf, flux, lim):
     assert rel(lsum[0], OR.wave_speed_sum(lams)) < 1e-13
 
 
+@pytest.mark.parametrize("solver, method", [("cyclic_reduction", "cr"), ("thomas", "thomas")])
 @pytest.mark.parametrize("precision, tol", [("fp64", 1e-10), ("mixed", 2e-5)])
 @pytest.mark.parametrize("name, kind, flowf, flux, lim", [CASES[0], CASES[5]], ids=[CASES[0][0], CASES[5][0]])
-def test_line_preconditioner_matches_oracle(name, kind, flowf, flux, lim, precision, tol):
-    print(f"[gpu] precond {name} {precision}", flush=True)
-    cfg = ImplicitConfig(cfl=25.0, cfl_max=1e3, precision=precision)
+def test_line_preconditioner_matches_oracle(name, kind, flowf, flux, lim, precision, tol, solver, method):
+    print(f"[gpu] precond {name} {precision} {solver}", flush=True)
+    cfg = ImplicitConfig(cfl=25.0, cfl_max=1e3, precision=precision, line_solver=solver)
     atlas, flow, num, st, ctx = _setup(kind, flowf, flux, lim, implicit=cfg)
     U = perturbed(OR.U0_of(st), 1e-2)
     ctx.set_cfl(25.0)
@@ -105,7 +107,7 @@ def test_line_preconditioner_matches_oracle(name, kind, flowf, flux, lim, precis
     rng = np.random.default_rng(3)
     r = rng.standard_normal(U.shape)
     lams = OR.wave_speeds(U, st)
-    ref = OS.LinePreconditioner(U, lams, 25.0, st, "thomas")(r)
+    ref = OS.LinePreconditioner(U, lams, 25.0, st, method)(r)
     got = ctx.precond_apply(r)[0]
     assert rel(got, ref) < tol, rel(got, ref)
 
