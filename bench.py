@@ -47,13 +47,47 @@ def algorithmic_bytes(name: str, ctx, m: int) -> float:
         # Q, dQ read once per cell; two face vectors; tangent face fluxes written (2 faces x nv)
         return ctx.n_cases * nc * (2 * nv * 8 + 4 * 8 + 2 * nv * 8)
     if name == "cgs2_update_dots":
-        j = m // 2
-        return ctx.n_cases * nc * nv * 8 * (j + 1 + 2)
+        # column j reads V_0..V_j and w, writes w: (j + 3) vectors, averaged over j = 0..m-1
+        return ctx.n_cases * nc * nv * 8 * ((m - 1) / 2 + 3)
     if name == "residual_faces":
         return ctx.n_cases * nc * (nv * 8 + 4 * 8 + 2 * nv * 8)
     if name == "tangent_cells":
         return ctx.n_cases * nc * (2 * nv * 8 + 2 * nv * 8 + 8 + nv * 8)
     raise KeyError(name)
+
+
+def path_bytes_per_cell(ctx) -> dict:
+    """Compulsory bytes per interior cell of each hot path (every array
+    touched once): residual reads U and the face vectors (d^2 values), writes
+    R, plus cell centres and volume (d + 1) when viscous and the wall distance
+    for SA; the tangent adds dU and V/dt; the line apply reads the 3 m^2
+    block entries (+3 SA) of the factor and r, writes z; the factor reads U,
+    the wave speeds and geometry and writes the factor."""
+    nv, d = ctx.nvar, 2
+    visc = ctx.viscous
+    sa = nv > 4
+    fb = 4 if ctx.config.precision == "mixed" else 8
+    blocks = 3 * 16 + (3 if sa else 0)
+    residual = 8 * (2 * nv + d * d + (d + 1 if visc else 0) + (1 if sa else 0))
+    return {"residual": residual, "tangent": residual + 8 * (nv + 1),
+            "precond_apply": blocks * fb + 16 * nv,
+            "precond_factor": 8 * (nv + d + d * d + (2 if sa else 0)) + blocks * fb}
+
+
+def path_rates(ctx) -> dict:
+    """Device time of one residual, tangent, line apply and line factor (all of
+    their launches, HIP events on the library's stream) and the achieved rate
+    of their compulsory bytes."""
+    ids = {"residual": "residual", "tangent": "tangent", "precond_apply": "precond_apply",
+           "precond_factor": "precond_factor"}
+    per = path_bytes_per_cell(ctx)
+    out = {}
+    for name, kid in ids.items():
+        ms = ctx.profile(kid, reps=20)
+        b = per[name] * ctx.atlas.cells * ctx.n_cases
+        out[name] = {"ms": round(ms, 5), "bytes_per_cell": per[name], "GB/s": round(b / (ms * 1e-3) / 1e9, 1),
+                     "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def cpu_baseline() -> dict:
@@ -130,7 +164,7 @@ def main(argv=None) -> int:
     # dominant kernel: the largest device time per Newton step among the hot kernels
     calls = {"precond_apply": 2 * (m + 1), "tangent_faces": 2 * m, "cgs2_update_dots": 2 * m,
              "residual_faces": 1, "tangent_cells": 2 * m}
-    times = {k: ctx.profile(k, reps=20) for k in calls}
+    times = {k: ctx.profile(k, reps=2 * m if k == "cgs2_update_dots" else 20) for k in calls}
     share = {k: times[k] * calls[k] for k in calls}
     dom = max(share, key=share.get)
     alg = algorithmic_bytes(dom, ctx, m)
@@ -139,6 +173,7 @@ def main(argv=None) -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "kernel_ms": round(times[dom], 5), "algorithmic_bytes": int(alg),
                 "kernel_ms_all": {k: round(v, 5) for k, v in times.items()}}
+    paths = path_rates(ctx)
     scal = ctx.scalars()[0]
     ctx.close()
     extra = {}
@@ -155,6 +190,7 @@ def main(argv=None) -> int:
                                "alpha 2.5, Re 6.5e6, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64",
                    "cells": cells, "krylov_subspace": m, "parallelism": "single GPU"},
         "roofline": roofline,
+        "paths": paths,
         "residual_first": float(hist[0, 0]), "residual_last": float(hist[-1, 0]), "cfl_last": float(scal[0]),
         **extra,
     }

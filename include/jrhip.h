@@ -133,8 +133,9 @@ int jr_forces(jr_ctx* ctx, double length, double centre_x, double centre_y, doub
 /* Average device time of one launch of a hot kernel on the device state,
  * timed with HIP events on the context stream: 0 residual (all launches),
  * 1 tangent (all launches), 2 line-preconditioner apply, 3 residual face
- * kernel, 4 tangent face kernel, 5 CGS2 update+dots at j = m/2, 6 line
- * factor, 7 tangent cell kernel.  No reference counterpart: measurement. */
+ * kernel, 4 tangent face kernel, 5 CGS2 update+dots (launch r at Arnoldi
+ * column j = r mod m: with reps a multiple of m, the average over a cycle),
+ * 6 line factor, 7 tangent cell kernel.  No reference counterpart: measurement. */
 int jr_profile(jr_ctx* ctx, int kernel, int reps, float* ms_per_launch);
 
 /* Device timing of the context's stream (HIP events): start, then stop

@@ -50,6 +50,7 @@ struct jr_ctx {
   Ctl* ctl = nullptr;
   double *hv = nullptr, *h2v = nullptr, *G = nullptr, *H = nullptr, *y = nullptr, *hy = nullptr, *hist = nullptr, *coef = nullptr;
   int hist_cap = 0;
+  int prof_rep = 0;  // launch counter of jr_profile
   Red red{};
   bool has_q = false;
 };
@@ -91,12 +92,8 @@ static dim3 grid1(size_t n, int per, int cases) { return dim3((unsigned)((n + pe
 template <int NV, bool VISC>
 static void launch_residual(jr_ctx* c, const double* U, double* out, int first_order) {
   const Plan& P = c->P;
-  hipLaunchKernelGGL((k_prim<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P.nc,
-                     c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
-  for (int s = 0; s < 2; ++s)
-    if (P.n_task[s] > 0)
-      hipLaunchKernelGGL((k_ghost<double, NV>), grid1(P.n_task[s], BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
-                         s, c->ph.gamma, c->ctl, c->Q, nullptr, 0);
+  hipLaunchKernelGGL((k_fill<double, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
+                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
   hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
                      c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Fi, c->Fj, 0);
   hipLaunchKernelGGL((k_cells<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
@@ -108,12 +105,8 @@ template <int NV, bool VISC>
 static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double* out, int first_order,
                            const double* vdt, int check_active) {
   const Plan& P = c->P;
-  hipLaunchKernelGGL((k_prim<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P.nc,
-                     c->ph.gamma, c->ctl, U, dU, nullptr, c->dQ, check_active);
-  for (int s = 0; s < 2; ++s)
-    if (P.n_task[s] > 0)
-      hipLaunchKernelGGL((k_ghost<Dual, NV>), grid1(P.n_task[s], BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, s,
-                         c->ph.gamma, c->ctl, c->Q, c->dQ, check_active);
+  hipLaunchKernelGGL((k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
+                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, dU, nullptr, c->dQ, check_active);
   hipLaunchKernelGGL((k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream,
                      P, c->ph, first_order, c->ctl, c->Q, c->dQ, c->Fi, c->Fj, check_active);
   hipLaunchKernelGGL((k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
@@ -158,6 +151,39 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
   }
 }
 
+// --- CGS2 of Arnoldi column j, instantiated for j / KC + 1 basis chunks --------
+
+template <int NV, int NCH>
+static void cgs2_passes(jr_ctx* c, int j, int passes) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  const size_t N = c->N, vstride = (size_t)c->n_cases * N;
+  if (passes & 1)
+    hipLaunchKernelGGL((k_dots<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W, j,
+                       c->red, c->hv);
+  if (passes & 2)
+    hipLaunchKernelGGL((k_update_dots<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
+                       j, c->hv, c->red, c->h2v);
+  if (passes & 4)
+    hipLaunchKernelGGL((k_update_norm<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
+                       j, c->hv, c->h2v, c->red, c->H, c->G, c->m);
+}
+
+template <int NV>
+static void cgs2_column(jr_ctx* c, int j, int passes = 7) {
+  switch (j / KC) {
+    case 0: cgs2_passes<NV, 1>(c, j, passes); break;
+    case 1: cgs2_passes<NV, 2>(c, j, passes); break;
+    case 2: cgs2_passes<NV, 3>(c, j, passes); break;
+    case 3: cgs2_passes<NV, 4>(c, j, passes); break;
+    case 4: cgs2_passes<NV, 5>(c, j, passes); break;
+    case 5: cgs2_passes<NV, 6>(c, j, passes); break;
+    case 6: cgs2_passes<NV, 7>(c, j, passes); break;
+    default: cgs2_passes<NV, 8>(c, j, passes); break;
+  }
+}
+static_assert(MMAX <= 8 * KC, "cgs2_column covers j < 64");
+
 // --- one Newton step ----------------------------------------------------------
 // Krylov basis layout: V[k][case][N], so V_k of every case is one contiguous
 // field batch (case stride N) that the apply and tangent kernels take as is.
@@ -175,12 +201,7 @@ static void gmres_and_update(jr_ctx* c) {
       const double* Vj = c->V + (size_t)j * vstride;
       launch_apply<R, NV>(c, Vj, c->Z, 0, 1);
       launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1);
-      hipLaunchKernelGGL((k_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W, j,
-                         c->red, c->hv);
-      hipLaunchKernelGGL((k_update_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                         j, c->hv, c->red, c->h2v);
-      hipLaunchKernelGGL((k_update_norm<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                         j, c->hv, c->h2v, c->red, c->H, c->G, c->m);
+      cgs2_column<NV>(c, j);
       hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
                          c->V + (size_t)(j + 1) * vstride, N);
     }
@@ -364,6 +385,37 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
         upload(c, &P.t_src[s], g->task_src[s], g->n_task[s]) ||
         upload(c, &P.t_kind[s], g->task_kind[s], g->n_task[s]) ||
         upload(c, &P.t_norm[s], g->task_normal[s], 2 * (size_t)g->n_task[s]))
+      return bail(0);
+  }
+  {
+    // Maps of the one-launch fill: which cells a stage writes, which stage-1
+    // destinations stage 2 overwrites, and the stage-1 task behind a stage-2
+    // source.  A stage writing one cell twice would be a race: refused.
+    std::vector<uint8_t> ghost(nc, 0), skip0((size_t)std::max(1, g->n_task[0]), 0);
+    std::vector<int> dest0(nc, -1), prev1((size_t)std::max(1, g->n_task[1]), -1);
+    std::vector<uint8_t> seen1(nc, 0);
+    for (int t = 0; t < g->n_task[0]; ++t) {
+      const int64_t d = g->task_dest[0][t];
+      if (dest0[d] >= 0) {
+        fail("jr_ctx_create: a ghost-fill stage writes one cell twice");
+        return bail(0);
+      }
+      dest0[d] = t;
+      ghost[d] = 1;
+    }
+    for (int t = 0; t < g->n_task[1]; ++t) {
+      const int64_t d = g->task_dest[1][t];
+      if (seen1[d]) {
+        fail("jr_ctx_create: a ghost-fill stage writes one cell twice");
+        return bail(0);
+      }
+      seen1[d] = 1;
+      ghost[d] = 1;
+      if (dest0[d] >= 0) skip0[dest0[d]] = 1;
+      prev1[t] = dest0[g->task_src[1][t]];
+    }
+    if (upload(c, &P.ghost, ghost.data(), nc) || upload(c, &P.t_skip0, skip0.data(), skip0.size()) ||
+        upload(c, &P.t_prev1, prev1.data(), prev1.size()))
       return bail(0);
   }
   P.n_lines = g->n_lines;
@@ -743,9 +795,8 @@ static void profile_launch(jr_ctx* c, int which) {
                          c->stream, P, c->ph, c->exact ? 0 : 1, c->ctl, c->Q, c->dQ, c->Fi, c->Fj, 0);
       break;
     case 5: {
-      int j = c->m / 2;
-      hipLaunchKernelGGL((k_update_dots<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                         j, c->hv, c->red, c->h2v);
+      const int j = c->prof_rep++ % c->m;   // cycles through the Arnoldi columns
+      cgs2_column<NV>(c, j, 2);
       break;
     }
     case 6: launch_factor<R, NV>(c, c->U); break;
@@ -797,6 +848,7 @@ extern "C" int jr_profile(jr_ctx* c, int which, int reps, float* ms) {
     TRY(put_ctl(c, h));
   }
   profile_dispatch(c, which);   // warm
+  c->prof_rep = 0;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (int r = 0; r < reps; ++r) profile_dispatch(c, which);
   HIPCHK(hipEventRecord(c->ev1, c->stream));

@@ -85,26 +85,33 @@ __global__ void __launch_bounds__(BLOCK) k_cycle_begin(int nc, Ctl* __restrict__
 }
 
 // --- CGS2 over the basis V_0..V_j ---------------------------------------------
-// Each element's basis values are loaded KC at a time, every load of a chunk
-// issued before the first is used (indices past the last vector are clamped
-// to it; those duplicates hit the cache and are masked out of the sums), so a
-// pass over j + 1 vectors costs ceil((j + 1) / KC) memory round trips per
-// element rather than j + 1.
+// The kernels of Arnoldi column j are instantiated for NCH = j / KC + 1
+// chunks of KC basis vectors: every basis value an element needs (NCH KC of
+// them, indices past j clamped to j, masked out of every sum) is loaded in
+// one batch before the first is used, so an element costs one memory round
+// trip whatever j is.  Chunks are processed in groups of at most GRP so the
+// batch fits in registers when m > GRP KC.
 constexpr int KC = 8;
+constexpr int GRP = 4;
 
-__device__ __forceinline__ void basis_chunk(const double* __restrict__ Vc, size_t vstride, size_t e, int k0, int kmax,
-                                            double (&v)[KC]) {
+template <int NCH>
+struct Cgs {
+  static constexpr int NK = NCH * KC;                                  // basis values per element
+  static constexpr int NP = NK <= 8 ? 8 : NK <= 16 ? 16 : NK <= 32 ? 32 : 64;   // padded for the reduction
+  static constexpr int NG = (NCH + GRP - 1) / GRP;                     // load batches per element
+  static constexpr int GS = NCH <= GRP ? NK : GRP * KC;                // values per batch
+  static constexpr int NKG = NG * GS;                                  // NK padded to whole batches (<= NP)
+};
+
+// v[k] = V_min(k0 + k, j)[e] for k < NB, issued together.
+template <int NB>
+__device__ __forceinline__ void basis_batch(const double* __restrict__ Vc, size_t vstride, size_t e, int k0, int j,
+                                            double (&v)[NB]) {
 #pragma unroll
-  for (int i = 0; i < KC; ++i) v[i] = Vc[(size_t)min(k0 + i, kmax) * vstride + e];
-  __builtin_amdgcn_sched_barrier(0);   // keep the KC loads ahead of their first use
+  for (int k = 0; k < NB; ++k) v[k] = Vc[(size_t)min(k0 + k, j) * vstride + e];
+  __builtin_amdgcn_sched_barrier(0);   // keep the batch ahead of its first use
 }
 
-// Sum of MM values (MM a power of two, at most 64) over the 64 lanes of a
-// wave by recursive halving: at the step with lane offset o each lane keeps
-// half of its values (the upper half when bit o of the lane is set) and adds
-// its partner's copy of that half.  Returns in every lane the wave total of
-// value lane / (64 / MM): MM - 1 + log2(64 / MM) exchanges instead of
-// 6 MM for MM separate butterflies.
 template <int H, int O, int MM>
 __device__ __forceinline__ void halve(double (&v)[MM], int lane) {
   if constexpr (H >= 1) {
@@ -181,77 +188,68 @@ __device__ bool finish_dots(double (&v)[MM], double* partials, unsigned* counter
   return true;
 }
 
-// CGS2 pass 1: h_k = V_k . w, k <= j (j < MM).
-template <int NV, int MM>
+// CGS2 pass 1: h_k = V_k . w, k <= j.
+template <int NV, int NCH>
 __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ V,
                                                 size_t vstride, size_t vcase, const double* __restrict__ W, int j,
                                                 Red red, double* __restrict__ hv) {
+  constexpr int NP = Cgs<NCH>::NP;
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double acc[MM];
+  double acc[NP];
 #pragma unroll
-  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NP; ++k) acc[k] = 0.0;
   JR_VEC_LOOP(N) {
     const double w = W[base + e];
 #pragma unroll
-    for (int c = 0; c < MM / KC; ++c)
-      if (c * KC <= j) {
-        double v[KC];
-        basis_chunk(Vc, vstride, e, c * KC, j, v);
+    for (int g = 0; g < Cgs<NCH>::NG; ++g) {
+      constexpr int GS = Cgs<NCH>::GS;
+      double v[GS];
+      basis_batch<GS>(Vc, vstride, e, g * GS, j, v);
 #pragma unroll
-        for (int i = 0; i < KC; ++i) acc[c * KC + i] += v[i] * w;
-      }
+      for (int k = 0; k < GS; ++k) acc[g * GS + k] += v[k] * w;
+    }
   }
-  __shared__ double tot[MM];
-  if (finish_dots<MM>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
+  __shared__ double tot[NP];
+  if (finish_dots<NP>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
     if (threadIdx.x <= j) hv[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
   }
 }
 
 // CGS2 pass 2: w <- w - sum_{k<=j} h_k V_k, then h2_k = V_k . w.
-template <int NV, int MM>
+template <int NV, int NCH>
 __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
                                                        double* __restrict__ W, int j, const double* __restrict__ hv,
                                                        Red red, double* __restrict__ h2v) {
+  constexpr int NK = Cgs<NCH>::NK, NP = Cgs<NCH>::NP;
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double h[MM];
+  double h[NK];
 #pragma unroll
-  for (int k = 0; k < MM; ++k) h[k] = k <= j ? hv[(size_t)cs * (MMAX + 1) + k] : 0.0;
-  double acc[MM];
+  for (int k = 0; k < NK; ++k) h[k] = k <= j ? hv[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  double acc[NP];
 #pragma unroll
-  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NP; ++k) acc[k] = 0.0;
   JR_VEC_LOOP(N) {
-    double vk[MM];
+    double vk[NK];
     double w = W[base + e];
+    basis_batch<NK>(Vc, vstride, e, 0, j, vk);
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < MM / KC; ++c)
-      if (c * KC <= j) {
-        double v[KC];
-        basis_chunk(Vc, vstride, e, c * KC, j, v);
-#pragma unroll
-        for (int i = 0; i < KC; ++i) {
-          vk[c * KC + i] = v[i];
-          if (c * KC + i <= j) s += h[c * KC + i] * v[i];
-        }
-      }
+    for (int k = 0; k < NK; ++k)
+      if (k <= j) s += h[k] * vk[k];
     w = w - s;
     W[base + e] = w;
 #pragma unroll
-    for (int c = 0; c < MM / KC; ++c)
-      if (c * KC <= j) {
-#pragma unroll
-        for (int i = 0; i < KC; ++i) acc[c * KC + i] += vk[c * KC + i] * w;
-      }
+    for (int k = 0; k < NK; ++k) acc[k] += vk[k] * w;
   }
-  __shared__ double tot[MM];
-  if (finish_dots<MM>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
+  __shared__ double tot[NP];
+  if (finish_dots<NP>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
     if (threadIdx.x <= j) h2v[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
   }
 }
@@ -260,34 +258,33 @@ __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __rest
 // The finishing workgroup also advances the Givens QR of the Hessenberg by
 // column j (krylov.py:121-131, the same operations in the same order as the
 // whole-matrix sweep).
-
-template <int NV, int MM>
+template <int NV, int NCH>
 __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, Ctl* __restrict__ ctl, const double* __restrict__ V,
                                                        size_t vstride, size_t vcase, double* __restrict__ W, int j,
                                                        const double* __restrict__ hv, const double* __restrict__ h2v,
                                                        Red red, double* __restrict__ H, double* __restrict__ G,
                                                        int m) {
+  constexpr int NKG = Cgs<NCH>::NKG, GS = Cgs<NCH>::GS;
   __shared__ double sh[1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
-  double h2[MM];
+  double h2[NKG];
 #pragma unroll
-  for (int k = 0; k < MM; ++k) h2[k] = k <= j ? h2v[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  for (int k = 0; k < NKG; ++k) h2[k] = k <= j ? h2v[(size_t)cs * (MMAX + 1) + k] : 0.0;
   double acc[1] = {0.0};
   JR_VEC_LOOP(N) {
     double w = W[base + e];
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < MM / KC; ++c)
-      if (c * KC <= j) {
-        double v[KC];
-        basis_chunk(Vc, vstride, e, c * KC, j, v);
+    for (int g = 0; g < Cgs<NCH>::NG; ++g) {
+      double v[GS];
+      basis_batch<GS>(Vc, vstride, e, g * GS, j, v);
 #pragma unroll
-        for (int i = 0; i < KC; ++i)
-          if (c * KC + i <= j) s += h2[c * KC + i] * v[i];
-      }
+      for (int k = 0; k < GS; ++k)
+        if (g * GS + k <= j) s += h2[g * GS + k] * v[k];
+    }
     w = w - s;
     W[base + e] = w;
     acc[0] += w * w;
@@ -403,7 +400,7 @@ __global__ void __launch_bounds__(BLOCK) k_combine(int nc, Ctl* __restrict__ ctl
     for (int c = 0; c < NCH; ++c)
       if (c * KC <= m) {
         double v[KC];
-        basis_chunk(Vc, vstride, e, c * KC, m, v);
+        basis_batch<KC>(Vc, vstride, e, c * KC, m, v);
 #pragma unroll
         for (int i = 0; i < KC; ++i) {
           const int k = c * KC + i;

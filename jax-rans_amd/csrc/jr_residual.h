@@ -22,44 +22,74 @@ template <> struct Io<Dual> {
   static __device__ __forceinline__ void st(double*, double* d, size_t i, Dual x) { d[i] = x.d; }
 };
 
-// Q (and dQ) <- primitive of U (and its tangent) on every canvas cell.
+// Primitives of U (and their tangent) with the ghost fill, in one launch
+// (residual.py:312-322, boundary.py:257-267).  Threads [0, nc) convert the
+// cells that no stage overwrites; the next n_task[0] threads are stage-1
+// tasks, the last n_task[1] stage-2 tasks.  Each stage is a gather from the
+// state before it, so a ghost value is computed from its source cell's
+// conservative state directly: prim(U[src]) through the task's condition, and
+// for a stage-2 task whose source is itself a stage-1 ghost, through that
+// stage-1 task first.  The host maps guarantee that every cell is written by
+// exactly one thread (stage-1 tasks that stage 2 overwrites are skipped).
 template <class T, int NV>
-__global__ void __launch_bounds__(BLOCK) k_prim(int nc, double gamma, const Ctl* __restrict__ ctl,
-                                                const double* __restrict__ U, const double* __restrict__ dU,
-                                                double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
-  int c = blockIdx.x * BLOCK + threadIdx.x;
-  int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
-  if (c >= nc) return;
-  size_t base = (size_t)cs * NV * nc + c;
-  T u[NV], q[NV];
-  for (int k = 0; k < NV; ++k) u[k] = Io<T>::ld(U, dU, base + (size_t)k * nc);
+__device__ __forceinline__ void prim_of(const double* __restrict__ U, const double* __restrict__ dU, size_t base,
+                                        int nc, int c, double gamma, T* q) {
+  T u[NV];
+  for (int k = 0; k < NV; ++k) u[k] = Io<T>::ld(U, dU, base + (size_t)k * nc + c);
   primitive<T, NV>(u, gamma, q);
-  for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc, q[k]);
 }
 
-// One ghost-fill stage: dest <- bc(src) or a copy of the partner cell.
 template <class T, int NV>
-__global__ void __launch_bounds__(BLOCK) k_ghost(Plan P, int stage, double gamma, const Ctl* __restrict__ ctl,
-                                                 double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
-  int t = blockIdx.x * BLOCK + threadIdx.x;
-  int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
-  if (t >= P.n_task[stage]) return;
-  int dst = P.t_dest[stage][t], src = P.t_src[stage][t], kind = P.t_kind[stage][t];
-  size_t base = (size_t)cs * NV * P.nc;
-  T q[NV], g[NV];
-  for (int k = 0; k < NV; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * P.nc + src);
+__device__ __forceinline__ void ghost_of(const Plan& P, int stage, int t, const T* q, const double* q_inf,
+                                         double gamma, T* g) {
+  const int kind = P.t_kind[stage][t];
   if (kind == K_COPY) {
     for (int k = 0; k < NV; ++k) g[k] = q[k];
   } else {
-    int n = P.n_task[stage];
-    bc_state<T, NV>(kind, q, P.t_norm[stage][t], P.t_norm[stage][n + t], ctl[cs].q_inf, gamma, g);
+    const int n = P.n_task[stage];
+    bc_state<T, NV>(kind, q, P.t_norm[stage][t], P.t_norm[stage][n + t], q_inf, gamma, g);
   }
-  for (int k = 0; k < NV; ++k) {
-    if (sizeof(T) == sizeof(double)) Q[base + (size_t)k * P.nc + dst] = val(g[k]);
-    else dQ[base + (size_t)k * P.nc + dst] = tan_of(g[k]);
+}
+
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl* __restrict__ ctl,
+                                                const double* __restrict__ U, const double* __restrict__ dU,
+                                                double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
+  int t = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  const double* q_inf = ctl[cs].q_inf;
+  T q[NV], g[NV];
+  int dst;
+  if (t < nc) {
+    if (P.ghost[t]) return;
+    prim_of<T, NV>(U, dU, base, nc, t, gamma, q);
+    for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc + t, q[k]);
+    return;
   }
+  t -= nc;
+  if (t < P.n_task[0]) {
+    if (P.t_skip0[t]) return;
+    prim_of<T, NV>(U, dU, base, nc, P.t_src[0][t], gamma, q);
+    ghost_of<T, NV>(P, 0, t, q, q_inf, gamma, g);
+    dst = P.t_dest[0][t];
+  } else {
+    t -= P.n_task[0];
+    if (t >= P.n_task[1]) return;
+    const int p = P.t_prev1[t];
+    if (p >= 0) {
+      T q0[NV];
+      prim_of<T, NV>(U, dU, base, nc, P.t_src[0][p], gamma, q0);
+      ghost_of<T, NV>(P, 0, p, q0, q_inf, gamma, q);
+    } else {
+      prim_of<T, NV>(U, dU, base, nc, P.t_src[1][t], gamma, q);
+    }
+    ghost_of<T, NV>(P, 1, t, q, q_inf, gamma, g);
+    dst = P.t_dest[1][t];
+  }
+  for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc + dst, g[k]);
 }
 
 template <class T, int NV>

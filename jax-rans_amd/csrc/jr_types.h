@@ -26,6 +26,9 @@ struct Plan {
   const int* t_src[2];
   const int* t_kind[2];
   const double* t_norm[2];         // [2][n]
+  const uint8_t* ghost;            // [nc] 1: written by a ghost-fill stage
+  const uint8_t* t_skip0;          // [n_task[0]] 1: the stage-2 fill overwrites this stage-1 destination
+  const int* t_prev1;              // [n_task[1]] stage-1 task whose destination is this task's source, or -1
   int n_lines, n_slots, max_len;   // line chains
   const int* l_start;              // [n_lines+1]
   const int* l_cell;               // [n_slots]

@@ -117,6 +117,7 @@ class Context:
             raise DeviceUnavailable("no HIP device is visible")
         self.atlas, self.n_cases = atlas, int(n_cases)
         self.nvar = 5 if sa else 4
+        self.viscous = bool(viscous)
         self.shape = (self.n_cases, self.nvar) + tuple(atlas.shape)
         self._keep = []
         g = GridDesc()
