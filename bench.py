@@ -3,7 +3,8 @@ step on one MI355X, on the 2D SA RANS case (Karman-Trefftz 10 deg section,
 C-grid 512 x 128 cells, M 0.73, alpha 2.5 deg, Re 6.5e6, Roe + smooth van
 Albada, GMRES(30) x 2 with the line preconditioner, FP64).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+    python bench.py --converge [CASE_DIR]
 
 Prints one JSON line.  A step is one full Newton step of the case (residual,
 wave speeds, line factor, GMRES, update) on the device; the state starts at
@@ -141,6 +142,27 @@ def run_steps(case, atlas, precision: str, steps: int, warmup: int):
     return ctx, dt, hist
 
 
+def converge(case_dir: Path, grid=None) -> dict:
+    """The case run to convergence through run() in FP64 and in mixed precision:
+    iterations, wall time and the force differences (mixed - FP64)."""
+    from jaxrans_amd import load_case, run
+    out = {}
+    for precision in ("fp64", "mixed"):
+        case = load_case(case_dir, (f"numerical.implicit.precision={json.dumps(precision)}",))
+        grid = case.grid() if grid is None else grid
+        t0 = time.perf_counter()
+        res = run(case, grid)
+        dt = time.perf_counter() - t0
+        out[precision] = {"reason": res.history.reason, "iterations": res.history.n_iters, "seconds": round(dt, 3),
+                          "residual_drop_orders": round(float(res.history.orders_dropped), 3),
+                          "cl": res.forces.cl, "cd": res.forces.cd, "cm": res.forces.cm,
+                          "solves_per_min": round(60.0 / dt, 3)}
+    f, m = out["fp64"], out["mixed"]
+    out["mixed_minus_fp64"] = {"iterations": m["iterations"] - f["iterations"], "cl": m["cl"] - f["cl"],
+                               "cd": m["cd"] - f["cd"], "cd_counts": (m["cd"] - f["cd"]) * 1e4, "cm": m["cm"] - f["cm"]}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,11 +170,19 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
+                    help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
+                         "convergence in FP64 and mixed precision and print iterations and force differences")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
               f"a batch of cases would shard over GPUs with no communication", file=sys.stderr)
         return 2
+    if args.converge:
+        line = {"metric": "mixed vs FP64 at convergence", "case": args.converge, "n_gpus": 1,
+                **converge(Path(args.converge))}
+        print(json.dumps(line), flush=True)
+        return 0
     from jaxrans_amd import load_case
     case = load_case(CASE)
     atlas = case.grid()

@@ -121,6 +121,12 @@ int jr_newton_init(jr_ctx* ctx, const double* cfl0);
  * hist_host[n_steps][n_cases] receives |R(U)| before each update. */
 int jr_newton_block(jr_ctx* ctx, int n_steps, double* hist_host);
 
+/* Execution option: replay a block of Newton steps from a HIP graph captured
+ * on the first block of that length (1), or launch every kernel on the
+ * context stream (0, the default: faster on MI355X with ROCm 7).  Results are
+ * identical.  No reference counterpart: execution. */
+int jr_set_graphs(jr_ctx* ctx, int enable);
+
 /* Per-case scalars [n_cases][8]: cfl, |R|_prev, |R|, linear reduction,
  * omega, accepted(last step), steps, rejected. */
 int jr_get_scalars(jr_ctx* ctx, double* out_host);

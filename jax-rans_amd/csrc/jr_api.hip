@@ -51,6 +51,11 @@ struct jr_ctx {
   double *hv = nullptr, *h2v = nullptr, *G = nullptr, *H = nullptr, *y = nullptr, *hy = nullptr, *hist = nullptr, *coef = nullptr;
   int hist_cap = 0;
   int prof_rep = 0;  // launch counter of jr_profile
+  // A block of Newton steps replayed from a HIP graph (captured on the first
+  // block of a given length; the history buffer is part of the capture).
+  bool use_graph = false;
+  int graph_steps = 0;
+  hipGraphExec_t graph = nullptr;
   Red red{};
   bool has_q = false;
 };
@@ -136,18 +141,19 @@ static void launch_factor(jr_ctx* c, const double* U) {
 }
 
 template <class R, int NV>
-static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, int check_active) {
+static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, int check_active,
+                         double* vout = nullptr) {
   const Plan& P = c->P;
   const size_t lds = (size_t)std::max(1, P.max_len) * NV * sizeof(R);
   if (c->thomas) {
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + 63) / 64);
     hipLaunchKernelGGL((k_apply<R, NV>), dim3(groups, c->n_cases), dim3(64), lds, c->stream, P, c->ctl,
-                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active);
+                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout);
   } else {
     const int T = c->cr_threads;
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
     hipLaunchKernelGGL((k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, c->stream, P, c->ctl,
-                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active);
+                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout);
   }
 }
 
@@ -198,13 +204,15 @@ static void gmres_and_update(jr_ctx* c) {
     hipLaunchKernelGGL((k_cycle_begin<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->Rv, c->V, N, c->restarts,
                        c->ktol, c->G);
     for (int j = 0; j < c->m; ++j) {
-      const double* Vj = c->V + (size_t)j * vstride;
-      launch_apply<R, NV>(c, Vj, c->Z, 0, 1);
+      // z = M^-1 V_j; for j > 0 the apply also forms V_j = w / |w| from the
+      // previous column's w
+      if (j == 0) launch_apply<R, NV>(c, c->V, c->Z, 0, 1);
+      else launch_apply<R, NV>(c, c->W, c->Z, 0, 1, c->V + (size_t)j * vstride);
       launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1);
       cgs2_column<NV>(c, j);
-      hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
-                         c->V + (size_t)(j + 1) * vstride, N);
     }
+    hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
+                       c->V + (size_t)c->m * vstride, N);
     hipLaunchKernelGGL(k_lstsq, dim3(c->n_cases), dim3(64), 0, c->stream, c->n_cases, c->ctl, c->H, c->G, c->m, c->y,
                        c->hy);
     hipLaunchKernelGGL((k_combine<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->y, c->hy,
@@ -491,8 +499,10 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   c->restarts = num->krylov_restarts;
   c->exact = num->jacobian_exact;
   c->mixed = num->mixed;
-  c->thomas = num->line_solver == 1;
-  c->cr_threads = std::min(512, std::max(64, (P.max_len + 63) / 64 * 64));
+  // cyclic reduction takes chains of up to CR_MAX_LEN cells (one thread per
+  // row); longer chains are solved by block Thomas, the same exact solve
+  c->thomas = num->line_solver == 1 || P.max_len > CR_MAX_LEN;
+  c->cr_threads = std::max(64, (P.max_len + 63) / 64 * 64);
   c->ktol = num->krylov_tol;
   c->max_rel = num->max_relative_update;
   c->ramp.cfl_growth = num->cfl_growth;
@@ -516,14 +526,17 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
   {
     void* p;
-    const size_t rows = (size_t)n_cases * 2 * P.n_slots;   // cyclic reduction: <= 2 rows per slot
-    if (dalloc(c, (char**)&p, rows * 48 * rsz)) return bail(0);
+    // per chain slot: factor CR_FAC (+CR_FAC_SA) values (block Thomas needs 48 + 3),
+    // cyclic-reduction workspace 96 (+6 SA)
+    const size_t slots = (size_t)n_cases * P.n_slots;
+    static_assert(CR_FAC >= 48 && CR_FAC_SA >= 3, "factor storage covers both line solvers");
+    if (dalloc(c, (char**)&p, slots * CR_FAC * rsz)) return bail(0);
     c->fac = p;
-    if (dalloc(c, (char**)&p, rows * 5 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, slots * CR_FAC_SA * rsz)) return bail(0);
     c->fac_sa = p;
-    if (dalloc(c, (char**)&p, rows * 48 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, slots * 96 * rsz)) return bail(0);
     c->ws = p;
-    if (dalloc(c, (char**)&p, rows * 3 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, slots * 6 * rsz)) return bail(0);
     c->ws_sa = p;
     if (dalloc(c, &c->blk, (size_t)n_cases * P.n_slots * BLKW)) return bail(0);
   }
@@ -583,11 +596,18 @@ int jr_ctx_destroy(jr_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hist) (void)hipFree(c->hist);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
+}
+
+static void drop_graph(jr_ctx* c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graph_steps = 0;
 }
 
 static int get_ctl(jr_ctx* c, std::vector<Ctl>& h) {
@@ -725,19 +745,52 @@ int jr_set_cfl(jr_ctx* c, const double* cfl) {
   return put_ctl(c, h);
 }
 
+int jr_set_graphs(jr_ctx* c, int enable) {
+  if (!c) return fail("jr_set_graphs: null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  c->use_graph = enable != 0;
+  return 0;
+}
+
 int jr_newton_block(jr_ctx* c, int n_steps, double* hist_host) {
   if (!c) return fail("jr_newton_block: null context");
   if (n_steps < 1) return fail("jr_newton_block: n_steps must be >= 1");
   if (!c->has_q) return fail("jr_newton_block: set the freestream first");
   if (n_steps > c->hist_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);   // the captured launches hold the old history pointer
     if (c->hist) (void)hipFree(c->hist);
     c->hist = nullptr;
     HIPCHK(hipMalloc(&c->hist, (size_t)n_steps * c->n_cases * sizeof(double)));
     c->hist_cap = n_steps;
   }
-  for (int s = 0; s < n_steps; ++s) {
-    step_dispatch(c, c->hist + (size_t)s * c->n_cases);
-    TRY(check_launch());
+  if (c->use_graph) {
+    if (!c->graph || c->graph_steps != n_steps) {
+      drop_graph(c);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < n_steps; ++s) step_dispatch(c, c->hist + (size_t)s * c->n_cases);
+      const hipError_t launched = hipGetLastError();
+      HIPCHK(hipStreamEndCapture(c->stream, &g));
+      if (launched != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(std::string("jr_newton_block: launch during capture: ") + hipGetErrorString(launched));
+      }
+      const hipError_t made = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (made != hipSuccess) {
+        c->graph = nullptr;
+        return fail(std::string("jr_newton_block: graph instantiation: ") + hipGetErrorString(made));
+      }
+      c->graph_steps = n_steps;
+    }
+    HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  } else {
+    for (int s = 0; s < n_steps; ++s) {
+      step_dispatch(c, c->hist + (size_t)s * c->n_cases);
+      TRY(check_launch());
+    }
   }
   if (hist_host) {
     HIPCHK(hipMemcpyAsync(hist_host, c->hist, (size_t)n_steps * c->n_cases * sizeof(double), hipMemcpyDeviceToHost,

@@ -4,55 +4,63 @@
 // about 2 log2 L synchronised steps instead of the 2 L sequential steps of
 // block Thomas.
 //
-// One workgroup per chain.  Level l of a chain has n_l rows (n_0 = L,
-// n_{l+1} = floor(n_l / 2)); row j of level l is the chain's cell
-// (j + 1) 2^l - 1.  Factor storage per chain: 2 L rows of 48 values of R at
-// chain row base 2 s0, level l at offset sum_{l' < l} n_l':
-//   even row: P^-1 (16), lower (16), upper (16)   (the eliminated rows)
-//   odd row:  alpha (16), gamma (16)              (the multipliers)
-//   the final single row: its inverse (16).
-// The scalar SA chain is stored alike, 5 values per row: inv, low, upp, alpha, gamma.
+// One workgroup per chain, at least L threads (L <= CR_MAX_LEN).  Level l of
+// a chain has n_l rows (n_0 = L, n_{l+1} = floor(n_l / 2)); row j of level l
+// is the chain's cell (j + 1) 2^l - 1.  The eliminated rows of every level
+// are the even rows (j = 2i), the multiplier rows the odd rows (j = 2i + 1);
+// counted over all levels, level by level, the chain has E = L - 1 even rows
+// (each level eliminates all but its odd rows) and O = sum_{l >= 1} n_l <= L - 1
+// odd rows, and one final row.  Odd row g and even row g are
+// owned by thread g, which loads all of its factor values before the first
+// synchronisation.
 //
-// Thread mapping: the odd rows of all levels, counted level by level, go to
-// threads 0, 1, ... in order, and so do the even rows, so with a workgroup of
-// at least L threads every thread owns at most one odd and one even row and
-// can fetch all of its factor blocks before the first synchronisation.
+// Factor storage per chain, structure of arrays so that the threads of a
+// wave read consecutive addresses (value k of row g at [k][g]), R values from
+// 80 s0 on (80 per chain slot suffice: 32 O + 48 E + 16 <= 80 L):
+//   odd rows  [32][O]: alpha (16), gamma (16)           (the multipliers)
+//   even rows [48][E]: P^-1 (16), lower (16), upper (16) (the eliminated rows)
+//   final row [16]:    its inverse
+// The scalar SA chain alike from 5 s0 on: odd [2][O] alpha, gamma; even
+// [3][E] 1/p, lower, upper; final 1/p.
 #pragma once
 #include "jr_solver.h"
 
 namespace jr {
 
 constexpr int CR_MAX_LEVELS = 16;
+constexpr int CR_MAX_LEN = 512;    // longest chain the cyclic-reduction kernels take (one thread per row)
+constexpr int CR_FAC = 80;         // factor values per chain slot (32 O + 48 E + 16 <= 80 L)
+constexpr int CR_FAC_SA = 5;       // SA factor values per chain slot (2 O + 3 E + 1 <= 5 L)
 
 struct CrLevels {
   int n[CR_MAX_LEVELS + 1];
-  int off[CR_MAX_LEVELS + 1];
   int odd0[CR_MAX_LEVELS + 1];    // first global odd-row index of the level
   int even0[CR_MAX_LEVELS + 1];   // first global even-row index of the level
   int nl;                         // levels with n > 1
 };
 
 __device__ __forceinline__ void cr_levels(int L, CrLevels& c) {
-  int n = L, off = 0, o = 0, e = 0, l = 0;
+  int n = L, o = 0, e = 0, l = 0;
   while (n > 1 && l < CR_MAX_LEVELS) {
     c.n[l] = n;
-    c.off[l] = off;
     c.odd0[l] = o;
     c.even0[l] = e;
-    off += n;
     o += n / 2;
     e += (n + 1) / 2;
     n /= 2;
     ++l;
   }
   c.n[l] = n;
-  c.off[l] = off;
   c.odd0[l] = o;
   c.even0[l] = e;
   c.nl = l;
 }
 
 __device__ __forceinline__ int cr_row(int l, int j) { return ((j + 1) << l) - 1; }
+
+// A workgroup-uniform value read from the LDS level table, made wave-uniform
+// so that the address arithmetic built on it stays in scalar registers.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // Level l and row j within the level (odd: j = 2i + 1, even: j = 2i) of the
 // g-th odd (or even) row counted over all levels; l = -1 if there is none.
@@ -91,129 +99,165 @@ __device__ __forceinline__ void inv4(const R* A, R* X) {
 }
 
 // Factor.  Level 0 reads the assembled chain blocks (k_blocks); deeper
-// levels (diag, lower, upper) live in the workspace ws (R, 48 per row, at the
-// chain's 2 s0 rows), double-buffered by level parity.
+// levels (diag, lower, upper) live in the workspace, two halves of [48][L]
+// (SA [3][L]) at 96 s0 (SA 6 s0), alternating with the level parity.
 template <class R, int NV>
-__global__ void __launch_bounds__(512) k_cr_factor(Plan P, const double* __restrict__ blk, R* __restrict__ fac,
-                                                   R* __restrict__ fac_sa, R* __restrict__ ws,
-                                                   R* __restrict__ ws_sa) {
+__global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* __restrict__ blk,
+                                                          R* __restrict__ fac, R* __restrict__ fac_sa,
+                                                          R* __restrict__ ws, R* __restrict__ ws_sa) {
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x;
   const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
   const int L = s1 - s0;
   __shared__ CrLevels c;   // level table in LDS: read with runtime level indices
   if (threadIdx.x == 0) cr_levels(L, c);
   __syncthreads();
+  const int O = uni(c.odd0[c.nl]), E = uni(c.even0[c.nl]), NL = uni(c.nl);
   const double* B = blk + (size_t)cs * P.n_slots * BLKW + (size_t)s0 * BLKW;
-  R* F = fac + ((size_t)cs * 2 * P.n_slots + 2 * (size_t)s0) * 48;
-  R* Fs = NV > 4 ? fac_sa + ((size_t)cs * 2 * P.n_slots + 2 * (size_t)s0) * 5 : nullptr;
-  R* W = ws + ((size_t)cs * 2 * P.n_slots + 2 * (size_t)s0) * 48;          // two halves of L rows
-  R* Ws = NV > 4 ? ws_sa + ((size_t)cs * 2 * P.n_slots + 2 * (size_t)s0) * 3 : nullptr;
-  // row accessors of the current level's (diag, lower, upper)
-  auto load_row = [&](int l, int j, R* d, R* a, R* u, R* sd, R* sa_, R* su) {
+  R* Fo = fac + ((size_t)cs * P.n_slots + s0) * CR_FAC;
+  R* Fe = Fo + (size_t)32 * O;
+  R* Ff = Fe + (size_t)48 * E;
+  R* So = NV > 4 ? fac_sa + ((size_t)cs * P.n_slots + s0) * CR_FAC_SA : nullptr;
+  R* Se = NV > 4 ? So + (size_t)2 * O : nullptr;
+  R* Sf = NV > 4 ? Se + (size_t)3 * E : nullptr;
+  R* W = ws + ((size_t)cs * P.n_slots + s0) * 96;
+  R* Ws = NV > 4 ? ws_sa + ((size_t)cs * P.n_slots + s0) * 6 : nullptr;
+  // block q (0 diagonal, 1 lower, 2 upper) of row j of level l: level 0 from
+  // the assembled chain blocks (diagonal = b[0] I), deeper levels from the
+  // workspace half of the level's parity
+  auto ld_blk = [&](int l, int j, int q, R* out) {
     if (l == 0) {
       const double* b = B + (size_t)j * BLKW;
-      for (int k = 0; k < 16; ++k) {
-        d[k] = (k % 5 == 0) ? (R)b[0] : R(0);
-        a[k] = (R)b[1 + k];
-        u[k] = (R)b[17 + k];
-      }
-      if constexpr (NV > 4) {
-        *sa_ = (R)b[33];
-        *sd = (R)b[34];
-        *su = (R)b[35];
-      }
+      if (q == 0)
+        for (int k = 0; k < 16; ++k) out[k] = (k % 5 == 0) ? (R)b[0] : R(0);
+      else
+        for (int k = 0; k < 16; ++k) out[k] = (R)b[1 + 16 * (q - 1) + k];
     } else {
-      const R* w = W + ((size_t)(l & 1) * L + j) * 48;
-      for (int k = 0; k < 16; ++k) {
-        d[k] = w[k];
-        a[k] = w[16 + k];
-        u[k] = w[32 + k];
-      }
-      if constexpr (NV > 4) {
-        const R* ws3 = Ws + ((size_t)(l & 1) * L + j) * 3;
-        *sd = ws3[0];
-        *sa_ = ws3[1];
-        *su = ws3[2];
+      const R* w = W + (size_t)(l & 1) * 48 * L + (size_t)(16 * q) * L + j;
+      for (int k = 0; k < 16; ++k) out[k] = w[(size_t)k * L];
+    }
+  };
+  // SA row (diagonal, lower, upper)
+  auto ld_sa = [&](int l, int j, R& sd, R& sa_, R& su) {
+    if constexpr (NV > 4) {
+      if (l == 0) {
+        const double* b = B + (size_t)j * BLKW;
+        sa_ = (R)b[33];
+        sd = (R)b[34];
+        su = (R)b[35];
+      } else {
+        const R* w3 = Ws + (size_t)(l & 1) * 3 * L + j;
+        sd = w3[0];
+        sa_ = w3[L];
+        su = w3[2 * (size_t)L];
       }
     }
   };
-  for (int l = 0; l <= c.nl; ++l) {
-    const int n = c.n[l];
-    R* Fl = F + (size_t)c.off[l] * 48;
-    R* Fsl = NV > 4 ? Fs + (size_t)c.off[l] * 5 : nullptr;
-    if (l == c.nl) {   // the last single row: its inverse
+  for (int l = 0; l <= NL; ++l) {
+    const int n = uni(c.n[l]);
+    if (l == NL) {   // the final single row: its inverse
       if (threadIdx.x == 0 && n == 1) {
-        R d[16], a[16], u[16], sd = R(0), sa_ = R(0), su = R(0);
-        load_row(l, 0, d, a, u, &sd, &sa_, &su);
-        inv4<R>(d, Fl);
-        if constexpr (NV > 4) Fsl[0] = R(1) / sd;
+        R d[16], x[16], sd = R(0), sa_ = R(0), su = R(0);
+        ld_blk(l, 0, 0, d);
+        inv4<R>(d, x);
+        for (int k = 0; k < 16; ++k) Ff[k] = x[k];
+        ld_sa(l, 0, sd, sa_, su);
+        if constexpr (NV > 4) Sf[0] = R(1) / sd;
       }
       break;
     }
+    const int e0 = uni(c.even0[l]), o0 = uni(c.odd0[l]);
     // even rows: inverse of the diagonal, lower and upper kept
     for (int i = threadIdx.x; i < (n + 1) / 2; i += T) {
-      const int j = 2 * i;
-      R d[16], a[16], u[16], sd = R(0), sa_ = R(0), su = R(0);
-      load_row(l, j, d, a, u, &sd, &sa_, &su);
-      R* o = Fl + (size_t)j * 48;
-      inv4<R>(d, o);
-      for (int k = 0; k < 16; ++k) {
-        o[16 + k] = a[k];
-        o[32 + k] = u[k];
+      R* o = Fe + e0 + i;
+      {
+        R d[16], x[16];
+        ld_blk(l, 2 * i, 0, d);
+        inv4<R>(d, x);
+        for (int k = 0; k < 16; ++k) o[(size_t)k * E] = x[k];
+      }
+      for (int q = 1; q <= 2; ++q) {
+        R a[16];
+        ld_blk(l, 2 * i, q, a);
+        for (int k = 0; k < 16; ++k) o[(size_t)(16 * q + k) * E] = a[k];
       }
       if constexpr (NV > 4) {
-        R* os = Fsl + (size_t)j * 5;
+        R sd = R(0), sa_ = R(0), su = R(0);
+        ld_sa(l, 2 * i, sd, sa_, su);
+        R* os = Se + e0 + i;
         os[0] = R(1) / sd;
-        os[1] = sa_;
-        os[2] = su;
+        os[E] = sa_;
+        os[2 * (size_t)E] = su;
       }
     }
     __threadfence_block();
     __syncthreads();
-    // odd rows: multipliers, then the next level's row
+    // odd rows: multipliers, then the next level's row i.  Operands are loaded
+    // one block at a time, just before their product, to bound live registers.
     for (int i = threadIdx.x; i < n / 2; i += T) {
       const int j = 2 * i + 1;
-      R d[16], a[16], u[16], sd = R(0), sa_ = R(0), su = R(0);
-      load_row(l, j, d, a, u, &sd, &sa_, &su);
-      const R* e0 = Fl + (size_t)(j - 1) * 48;                        // even row 2i: inv, low, upp
       const bool has1 = j + 1 < n;
-      const R* e1 = Fl + (size_t)(j + 1) * 48;                        // even row 2i+2
-      R alpha[16], gam[16], t[16];
-      mm4<R>(a, e0, alpha);                                           // alpha = a_odd inv_even
-      if (has1) mm4<R>(u, e1, gam);
-      else
-        for (int k = 0; k < 16; ++k) gam[k] = R(0);
-      R* o = Fl + (size_t)j * 48;
-      for (int k = 0; k < 16; ++k) {
-        o[k] = alpha[k];
-        o[16 + k] = gam[k];
+      const R* q0 = Fe + e0 + i;                     // even row j - 1
+      const R* q1 = Fe + e0 + (has1 ? i + 1 : i);    // even row j + 1
+      auto ld16 = [&](const R* q, int off, R* out) {
+        for (int k = 0; k < 16; ++k) out[k] = q[(size_t)(off + k) * E];
+      };
+      R alpha[16], gam[16], op[16], t[16];
+      {
+        R a[16];
+        ld_blk(l, j, 1, a);
+        ld16(q0, 0, op);
+        mm4<R>(a, op, alpha);                                         // alpha = a_odd P^-1_even(j-1)
       }
-      // next level row i: a' = -alpha low_e0, d' = d - alpha upp_e0 - gamma low_e1, c' = -gamma upp_e1
-      R* w = W + ((size_t)((l + 1) & 1) * L + i) * 48;
-      mm4<R>(alpha, e0 + 16, t);
-      for (int k = 0; k < 16; ++k) w[16 + k] = -t[k];
-      mm4<R>(alpha, e0 + 32, t);
-      for (int k = 0; k < 16; ++k) w[k] = d[k] - t[k];
       if (has1) {
-        mm4<R>(gam, e1 + 16, t);
-        for (int k = 0; k < 16; ++k) w[k] = w[k] - t[k];
-        mm4<R>(gam, e1 + 32, t);
-        for (int k = 0; k < 16; ++k) w[32 + k] = -t[k];
+        R u[16];
+        ld_blk(l, j, 2, u);
+        ld16(q1, 0, op);
+        mm4<R>(u, op, gam);                                           // gamma = c_odd P^-1_even(j+1)
       } else {
-        for (int k = 0; k < 16; ++k) w[32 + k] = R(0);
+        for (int k = 0; k < 16; ++k) gam[k] = R(0);
       }
+      R* o = Fo + o0 + i;
+      for (int k = 0; k < 16; ++k) {
+        o[(size_t)k * O] = alpha[k];
+        o[(size_t)(16 + k) * O] = gam[k];
+      }
+      __builtin_amdgcn_sched_barrier(0);   // stage boundary: keep the later loads from being hoisted
+      // next level row i: a' = -alpha low0, d' = d - alpha upp0 - gamma low1, c' = -gamma upp1
+      R* w = W + (size_t)((l + 1) & 1) * 48 * L + i;
+      ld16(q0, 16, op);
+      mm4<R>(alpha, op, t);
+      for (int k = 0; k < 16; ++k) w[(size_t)(16 + k) * L] = -t[k];
+      __builtin_amdgcn_sched_barrier(0);
+      R d[16];
+      ld_blk(l, j, 0, d);
+      ld16(q0, 32, op);
+      mm4<R>(alpha, op, t);
+      for (int k = 0; k < 16; ++k) d[k] = d[k] - t[k];
+      if (has1) {
+        ld16(q1, 16, op);
+        mm4<R>(gam, op, t);
+        for (int k = 0; k < 16; ++k) d[k] = d[k] - t[k];
+        ld16(q1, 32, op);
+        mm4<R>(gam, op, t);
+        for (int k = 0; k < 16; ++k) w[(size_t)(32 + k) * L] = -t[k];
+      } else {
+        for (int k = 0; k < 16; ++k) w[(size_t)(32 + k) * L] = R(0);
+      }
+      for (int k = 0; k < 16; ++k) w[(size_t)k * L] = d[k];
       if constexpr (NV > 4) {
-        const R* es0 = Fsl + (size_t)(j - 1) * 5;
-        const R* es1 = Fsl + (size_t)(j + 1) * 5;
-        R al = sa_ * es0[0];
-        R ga = has1 ? su * es1[0] : R(0);
-        R* os = Fsl + (size_t)j * 5;
-        os[3] = al;
-        os[4] = ga;
-        R* ws3 = Ws + ((size_t)((l + 1) & 1) * L + i) * 3;
-        ws3[0] = sd - al * es0[2] - (has1 ? ga * es1[1] : R(0));
-        ws3[1] = -(al * es0[1]);
-        ws3[2] = has1 ? -(ga * es1[2]) : R(0);
+        R sd = R(0), sa_ = R(0), su = R(0);
+        ld_sa(l, j, sd, sa_, su);
+        const R* es0 = Se + e0 + i;
+        const R* es1 = Se + e0 + (has1 ? i + 1 : i);
+        const R al = sa_ * es0[0];
+        const R ga = has1 ? su * es1[0] : R(0);
+        R* os = So + o0 + i;
+        os[0] = al;
+        os[O] = ga;
+        R* w3 = Ws + (size_t)((l + 1) & 1) * 3 * L + i;
+        w3[0] = sd - al * es0[2 * (size_t)E] - (has1 ? ga * es1[E] : R(0));
+        w3[L] = -(al * es0[E]);
+        w3[2 * (size_t)L] = has1 ? -(ga * es1[2 * (size_t)E]) : R(0);
       }
     }
     __threadfence_block();
@@ -223,34 +267,43 @@ __global__ void __launch_bounds__(512) k_cr_factor(Plan P, const double* __restr
 
 // z <- M^-1 r (accumulate: z += M^-1 r).  Workgroups past the chains copy
 // the cells on no chain (identity there), blockDim.x per workgroup.
-// Dynamic LDS: max_len * NV values of R.
+// Dynamic LDS: max_len * NV values of R.  With vout set, the input is the
+// Arnoldi vector w / |w|, also written to vout (apply_input).
 template <class R, int NV>
-__global__ void __launch_bounds__(512) k_cr_apply(Plan P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
-                                                  const R* __restrict__ fac_sa, const double* __restrict__ r,
-                                                  double* __restrict__ z, int accumulate, int check_active) {
+__global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __restrict__ ctl,
+                                                         const R* __restrict__ fac, const R* __restrict__ fac_sa,
+                                                         const double* __restrict__ r, double* __restrict__ z,
+                                                         int accumulate, int check_active, double* __restrict__ vout) {
   extern __shared__ unsigned char dyn[];
   R* xb = reinterpret_cast<R*>(dyn);
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x, t = threadIdx.x;
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
+  const double den = apply_den(ctl, cs, vout);
   if (chain >= P.n_lines) {
     const int k = (chain - P.n_lines) * T + t;
     if (k >= P.n_free) return;
     const int cell = P.free_cell[k];
     for (int v = 0; v < NV; ++v) {
       size_t o = base + (size_t)v * nc + cell;
-      z[o] = accumulate ? z[o] + r[o] : r[o];
+      const double x = apply_input(r, vout, o, den);
+      z[o] = accumulate ? z[o] + x : x;
     }
     return;
   }
   const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
   const int L = s1 - s0;
   __shared__ CrLevels c;   // level table in LDS: read with runtime level indices
-  if (threadIdx.x == 0) cr_levels(L, c);
+  if (t == 0) cr_levels(L, c);
   __syncthreads();
-  const R* F = fac + ((size_t)cs * 2 * P.n_slots + 2 * (size_t)s0) * 48;
-  const R* Fs = NV > 4 ? fac_sa + ((size_t)cs * 2 * P.n_slots + 2 * (size_t)s0) * 5 : nullptr;
+  const int O = uni(c.odd0[c.nl]), E = uni(c.even0[c.nl]), NL = uni(c.nl);
+  const R* Fo = fac + ((size_t)cs * P.n_slots + s0) * CR_FAC;
+  const R* Fe = Fo + (size_t)32 * O;
+  const R* Ff = Fe + (size_t)48 * E;
+  const R* So = NV > 4 ? fac_sa + ((size_t)cs * P.n_slots + s0) * CR_FAC_SA : nullptr;
+  const R* Se = NV > 4 ? So + (size_t)2 * O : nullptr;
+  const R* Sf = NV > 4 ? Se + (size_t)3 * E : nullptr;
   // this thread's odd row (reduction) and even row (back substitution)
   int lo_l, lo_j, le_l, le_j;
   cr_find(c, t, true, lo_l, lo_j);
@@ -258,35 +311,31 @@ __global__ void __launch_bounds__(512) k_cr_apply(Plan P, const Ctl* __restrict_
   R oa[32], ev[48];
   R soa[2] = {R(0), R(0)}, sev[3] = {R(0), R(0), R(0)};
   if (lo_l >= 0) {
-    const R* p = F + ((size_t)c.off[lo_l] + lo_j) * 48;
 #pragma unroll
-    for (int k = 0; k < 32; ++k) oa[k] = p[k];
+    for (int k = 0; k < 32; ++k) oa[k] = Fo[(size_t)k * O + t];
     if constexpr (NV > 4) {
-      const R* ps = Fs + ((size_t)c.off[lo_l] + lo_j) * 5;
-      soa[0] = ps[3];
-      soa[1] = ps[4];
+      soa[0] = So[t];
+      soa[1] = So[O + t];
     }
   }
   if (le_l >= 0) {
-    const R* p = F + ((size_t)c.off[le_l] + le_j) * 48;
 #pragma unroll
-    for (int k = 0; k < 48; ++k) ev[k] = p[k];
+    for (int k = 0; k < 48; ++k) ev[k] = Fe[(size_t)k * E + t];
     if constexpr (NV > 4) {
-      const R* ps = Fs + ((size_t)c.off[le_l] + le_j) * 5;
-      sev[0] = ps[0];
-      sev[1] = ps[1];
-      sev[2] = ps[2];
+      sev[0] = Se[t];
+      sev[1] = Se[E + t];
+      sev[2] = Se[2 * (size_t)E + t];
     }
   }
   for (int k = t; k < L * NV; k += T) {
     const int row = k / NV, v = k - row * NV;
-    xb[k] = (R)r[base + (size_t)v * nc + P.l_cell[s0 + row]];
+    xb[k] = (R)apply_input(r, vout, base + (size_t)v * nc + P.l_cell[s0 + row], den);
   }
   __syncthreads();
   // reduction: b_odd -= alpha b_even(j-1) + gamma b_even(j+1)
-  for (int l = 0; l < c.nl; ++l) {
-    const int n = c.n[l];
-    auto reduce = [&](int j, const R* A, const R* G, const R* sA) {
+  for (int l = 0; l < NL; ++l) {
+    if (lo_l == l) {
+      const int j = lo_j, n = c.n[l];
       R* bo = xb + (size_t)cr_row(l, j) * NV;
       const R* b0 = xb + (size_t)cr_row(l, j - 1) * NV;
       const bool has1 = j + 1 < n;
@@ -294,46 +343,35 @@ __global__ void __launch_bounds__(512) k_cr_apply(Plan P, const Ctl* __restrict_
       R out[4];
       for (int i = 0; i < 4; ++i) {
         R acc = bo[i];
-        for (int q = 0; q < 4; ++q) acc -= A[i * 4 + q] * b0[q];
+        for (int q = 0; q < 4; ++q) acc -= oa[i * 4 + q] * b0[q];
         if (has1)
-          for (int q = 0; q < 4; ++q) acc -= G[i * 4 + q] * b1[q];
+          for (int q = 0; q < 4; ++q) acc -= oa[16 + i * 4 + q] * b1[q];
         out[i] = acc;
       }
       R os = R(0);
-      if constexpr (NV > 4) os = bo[4] - sA[0] * b0[4] - (has1 ? sA[1] * b1[4] : R(0));
+      if constexpr (NV > 4) os = bo[4] - soa[0] * b0[4] - (has1 ? soa[1] * b1[4] : R(0));
       for (int i = 0; i < 4; ++i) bo[i] = out[i];
       if constexpr (NV > 4) bo[4] = os;
-    };
-    if (lo_l == l) reduce(lo_j, oa, oa + 16, soa);
-    // rows beyond the workgroup (chains longer than it): factor read in place
-    for (int g = t + T; g < c.odd0[c.nl]; g += T) {
-      int ll, jj;
-      cr_find(c, g, true, ll, jj);
-      if (ll != l) continue;
-      const R* p = F + ((size_t)c.off[ll] + jj) * 48;   // read in place
-      reduce(jj, p, p + 16, NV > 4 ? Fs + ((size_t)c.off[ll] + jj) * 5 + 3 : nullptr);
     }
     __syncthreads();
   }
-  // the last single row
+  // the final single row
   if (t == 0) {
-    const int l = c.nl;
-    R* b = xb + (size_t)cr_row(l, 0) * NV;
-    const R* inv = F + (size_t)c.off[l] * 48;
+    R* b = xb + (size_t)cr_row(NL, 0) * NV;
     R x[4];
     for (int i = 0; i < 4; ++i) {
       R acc = R(0);
-      for (int q = 0; q < 4; ++q) acc += inv[i * 4 + q] * b[q];
+      for (int q = 0; q < 4; ++q) acc += Ff[i * 4 + q] * b[q];
       x[i] = acc;
     }
     for (int i = 0; i < 4; ++i) b[i] = x[i];
-    if constexpr (NV > 4) b[4] = Fs[(size_t)c.off[l] * 5] * b[4];
+    if constexpr (NV > 4) b[4] = Sf[0] * b[4];
   }
   __syncthreads();
   // back substitution: x_even = P^-1 (b_even - low x(j-1) - upp x(j+1))
-  for (int l = c.nl - 1; l >= 0; --l) {
-    const int n = c.n[l];
-    auto back = [&](int j, const R* E, const R* sE) {
+  for (int l = NL - 1; l >= 0; --l) {
+    if (le_l == l) {
+      const int j = le_j, n = c.n[l];
       R* be = xb + (size_t)cr_row(l, j) * NV;
       const bool has0 = j > 0, has1 = j + 1 < n;
       const R* x0 = xb + (size_t)cr_row(l, has0 ? j - 1 : j) * NV;
@@ -342,29 +380,22 @@ __global__ void __launch_bounds__(512) k_cr_apply(Plan P, const Ctl* __restrict_
       for (int i = 0; i < 4; ++i) {
         R acc = be[i];
         if (has0)
-          for (int q = 0; q < 4; ++q) acc -= E[16 + i * 4 + q] * x0[q];
+          for (int q = 0; q < 4; ++q) acc -= ev[16 + i * 4 + q] * x0[q];
         if (has1)
-          for (int q = 0; q < 4; ++q) acc -= E[32 + i * 4 + q] * x1[q];
+          for (int q = 0; q < 4; ++q) acc -= ev[32 + i * 4 + q] * x1[q];
         tmp[i] = acc;
       }
       R xs = R(0);
-      if constexpr (NV > 4) xs = sE[0] * (be[4] - (has0 ? sE[1] * x0[4] : R(0)) - (has1 ? sE[2] * x1[4] : R(0)));
+      if constexpr (NV > 4)
+        xs = sev[0] * (be[4] - (has0 ? sev[1] * x0[4] : R(0)) - (has1 ? sev[2] * x1[4] : R(0)));
       R x[4];
       for (int i = 0; i < 4; ++i) {
         R acc = R(0);
-        for (int q = 0; q < 4; ++q) acc += E[i * 4 + q] * tmp[q];
+        for (int q = 0; q < 4; ++q) acc += ev[i * 4 + q] * tmp[q];
         x[i] = acc;
       }
       for (int i = 0; i < 4; ++i) be[i] = x[i];
       if constexpr (NV > 4) be[4] = xs;
-    };
-    if (le_l == l) back(le_j, ev, sev);
-    for (int g = t + T; g < c.even0[c.nl]; g += T) {
-      int ll, jj;
-      cr_find(c, g, false, ll, jj);
-      if (ll != l) continue;
-      const R* p = F + ((size_t)c.off[ll] + jj) * 48;   // read in place
-      back(jj, p, NV > 4 ? Fs + ((size_t)c.off[ll] + jj) * 5 : nullptr);
     }
     __syncthreads();
   }

@@ -102,7 +102,7 @@ __device__ __forceinline__ void load_q(const double* Q, const double* dQ, size_t
 // Faces not adjacent to an interior cell are zero.  Rows 0..3 hold the
 // inviscid minus the viscous flux, row 4 (SA) the nu_hat face flux.
 template <class T, int NV, bool VISC>
-__global__ void __launch_bounds__(BLOCK) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
+__global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
                                                  double* __restrict__ Fi, double* __restrict__ Fj, int check_active) {
   int f = blockIdx.x * BLOCK + threadIdx.x;
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(BLOCK) k_faces(Plan P, Physics ph, int first_o
 // Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere.
 // Tangent mode writes  out = vdt * dU + dR  (vdt may be null: no pseudo-time term).
 template <class T, int NV>
-__global__ void __launch_bounds__(BLOCK) k_cells(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+__global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ U, const double* __restrict__ dU,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
                                                  const double* __restrict__ Fi, const double* __restrict__ Fj,

@@ -418,10 +418,28 @@ __global__ void __launch_bounds__(64) k_thomas(Plan P, const double* __restrict_
 // substitutions (linalg.py:103-132).  Workgroups past the chains copy the cells
 // on no chain (identity there, preconditioners.py:110-115), 64 per workgroup.
 // Dynamic LDS: max_len * NV values of R (the forward sweep's d).
+// Input of the line apply: r itself, or -- when vout is set -- the Arnoldi
+// vector w / max(|w|, tiny) (krylov.py:73-74), which is also written to vout
+// (the next basis vector), so normalisation costs no pass of its own.
+__device__ __forceinline__ double apply_input(const double* __restrict__ r, double* __restrict__ vout, size_t o,
+                                              double den) {
+  if (vout) {
+    const double x = r[o] / den;
+    vout[o] = x;
+    return x;
+  }
+  return r[o];
+}
+
+__device__ __forceinline__ double apply_den(const Ctl* __restrict__ ctl, int cs, const double* vout) {
+  return vout ? fmax(ctl[cs].wnorm, 2.2250738585072014e-308) : 1.0;
+}
+
 template <class R, int NV>
 __global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
                                               const R* __restrict__ fac_sa, const double* __restrict__ r,
-                                              double* __restrict__ z, int accumulate, int check_active) {
+                                              double* __restrict__ z, int accumulate, int check_active,
+                                              double* __restrict__ vout) {
   extern __shared__ unsigned char dyn[];
   R* dbuf = reinterpret_cast<R*>(dyn);
   __shared__ R fb[CH * 34];       // forward: L, P^-1, l, 1/p per slot; backward: C, c
@@ -431,13 +449,15 @@ __global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ct
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
+  const double den = apply_den(ctl, cs, vout);
   if (chain >= P.n_lines) {
     const int k = (chain - P.n_lines) * 64 + lane;
     if (k >= P.n_free) return;
     const int cell = P.free_cell[k];
     for (int v = 0; v < NV; ++v) {
       size_t o = base + (size_t)v * nc + cell;
-      z[o] = accumulate ? z[o] + r[o] : r[o];
+      const double x = apply_input(r, vout, o, den);
+      z[o] = accumulate ? z[o] + x : x;
     }
     return;
   }
@@ -470,7 +490,7 @@ __global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ct
       R v = R(0);
       if (idx < m * NV) {
         int k = idx / NV, var = idx - k * NV;
-        v = (R)r[base + (size_t)var * nc + P.l_cell[sc + k]];
+        v = (R)apply_input(r, vout, base + (size_t)var * nc + P.l_cell[sc + k], den);
       }
       pr[t] = v;
     }

@@ -24,7 +24,7 @@ EXPORTS = (
     "jr_residual_dev", "jr_tangent", "jr_tangent_dev", "jr_wave_speeds", "jr_precond_factor", "jr_precond_apply",
     "jr_newton_init", "jr_newton_block", "jr_get_scalars", "jr_set_cfl", "jr_forces", "jr_timer_start",
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
-    "jr_device_count", "jr_profile", "jr_last_error",
+    "jr_device_count", "jr_profile", "jr_set_graphs", "jr_last_error",
 )
 
 
@@ -256,6 +256,11 @@ class Context:
         hist = np.empty((n_steps, self.n_cases))
         _check(self._lib.jr_newton_block(self._h, int(n_steps), _ptr(hist, C.c_double)))
         return hist
+
+    def set_graphs(self, enable: bool) -> None:
+        """Replay Newton blocks from a captured HIP graph, or launch every
+        kernel on the context stream (the default)."""
+        _check(self._lib.jr_set_graphs(self._h, int(bool(enable))))
 
     def scalars(self) -> np.ndarray:
         out = np.empty((self.n_cases, 8))
