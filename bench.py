@@ -151,7 +151,13 @@ def converge(case_dir: Path, grid=None) -> dict:
         case = load_case(case_dir, (f"numerical.implicit.precision={json.dumps(precision)}",))
         grid = case.grid() if grid is None else grid
         t0 = time.perf_counter()
-        res = run(case, grid)
+
+        def progress(it, last, orders, f, precision=precision):
+            if it % (50 * case.block_iters) == 0:
+                print(f"[converge {precision}] {it} iterations, residual {last:.3e} ({orders:.2f} orders), "
+                      f"CL {f.cl:+.8f} CD {f.cd:+.8f}", file=sys.stderr, flush=True)
+
+        res = run(case, grid, progress)
         dt = time.perf_counter() - t0
         out[precision] = {"reason": res.history.reason, "iterations": res.history.n_iters, "seconds": round(dt, 3),
                           "residual_drop_orders": round(float(res.history.orders_dropped), 3),

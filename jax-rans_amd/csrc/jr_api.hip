@@ -57,6 +57,7 @@ struct jr_ctx {
   int graph_steps = 0;
   hipGraphExec_t graph = nullptr;
   Red red{};
+  Cg cg{};           // CGS2 partial sums
   bool has_q = false;
 };
 
@@ -148,12 +149,14 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
   if (c->thomas) {
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + 63) / 64);
     hipLaunchKernelGGL((k_apply<R, NV>), dim3(groups, c->n_cases), dim3(64), lds, c->stream, P, c->ctl,
-                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout);
+                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, c->cg.p3,
+                       c->cg.nblk);
   } else {
     const int T = c->cr_threads;
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
     hipLaunchKernelGGL((k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, c->stream, P, c->ctl,
-                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout);
+                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, c->cg.p3,
+                       c->cg.nblk);
   }
 }
 
@@ -166,13 +169,13 @@ static void cgs2_passes(jr_ctx* c, int j, int passes) {
   const size_t N = c->N, vstride = (size_t)c->n_cases * N;
   if (passes & 1)
     hipLaunchKernelGGL((k_dots<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W, j,
-                       c->red, c->hv);
+                       c->cg, c->hv, c->h2v, c->H, c->G, c->m);
   if (passes & 2)
     hipLaunchKernelGGL((k_update_dots<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                       j, c->hv, c->red, c->h2v);
+                       j, c->hv, c->cg);
   if (passes & 4)
     hipLaunchKernelGGL((k_update_norm<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                       j, c->hv, c->h2v, c->red, c->H, c->G, c->m);
+                       j, c->h2v, c->cg);
 }
 
 template <int NV>
@@ -212,9 +215,9 @@ static void gmres_and_update(jr_ctx* c) {
       cgs2_column<NV>(c, j);
     }
     hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
-                       c->V + (size_t)c->m * vstride, N);
+                       c->V + (size_t)c->m * vstride, N, c->cg);
     hipLaunchKernelGGL(k_lstsq, dim3(c->n_cases), dim3(64), 0, c->stream, c->n_cases, c->ctl, c->H, c->G, c->m, c->y,
-                       c->hy);
+                       c->hy, c->cg, c->hv, c->h2v);
     hipLaunchKernelGGL((k_combine<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->y, c->hy,
                        c->m, c->T, c->Act, c->Rv, c->R, c->red);
     launch_apply<R, NV>(c, c->T, c->X, 1, 1);
@@ -511,7 +514,7 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   c->ramp.cfl_max = num->cfl_max;
   c->ramp.krylov_success = num->krylov_success;
   c->N = (size_t)c->nv * nc;
-  c->nblk = (int)std::min<size_t>(512, std::max<size_t>(1, (c->N + BLOCK * 2 - 1) / (BLOCK * 2)));
+  c->nblk = (int)std::min<size_t>(CG_BLK, std::max<size_t>(1, (c->N + BLOCK * 2 - 1) / (BLOCK * 2)));
 
   const size_t F = (size_t)n_cases * c->N;
   const size_t nf = (size_t)P.nfi + P.nfj;
@@ -545,6 +548,10 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
       dalloc(c, &c->G, (size_t)n_cases * GIV) || dalloc(c, &c->y, (size_t)n_cases * (MMAX + 1)) ||
       dalloc(c, &c->hy, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->coef, (size_t)n_cases * 8))
     return bail(0);
+  c->cg.nblk = c->nblk;
+  if (dalloc(c, &c->cg.p1, (size_t)n_cases * MMAX * c->nblk) ||
+      dalloc(c, &c->cg.p2, (size_t)n_cases * MMAX * c->nblk) || dalloc(c, &c->cg.p3, (size_t)n_cases * c->nblk))
+    return bail(0);
   c->red.stride = 1024 * (MMAX + 2);
   if (dalloc(c, &c->red.partials, (size_t)n_cases * c->red.stride) ||
       dalloc(c, &c->red.counters, (size_t)n_cases * 4))
@@ -557,7 +564,8 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
                         (const void*)c->sam, (const void*)c->sad, (const void*)c->lsum, (const void*)c->vdt,
                         (const void*)c->sa_diag, (const void*)c->ctl, (const void*)c->hv, (const void*)c->h2v,
                         (const void*)c->G, (const void*)c->H, (const void*)c->y, (const void*)c->hy,
-                        (const void*)c->coef, (const void*)c->red.partials, (const void*)c->red.counters})
+                        (const void*)c->coef, (const void*)c->red.partials, (const void*)c->red.counters,
+                        (const void*)c->cg.p1, (const void*)c->cg.p2, (const void*)c->cg.p3})
     if (!p) {
       fail("jr_ctx_create: a device buffer was not allocated");
       return bail(0);

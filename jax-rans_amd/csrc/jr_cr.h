@@ -273,14 +273,15 @@ template <class R, int NV>
 __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __restrict__ ctl,
                                                          const R* __restrict__ fac, const R* __restrict__ fac_sa,
                                                          const double* __restrict__ r, double* __restrict__ z,
-                                                         int accumulate, int check_active, double* __restrict__ vout) {
+                                                         int accumulate, int check_active, double* __restrict__ vout,
+                                                         const double* __restrict__ p3, int nblk) {
   extern __shared__ unsigned char dyn[];
   R* xb = reinterpret_cast<R*>(dyn);
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x, t = threadIdx.x;
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  const double den = apply_den(ctl, cs, vout);
+  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
   if (chain >= P.n_lines) {
     const int k = (chain - P.n_lines) * T + t;
     if (k >= P.n_free) return;

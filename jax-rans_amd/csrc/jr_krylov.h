@@ -137,65 +137,122 @@ __device__ __forceinline__ double wave_reduce_scatter(double (&v)[MM]) {
   return s;
 }
 
-// Workgroup sums of MM values -> per-block partials ([k][block], coalesced
-// for the finishing pass) -> the last workgroup to arrive adds them in block
-// order into the shared array tot[MM].  Returns true in every thread of that
-// workgroup.  Deterministic: the order of every sum is fixed by the launch.
-template <int MM>
-__device__ bool finish_dots(double (&v)[MM], double* partials, unsigned* counter, double* tot) {
-  __shared__ double sh[MM][4];
-  __shared__ int is_last;
-  constexpr int SPAN = 64 / MM;
+// --- Partial sums of the CGS2 passes ---------------------------------------------
+// A CGS2 pass ends with each workgroup writing its sums to a partials array
+// (no fence, no arrival counter); the kernel that needs the totals adds the
+// partials at its start, every workgroup the same partials in the same fixed
+// order, so every consumer sees bit-identical totals and runs are
+// reproducible.  The kernel boundary makes the partials visible.
+constexpr int CG_BLK = 256;   // workgroups per case of the vector kernels (at most)
+
+struct Cg {
+  double* p1;   // [case][MMAX][nblk]  h_k partials (pass 1)
+  double* p2;   // [case][MMAX][nblk]  h2_k partials (pass 2)
+  double* p3;   // [case][nblk]        |w|^2 partials (pass 3)
+  int nblk;
+};
+
+// Workgroup totals of NP values, written as this workgroup's partials
+// parts[k][blockIdx.x] for k < nk.
+template <int NP>
+__device__ __forceinline__ void put_partials(double (&v)[NP], double* __restrict__ parts, int nblk, int nk) {
+  __shared__ double sh[NP][4];
+  constexpr int SPAN = 64 / NP;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nblk = gridDim.x, blk = blockIdx.x;
-  {
-    const double s = wave_reduce_scatter<MM>(v);
-    if (lane % SPAN == 0) sh[lane / SPAN][w] = s;
-  }
+  const double s = wave_reduce_scatter<NP>(v);
+  if (lane % SPAN == 0) sh[lane / SPAN][w] = s;
   __syncthreads();
-  if (threadIdx.x < MM) {
+  if (threadIdx.x < nk) {
     const int k = threadIdx.x;
-    partials[(size_t)k * nblk + blk] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
-    __threadfence();
+    parts[(size_t)k * nblk + blockIdx.x] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = prev == (unsigned)(nblk - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double acc[MM];
+}
+
+// Totals tot[k], k <= j, of the partials parts[k][b] of the previous pass:
+// thread t adds workgroups t, t + BLOCK, ..., then the workgroup reduce-
+// scatters.  Rows past j are not read.  Called by every thread.
+template <int NP>
+__device__ __forceinline__ void get_totals(const double* __restrict__ parts, int nblk, int j, double* tot) {
+  __shared__ double sh[NP][4];
+  constexpr int SPAN = 64 / NP;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc[NP];
 #pragma unroll
-  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NP; ++k) acc[k] = 0.0;
   for (int b = threadIdx.x; b < nblk; b += BLOCK) {
+    double v[NP];
 #pragma unroll
-    for (int k = 0; k < MM; ++k)
-      acc[k] += __hip_atomic_load(&partials[(size_t)k * nblk + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < NP; ++k) v[k] = parts[(size_t)min(k, j) * nblk + b];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) acc[k] += k <= j ? v[k] : 0.0;
   }
-  {
-    const double s = wave_reduce_scatter<MM>(acc);
-    if (lane % SPAN == 0) sh[lane / SPAN][w] = s;
-  }
+  const double s = wave_reduce_scatter<NP>(acc);
+  if (lane % SPAN == 0) sh[lane / SPAN][w] = s;
   __syncthreads();
-  if (threadIdx.x < MM) {
+  if (threadIdx.x < NP) {
     const int k = threadIdx.x;
     tot[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
   }
-  if (threadIdx.x == 0) *counter = 0u;
   __syncthreads();
-  return true;
 }
 
-// CGS2 pass 1: h_k = V_k . w, k <= j.
+// Column j of the Hessenberg (krylov.py:69-76): H[:, j] = h + h2,
+// H[j+1, j] = |w|, zero below, then the Givens QR advanced by column j
+// (krylov.py:121-131, the same operations in the same order as the
+// whole-matrix sweep).  Run by one workgroup after the column's pass 3.
+// G holds the rotated columns (MMAX+1) x MMAX, cos[MMAX], sin[MMAX], g[MMAX+1].
+__device__ void column_finish(int cs, int j, int m, const double* __restrict__ p3, int nblk,
+                              const double* __restrict__ hv, const double* __restrict__ h2v, double* __restrict__ H,
+                              double* __restrict__ G) {
+  __shared__ double gcol[MMAX + 1];
+  const double nw = column_norm(p3, nblk);
+  double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;   // H[row][col], row-major (MMAX+1) x MMAX
+  for (int k = threadIdx.x; k <= j; k += blockDim.x)
+    Hc[k * MMAX + j] = hv[(size_t)cs * (MMAX + 1) + k] + h2v[(size_t)cs * (MMAX + 1) + k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Hc[(j + 1) * MMAX + j] = nw;
+    for (int k = j + 2; k <= m; ++k) Hc[k * MMAX + j] = 0.0;
+    double* Gc = G + (size_t)cs * GIV;
+    double* Rr = Gc;                                   // rotated columns
+    double* cs_ = Gc + (size_t)(MMAX + 1) * MMAX;
+    double* sn = cs_ + MMAX;
+    double* g = sn + MMAX;
+    double* col = gcol;
+    for (int k = 0; k <= j + 1; ++k) col[k] = Hc[k * MMAX + j];
+    for (int i = 0; i < j; ++i) {
+      double a = col[i], c = col[i + 1];
+      col[i] = cs_[i] * a + sn[i] * c;
+      col[i + 1] = cs_[i] * c - sn[i] * a;
+    }
+    double a = col[j], c = col[j + 1];
+    double r = sqrt(a * a + c * c);
+    double safe = r > 0 ? r : 1.0;
+    cs_[j] = r > 0 ? a / safe : 1.0;
+    sn[j] = r > 0 ? c / safe : 0.0;
+    col[j] = r;
+    col[j + 1] = 0.0;
+    const double gj = g[j];
+    g[j] = cs_[j] * gj;
+    g[j + 1] = -sn[j] * gj;
+    for (int k = 0; k <= j + 1; ++k) Rr[k * MMAX + j] = col[k];
+  }
+  __syncthreads();
+}
+
+// CGS2 pass 1: h_k = V_k . w, k <= j.  Workgroup 0 first finishes column
+// j - 1 (its pass 3 ended the previous kernel sequence).
 template <int NV, int NCH>
 __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ V,
                                                 size_t vstride, size_t vcase, const double* __restrict__ W, int j,
-                                                Red red, double* __restrict__ hv) {
+                                                Cg cg, const double* __restrict__ hv,
+                                                const double* __restrict__ h2v, double* __restrict__ H,
+                                                double* __restrict__ G, int m) {
   constexpr int NP = Cgs<NCH>::NP;
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
+  if (blockIdx.x == 0 && j > 0)
+    column_finish(cs, j - 1, m, cg.p3 + (size_t)cs * cg.nblk, cg.nblk, hv, h2v, H, G);
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
   double acc[NP];
@@ -212,26 +269,27 @@ __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ 
       for (int k = 0; k < GS; ++k) acc[g * GS + k] += v[k] * w;
     }
   }
-  __shared__ double tot[NP];
-  if (finish_dots<NP>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
-    if (threadIdx.x <= j) hv[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
-  }
+  put_partials<NP>(acc, cg.p1 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j + 1);
 }
 
-// CGS2 pass 2: w <- w - sum_{k<=j} h_k V_k, then h2_k = V_k . w.
+// CGS2 pass 2: h = the pass-1 totals; w <- w - sum_{k<=j} h_k V_k, then
+// h2_k = V_k . w.  Workgroup 0 records h.
 template <int NV, int NCH>
 __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
-                                                       double* __restrict__ W, int j, const double* __restrict__ hv,
-                                                       Red red, double* __restrict__ h2v) {
+                                                       double* __restrict__ W, int j, double* __restrict__ hv,
+                                                       Cg cg) {
   constexpr int NK = Cgs<NCH>::NK, NP = Cgs<NCH>::NP;
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
+  __shared__ double tot[NP];
+  get_totals<NP>(cg.p1 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j, tot);
+  if (blockIdx.x == 0 && threadIdx.x <= j) hv[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
   double h[NK];
 #pragma unroll
-  for (int k = 0; k < NK; ++k) h[k] = k <= j ? hv[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  for (int k = 0; k < NK; ++k) h[k] = k <= j ? tot[k] : 0.0;
   double acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) acc[k] = 0.0;
@@ -248,31 +306,29 @@ __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __rest
 #pragma unroll
     for (int k = 0; k < NK; ++k) acc[k] += vk[k] * w;
   }
-  __shared__ double tot[NP];
-  if (finish_dots<NP>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, tot)) {
-    if (threadIdx.x <= j) h2v[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
-  }
+  put_partials<NP>(acc, cg.p2 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j + 1);
 }
 
-// CGS2 pass 3: w <- w - sum h2_k V_k, |w|; H[:, j] = h + h2, H[j+1, j] = |w|.
-// The finishing workgroup also advances the Givens QR of the Hessenberg by
-// column j (krylov.py:121-131, the same operations in the same order as the
-// whole-matrix sweep).
+// CGS2 pass 3: h2 = the pass-2 totals; w <- w - sum h2_k V_k and |w|^2.
+// Workgroup 0 records h2.  The column is finished (H, Givens) by the next
+// column's pass 1, or by the cycle's least squares for the last column.
 template <int NV, int NCH>
-__global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, Ctl* __restrict__ ctl, const double* __restrict__ V,
-                                                       size_t vstride, size_t vcase, double* __restrict__ W, int j,
-                                                       const double* __restrict__ hv, const double* __restrict__ h2v,
-                                                       Red red, double* __restrict__ H, double* __restrict__ G,
-                                                       int m) {
-  constexpr int NKG = Cgs<NCH>::NKG, GS = Cgs<NCH>::GS;
+__global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __restrict__ ctl,
+                                                       const double* __restrict__ V, size_t vstride, size_t vcase,
+                                                       double* __restrict__ W, int j, double* __restrict__ h2v,
+                                                       Cg cg) {
+  constexpr int NKG = Cgs<NCH>::NKG, GS = Cgs<NCH>::GS, NP = Cgs<NCH>::NP;
   __shared__ double sh[1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
+  __shared__ double tot[NP];
+  get_totals<NP>(cg.p2 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j, tot);
+  if (blockIdx.x == 0 && threadIdx.x <= j) h2v[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
   double h2[NKG];
 #pragma unroll
-  for (int k = 0; k < NKG; ++k) h2[k] = k <= j ? h2v[(size_t)cs * (MMAX + 1) + k] : 0.0;
+  for (int k = 0; k < NKG; ++k) h2[k] = k <= j ? tot[min(k, NP - 1)] : 0.0;
   double acc[1] = {0.0};
   JR_VEC_LOOP(N) {
     double w = W[base + e];
@@ -289,66 +345,34 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, Ctl* __restrict__
     W[base + e] = w;
     acc[0] += w * w;
   }
-  __shared__ double tot[1];
-  __shared__ double gcol[MMAX + 1];
-  if (finish_sum<1>(acc, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
-    double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;   // H[row][col], row-major (MMAX+1) x MMAX
-    for (int k = threadIdx.x; k <= j; k += BLOCK)
-      Hc[k * MMAX + j] = hv[(size_t)cs * (MMAX + 1) + k] + h2v[(size_t)cs * (MMAX + 1) + k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double nw = sqrt(tot[0]);
-      Hc[(j + 1) * MMAX + j] = nw;
-      for (int k = j + 2; k <= m; ++k) Hc[k * MMAX + j] = 0.0;
-      ctl[cs].wnorm = nw;
-      double* Gc = G + (size_t)cs * GIV;
-      double* Rr = Gc;                                   // rotated columns
-      double* cs_ = Gc + (size_t)(MMAX + 1) * MMAX;
-      double* sn = cs_ + MMAX;
-      double* g = sn + MMAX;
-      double* col = gcol;
-      for (int k = 0; k <= j + 1; ++k) col[k] = Hc[k * MMAX + j];
-      for (int i = 0; i < j; ++i) {
-        double a = col[i], c = col[i + 1];
-        col[i] = cs_[i] * a + sn[i] * c;
-        col[i + 1] = cs_[i] * c - sn[i] * a;
-      }
-      double a = col[j], c = col[j + 1];
-      double r = sqrt(a * a + c * c);
-      double safe = r > 0 ? r : 1.0;
-      cs_[j] = r > 0 ? a / safe : 1.0;
-      sn[j] = r > 0 ? c / safe : 0.0;
-      col[j] = r;
-      col[j + 1] = 0.0;
-      const double gj = g[j];
-      g[j] = cs_[j] * gj;
-      g[j + 1] = -sn[j] * gj;
-      for (int k = 0; k <= j + 1; ++k) Rr[k * MMAX + j] = col[k];
-    }
-  }
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) cg.p3[(size_t)cs * cg.nblk + blockIdx.x] = acc[0];
 }
 
-// V_{j+1} = w / max(|w|, tiny)
+// V_{j+1} = w / max(|w|, tiny) (used for the last column of a cycle; the
+// other columns are normalised by the next line apply).
 template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_normalize(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ W,
-                                                     double* __restrict__ Vj1, size_t vcase) {
+                                                     double* __restrict__ Vj1, size_t vcase, Cg cg) {
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
-  const double den = fmax(ctl[cs].wnorm, 2.2250738585072014e-308);
+  const double den = fmax(column_norm(cg.p3 + (size_t)cs * cg.nblk, cg.nblk), 2.2250738585072014e-308);
   double* v = Vj1 + (size_t)cs * vcase;
   JR_VEC_LOOP(N) v[e] = W[base + e] / den;
 }
 
-// Least squares of the cycle (krylov.py:100-137): the Givens sweep was
-// advanced column by column during the Arnoldi process; here the pivot-drop
-// back substitution (a direction whose pivot is below eps (m+1) times the
-// largest is dropped) and H y.  One workgroup per case.
-__global__ void k_lstsq(int n_cases, const Ctl* __restrict__ ctl, const double* __restrict__ H,
-                        const double* __restrict__ G, int m, double* __restrict__ y, double* __restrict__ hy) {
+// Least squares of the cycle (krylov.py:100-137): finishes the last column
+// (H, Givens), then the pivot-drop back substitution (a direction whose pivot
+// is below eps (m+1) times the largest is dropped) and H y.  One workgroup
+// of 64 threads per case.
+__global__ void k_lstsq(int n_cases, const Ctl* __restrict__ ctl, double* __restrict__ H, double* __restrict__ G,
+                        int m, double* __restrict__ y, double* __restrict__ hy, Cg cg,
+                        const double* __restrict__ hv, const double* __restrict__ h2v) {
   __shared__ double ys[MMAX];
   const int cs = blockIdx.x;
   if (!ctl[cs].active) return;
+  column_finish(cs, m - 1, m, cg.p3 + (size_t)cs * cg.nblk, cg.nblk, hv, h2v, H, G);
   const double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;
   const double* Gc = G + (size_t)cs * GIV;
   const double* Rr = Gc;

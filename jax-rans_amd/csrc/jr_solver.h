@@ -418,6 +418,28 @@ __global__ void __launch_bounds__(64) k_thomas(Plan P, const double* __restrict_
 // substitutions (linalg.py:103-132).  Workgroups past the chains copy the cells
 // on no chain (identity there, preconditioners.py:110-115), 64 per workgroup.
 // Dynamic LDS: max_len * NV values of R (the forward sweep's d).
+// Sum of n partials in an order independent of the launch shape: lane l adds
+// p[l], p[l + 64], ... in turn, then the xor butterfly leaves the same total in
+// every lane.  Called by one whole wave.
+__device__ __forceinline__ double wave_sum_parts(const double* __restrict__ p, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int b = lane; b < n; b += 64) s += p[b];
+  return wave_sum(s);
+}
+
+// |w| of Arnoldi column j from its pass-3 partials: the first wave sums them,
+// the workgroup reads the result.  Called by every thread of the workgroup.
+__device__ __forceinline__ double column_norm(const double* __restrict__ p3, int nblk) {
+  __shared__ double s_norm;
+  if (threadIdx.x < 64) {
+    const double t = wave_sum_parts(p3, nblk);
+    if (threadIdx.x == 0) s_norm = sqrt(t);
+  }
+  __syncthreads();
+  return s_norm;
+}
+
 // Input of the line apply: r itself, or -- when vout is set -- the Arnoldi
 // vector w / max(|w|, tiny) (krylov.py:73-74), which is also written to vout
 // (the next basis vector), so normalisation costs no pass of its own.
@@ -431,15 +453,17 @@ __device__ __forceinline__ double apply_input(const double* __restrict__ r, doub
   return r[o];
 }
 
-__device__ __forceinline__ double apply_den(const Ctl* __restrict__ ctl, int cs, const double* vout) {
-  return vout ? fmax(ctl[cs].wnorm, 2.2250738585072014e-308) : 1.0;
+// |w| of the Arnoldi vector from its per-workgroup partials (p3 of this case,
+// nblk values) when the apply normalises; called by every thread.
+__device__ __forceinline__ double apply_den(const double* __restrict__ p3, int nblk, const double* vout) {
+  return vout ? fmax(column_norm(p3, nblk), 2.2250738585072014e-308) : 1.0;
 }
 
 template <class R, int NV>
 __global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
                                               const R* __restrict__ fac_sa, const double* __restrict__ r,
                                               double* __restrict__ z, int accumulate, int check_active,
-                                              double* __restrict__ vout) {
+                                              double* __restrict__ vout, const double* __restrict__ p3, int nblk) {
   extern __shared__ unsigned char dyn[];
   R* dbuf = reinterpret_cast<R*>(dyn);
   __shared__ R fb[CH * 34];       // forward: L, P^-1, l, 1/p per slot; backward: C, c
@@ -449,7 +473,7 @@ __global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ct
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  const double den = apply_den(ctl, cs, vout);
+  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
   if (chain >= P.n_lines) {
     const int k = (chain - P.n_lines) * 64 + lane;
     if (k >= P.n_free) return;

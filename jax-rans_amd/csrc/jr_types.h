@@ -70,7 +70,6 @@ struct Ctl {
   double omega;        // relaxation factor
   double lin_red;
   double beta;         // ||r|| at the start of the current GMRES cycle
-  double wnorm;        // |w| of the latest Arnoldi vector before normalisation
   int cycles;          // GMRES cycles done this step
   int active;          // GMRES cycle running
   int accept;          // last step accepted
