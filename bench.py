@@ -5,6 +5,7 @@ Albada, GMRES(30) x 2 with the line preconditioner, FP64).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
     python bench.py --converge [CASE_DIR]
+    python bench.py --calibrate-copy
 
 Prints one JSON line.  A step is one full Newton step of the case (residual,
 wave speeds, line factor, GMRES, update) on the device; the state starts at
@@ -31,29 +32,65 @@ import numpy as np  # noqa: E402
 
 CASE = ROOT / "cases" / "kt10_c_sa"
 HBM_PEAK_GBS = 8000.0
+CPU_STEPS = 4          # oracle Newton steps timed for the CPU baseline (about 12 s)
 
 
-def algorithmic_bytes(name: str, ctx, m: int) -> float:
-    """Compulsory bytes of one launch (each array touched once), FP64 unless
-    the line factor is FP32 (``mixed``)."""
+def cr_factor_values(lengths) -> tuple[int, int]:
+    """Values the block cyclic reduction keeps per chain of each length L:
+    32 per odd row, 48 per even row over all levels, 16 for the final row
+    (SA: 2, 3, 1)."""
+    flow = sa = 0
+    for L in lengths:
+        n, odd, even = int(L), 0, 0
+        while n > 1:
+            odd += n // 2
+            even += (n + 1) // 2
+            n //= 2
+        flow += 32 * odd + 48 * even + 16
+        sa += 2 * odd + 3 * even + 1
+    return flow, sa
+
+
+def algorithmic_bytes(name: str, ctx, m: int, restarts: int) -> float:
+    """Compulsory bytes of one launch of a hot kernel (every array it must
+    touch, once), averaged over the launches of a Newton step.  FP64 fields;
+    the line factor in FP32 for ``mixed``."""
     at = ctx.atlas
     nv, nc = ctx.nvar, at.n_canvas
-    chain_cells = at.lines.cells.size
-    free = nc - chain_cells
+    N = nv * nc * 8                        # one canvas field, bytes
     fb = 4 if ctx.config.precision == "mixed" else 8
+    sa = nv > 4
+    visc = ctx.viscous
+    lengths = np.diff(at.lines.start)
     if name == "precond_apply":
-        # factor L, P^-1, C (3 m^2) + SA l, p^-1, c (3), r read + z write on every cell
-        return ctx.n_cases * (chain_cells * (3 * 16 * fb + 3 * fb) + (chain_cells + free) * 2 * nv * 8)
+        flow, sav = cr_factor_values(lengths)
+        if ctx.config.line_solver == "thomas" or lengths.max() > 512:
+            flow, sav = 48 * lengths.sum(), 3 * lengths.sum()
+        factor = (flow + (sav if sa else 0)) * fb
+        # per step: m - 1 applies per cycle read w, write V_j and z; one reads V_0,
+        # writes z; the cycle end reads T and X, writes X
+        per_cycle = (m - 1) * 3 + 2 + 3
+        return ctx.n_cases * (factor + N * per_cycle / (m + 1))
     if name == "tangent_faces":
-        # Q, dQ read once per cell; two face vectors; tangent face fluxes written (2 faces x nv)
-        return ctx.n_cases * nc * (2 * nv * 8 + 4 * 8 + 2 * nv * 8)
-    if name == "cgs2_update_dots":
-        # column j reads V_0..V_j and w, writes w: (j + 3) vectors, averaged over j = 0..m-1
-        return ctx.n_cases * nc * nv * 8 * ((m - 1) / 2 + 3)
-    if name == "residual_faces":
-        return ctx.n_cases * nc * (nv * 8 + 4 * 8 + 2 * nv * 8)
+        # Q and dQ of every cell, both face-vector families, cell centres (viscous),
+        # the tangent face fluxes of both families written
+        return ctx.n_cases * nc * 8 * (2 * nv + 4 + (2 if visc else 0) + 2 * nv)
+    if name == "tangent_fill":
+        return ctx.n_cases * nc * 8 * (2 * nv + nv)       # U, dU read; dQ written
     if name == "tangent_cells":
-        return ctx.n_cases * nc * (2 * nv * 8 + 2 * nv * 8 + 8 + nv * 8)
+        # face fluxes of both families, dU, V/dt; SA source: Q, dQ, face vectors,
+        # volume, wall distance; J dU written
+        extra = (2 * nv + 4 + 2) if sa else 0
+        return ctx.n_cases * nc * 8 * (2 * nv + nv + 1 + extra + nv)
+    jmean = (m - 1) / 2
+    if name == "cgs2_dots":
+        return ctx.n_cases * N * (jmean + 1 + 1)          # V_0..V_j, w
+    if name in ("cgs2_update_dots", "cgs2_update_norm"):
+        return ctx.n_cases * N * (jmean + 1 + 2)          # V_0..V_j, w read and written
+    if name == "precond_factor":
+        flow, sav = cr_factor_values(lengths)
+        slots = lengths.sum()
+        return ctx.n_cases * (slots * 36 * 8 + (flow + (sav if sa else 0)) * fb)
     raise KeyError(name)
 
 
@@ -91,8 +128,25 @@ def path_rates(ctx) -> dict:
     return out
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a plain copy),
+    or None when it holds no line for the kernel."""
+    path = ROOT / "profiles" / "pmc_traffic.txt"
+    if not path.is_file():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline() -> dict:
-    """One oracle Newton step of the same case, timed in a one-thread child process."""
+    """CPU_STEPS oracle Newton steps of the same case, timed in a one-thread child process."""
     code = r"""
 import os, sys, time, json
 sys.path[:0] = [sys.argv[1] + '/jax-rans_amd', sys.argv[1]]
@@ -104,21 +158,24 @@ at = case.grid()
 st = OR.setup_for(at, case.flow, case.numerics)
 s = OS.NewtonState(OR.U0_of(st), case.implicit.cfl)
 np.seterr(all='ignore')
+steps = int(sys.argv[2])
 t = time.perf_counter()
-s, n, info = OS.newton_step(s, st, case.implicit)
+for _ in range(steps):
+    s, n, info = OS.newton_step(s, st, case.implicit)
 dt = time.perf_counter() - t
-print(json.dumps({"seconds": dt, "cells": at.cells}))
+print(json.dumps({"seconds": dt, "cells": at.cells, "steps": steps}))
 """
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-    out = subprocess.run([sys.executable, "-c", code, str(ROOT)], env=env, capture_output=True, text=True,
-                         timeout=900)
+    out = subprocess.run([sys.executable, "-c", code, str(ROOT), str(CPU_STEPS)], env=env, capture_output=True,
+                         text=True, timeout=900)
     if out.returncode != 0:
         return {"value": None, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
                 "sample": "oracle Newton step failed: " + out.stderr[-300:]}
     r = json.loads(out.stdout.strip().splitlines()[-1])
-    return {"value": r["cells"] / r["seconds"] / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"one FP64 Newton step (GMRES(30) x 2, line preconditioner) of the NumPy oracle on the "
-                      f"same {r['cells']}-cell case from the freestream, {r['seconds']:.1f} s, one thread"}
+    return {"value": r["cells"] * r["steps"] / r["seconds"] / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{r['steps']} FP64 Newton steps (GMRES(30) x 2, line preconditioner) of the NumPy oracle on "
+                      f"the same {r['cells']}-cell case from the freestream, {r['seconds']:.1f} s, one thread"}
 
 
 def run_steps(case, atlas, precision: str, steps: int, warmup: int):
@@ -176,6 +233,10 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="instead of the step benchmark: one plain 512 MiB device-to-device copy (after a "
+                         "512 MiB eviction fill), the reference dispatch for calibrating rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
@@ -184,6 +245,19 @@ def main(argv=None) -> int:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
               f"a batch of cases would shard over GPUs with no communication", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        import torch
+        n = 64 * 2**20                                   # 512 MiB of float64
+        src = torch.full((n,), 1.0, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        evict = torch.full((n,), 2.0, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        ok = bool(torch.equal(dst[:: 2**16], src[:: 2**16]))
+        print(json.dumps({"calibration_copy_bytes": n * 8, "checked": ok, "evict_sum": float(evict[:8].sum())}),
+              flush=True)
+        return 0 if ok else 1
     if args.converge:
         line = {"metric": "mixed vs FP64 at convergence", "case": args.converge, "n_gpus": 1,
                 **converge(Path(args.converge))}
@@ -196,19 +270,23 @@ def main(argv=None) -> int:
     ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup)
     ms_per_step = 1e3 * dt / args.steps
     value = cells * args.steps / dt / 1e6
-    m = case.implicit.krylov_subspace
-    # dominant kernel: the largest device time per Newton step among the hot kernels
-    calls = {"precond_apply": 2 * (m + 1), "tangent_faces": 2 * m, "cgs2_update_dots": 2 * m,
-             "residual_faces": 1, "tangent_cells": 2 * m}
-    times = {k: ctx.profile(k, reps=2 * m if k == "cgs2_update_dots" else 20) for k in calls}
-    share = {k: times[k] * calls[k] for k in calls}
-    dom = max(share, key=share.get)
-    alg = algorithmic_bytes(dom, ctx, m)
-    achieved = alg / (times[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "kernel_ms": round(times[dom], 5), "algorithmic_bytes": int(alg),
-                "kernel_ms_all": {k: round(v, 5) for k, v in times.items()}}
+    m, restarts = case.implicit.krylov_subspace, case.implicit.krylov_restarts
+    # Live probe: the same Newton steps again, every launch of the hot kernels
+    # bracketed by HIP events on the library's stream.
+    probe = ctx.probe_block(args.steps)
+    kernels = {}
+    for k, v in probe.items():
+        alg = algorithmic_bytes(k, ctx, m, restarts)
+        kernels[k] = {"launches_per_step": v["launches"] / args.steps, "ms": round(v["ms"], 5),
+                      "ms_per_step": round(v["ms"] * v["launches"] / args.steps, 4),
+                      "algorithmic_bytes": int(alg), "GB/s": round(alg / (v["ms"] * 1e-3) / 1e9, 1)}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    kd = kernels[dom]
+    achieved = kd["GB/s"]
+    traffic = measured_traffic(dom)
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "kernel_ms": kd["ms"], "algorithmic_bytes": kd["algorithmic_bytes"]}
     paths = path_rates(ctx)
     scal = ctx.scalars()[0]
     ctx.close()
@@ -226,6 +304,7 @@ def main(argv=None) -> int:
                                "alpha 2.5, Re 6.5e6, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64",
                    "cells": cells, "krylov_subspace": m, "parallelism": "single GPU"},
         "roofline": roofline,
+        "kernels": kernels,
         "paths": paths,
         "residual_first": float(hist[0, 0]), "residual_last": float(hist[-1, 0]), "cfl_last": float(scal[0]),
         **extra,

@@ -127,6 +127,15 @@ int jr_newton_block(jr_ctx* ctx, int n_steps, double* hist_host);
  * identical.  No reference counterpart: execution. */
 int jr_set_graphs(jr_ctx* ctx, int enable);
 
+/* Event probe of the hot kernels during real Newton steps: arm with a mask
+ * of kernel ids (bit 0 line apply, 1 tangent face kernel, 2 tangent fill,
+ * 3 tangent cell kernel, 4 CGS2 pass 1, 5 pass 2, 6 pass 3, 7 line factor)
+ * and room for max_launches bracketed launches; read returns per id the
+ * summed device milliseconds and the launch count, and disarms.  Needs graphs
+ * off.  No reference counterpart: measurement. */
+int jr_probe_arm(jr_ctx* ctx, unsigned mask, int max_launches);
+int jr_probe_read(jr_ctx* ctx, double* ms_total /*[8]*/, int* launches /*[8]*/);
+
 /* Per-case scalars [n_cases][8]: cfl, |R|_prev, |R|, linear reduction,
  * omega, accepted(last step), steps, rejected. */
 int jr_get_scalars(jr_ctx* ctx, double* out_host);

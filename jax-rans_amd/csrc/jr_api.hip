@@ -2,6 +2,7 @@
 // launch sequences of the residual, its tangent, the line preconditioner and
 // whole Newton steps.  Declarations and the reference interfaces each entry
 // point replaces: include/jrhip.h.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -58,6 +59,12 @@ struct jr_ctx {
   hipGraphExec_t graph = nullptr;
   Red red{};
   Cg cg{};           // CGS2 partial sums
+  // Event probe: when armed, launches of the selected kernels are bracketed
+  // by event pairs from a preallocated pool (measurement only).
+  unsigned probe_mask = 0;
+  std::vector<hipEvent_t> probe_ev;
+  std::vector<int> probe_id;      // kernel id of each recorded pair
+  int probe_used = 0;
   bool has_q = false;
 };
 
@@ -93,6 +100,30 @@ static int upload(jr_ctx* c, const D** dst, const S* src, size_t count) {
 
 static dim3 grid1(size_t n, int per, int cases) { return dim3((unsigned)((n + per - 1) / per), (unsigned)cases); }
 
+// Kernel ids of the event probe (jr_probe_arm).
+enum ProbeId { PR_APPLY = 0, PR_FACES = 1, PR_FILL = 2, PR_CELLS = 3, PR_DOTS = 4, PR_UPDATE_DOTS = 5,
+               PR_UPDATE_NORM = 6, PR_FACTOR = 7, PR_N = 8 };
+
+// Event pair for the next launch when kernel `id` is probed and the pool has
+// room (-1 otherwise).  The probed launch goes through hipExtLaunchKernelGGL,
+// whose events are stamped by the dispatch itself: kernel start to kernel end.
+static int probe_slot(jr_ctx* c, int id) {
+  if (!(c->probe_mask & (1u << id)) || 2 * (c->probe_used + 1) > (int)c->probe_ev.size()) return -1;
+  const int k = c->probe_used++;
+  c->probe_id[k] = id;
+  return k;
+}
+
+#define JR_LAUNCH(ID, KERNEL, GRID, BLOCKDIM, SHMEM, ...)                                                     \
+  do {                                                                                                       \
+    const int pk_ = probe_slot(c, ID);                                                                       \
+    if (pk_ < 0)                                                                                             \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCKDIM, SHMEM, c->stream, __VA_ARGS__);                             \
+    else                                                                                                     \
+      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCKDIM, SHMEM, c->stream, c->probe_ev[2 * pk_],                  \
+                            c->probe_ev[2 * pk_ + 1], 0, __VA_ARGS__);                                       \
+  } while (0)
+
 // --- residual and tangent launch sequences -----------------------------------
 
 template <int NV, bool VISC>
@@ -111,12 +142,13 @@ template <int NV, bool VISC>
 static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double* out, int first_order,
                            const double* vdt, int check_active) {
   const Plan& P = c->P;
-  hipLaunchKernelGGL((k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
-                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, dU, nullptr, c->dQ, check_active);
-  hipLaunchKernelGGL((k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream,
-                     P, c->ph, first_order, c->ctl, c->Q, c->dQ, c->Fi, c->Fj, check_active);
-  hipLaunchKernelGGL((k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     c->ctl, U, dU, c->Q, c->dQ, c->Fi, c->Fj, vdt, out, check_active);
+  JR_LAUNCH(PR_FILL, (k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
+            dim3(BLOCK), 0, P, c->ph.gamma, c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
+  JR_LAUNCH(PR_FACES, (k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
+            first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, c->Fi, c->Fj, check_active);
+  JR_LAUNCH(PR_CELLS, (k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
+            (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Fi,
+            (const double*)c->Fj, vdt, out, check_active);
 }
 
 template <int NV>
@@ -134,11 +166,11 @@ static void launch_factor(jr_ctx* c, const double* U) {
   hipLaunchKernelGGL((k_blocks<NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl,
                      U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
   if (c->thomas)
-    hipLaunchKernelGGL((k_thomas<R, NV>), dim3(P.n_lines, c->n_cases), dim3(64), 0, c->stream, P, c->blk, (R*)c->fac,
-                       (R*)c->fac_sa);
+    JR_LAUNCH(PR_FACTOR, (k_thomas<R, NV>), dim3(P.n_lines, c->n_cases), dim3(64), 0, P, (const double*)c->blk,
+              (R*)c->fac, (R*)c->fac_sa);
   else
-    hipLaunchKernelGGL((k_cr_factor<R, NV>), dim3(P.n_lines, c->n_cases), dim3(c->cr_threads), 0, c->stream, P, c->blk,
-                       (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
+    JR_LAUNCH(PR_FACTOR, (k_cr_factor<R, NV>), dim3(P.n_lines, c->n_cases), dim3(c->cr_threads), 0, P,
+              (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
 }
 
 template <class R, int NV>
@@ -148,15 +180,15 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
   const size_t lds = (size_t)std::max(1, P.max_len) * NV * sizeof(R);
   if (c->thomas) {
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + 63) / 64);
-    hipLaunchKernelGGL((k_apply<R, NV>), dim3(groups, c->n_cases), dim3(64), lds, c->stream, P, c->ctl,
-                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, c->cg.p3,
-                       c->cg.nblk);
+    JR_LAUNCH(PR_APPLY, (k_apply<R, NV>), dim3(groups, c->n_cases), dim3(64), lds, P, (const Ctl*)c->ctl,
+              (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, (const double*)c->cg.p3,
+              c->cg.nblk);
   } else {
     const int T = c->cr_threads;
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
-    hipLaunchKernelGGL((k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, c->stream, P, c->ctl,
-                       (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, c->cg.p3,
-                       c->cg.nblk);
+    JR_LAUNCH(PR_APPLY, (k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, P, (const Ctl*)c->ctl,
+              (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, (const double*)c->cg.p3,
+              c->cg.nblk);
   }
 }
 
@@ -168,14 +200,14 @@ static void cgs2_passes(jr_ctx* c, int j, int passes) {
   const dim3 vg(c->nblk, c->n_cases);
   const size_t N = c->N, vstride = (size_t)c->n_cases * N;
   if (passes & 1)
-    hipLaunchKernelGGL((k_dots<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W, j,
-                       c->cg, c->hv, c->h2v, c->H, c->G, c->m);
+    JR_LAUNCH(PR_DOTS, (k_dots<NV, NCH>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl, (const double*)c->V, vstride,
+              N, (const double*)c->W, j, c->cg, (const double*)c->hv, (const double*)c->h2v, c->H, c->G, c->m);
   if (passes & 2)
-    hipLaunchKernelGGL((k_update_dots<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                       j, c->hv, c->cg);
+    JR_LAUNCH(PR_UPDATE_DOTS, (k_update_dots<NV, NCH>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl,
+              (const double*)c->V, vstride, N, c->W, j, c->hv, c->cg);
   if (passes & 4)
-    hipLaunchKernelGGL((k_update_norm<NV, NCH>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->W,
-                       j, c->h2v, c->cg);
+    JR_LAUNCH(PR_UPDATE_NORM, (k_update_norm<NV, NCH>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl,
+              (const double*)c->V, vstride, N, c->W, j, c->h2v, c->cg);
 }
 
 template <int NV>
@@ -605,6 +637,7 @@ int jr_ctx_destroy(jr_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hist) (void)hipFree(c->hist);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  for (hipEvent_t e : c->probe_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -758,6 +791,40 @@ int jr_set_graphs(jr_ctx* c, int enable) {
   HIPCHK(hipStreamSynchronize(c->stream));
   drop_graph(c);
   c->use_graph = enable != 0;
+  return 0;
+}
+
+int jr_probe_arm(jr_ctx* c, unsigned mask, int max_launches) {
+  if (!c) return fail("jr_probe_arm: null context");
+  if (max_launches < 0 || max_launches > (1 << 20)) return fail("jr_probe_arm: max_launches out of range");
+  if (mask && c->use_graph) return fail("jr_probe_arm: the probe needs kernel-by-kernel launches (graphs off)");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  while ((int)c->probe_ev.size() < 2 * max_launches) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->probe_ev.push_back(e);
+  }
+  c->probe_id.assign(c->probe_ev.size() / 2, -1);
+  c->probe_mask = mask & ((1u << PR_N) - 1);
+  c->probe_used = 0;
+  return 0;
+}
+
+int jr_probe_read(jr_ctx* c, double* ms_total, int* launches) {
+  if (!c || !ms_total || !launches) return fail("jr_probe_read: null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < PR_N; ++k) {
+    ms_total[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (int k = 0; k < c->probe_used; ++k) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->probe_ev[2 * k], c->probe_ev[2 * k + 1]));
+    ms_total[c->probe_id[k]] += ms;
+    launches[c->probe_id[k]] += 1;
+  }
+  c->probe_mask = 0;
+  c->probe_used = 0;
   return 0;
 }
 

@@ -24,7 +24,7 @@ EXPORTS = (
     "jr_residual_dev", "jr_tangent", "jr_tangent_dev", "jr_wave_speeds", "jr_precond_factor", "jr_precond_apply",
     "jr_newton_init", "jr_newton_block", "jr_get_scalars", "jr_set_cfl", "jr_forces", "jr_timer_start",
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
-    "jr_device_count", "jr_profile", "jr_set_graphs", "jr_last_error",
+    "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_last_error",
 )
 
 
@@ -261,6 +261,22 @@ class Context:
         """Replay Newton blocks from a captured HIP graph, or launch every
         kernel on the context stream (the default)."""
         _check(self._lib.jr_set_graphs(self._h, int(bool(enable))))
+
+    PROBE = ("precond_apply", "tangent_faces", "tangent_fill", "tangent_cells", "cgs2_dots", "cgs2_update_dots",
+             "cgs2_update_norm", "precond_factor")
+
+    def probe_block(self, n_steps: int, kernels=PROBE, max_launches: int = 16384) -> dict:
+        """``n_steps`` Newton steps with every launch of ``kernels`` bracketed by
+        HIP events on the library's stream: per kernel the launch count and the
+        mean device milliseconds per launch."""
+        mask = sum(1 << self.PROBE.index(k) for k in kernels)
+        _check(self._lib.jr_probe_arm(self._h, C.c_uint(mask), int(max_launches)))
+        self.newton_block(n_steps)
+        ms = np.zeros(8)
+        n = np.zeros(8, np.int32)
+        _check(self._lib.jr_probe_read(self._h, _ptr(ms, C.c_double), _ptr(n, C.c_int)))
+        return {k: {"launches": int(n[i]), "ms": float(ms[i] / n[i]) if n[i] else float("nan")}
+                for i, k in enumerate(self.PROBE) if k in kernels}
 
     def scalars(self) -> np.ndarray:
         out = np.empty((self.n_cases, 8))
