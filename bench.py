@@ -67,10 +67,14 @@ def algorithmic_bytes(name: str, ctx, m: int, restarts: int) -> float:
         if ctx.config.line_solver == "thomas" or lengths.max() > 512:
             flow, sav = 48 * lengths.sum(), 3 * lengths.sum()
         factor = (flow + (sav if sa else 0)) * fb
-        # per step: m - 1 applies per cycle read w, write V_j and z; one reads V_0,
-        # writes z; the cycle end reads T and X, writes X
-        per_cycle = (m - 1) * 3 + 2 + 3
-        return ctx.n_cases * (factor + N * per_cycle / (m + 1))
+        # per cycle: m - 1 applies read w, write V_j and z; one reads V_0, writes z;
+        # each of these m also reads U and writes the tangent's dQ (ghost cells
+        # through their nv x nv fill Jacobians); the cycle end reads T and X, writes X
+        cr = ctx.config.line_solver != "thomas" and lengths.max() <= 512
+        ghosts = len(at.stages[0].dest) + len(at.stages[1].dest)
+        tangent_in = (2 * N + ghosts * nv * nv * 8) if cr else 0
+        per_cycle = (m - 1) * 3 * N + 2 * N + m * tangent_in + 3 * N
+        return ctx.n_cases * (factor + per_cycle / (m + 1))
     if name == "tangent_faces":
         # Q and dQ of every cell, both face-vector families, cell centres (viscous),
         # the tangent face fluxes of both families written
@@ -276,6 +280,8 @@ def main(argv=None) -> int:
     probe = ctx.probe_block(args.steps)
     kernels = {}
     for k, v in probe.items():
+        if not v["launches"]:
+            continue                       # not launched on this path (e.g. the fill, folded into the apply)
         alg = algorithmic_bytes(k, ctx, m, restarts)
         kernels[k] = {"launches_per_step": v["launches"] / args.steps, "ms": round(v["ms"], 5),
                       "ms_per_step": round(v["ms"] * v["launches"] / args.steps, 4),

@@ -59,6 +59,7 @@ struct jr_ctx {
   hipGraphExec_t graph = nullptr;
   Red red{};
   Cg cg{};           // CGS2 partial sums
+  double* gjac = nullptr;   // [n_cases][n_final][nv][nv] linearised ghost fill
   // Event probe: when armed, launches of the selected kernels are bracketed
   // by event pairs from a preallocated pool (measurement only).
   unsigned probe_mask = 0;
@@ -138,12 +139,15 @@ static void launch_residual(jr_ctx* c, const double* U, double* out, int first_o
 }
 
 // Requires c->Q (primitives with ghosts) for the same U.
+// dq_ready: the preceding line apply already wrote dQ on every cell (ghosts
+// through the linearised fill), so the fill kernel is not needed.
 template <int NV, bool VISC>
 static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double* out, int first_order,
-                           const double* vdt, int check_active) {
+                           const double* vdt, int check_active, bool dq_ready = false) {
   const Plan& P = c->P;
-  JR_LAUNCH(PR_FILL, (k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
-            dim3(BLOCK), 0, P, c->ph.gamma, c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
+  if (!dq_ready)
+    JR_LAUNCH(PR_FILL, (k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
+              dim3(BLOCK), 0, P, c->ph.gamma, c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
   JR_LAUNCH(PR_FACES, (k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
             first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, c->Fi, c->Fj, check_active);
   JR_LAUNCH(PR_CELLS, (k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
@@ -175,7 +179,7 @@ static void launch_factor(jr_ctx* c, const double* U) {
 
 template <class R, int NV>
 static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, int check_active,
-                         double* vout = nullptr) {
+                         double* vout = nullptr, double* dq_out = nullptr) {
   const Plan& P = c->P;
   const size_t lds = (size_t)std::max(1, P.max_len) * NV * sizeof(R);
   if (c->thomas) {
@@ -188,7 +192,7 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
     JR_LAUNCH(PR_APPLY, (k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, P, (const Ctl*)c->ctl,
               (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, (const double*)c->cg.p3,
-              c->cg.nblk);
+              c->cg.nblk, dq_out, (const double*)c->U, c->ph.gamma, (const double*)c->gjac);
   }
 }
 
@@ -241,9 +245,12 @@ static void gmres_and_update(jr_ctx* c) {
     for (int j = 0; j < c->m; ++j) {
       // z = M^-1 V_j; for j > 0 the apply also forms V_j = w / |w| from the
       // previous column's w
-      if (j == 0) launch_apply<R, NV>(c, c->V, c->Z, 0, 1);
-      else launch_apply<R, NV>(c, c->W, c->Z, 0, 1, c->V + (size_t)j * vstride);
-      launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1);
+      // the cyclic-reduction apply also writes the tangent's dQ (ghost cells
+      // through the linearised fill), so the tangent needs no fill launch
+      double* dq = (!c->thomas && P.n_final > 0) ? c->dQ : nullptr;
+      if (j == 0) launch_apply<R, NV>(c, c->V, c->Z, 0, 1, nullptr, dq);
+      else launch_apply<R, NV>(c, c->W, c->Z, 0, 1, c->V + (size_t)j * vstride, dq);
+      launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1, dq != nullptr);
       cgs2_column<NV>(c, j);
     }
     hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
@@ -270,6 +277,9 @@ static void newton_step(jr_ctx* c, double* hist_row) {
   const Plan& P = c->P;
   const dim3 vg(c->nblk, c->n_cases);
   launch_residual<NV, VISC>(c, c->U, c->R, 0);
+  if (!c->thomas && P.n_final > 0)
+    hipLaunchKernelGGL((k_ghost_jac<NV>), grid1(P.n_final, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
+                       c->ph.gamma, c->ctl, c->Q, c->gjac);
   hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
                      hist_row, c->n_cases);
   launch_speeds<NV>(c, c->U);
@@ -460,6 +470,45 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
     if (upload(c, &P.ghost, ghost.data(), nc) || upload(c, &P.t_skip0, skip0.data(), skip0.size()) ||
         upload(c, &P.t_prev1, prev1.data(), prev1.size()))
       return bail(0);
+    // Final fill tasks (the last writer of each ghost cell) and their root
+    // cells: a stage-1 task's source, a stage-2 task's source, or -- when that
+    // is a stage-1 ghost -- the source of that stage-1 task.  Every root must
+    // be a non-ghost cell for the linearised fill; otherwise it stays off.
+    std::vector<int> fq_task, fq_dest, fq_root;
+    bool roots_ok = true;
+    for (int t = 0; t < g->n_task[0]; ++t) {
+      if (skip0[t]) continue;
+      fq_task.push_back(t);
+      fq_dest.push_back((int)g->task_dest[0][t]);
+      fq_root.push_back((int)g->task_src[0][t]);
+    }
+    for (int t = 0; t < g->n_task[1]; ++t) {
+      fq_task.push_back(g->n_task[0] + t);
+      fq_dest.push_back((int)g->task_dest[1][t]);
+      fq_root.push_back(prev1[t] >= 0 ? (int)g->task_src[0][prev1[t]] : (int)g->task_src[1][t]);
+    }
+    for (int r : fq_root)
+      if (ghost[r]) roots_ok = false;
+    const int nf = (int)fq_task.size();
+    std::vector<int> dep_start(nc + 1, 0), dep_q(std::max(1, nf));
+    for (int q = 0; q < nf; ++q) dep_start[fq_root[q] + 1] += 1;
+    for (size_t i = 0; i < nc; ++i) dep_start[i + 1] += dep_start[i];
+    {
+      std::vector<int> fillp(dep_start.begin(), dep_start.end() - 1);
+      for (int q = 0; q < nf; ++q) dep_q[fillp[fq_root[q]]++] = q;
+    }
+    P.n_final = roots_ok ? nf : 0;
+    if (nf == 0) {
+      fq_task.push_back(0);
+      fq_dest.push_back(0);
+      fq_root.push_back(0);
+    }
+    if (upload(c, &P.gq_task, fq_task.data(), fq_task.size()) ||
+        upload(c, &P.gq_dest, fq_dest.data(), fq_dest.size()) ||
+        upload(c, &P.gq_root, fq_root.data(), fq_root.size()) ||
+        upload(c, &P.gdep_start, dep_start.data(), dep_start.size()) ||
+        upload(c, &P.gdep_q, dep_q.data(), dep_q.size()))
+      return bail(0);
   }
   P.n_lines = g->n_lines;
   P.n_slots = g->n_slots;
@@ -580,6 +629,7 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
       dalloc(c, &c->G, (size_t)n_cases * GIV) || dalloc(c, &c->y, (size_t)n_cases * (MMAX + 1)) ||
       dalloc(c, &c->hy, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->coef, (size_t)n_cases * 8))
     return bail(0);
+  if (dalloc(c, &c->gjac, (size_t)n_cases * std::max(1, P.n_final) * c->nv * c->nv)) return bail(0);
   c->cg.nblk = c->nblk;
   if (dalloc(c, &c->cg.p1, (size_t)n_cases * MMAX * c->nblk) ||
       dalloc(c, &c->cg.p2, (size_t)n_cases * MMAX * c->nblk) || dalloc(c, &c->cg.p3, (size_t)n_cases * c->nblk))
@@ -597,7 +647,8 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
                         (const void*)c->sa_diag, (const void*)c->ctl, (const void*)c->hv, (const void*)c->h2v,
                         (const void*)c->G, (const void*)c->H, (const void*)c->y, (const void*)c->hy,
                         (const void*)c->coef, (const void*)c->red.partials, (const void*)c->red.counters,
-                        (const void*)c->cg.p1, (const void*)c->cg.p2, (const void*)c->cg.p3})
+                        (const void*)c->cg.p1, (const void*)c->cg.p2, (const void*)c->cg.p3,
+                        (const void*)c->gjac})
     if (!p) {
       fail("jr_ctx_create: a device buffer was not allocated");
       return bail(0);

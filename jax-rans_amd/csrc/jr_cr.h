@@ -268,33 +268,56 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* 
 // z <- M^-1 r (accumulate: z += M^-1 r).  Workgroups past the chains copy
 // the cells on no chain (identity there), blockDim.x per workgroup.
 // Dynamic LDS: max_len * NV values of R.  With vout set, the input is the
-// Arnoldi vector w / |w|, also written to vout (apply_input).
+// Arnoldi vector w / |w|, also written to vout (as apply_input).  With dq_out
+// set, the tangent of the primitives at (U, z) is written there too on every
+// cell, ghosts through the linearised fill (gjac): the input of the
+// Jacobian-vector product that follows.
 template <class R, int NV>
 __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __restrict__ ctl,
                                                          const R* __restrict__ fac, const R* __restrict__ fac_sa,
                                                          const double* __restrict__ r, double* __restrict__ z,
                                                          int accumulate, int check_active, double* __restrict__ vout,
-                                                         const double* __restrict__ p3, int nblk) {
+                                                         const double* __restrict__ p3, int nblk,
+                                                         double* __restrict__ dq_out, const double* __restrict__ Uc,
+                                                         double gamma, const double* __restrict__ gjac) {
   extern __shared__ unsigned char dyn[];
   R* xb = reinterpret_cast<R*>(dyn);
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x, t = threadIdx.x;
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
-  if (chain >= P.n_lines) {
+  if (chain >= P.n_lines) {   // cells on no chain: the identity
     const int k = (chain - P.n_lines) * T + t;
-    if (k >= P.n_free) return;
-    const int cell = P.free_cell[k];
+    const int cell = k < P.n_free ? P.free_cell[k] : -1;
+    double x[NV], u[NV];
+    if (cell >= 0)
+      for (int v = 0; v < NV; ++v) {
+        x[v] = r[base + (size_t)v * nc + cell];
+        if (dq_out) u[v] = Uc[base + (size_t)v * nc + cell];
+      }
+    const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);   // every thread: a barrier inside
+    if (cell < 0) return;
     for (int v = 0; v < NV; ++v) {
-      size_t o = base + (size_t)v * nc + cell;
-      const double x = apply_input(r, vout, o, den);
-      z[o] = accumulate ? z[o] + x : x;
+      const size_t o = base + (size_t)v * nc + cell;
+      x[v] = vout ? x[v] / den : x[v];
+      if (vout) vout[o] = x[v];
+      z[o] = accumulate ? z[o] + x[v] : x[v];
     }
+    if (dq_out && !P.ghost[cell])
+      write_tangent_q<NV>(P, u, x, gamma, dq_out, base, cell, gjac + (size_t)cs * P.n_final * NV * NV);
     return;
   }
   const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
   const int L = s1 - s0;
+  // thread t owns row t of the chain (T >= L): its right-hand side is loaded
+  // first, all NV values at once, and stays in flight while the rest is set up
+  const int cell = t < L ? P.l_cell[s0 + t] : -1;
+  double xr[NV], u[NV];
+  if (cell >= 0)
+    for (int v = 0; v < NV; ++v) {
+      xr[v] = r[base + (size_t)v * nc + cell];
+      if (dq_out) u[v] = Uc[base + (size_t)v * nc + cell];
+    }
   __shared__ CrLevels c;   // level table in LDS: read with runtime level indices
   if (t == 0) cr_levels(L, c);
   __syncthreads();
@@ -328,10 +351,13 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __re
       sev[2] = Se[2 * (size_t)E + t];
     }
   }
-  for (int k = t; k < L * NV; k += T) {
-    const int row = k / NV, v = k - row * NV;
-    xb[k] = (R)apply_input(r, vout, base + (size_t)v * nc + P.l_cell[s0 + row], den);
-  }
+  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);   // every thread: a barrier inside
+  if (cell >= 0)
+    for (int v = 0; v < NV; ++v) {
+      const double xv = vout ? xr[v] / den : xr[v];
+      if (vout) vout[base + (size_t)v * nc + cell] = xv;
+      xb[t * NV + v] = (R)xv;
+    }
   __syncthreads();
   // reduction: b_odd -= alpha b_even(j-1) + gamma b_even(j+1)
   for (int l = 0; l < NL; ++l) {
@@ -400,11 +426,14 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __re
     }
     __syncthreads();
   }
-  for (int k = t; k < L * NV; k += T) {
-    const int row = k / NV, v = k - row * NV;
-    size_t o = base + (size_t)v * nc + P.l_cell[s0 + row];
-    double x = (double)xb[k];
-    z[o] = accumulate ? z[o] + x : x;
+  if (cell >= 0) {
+    for (int v = 0; v < NV; ++v) {
+      const size_t o = base + (size_t)v * nc + cell;
+      xr[v] = (double)xb[t * NV + v];
+      z[o] = accumulate ? z[o] + xr[v] : xr[v];
+    }
+    if (dq_out && !P.ghost[cell])
+      write_tangent_q<NV>(P, u, xr, gamma, dq_out, base, cell, gjac + (size_t)cs * P.n_final * NV * NV);
   }
 }
 

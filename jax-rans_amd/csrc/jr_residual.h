@@ -92,6 +92,44 @@ __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl*
   for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc + dst, g[k]);
 }
 
+// Jacobian of every final ghost value with respect to its root cell's
+// primitives, at the state of the residual (Q): column k is the tangent of
+// the fill with a unit tangent on primitive k -- the same dual-number fill
+// (ghost_of, through the stage-1 task first for a corner), so applying it to
+// a root tangent reproduces the fill's tangent.  jac[case][q][NV][NV]
+// (row = ghost primitive, column = root primitive).  Once per Newton step.
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_ghost_jac(Plan P, double gamma, const Ctl* __restrict__ ctl,
+                                                     const double* __restrict__ Q, double* __restrict__ jac) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (q >= P.n_final) return;
+  const int nc = P.nc, n0 = P.n_task[0];
+  const size_t base = (size_t)cs * NV * nc;
+  const double* q_inf = ctl[cs].q_inf;
+  const int task = P.gq_task[q], root = P.gq_root[q];
+  double qr[NV];
+  for (int k = 0; k < NV; ++k) qr[k] = Q[base + (size_t)k * nc + root];
+  double* J = jac + ((size_t)cs * P.n_final + q) * NV * NV;
+  for (int col = 0; col < NV; ++col) {
+    Dual s[NV], g[NV];
+    for (int k = 0; k < NV; ++k) s[k] = mkd(qr[k], k == col ? 1.0 : 0.0);
+    if (task < n0) {
+      ghost_of<Dual, NV>(P, 0, task, s, q_inf, gamma, g);
+    } else {
+      const int t = task - n0, p = P.t_prev1[t];
+      if (p >= 0) {
+        Dual s1[NV];
+        ghost_of<Dual, NV>(P, 0, p, s, q_inf, gamma, s1);
+        ghost_of<Dual, NV>(P, 1, t, s1, q_inf, gamma, g);
+      } else {
+        ghost_of<Dual, NV>(P, 1, t, s, q_inf, gamma, g);
+      }
+    }
+    for (int r = 0; r < NV; ++r) J[r * NV + col] = g[r].d;
+  }
+}
+
 template <class T, int NV>
 __device__ __forceinline__ void load_q(const double* Q, const double* dQ, size_t base, int nc, int c, T* q) {
   for (int k = 0; k < NV; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + c);
@@ -150,11 +188,9 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
   if (first_order) {
     for (int k = 0; k < 4; ++k) { qL[k] = qm[k]; qR[k] = qp[k]; }
   } else {
-    T qmm[4], qpp[4];
-    for (int k = 0; k < 4; ++k) {
-      qmm[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + cL - dL);
-      qpp[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + cR + dL);
-    }
+    T qmm[NV], qpp[NV];
+    load_q<T, NV>(Q, dQ, base, nc, cL - dL, qmm);
+    load_q<T, NV>(Q, dQ, base, nc, cR + dL, qpp);
     if (ph.kappa != 0.0) {
       double lo = 0.25 * (1.0 - ph.kappa), hi = 0.25 * (1.0 + ph.kappa);
       for (int k = 0; k < 4; ++k) {

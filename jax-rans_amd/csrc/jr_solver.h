@@ -453,6 +453,38 @@ __device__ __forceinline__ double apply_input(const double* __restrict__ r, doub
   return r[o];
 }
 
+// dq = d(primitive)/dU (U) z at one cell, the same dual-number conversion the
+// fill kernel applies, written to dq[at + k nc].
+template <int NV>
+__device__ __forceinline__ void tangent_primitive(const double* u, const double* z, double gamma, double* out) {
+  Dual ud[NV], qd[NV];
+  for (int k = 0; k < NV; ++k) ud[k] = mkd(u[k], z[k]);
+  primitive<Dual, NV>(ud, gamma, qd);
+  for (int k = 0; k < NV; ++k) out[k] = qd[k].d;
+}
+
+// Tangent primitives of non-ghost cell `cell` (dq[at + k nc]) and of the
+// final ghost cells rooted there (linearised fill: dq_ghost = J dq_root).
+template <int NV>
+__device__ __forceinline__ void write_tangent_q(const Plan& P, const double* u, const double* z, double gamma,
+                                                double* __restrict__ dq, size_t base, int cell,
+                                                const double* __restrict__ jac_case) {
+  const int nc = P.nc;
+  double d[NV];
+  tangent_primitive<NV>(u, z, gamma, d);
+  for (int k = 0; k < NV; ++k) dq[base + (size_t)k * nc + cell] = d[k];
+  for (int e = P.gdep_start[cell]; e < P.gdep_start[cell + 1]; ++e) {
+    const int q = P.gdep_q[e];
+    const double* J = jac_case + (size_t)q * NV * NV;
+    const int dest = P.gq_dest[q];
+    for (int r = 0; r < NV; ++r) {
+      double acc = 0.0;
+      for (int k = 0; k < NV; ++k) acc += J[r * NV + k] * d[k];
+      dq[base + (size_t)r * nc + dest] = acc;
+    }
+  }
+}
+
 // |w| of the Arnoldi vector from its per-workgroup partials (p3 of this case,
 // nblk values) when the apply normalises; called by every thread.
 __device__ __forceinline__ double apply_den(const double* __restrict__ p3, int nblk, const double* vout) {

@@ -29,6 +29,15 @@ struct Plan {
   const uint8_t* ghost;            // [nc] 1: written by a ghost-fill stage
   const uint8_t* t_skip0;          // [n_task[0]] 1: the stage-2 fill overwrites this stage-1 destination
   const int* t_prev1;              // [n_task[1]] stage-1 task whose destination is this task's source, or -1
+  // Linearised ghost fill of the tangent: the final fill tasks (stage-1 tasks
+  // that stage 2 keeps, and every stage-2 task), each a function of one
+  // non-ghost root cell; per cell, the final tasks rooted there.
+  int n_final;                     // 0: not available (the tangent runs the fill kernel)
+  const int* gq_task;              // [n_final] stage-1 t, or n_task[0] + stage-2 t
+  const int* gq_dest;              // [n_final]
+  const int* gq_root;              // [n_final]
+  const int* gdep_start;           // [nc + 1] CSR over root cells
+  const int* gdep_q;               // [n_final] final task indices, grouped by root
   int n_lines, n_slots, max_len;   // line chains
   const int* l_start;              // [n_lines+1]
   const int* l_cell;               // [n_slots]
