@@ -250,11 +250,12 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
     pr[0] = qp[1]; pr[1] = qp[2]; pr[2] = temperature(qp, gamma);
     if (NPHI > 3) { pl[3] = qm[NV - 1]; pr[3] = qp[NV - 1]; }
     T gx[4], gy[4];
+    const double idet = 1.0 / det;
     for (int k = 0; k < NPHI; ++k) {
       T dp1 = pr[k] - pl[k];
       T dp2 = pn1[k] - pn0[k];
-      gx[k] = (dp1 * d2y - dp2 * d1y) / det;
-      gy[k] = (d1x * dp2 - d2x * dp1) / det;
+      gx[k] = (dp1 * d2y - dp2 * d1y) * idet;
+      gy[k] = (d1x * dp2 - d2x * dp1) * idet;
     }
     T gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
     T gT[2] = {gx[2], gy[2]};

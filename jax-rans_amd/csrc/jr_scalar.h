@@ -29,14 +29,20 @@ __device__ __forceinline__ Dual operator-(Dual a) { return mkd(-a.v, -a.d); }
 __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return mkd(a.v * b.v, a.d * b.v + a.v * b.d); }
 __device__ __forceinline__ Dual operator*(Dual a, double b) { return mkd(a.v * b, a.d * b); }
 __device__ __forceinline__ Dual operator*(double a, Dual b) { return mkd(a * b.v, a * b.d); }
+// Division through one reciprocal: an FP64 division expands to about ten
+// instructions, a multiply to one.  The value differs from a / b by at most
+// an ulp, which the tangent kernels only use for their own coefficients.
 __device__ __forceinline__ Dual operator/(Dual a, Dual b) {
-  double q = a.v / b.v;
-  return mkd(q, (a.d - q * b.d) / b.v);
+  const double r = 1.0 / b.v, q = a.v * r;
+  return mkd(q, (a.d - q * b.d) * r);
 }
-__device__ __forceinline__ Dual operator/(Dual a, double b) { return mkd(a.v / b, a.d / b); }
+__device__ __forceinline__ Dual operator/(Dual a, double b) {
+  const double r = 1.0 / b;
+  return mkd(a.v * r, a.d * r);
+}
 __device__ __forceinline__ Dual operator/(double a, Dual b) {
-  double q = a / b.v;
-  return mkd(q, -q * b.d / b.v);
+  const double r = 1.0 / b.v, q = a * r;
+  return mkd(q, -q * b.d * r);
 }
 
 __device__ __forceinline__ double val(double x) { return x; }
