@@ -67,7 +67,8 @@ def algorithmic_bytes(name: str, ctx, m: int, restarts: int) -> float:
         if ctx.config.line_solver == "thomas" or lengths.max() > 512:
             flow, sav = 48 * lengths.sum(), 3 * lengths.sum()
         factor = (flow + (sav if sa else 0)) * fb
-        # per cycle: m - 1 applies read w, write V_j and z; one reads V_0, writes z;
+        # per cycle (launches per step = cycles x (m + 1)):
+        # m - 1 applies read w, write V_j and z; one reads V_0, writes z;
         # each of these m also reads U and writes the tangent's dQ (ghost cells
         # through their nv x nv fill Jacobians); the cycle end reads T and X, writes X
         cr = ctx.config.line_solver != "thomas" and lengths.max() <= 512
@@ -182,17 +183,29 @@ print(json.dumps({"seconds": dt, "cells": at.cells, "steps": steps}))
                       f"the same {r['cells']}-cell case from the freestream, {r['seconds']:.1f} s, one thread"}
 
 
-def run_steps(case, atlas, precision: str, steps: int, warmup: int):
+def polar_flows(flow, n: int):
+    """The batched-polar freestreams: alpha ~ U[-4, 12] deg, Mach ~ U[0.3, 0.78],
+    Re 6e6, drawn with numpy.random.default_rng(2)."""
+    from dataclasses import replace
+    rng = np.random.default_rng(2)
+    alphas = rng.uniform(-4.0, 12.0, n)
+    machs = rng.uniform(0.3, 0.78, n)
+    return [replace(flow, alpha_deg=float(a), mach=float(m), reynolds=6e6) for a, m in zip(alphas, machs)]
+
+
+def run_steps(case, atlas, precision: str, steps: int, warmup: int, batch: int = 0):
     from dataclasses import replace
 
     from jaxrans_amd.core.state import q_infinity, uniform_canvas
     from jaxrans_amd.device import context_for
     cfg = replace(case.implicit, precision=precision)
-    ctx = context_for(atlas, case.flow, case.numerics, cfg)
-    q = q_infinity(case.flow)
-    ctx.set_freestream(q, case.flow.mu_inf)
-    ctx.set_state(uniform_canvas(q, atlas.shape, case.flow.gamma))
-    ctx.newton_init(cfg.cfl)
+    flows = polar_flows(case.flow, batch) if batch else [case.flow]
+    n = len(flows)
+    ctx = context_for(atlas, case.flow, case.numerics, cfg, n_cases=n)
+    qs = np.stack([q_infinity(f) for f in flows])
+    ctx.set_freestream(qs, np.array([f.mu_inf for f in flows]))
+    ctx.set_state(np.stack([uniform_canvas(q, atlas.shape, f.gamma) for q, f in zip(qs, flows)]))
+    ctx.newton_init(np.full(n, cfg.cfl))
     if warmup:
         ctx.newton_block(warmup)
     ctx.synchronize()
@@ -241,6 +254,10 @@ def main(argv=None) -> int:
                     help="instead of the step benchmark: one plain 512 MiB device-to-device copy (after a "
                          "512 MiB eviction fill), the reference dispatch for calibrating rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--polar", type=int, metavar="B", default=0,
+                    help="instead of the single case: B cases of the benchmark grid in one context (alpha "
+                         "uniform on [-4, 12] deg, Mach uniform on [0.3, 0.78], Re 6e6, numpy default_rng(2)), "
+                         "every launch shared by the batch")
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
@@ -270,19 +287,28 @@ def main(argv=None) -> int:
     from jaxrans_amd import load_case
     case = load_case(CASE)
     atlas = case.grid()
-    cells = atlas.cells
-    ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup)
+    B = max(1, args.polar)
+    cells = atlas.cells * B
+    ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup, args.polar)
     ms_per_step = 1e3 * dt / args.steps
     value = cells * args.steps / dt / 1e6
     m, restarts = case.implicit.krylov_subspace, case.implicit.krylov_restarts
     # Live probe: the same Newton steps again, every launch of the hot kernels
     # bracketed by HIP events on the library's stream.
+    cyc0 = ctx.scalars()[:, 8].copy()
     probe = ctx.probe_block(args.steps)
+    # A later GMRES cycle is launched when some case still needs it; cases that
+    # met their tolerance exit its kernels at once.  The Arnoldi kernels' bytes
+    # count the active (case, column) pairs: cycles run x m over cases x columns launched.
+    columns = probe["tangent_faces"]["launches"]
+    active = float((ctx.scalars()[:, 8] - cyc0).sum() * m / max(1, ctx.n_cases * columns))
+    arnoldi = ("precond_apply", "tangent_faces", "tangent_fill", "tangent_cells", "cgs2_dots", "cgs2_update_dots",
+               "cgs2_update_norm")
     kernels = {}
     for k, v in probe.items():
         if not v["launches"]:
             continue                       # not launched on this path (e.g. the fill, folded into the apply)
-        alg = algorithmic_bytes(k, ctx, m, restarts)
+        alg = algorithmic_bytes(k, ctx, m, restarts) * (active if k in arnoldi else 1.0)
         kernels[k] = {"launches_per_step": v["launches"] / args.steps, "ms": round(v["ms"], 5),
                       "ms_per_step": round(v["ms"] * v["launches"] / args.steps, 4),
                       "algorithmic_bytes": int(alg), "GB/s": round(alg / (v["ms"] * 1e-3) / 1e9, 1)}
@@ -291,14 +317,15 @@ def main(argv=None) -> int:
     achieved = kd["GB/s"]
     traffic = measured_traffic(dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "kernel_ms": kd["ms"], "algorithmic_bytes": kd["algorithmic_bytes"]}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic if not args.polar else None,
+                "kernel": dom, "kernel_ms": kd["ms"], "algorithmic_bytes": kd["algorithmic_bytes"],
+                "active_gmres_fraction": round(active, 4)}
     paths = path_rates(ctx)
     scal = ctx.scalars()[0]
     ctx.close()
     extra = {}
     if not args.no_mixed:
-        cm, dtm, histm = run_steps(case, atlas, "mixed", args.steps, args.warmup)
+        cm, dtm, histm = run_steps(case, atlas, "mixed", args.steps, args.warmup, args.polar)
         extra = {"mixed_value": round(cells * args.steps / dtm / 1e6, 4), "mixed_ms_per_step": round(1e3 * dtm / args.steps, 3),
                  "mixed_last_residual": float(histm[-1, 0])}
         cm.close()
@@ -306,9 +333,12 @@ def main(argv=None) -> int:
         "metric": "Newton iters x Mcells/s", "value": round(value, 4), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "2D SA RANS, Karman-Trefftz 10 deg, C-grid 512x128 (65536 cells), M 0.73, "
-                               "alpha 2.5, Re 6.5e6, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64",
-                   "cells": cells, "krylov_subspace": m, "parallelism": "single GPU"},
+        "config": {"workload": ("2D SA RANS, Karman-Trefftz 10 deg, C-grid 512x128 (65536 cells), M 0.73, "
+                                "alpha 2.5, Re 6.5e6, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64")
+                   if not args.polar else
+                   (f"batched polar: {B} cases on the same 512x128 C-grid, alpha U[-4,12], M U[0.3,0.78], "
+                    "Re 6e6, SA, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64"),
+                   "cells": cells, "cases": B, "krylov_subspace": m, "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels": kernels,
         "paths": paths,

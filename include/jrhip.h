@@ -136,8 +136,8 @@ int jr_set_graphs(jr_ctx* ctx, int enable);
 int jr_probe_arm(jr_ctx* ctx, unsigned mask, int max_launches);
 int jr_probe_read(jr_ctx* ctx, double* ms_total /*[8]*/, int* launches /*[8]*/);
 
-/* Per-case scalars [n_cases][8]: cfl, |R|_prev, |R|, linear reduction,
- * omega, accepted(last step), steps, rejected. */
+/* Per-case scalars [n_cases][9]: cfl, |R|_prev, |R|, linear reduction,
+ * omega, accepted(last step), steps, rejected, GMRES cycles run so far. */
 int jr_get_scalars(jr_ctx* ctx, double* out_host);
 int jr_set_cfl(jr_ctx* ctx, const double* cfl);
 

@@ -196,6 +196,18 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
   }
 }
 
+static int get_ctl(jr_ctx* c, std::vector<Ctl>& h);
+
+// Whether any case's GMRES cycle is active (reads the control blocks: a
+// stream synchronisation).  On a read error it answers true.
+static bool any_active(jr_ctx* c) {
+  std::vector<Ctl> h;
+  if (get_ctl(c, h) != 0) return true;
+  for (const Ctl& x : h)
+    if (x.active) return true;
+  return false;
+}
+
 // --- CGS2 of Arnoldi column j, instantiated for j / KC + 1 basis chunks --------
 
 template <int NV, int NCH>
@@ -242,6 +254,10 @@ static void gmres_and_update(jr_ctx* c) {
   for (int cyc = 0; cyc < c->restarts; ++cyc) {
     hipLaunchKernelGGL((k_cycle_begin<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->Rv, c->V, N, c->restarts,
                        c->ktol, c->G);
+    // A later cycle runs only if some case has not met its tolerance
+    // (krylov.py:85-87); when none has, its ~5m launches would all exit at
+    // once, so the host reads the flags and skips them (not under capture).
+    if (cyc > 0 && !c->use_graph && !any_active(c)) break;
     for (int j = 0; j < c->m; ++j) {
       // z = M^-1 V_j; for j > 0 the apply also forms V_j = w / |w| from the
       // previous column's w
@@ -931,7 +947,7 @@ int jr_get_scalars(jr_ctx* c, double* out) {
   std::vector<Ctl> h;
   TRY(get_ctl(c, h));
   for (int k = 0; k < c->n_cases; ++k) {
-    double* o = out + k * 8;
+    double* o = out + k * 9;
     o[0] = h[k].cfl;
     o[1] = h[k].res_prev;
     o[2] = h[k].norm;
@@ -940,6 +956,7 @@ int jr_get_scalars(jr_ctx* c, double* out) {
     o[5] = h[k].accept;
     o[6] = h[k].steps;
     o[7] = h[k].rejected;
+    o[8] = h[k].cycles_total;
   }
   return 0;
 }

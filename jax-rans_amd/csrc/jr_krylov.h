@@ -444,6 +444,7 @@ __global__ void __launch_bounds__(BLOCK) k_combine(int nc, Ctl* __restrict__ ctl
     if (threadIdx.x == 0) {
       ctl[cs].rnorm = sqrt(tot[0]);
       ctl[cs].cycles += 1;
+      ctl[cs].cycles_total += 1;
     }
   }
 }

@@ -84,7 +84,7 @@ struct Ctl {
   int accept;          // last step accepted
   int steps;           // Newton steps taken
   int rejected;
-  int pad_;
+  int cycles_total;    // GMRES cycles run since the context was made (measurement)
 };
 
 }  // namespace jr

@@ -279,7 +279,7 @@ class Context:
                 for i, k in enumerate(self.PROBE) if k in kernels}
 
     def scalars(self) -> np.ndarray:
-        out = np.empty((self.n_cases, 8))
+        out = np.empty((self.n_cases, 9))
         _check(self._lib.jr_get_scalars(self._h, _ptr(out, C.c_double)))
         return out
 

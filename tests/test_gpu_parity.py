@@ -190,3 +190,40 @@ def test_graph_replays_match_eager_launches(name, kind, flowf, flux, lim):
         assert np.array_equal(graph.scalars(), eager.scalars()), block
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("name, kind, flowf, flux, lim", [CASES[1], CASES[5]], ids=[CASES[1][0], CASES[5][0]])
+def test_batched_cases_equal_each_case_alone(name, kind, flowf, flux, lim):
+    """Three freestreams (alpha, Mach, Reynolds differ) stepped together in one
+    context give, case by case, bit for bit the states, residual histories and
+    forces of the same cases stepped alone (the batched polar of
+    src/jaxrans/polar.py:93-112, the per-slot freestream of sweep.py:196-201)."""
+    from dataclasses import replace
+
+    from jaxrans_amd.core.state import q_infinity
+    print(f"[gpu] batch {name}", flush=True)
+    cfg = ImplicitConfig(krylov_subspace=12, krylov_restarts=2)
+    atlas, flow, num, st, _ = _setup(kind, flowf, flux, lim, implicit=cfg)
+    _.close()
+    flows = [replace(flow, alpha_deg=a, mach=m, reynolds=None if flow.inviscid else r)
+             for a, m, r in [(-2.0, 0.45, 4e6), (1.25, 0.5, 6.5e6), (4.0, 0.6, 9e6)]]
+    qs = np.stack([q_infinity(f) for f in flows])
+    mus = None if flow.inviscid else np.array([f.mu_inf for f in flows])
+    U = np.stack([perturbed(OR.U0_of(OR.setup_for(atlas, f, num)), 1e-2) for f in flows])
+    batch = context_for(atlas, flow, num, implicit=cfg, n_cases=3)
+    batch.set_freestream(qs, mus)
+    batch.set_state(U)
+    batch.newton_init(np.full(3, cfg.cfl))
+    hb = batch.newton_block(4)
+    Ub, Fb = batch.get_state(), batch.forces()
+    batch.close()
+    for k, f in enumerate(flows):
+        alone = context_for(atlas, f, num, implicit=cfg)
+        alone.set_freestream(qs[k], None if mus is None else mus[k])
+        alone.set_state(U[k])
+        alone.newton_init(cfg.cfl)
+        ha = alone.newton_block(4)
+        assert np.array_equal(ha[:, 0], hb[:, k]), (k, ha[:, 0], hb[:, k])
+        assert np.array_equal(alone.get_state()[0], Ub[k]), k
+        assert np.array_equal(alone.forces()[0], Fb[k]), k
+        alone.close()
