@@ -60,6 +60,7 @@ struct jr_ctx {
   Red red{};
   Cg cg{};           // CGS2 partial sums
   double* gjac = nullptr;   // [n_cases][n_final][nv][nv] linearised ghost fill
+  bool dq_fused = false;    // the line apply writes the tangent input (cyclic reduction, fill linearised)
   // Event probe: when armed, launches of the selected kernels are bracketed
   // by event pairs from a preallocated pool (measurement only).
   unsigned probe_mask = 0;
@@ -190,7 +191,8 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
   } else {
     const int T = c->cr_threads;
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
-    JR_LAUNCH(PR_APPLY, (k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds, P, (const Ctl*)c->ctl,
+    const size_t lds_cr = cr_apply_lds<R, NV>(P.max_len, P.max_dep, dq_out != nullptr);
+    JR_LAUNCH(PR_APPLY, (k_cr_apply<R, NV>), dim3(groups, c->n_cases), dim3(T), lds_cr, P, (const Ctl*)c->ctl,
               (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, (const double*)c->cg.p3,
               c->cg.nblk, dq_out, (const double*)c->U, c->ph.gamma, (const double*)c->gjac);
   }
@@ -263,7 +265,7 @@ static void gmres_and_update(jr_ctx* c) {
       // previous column's w
       // the cyclic-reduction apply also writes the tangent's dQ (ghost cells
       // through the linearised fill), so the tangent needs no fill launch
-      double* dq = (!c->thomas && P.n_final > 0) ? c->dQ : nullptr;
+      double* dq = c->dq_fused ? c->dQ : nullptr;
       if (j == 0) launch_apply<R, NV>(c, c->V, c->Z, 0, 1, nullptr, dq);
       else launch_apply<R, NV>(c, c->W, c->Z, 0, 1, c->V + (size_t)j * vstride, dq);
       launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1, dq != nullptr);
@@ -293,7 +295,7 @@ static void newton_step(jr_ctx* c, double* hist_row) {
   const Plan& P = c->P;
   const dim3 vg(c->nblk, c->n_cases);
   launch_residual<NV, VISC>(c, c->U, c->R, 0);
-  if (!c->thomas && P.n_final > 0)
+  if (c->dq_fused)
     hipLaunchKernelGGL((k_ghost_jac<NV>), grid1(P.n_final, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
                        c->ph.gamma, c->ctl, c->Q, c->gjac);
   hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
@@ -456,6 +458,8 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
         upload(c, &P.t_norm[s], g->task_normal[s], 2 * (size_t)g->n_task[s]))
       return bail(0);
   }
+  std::vector<int> fq_task, fq_dest, fq_root;   // final fill tasks (below)
+  bool roots_ok = true;
   {
     // Maps of the one-launch fill: which cells a stage writes, which stage-1
     // destinations stage 2 overwrites, and the stage-1 task behind a stage-2
@@ -490,8 +494,6 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
     // cells: a stage-1 task's source, a stage-2 task's source, or -- when that
     // is a stage-1 ghost -- the source of that stage-1 task.  Every root must
     // be a non-ghost cell for the linearised fill; otherwise it stays off.
-    std::vector<int> fq_task, fq_dest, fq_root;
-    bool roots_ok = true;
     for (int t = 0; t < g->n_task[0]; ++t) {
       if (skip0[t]) continue;
       fq_task.push_back(t);
@@ -505,26 +507,6 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
     }
     for (int r : fq_root)
       if (ghost[r]) roots_ok = false;
-    const int nf = (int)fq_task.size();
-    std::vector<int> dep_start(nc + 1, 0), dep_q(std::max(1, nf));
-    for (int q = 0; q < nf; ++q) dep_start[fq_root[q] + 1] += 1;
-    for (size_t i = 0; i < nc; ++i) dep_start[i + 1] += dep_start[i];
-    {
-      std::vector<int> fillp(dep_start.begin(), dep_start.end() - 1);
-      for (int q = 0; q < nf; ++q) dep_q[fillp[fq_root[q]]++] = q;
-    }
-    P.n_final = roots_ok ? nf : 0;
-    if (nf == 0) {
-      fq_task.push_back(0);
-      fq_dest.push_back(0);
-      fq_root.push_back(0);
-    }
-    if (upload(c, &P.gq_task, fq_task.data(), fq_task.size()) ||
-        upload(c, &P.gq_dest, fq_dest.data(), fq_dest.size()) ||
-        upload(c, &P.gq_root, fq_root.data(), fq_root.size()) ||
-        upload(c, &P.gdep_start, dep_start.data(), dep_start.size()) ||
-        upload(c, &P.gdep_q, dep_q.data(), dep_q.size()))
-      return bail(0);
   }
   P.n_lines = g->n_lines;
   P.n_slots = g->n_slots;
@@ -557,6 +539,59 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
     }
   }
   P.max_len = max_len;
+  {
+    // Linearised-fill dependents: a final task rooted on a chain goes to that
+    // chain's list (with the root's row in the chain: the apply's workgroup
+    // has the root's tangent); one rooted on no chain to its root cell's list.
+    const int nf = (int)fq_task.size();
+    std::vector<int> chain_of(nc, -1), row_of(nc, -1);
+    for (int l = 0; l < g->n_lines; ++l)
+      for (int64_t sl = g->line_start[l]; sl < g->line_start[l + 1]; ++sl) {
+        chain_of[g->line_cells[sl]] = l;
+        row_of[g->line_cells[sl]] = (int)(sl - g->line_start[l]);
+      }
+    std::vector<int> cq_start(g->n_lines + 1, 0), cq_q(std::max(1, nf)), gq_row(std::max(1, nf), -1);
+    std::vector<int> dep_start(nc + 1, 0), dep_q(std::max(1, nf));
+    for (int q = 0; q < nf; ++q) {
+      const int r = fq_root[q];
+      if (chain_of[r] >= 0) {
+        cq_start[chain_of[r] + 1] += 1;
+        gq_row[q] = row_of[r];
+      } else {
+        dep_start[r + 1] += 1;
+      }
+    }
+    int max_dep = 0;
+    for (int l = 0; l < g->n_lines; ++l) {
+      max_dep = std::max(max_dep, cq_start[l + 1]);
+      cq_start[l + 1] += cq_start[l];
+    }
+    for (size_t i = 0; i < nc; ++i) dep_start[i + 1] += dep_start[i];
+    {
+      std::vector<int> fc(cq_start.begin(), cq_start.end() - 1), fd(dep_start.begin(), dep_start.end() - 1);
+      for (int q = 0; q < nf; ++q) {
+        const int r = fq_root[q];
+        if (chain_of[r] >= 0) cq_q[fc[chain_of[r]]++] = q;
+        else dep_q[fd[r]++] = q;
+      }
+    }
+    P.n_final = roots_ok ? nf : 0;
+    P.max_dep = max_dep;
+    if (nf == 0) {
+      fq_task.push_back(0);
+      fq_dest.push_back(0);
+      fq_root.push_back(0);
+    }
+    if (upload(c, &P.gq_task, fq_task.data(), fq_task.size()) ||
+        upload(c, &P.gq_dest, fq_dest.data(), fq_dest.size()) ||
+        upload(c, &P.gq_root, fq_root.data(), fq_root.size()) ||
+        upload(c, &P.gq_row, gq_row.data(), gq_row.size()) ||
+        upload(c, &P.cq_start, cq_start.data(), cq_start.size()) ||
+        upload(c, &P.cq_q, cq_q.data(), cq_q.size()) ||
+        upload(c, &P.gdep_start, dep_start.data(), dep_start.size()) ||
+        upload(c, &P.gdep_q, dep_q.data(), dep_q.size()))
+      return bail(0);
+  }
   if ((size_t)max_len * 5 * sizeof(double) > 60 * 1024) {
     fail("jr_ctx_create: a line chain longer than " + std::to_string(60 * 1024 / 40) + " cells is not supported");
     return bail(0);
@@ -603,6 +638,11 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   // row); longer chains are solved by block Thomas, the same exact solve
   c->thomas = num->line_solver == 1 || P.max_len > CR_MAX_LEN;
   c->cr_threads = std::max(64, (P.max_len + 63) / 64 * 64);
+  {
+    const size_t need = c->nv == 5 ? cr_apply_lds<double, 5>(P.max_len, P.max_dep, true)
+                                   : cr_apply_lds<double, 4>(P.max_len, P.max_dep, true);
+    c->dq_fused = !c->thomas && P.n_final > 0 && need <= 60 * 1024;
+  }
   c->ktol = num->krylov_tol;
   c->max_rel = num->max_relative_update;
   c->ramp.cfl_growth = num->cfl_growth;

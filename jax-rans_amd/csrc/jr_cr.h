@@ -31,6 +31,7 @@ constexpr int CR_MAX_LEVELS = 16;
 constexpr int CR_MAX_LEN = 512;    // longest chain the cyclic-reduction kernels take (one thread per row)
 constexpr int CR_FAC = 80;         // factor values per chain slot (32 O + 48 E + 16 <= 80 L)
 constexpr int CR_FAC_SA = 5;       // SA factor values per chain slot (2 O + 3 E + 1 <= 5 L)
+constexpr int CR_DEP_LDS = 160;    // ghost Jacobians of one chain staged in LDS (the rest read in the tail)
 
 struct CrLevels {
   int n[CR_MAX_LEVELS + 1];
@@ -265,13 +266,25 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* 
   }
 }
 
+// Dynamic LDS of k_cr_apply: the right-hand sides (max_len x NV values of R)
+// and, when the tangent input is written, the chain's tangents and the ghost
+// Jacobians rooted on it.
+template <class R, int NV>
+__host__ __device__ inline size_t cr_apply_lds(int max_len, int max_dep, bool dq) {
+  const size_t xb = ((size_t)(max_len > 0 ? max_len : 1) * NV * sizeof(R) + 7) / 8 * 8;
+  if (!dq) return xb;
+  const size_t k = (size_t)(max_dep < CR_DEP_LDS ? max_dep : CR_DEP_LDS);
+  return xb + (size_t)max_len * NV * sizeof(double) + k * (NV * NV * sizeof(double) + 2 * sizeof(int));
+}
+
 // z <- M^-1 r (accumulate: z += M^-1 r).  Workgroups past the chains copy
 // the cells on no chain (identity there), blockDim.x per workgroup.
 // Dynamic LDS: max_len * NV values of R.  With vout set, the input is the
 // Arnoldi vector w / |w|, also written to vout (as apply_input).  With dq_out
 // set, the tangent of the primitives at (U, z) is written there too on every
 // cell, ghosts through the linearised fill (gjac): the input of the
-// Jacobian-vector product that follows.
+// Jacobian-vector product that follows.  Dynamic LDS then also holds the
+// chain's tangents and ghost Jacobians (cr_apply_lds).
 template <class R, int NV>
 __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __restrict__ ctl,
                                                          const R* __restrict__ fac, const R* __restrict__ fac_sa,
@@ -313,11 +326,35 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __re
   // first, all NV values at once, and stays in flight while the rest is set up
   const int cell = t < L ? P.l_cell[s0 + t] : -1;
   double xr[NV], u[NV];
+  bool is_ghost = false;
   if (cell >= 0)
     for (int v = 0; v < NV; ++v) {
       xr[v] = r[base + (size_t)v * nc + cell];
       if (dq_out) u[v] = Uc[base + (size_t)v * nc + cell];
     }
+  if (dq_out && cell >= 0) is_ghost = P.ghost[cell] != 0;
+  // Linearised fill of the ghosts rooted on this chain: their Jacobians,
+  // destinations and root rows go to LDS now, so the tail has no global loads.
+  // Dynamic LDS after xb: dqs [max_len][NV] double, Js [max_dep][NV*NV] double,
+  // jd, jrow [max_dep] int.
+  const size_t xb_bytes = ((size_t)P.max_len * NV * sizeof(R) + 7) / 8 * 8;
+  const int kl = P.max_dep < CR_DEP_LDS ? P.max_dep : CR_DEP_LDS;   // staged per chain
+  double* dqs = reinterpret_cast<double*>(dyn + xb_bytes);
+  double* Js = dqs + (size_t)P.max_len * NV;
+  int* jd = reinterpret_cast<int*>(Js + (size_t)kl * NV * NV);
+  int* jrow = jd + kl;
+  int nd = 0, d0 = 0;
+  const double* Jc = gjac + (size_t)cs * P.n_final * NV * NV;
+  if (dq_out) {
+    d0 = P.cq_start[chain];
+    nd = P.cq_start[chain + 1] - d0;
+    for (int i = t; i < nd && i < kl; i += T) {
+      const int q = P.cq_q[d0 + i];
+      for (int k = 0; k < NV * NV; ++k) Js[(size_t)i * NV * NV + k] = Jc[(size_t)q * NV * NV + k];
+      jd[i] = P.gq_dest[q];
+      jrow[i] = P.gq_row[q];
+    }
+  }
   __shared__ CrLevels c;   // level table in LDS: read with runtime level indices
   if (t == 0) cr_levels(L, c);
   __syncthreads();
@@ -432,8 +469,36 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __re
       xr[v] = (double)xb[t * NV + v];
       z[o] = accumulate ? z[o] + xr[v] : xr[v];
     }
-    if (dq_out && !P.ghost[cell])
-      write_tangent_q<NV>(P, u, xr, gamma, dq_out, base, cell, gjac + (size_t)cs * P.n_final * NV * NV);
+    if (dq_out) {
+      double d[NV];
+      tangent_primitive<NV>(u, xr, gamma, d);
+      for (int v = 0; v < NV; ++v) dqs[t * NV + v] = d[v];
+      if (!is_ghost)
+        for (int v = 0; v < NV; ++v) dq_out[base + (size_t)v * nc + cell] = d[v];
+    }
+  }
+  if (nd > 0) {   // the ghosts rooted on this chain: dQ_ghost = J dQ_root
+    __syncthreads();
+    for (int i = t; i < nd; i += T) {
+      const double* J;
+      const double* d;
+      int dest;
+      if (i < kl) {
+        J = Js + (size_t)i * NV * NV;
+        d = dqs + (size_t)jrow[i] * NV;
+        dest = jd[i];
+      } else {   // past the staged ones: indices and Jacobian from global memory
+        const int q = P.cq_q[d0 + i];
+        J = Jc + (size_t)q * NV * NV;
+        d = dqs + (size_t)P.gq_row[q] * NV;
+        dest = P.gq_dest[q];
+      }
+      for (int rr = 0; rr < NV; ++rr) {
+        double acc = 0.0;
+        for (int k = 0; k < NV; ++k) acc += J[rr * NV + k] * d[k];
+        dq_out[base + (size_t)rr * nc + dest] = acc;
+      }
+    }
   }
 }
 

@@ -36,7 +36,11 @@ struct Plan {
   const int* gq_task;              // [n_final] stage-1 t, or n_task[0] + stage-2 t
   const int* gq_dest;              // [n_final]
   const int* gq_root;              // [n_final]
-  const int* gdep_start;           // [nc + 1] CSR over root cells
+  const int* gq_row;               // [n_final] row of the root in its chain, -1 if on no chain
+  const int* cq_start;             // [n_lines + 1] CSR over chains: final tasks rooted on the chain
+  const int* cq_q;                 // [n_final]
+  int max_dep;                     // most final tasks rooted on one chain
+  const int* gdep_start;           // [nc + 1] CSR over root cells on no chain
   const int* gdep_q;               // [n_final] final task indices, grouped by root
   int n_lines, n_slots, max_len;   // line chains
   const int* l_start;              // [n_lines+1]
