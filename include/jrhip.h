@@ -136,6 +136,26 @@ int jr_set_graphs(jr_ctx* ctx, int enable);
 int jr_probe_arm(jr_ctx* ctx, unsigned mask, int max_launches);
 int jr_probe_read(jr_ctx* ctx, double* ms_total /*[8]*/, int* launches /*[8]*/);
 
+/* Asynchronous block of n_steps Newton steps (serving many contexts from one
+ * host thread): enqueued on the context stream with no host synchronisation
+ * (every GMRES cycle is launched; cycles no case needs exit on the device,
+ * so results equal jr_newton_block's); poll returns 1 when the block is done
+ * (0 not yet, < 0 error); wait blocks until then and copies the residual
+ * history [n_steps][n_cases].  With force_ref = {length, x_ref, y_ref} the
+ * block ends with jr_forces' coefficients [n_cases][8], which wait copies to
+ * forces_host (NULL to skip; force_ref NULL and forces_host non-NULL is an
+ * error).  One pending block per context.  The blocks of
+ * driver.py:87-104, split so the host can keep several solves in flight.
+ * No reference counterpart for the split itself: execution. */
+int jr_newton_block_async(jr_ctx* ctx, int n_steps, const double* force_ref);
+int jr_newton_block_poll(jr_ctx* ctx);
+int jr_newton_block_wait(jr_ctx* ctx, double* hist_host, double* forces_host);
+
+/* Device memory held by a context, and the device's free / total memory
+ * (admission of served solves by free HBM).  No reference counterpart. */
+int jr_ctx_bytes(jr_ctx* ctx, size_t* bytes);
+int jr_mem_info(size_t* free_bytes, size_t* total_bytes);
+
 /* Per-case scalars [n_cases][9]: cfl, |R|_prev, |R|, linear reduction,
  * omega, accepted(last step), steps, rejected, GMRES cycles run so far. */
 int jr_get_scalars(jr_ctx* ctx, double* out_host);

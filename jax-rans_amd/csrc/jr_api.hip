@@ -55,6 +55,15 @@ struct jr_ctx {
   // A block of Newton steps replayed from a HIP graph (captured on the first
   // block of a given length; the history buffer is part of the capture).
   bool use_graph = false;
+  // Asynchronous blocks (serving): no host synchronisation inside the block,
+  // the residual history lands in pinned host memory behind ev_block.
+  bool async_mode = false;
+  int pending_steps = 0;
+  bool pending_forces = false;
+  double* hist_pinned = nullptr;   // [pinned_cap][n_cases] history, then [n_cases][8] coefficients
+  int pinned_cap = 0;
+  hipEvent_t ev_block = nullptr;
+  size_t bytes = 0;         // device memory this context holds
   int graph_steps = 0;
   hipGraphExec_t graph = nullptr;
   Red red{};
@@ -77,6 +86,7 @@ static int dalloc(jr_ctx* c, T** p, size_t count) {
   HIPCHK(hipMalloc(&q, count * sizeof(T)));
   HIPCHK(hipMemset(q, 0, count * sizeof(T)));
   c->allocs.push_back(q);
+  c->bytes += count * sizeof(T);
   *p = static_cast<T*>(q);
   return 0;
 }
@@ -259,7 +269,7 @@ static void gmres_and_update(jr_ctx* c) {
     // A later cycle runs only if some case has not met its tolerance
     // (krylov.py:85-87); when none has, its ~5m launches would all exit at
     // once, so the host reads the flags and skips them (not under capture).
-    if (cyc > 0 && !c->use_graph && !any_active(c)) break;
+    if (cyc > 0 && !c->use_graph && !c->async_mode && !any_active(c)) break;
     for (int j = 0; j < c->m; ++j) {
       // z = M^-1 V_j; for j > 0 the apply also forms V_j = w / |w| from the
       // previous column's w
@@ -745,6 +755,8 @@ int jr_ctx_destroy(jr_ctx* c) {
   if (c->hist) (void)hipFree(c->hist);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (hipEvent_t e : c->probe_ev) (void)hipEventDestroy(e);
+  if (c->hist_pinned) (void)hipHostFree(c->hist_pinned);
+  if (c->ev_block) (void)hipEventDestroy(c->ev_block);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -935,8 +947,82 @@ int jr_probe_read(jr_ctx* c, double* ms_total, int* launches) {
   return 0;
 }
 
+int jr_newton_block_async(jr_ctx* c, int n_steps, const double* force_ref) {
+  if (!c) return fail("jr_newton_block_async: null context");
+  if (n_steps < 1) return fail("jr_newton_block_async: n_steps must be >= 1");
+  if (!c->has_q) return fail("jr_newton_block_async: set the freestream first");
+  if (c->pending_steps) return fail("jr_newton_block_async: a block is already pending on this context");
+  if (c->use_graph) return fail("jr_newton_block_async: graphs are for jr_newton_block");
+  if (n_steps > c->hist_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    if (c->hist) (void)hipFree(c->hist);
+    c->hist = nullptr;
+    HIPCHK(hipMalloc(&c->hist, (size_t)n_steps * c->n_cases * sizeof(double)));
+    c->hist_cap = n_steps;
+  }
+  if (n_steps > c->pinned_cap) {
+    if (c->hist_pinned) (void)hipHostFree(c->hist_pinned);
+    c->hist_pinned = nullptr;
+    HIPCHK(hipHostMalloc((void**)&c->hist_pinned, (size_t)(n_steps + 8) * c->n_cases * sizeof(double)));
+    c->pinned_cap = n_steps;
+  }
+  if (!c->ev_block) HIPCHK(hipEventCreateWithFlags(&c->ev_block, hipEventDisableTiming));
+  c->async_mode = true;
+  for (int s = 0; s < n_steps; ++s) step_dispatch(c, c->hist + (size_t)s * c->n_cases);
+  c->async_mode = false;
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(c->hist_pinned, c->hist, (size_t)n_steps * c->n_cases * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  if (force_ref) {
+    forces_dispatch(c, force_ref[0], force_ref[1], force_ref[2]);
+    TRY(check_launch());
+    HIPCHK(hipMemcpyAsync(c->hist_pinned + (size_t)c->pinned_cap * c->n_cases, c->coef,
+                          (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipEventRecord(c->ev_block, c->stream));
+  c->pending_steps = n_steps;
+  c->pending_forces = force_ref != nullptr;
+  return 0;
+}
+
+int jr_newton_block_poll(jr_ctx* c) {
+  if (!c) return fail("jr_newton_block_poll: null context");
+  if (!c->pending_steps) return 1;
+  const hipError_t e = hipEventQuery(c->ev_block);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(std::string("jr_newton_block_poll: ") + hipGetErrorString(e));
+}
+
+int jr_newton_block_wait(jr_ctx* c, double* hist_host, double* forces_host) {
+  if (!c) return fail("jr_newton_block_wait: null context");
+  if (!c->pending_steps) return fail("jr_newton_block_wait: no block pending");
+  if (forces_host && !c->pending_forces) return fail("jr_newton_block_wait: the block was started without forces");
+  HIPCHK(hipEventSynchronize(c->ev_block));
+  if (hist_host) std::memcpy(hist_host, c->hist_pinned, (size_t)c->pending_steps * c->n_cases * sizeof(double));
+  if (forces_host)
+    std::memcpy(forces_host, c->hist_pinned + (size_t)c->pinned_cap * c->n_cases, (size_t)c->n_cases * 8 * sizeof(double));
+  c->pending_steps = 0;
+  c->pending_forces = false;
+  return 0;
+}
+
+int jr_ctx_bytes(jr_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return fail("jr_ctx_bytes: null argument");
+  *bytes = c->bytes + (size_t)c->hist_cap * c->n_cases * sizeof(double);
+  return 0;
+}
+
+int jr_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail("jr_mem_info: null argument");
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  return 0;
+}
+
 int jr_newton_block(jr_ctx* c, int n_steps, double* hist_host) {
   if (!c) return fail("jr_newton_block: null context");
+  if (c->pending_steps) return fail("jr_newton_block: an asynchronous block is pending: wait for it first");
   if (n_steps < 1) return fail("jr_newton_block: n_steps must be >= 1");
   if (!c->has_q) return fail("jr_newton_block: set the freestream first");
   if (n_steps > c->hist_cap) {

@@ -24,7 +24,8 @@ EXPORTS = (
     "jr_residual_dev", "jr_tangent", "jr_tangent_dev", "jr_wave_speeds", "jr_precond_factor", "jr_precond_apply",
     "jr_newton_init", "jr_newton_block", "jr_get_scalars", "jr_set_cfl", "jr_forces", "jr_timer_start",
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
-    "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_last_error",
+    "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
+    "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_last_error",
 )
 
 
@@ -85,6 +86,13 @@ def library(path: str | os.PathLike | None = None):
     if path is None:
         _LIB = lib
     return lib
+
+
+def mem_info() -> tuple[int, int]:
+    """(free, total) bytes of device memory."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(library().jr_mem_info(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
 
 
 def _check(rc: int) -> None:
@@ -256,6 +264,43 @@ class Context:
         hist = np.empty((n_steps, self.n_cases))
         _check(self._lib.jr_newton_block(self._h, int(n_steps), _ptr(hist, C.c_double)))
         return hist
+
+    def newton_block_async(self, n_steps: int, forces_ref=None) -> None:
+        """Enqueue ``n_steps`` Newton steps and return at once (see ``wait``).
+        ``forces_ref = (length, (x, y))`` also enqueues the force coefficients
+        of the block's final state."""
+        ref = None
+        if forces_ref is not None:
+            length, (x, y) = forces_ref
+            ref = np.array([length, x, y], np.float64)
+        _check(self._lib.jr_newton_block_async(self._h, int(n_steps), None if ref is None else _ptr(ref, C.c_double)))
+        self._pending = int(n_steps)
+        self._pending_forces = ref is not None
+
+    def poll(self) -> bool:
+        """Whether the pending asynchronous block has finished."""
+        rc = self._lib.jr_newton_block_poll(self._h)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def wait(self):
+        """Finish the pending asynchronous block: its residual history
+        ``(n_steps, n_cases)``, and with ``forces_ref`` given to
+        ``newton_block_async`` the coefficients ``(n_cases, 8)`` as well
+        (a tuple then)."""
+        hist = np.empty((getattr(self, "_pending", 0), self.n_cases))
+        forces = np.empty((self.n_cases, 8)) if getattr(self, "_pending_forces", False) else None
+        _check(self._lib.jr_newton_block_wait(self._h, _ptr(hist, C.c_double),
+                                              None if forces is None else _ptr(forces, C.c_double)))
+        self._pending, self._pending_forces = 0, False
+        return hist if forces is None else (hist, forces)
+
+    @property
+    def device_bytes(self) -> int:
+        out = C.c_size_t()
+        _check(self._lib.jr_ctx_bytes(self._h, C.byref(out)))
+        return int(out.value)
 
     def set_graphs(self, enable: bool) -> None:
         """Replay Newton blocks from a captured HIP graph, or launch every
