@@ -5,6 +5,7 @@ Albada, GMRES(30) x 2 with the line preconditioner, FP64).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
     python bench.py --converge [CASE_DIR]
+    python bench.py --serve N [--rate R] [--forces-tol T] [--in-flight K]
     python bench.py --calibrate-copy
 
 Prints one JSON line.  A step is one full Newton step of the case (residual,
@@ -243,6 +244,69 @@ def converge(case_dir: Path, grid=None) -> dict:
     return out
 
 
+def _serve_grid(args):
+    from jaxrans_amd import serve
+    req, cfg = args
+    return req.rid, serve.request_grid(req, cfg)
+
+
+SERVE_MODES = {"fp64_serial": ("fp64", False), "fp64_served": ("fp64", True), "mixed_served": ("mixed", True),
+               "mixed_serial": ("mixed", False)}
+
+
+def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iters: int, driver: str = "threads",
+                modes=("fp64_serial", "fp64_served", "mixed_served")) -> dict:
+    """The first ``n`` requests of the 256-request stream (Poisson arrivals at
+    ``rate`` per second, numpy default_rng(3)) served three ways: the plain
+    FP64 path one solve at a time; served FP64; served mixed, ``in_flight``
+    solves on the device together.  Grids are built beforehand in a process
+    pool (CPU grid generation is not part of a solve's latency here).
+    Solves end when their forces settle (CL, CD, CM each within
+    ``forces_tol`` over the last quarter of the blocks) or at
+    ``max_iters``."""
+    import multiprocessing as mp
+
+    from jaxrans_amd import serve
+    reqs = serve.make_requests(256, rate)[:n]
+    cfg = serve.default_config("fp64", max_iters=max_iters, forces_tol=forces_tol, max_in_flight=in_flight,
+                               driver=driver)
+    t0 = time.perf_counter()
+    workers = max(1, min(16, os.cpu_count() or 1, n))
+    with mp.get_context("fork").Pool(workers) as pool:
+        atlases = dict(pool.map(_serve_grid, [(r, cfg) for r in reqs], chunksize=1))
+    print(f"[serve] {n} grids built in {time.perf_counter() - t0:.1f} s on {workers} processes", file=sys.stderr,
+          flush=True)
+    runs = {}
+    for label in modes:
+        precision, served = SERVE_MODES[label]
+        c = serve.with_precision(cfg, precision)
+        t = time.perf_counter()
+        def done(o, label=label):
+            print(f"[serve {label}] request {o.rid} ({o.cells} cells): {o.reason} after {o.iterations} steps, "
+                  f"CL {o.cl:+.7f}, latency {o.latency:.2f} s", file=sys.stderr, flush=True)
+
+        outs = (serve.serve if served else serve.serve_serial)(reqs, atlases, c, on_done=done)
+        wall = time.perf_counter() - t
+        runs[label] = ({o.rid: o for o in outs}, serve.summary(outs, wall))
+        print(f"[serve] {label}: {json.dumps(runs[label][1])}", file=sys.stderr, flush=True)
+    base = runs[modes[0]][0]
+    out = {}
+    for label, (by_id, summ) in runs.items():
+        dcl = max(abs(by_id[k].cl - base[k].cl) for k in base)
+        dcd = max(abs(by_id[k].cd - base[k].cd) for k in base)
+        dit = [by_id[k].iterations - base[k].iterations for k in base]
+        out[label] = {**summ, f"max_abs_dcl_vs_{modes[0]}": dcl, f"max_abs_dcd_counts_vs_{modes[0]}": dcd * 1e4,
+                      f"mean_diterations_vs_{modes[0]}": float(np.mean(dit))}
+    for label in modes[1:]:
+        out[f"solves_per_min_{label}_over_{modes[0]}"] = round(
+            out[label]["solves_per_min"] / out[modes[0]]["solves_per_min"], 3)
+    out["requests"] = [{"rid": r.rid, "cells": list(r.cells), "mach": round(r.mach, 4), "alpha": round(r.alpha_deg, 3),
+                        "re": r.reynolds, **{lab: [by_id[r.rid].iterations, by_id[r.rid].reason,
+                                                   round(by_id[r.rid].cl, 7), round(by_id[r.rid].latency, 2)]
+                                             for lab, (by_id, _) in runs.items()}} for r in reqs]
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +325,17 @@ def main(argv=None) -> int:
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
+    ap.add_argument("--serve", type=int, metavar="N", default=0,
+                    help="instead of the step benchmark: the served stream, first N of its 256 requests")
+    ap.add_argument("--rate", type=float, default=0.5, help="--serve: Poisson arrival rate, requests per second")
+    ap.add_argument("--forces-tol", type=float, default=1e-5, help="--serve: settle tolerance on CL, CD, CM")
+    ap.add_argument("--in-flight", type=int, default=8, help="--serve: solves on the device together")
+    ap.add_argument("--max-iters", type=int, default=4000, help="--serve: Newton budget per solve")
+    ap.add_argument("--driver", choices=("threads", "async"), default="threads",
+                    help="--serve: a host thread per solve, or one thread polling asynchronous blocks")
+    ap.add_argument("--serve-modes", default="fp64_serial,fp64_served,mixed_served",
+                    help=f"--serve: comma-separated runs of {sorted(SERVE_MODES)}; the first is the reference "
+                         f"for the force differences")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
@@ -279,6 +354,18 @@ def main(argv=None) -> int:
         print(json.dumps({"calibration_copy_bytes": n * 8, "checked": ok, "evict_sum": float(evict[:8].sum())}),
               flush=True)
         return 0 if ok else 1
+    if args.serve:
+        line = {"metric": "solves/min + p50/p95/p99 latency, served stream", "n_gpus": 1,
+                "config": {"workload": f"served stream: first {args.serve} of 256 requests, Poisson {args.rate}/s, "
+                                       "Karman-Trefftz (x0 U[-0.12,-0.04], y0 U[0,0.06], tau U[5,15]), C-grids "
+                                       "{257x65, 513x129, 769x193, 1025x257} nodes, M U[0.3,0.8], alpha U[-2,6], "
+                                       "Re log-U[1e6,2e7], SA, Roe + van_albada_smooth, GMRES(30)x2, line_j",
+                           "forces_tol": args.forces_tol, "max_iters": args.max_iters, "in_flight": args.in_flight,
+                           "driver": args.driver},
+                **serve_bench(args.serve, args.rate, args.forces_tol, args.in_flight, args.max_iters, args.driver,
+                              tuple(args.serve_modes.split(",")))}
+        print(json.dumps(line), flush=True)
+        return 0
     if args.converge:
         line = {"metric": "mixed vs FP64 at convergence", "case": args.converge, "n_gpus": 1,
                 **converge(Path(args.converge))}

@@ -1,0 +1,85 @@
+"""The served stream on the device: asynchronous blocks equal blocking ones,
+and each request's result is the same alone and in the mix."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import c_atlas, perturbed
+from jaxrans_amd import serve
+from jaxrans_amd.core.config import FlowConditions, ImplicitConfig, NumericsConfig
+from jaxrans_amd.device import context_for, mem_info
+from oracle import residual as OR
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_async_blocks_equal_blocking_blocks(precision):
+    """Blocks enqueued asynchronously (every GMRES cycle launched, cycles no
+    case needs exiting on the device) give bit for bit the histories, states,
+    scalars and forces of jr_newton_block (which skips such cycles on the
+    host)."""
+    print(f"[gpu] async blocks {precision}", flush=True)
+    atlas = c_atlas()
+    flow = FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2")
+    num = NumericsConfig(flux="roe", limiter="van_albada_smooth", solver="implicit")
+    cfg = ImplicitConfig(krylov_subspace=12, krylov_restarts=2, precision=precision)
+    st = OR.setup_for(atlas, flow, num)
+    U = perturbed(OR.U0_of(st), 1e-2)
+    ctxs = []
+    for _ in range(2):
+        c = context_for(atlas, flow, num, implicit=cfg)
+        c.set_freestream(st.q_inf, flow.mu_inf)
+        c.set_state(U)
+        c.newton_init(cfg.cfl)
+        ctxs.append(c)
+    blocking, asynchronous = ctxs
+    ref = (1.0, (0.25, 0.0))
+    for block in range(3):
+        hb = blocking.newton_block(4)
+        fb = blocking.forces(ref[0], ref[1])
+        asynchronous.newton_block_async(4, ref)
+        while not asynchronous.poll():
+            pass
+        ha, fa = asynchronous.wait()
+        assert np.array_equal(ha, hb), (block, ha, hb)
+        assert np.array_equal(fa, fb), block
+        assert np.array_equal(asynchronous.get_state(), blocking.get_state()), block
+        assert np.array_equal(asynchronous.scalars(), blocking.scalars()), block
+    # a block without forces returns the history alone
+    asynchronous.newton_block_async(2)
+    assert asynchronous.wait().shape == (2, 1)
+    for c in ctxs:
+        c.close()
+
+
+def _small_stream():
+    sizes = ((96, 24), (128, 32), (160, 32))
+    reqs = serve.make_requests(5, 0.0, seed=11, sizes=sizes)
+    cfg = serve.default_config("mixed", wall_spacing=1e-3, far_field=15.0, block_iters=10, max_iters=60,
+                               forces_tol=None, max_in_flight=5, reserve_bytes=0)
+    return reqs, cfg, {r.rid: serve.request_grid(r, cfg) for r in reqs}
+
+
+@pytest.mark.parametrize("driver", ["threads", "async"])
+def test_served_results_are_the_same_alone_and_in_the_mix(driver):
+    """Five requests of two grid sizes, all arriving at once, served five
+    in flight (their blocks overlap on the device, each on its own stream)
+    and then one at a time through the plain path: each request ends with the same iterations,
+    reason and forces, bit for bit."""
+    from dataclasses import replace
+    print(f"[gpu] served mix vs alone, {driver}", flush=True)
+    reqs, cfg, atlases = _small_stream()
+    cfg = replace(cfg, driver=driver)
+    free0, total = mem_info()
+    assert 0 < free0 <= total
+    mixed = {o.rid: o for o in serve.serve(reqs, atlases, cfg)}
+    alone = {o.rid: o for o in serve.serve_serial(reqs, atlases, cfg)}
+    assert sorted(mixed) == sorted(alone) == [r.rid for r in reqs]
+    for rid in mixed:
+        a, m = alone[rid], mixed[rid]
+        assert (a.iterations, a.reason, a.cells) == (m.iterations, m.reason, m.cells), rid
+        assert (a.cl, a.cd, a.cm) == (m.cl, m.cd, m.cm), (rid, a, m)
+        assert np.isfinite(a.cl) and a.iterations == cfg.max_iters
