@@ -302,7 +302,7 @@ def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iter
             out[label]["solves_per_min"] / out[modes[0]]["solves_per_min"], 3)
     out["requests"] = [{"rid": r.rid, "cells": list(r.cells), "mach": round(r.mach, 4), "alpha": round(r.alpha_deg, 3),
                         "re": r.reynolds, **{lab: [by_id[r.rid].iterations, by_id[r.rid].reason,
-                                                   round(by_id[r.rid].cl, 7), round(by_id[r.rid].latency, 2)]
+                                                   round(by_id[r.rid].cl, 10), round(by_id[r.rid].cd, 10), round(by_id[r.rid].latency, 2)]
                                              for lab, (by_id, _) in runs.items()}} for r in reqs]
     return out
 
