@@ -56,6 +56,44 @@ typedef struct {
   const double* wall_mid;      /* [2][n_walls] */
 } jr_grid_desc;
 
+/* The 3D atlas (src/jaxrans/mesh/atlas3d.py:119-287), host arrays.  Canvas
+ * cell (i, j, k) at flat index (i NJ + j) NK + k; the faces of family a lie
+ * on the cell grid with one more along a, flat in the same order; chain
+ * links and the composed ghost fill index the families concatenated
+ * (i | j | k).  The fill is given composed: ghost g takes the primitives of
+ * interior cell ghost_root[g] through boundary tasks ghost_ops[g][0 ..
+ * ghost_nops[g]) in order (the reference's staged fill, evaluated per ghost). */
+typedef struct {
+  int32_t NI, NJ, NK;
+  const double* volume;        /* [NI*NJ*NK] */
+  const double* face[3];       /* family a: [3][nf_a] area vectors */
+  const double* centres;       /* [3][(NI+2)*(NJ+2)*(NK+2)] one-ghost edge-padded cell centres */
+  const double* distance;      /* [NI*NJ*NK] wall distance */
+  const uint8_t* interior;     /* [NI*NJ*NK] */
+  const int64_t* neighbours;   /* [NI*NJ*NK] */
+  int32_t n_ghost;
+  const int64_t* ghost_dest;
+  const int64_t* ghost_root;
+  const int64_t* ghost_nops;
+  const int64_t* ghost_ops;    /* [n_ghost][4] task ids, -1 unused */
+  int32_t n_tasks;
+  const int32_t* task_kind;    /* as jr_grid_desc */
+  const double* task_normal;   /* [3][n_tasks] outward unit normals */
+  int32_t n_lines, n_slots;
+  const int64_t* line_start;   /* [n_lines+1] */
+  const int64_t* line_cells;   /* [n_slots] */
+  const int64_t* line_up_face; /* concatenated face index */
+  const int64_t* line_up_sign;
+  const int64_t* line_down_face;
+  const int64_t* line_down_sign;
+  int32_t n_walls;
+  const int64_t* wall_cell;
+  const int64_t* wall_family;  /* normal axis 0, 1, 2 */
+  const int64_t* wall_face;    /* flat index in that family */
+  const double* wall_into;
+  const double* wall_mid;      /* [3][n_walls] */
+} jr_grid_desc3;
+
 /* Discretization and solver settings (src/jaxrans/core/config.py:112-212). */
 typedef struct {
   double gamma;
@@ -80,9 +118,17 @@ typedef struct {
 /* Context (replaces make_residual + make_newton_step set-up,
  * src/jaxrans/numerics/residual.py:273-334, solvers/implicit.py:84-98). */
 int jr_ctx_create(const jr_grid_desc* grid, const jr_numerics* num, int n_cases, jr_ctx** out);
+
+/* A context on a 3D canvas (the reference builds the same operators from a 3D
+ * atlas, src/jaxrans/numerics/residual.py:273-334 with mesh/atlas3d.py).
+ * Every entry point below works on it, fields [n_cases][nvar][NI][NJ][NK]
+ * with nvar 5 (rho, rho u, rho v, rho w, E) or 6 (+ rho nu_hat); the line
+ * preconditioner solves its chains by block Thomas (5x5 flow blocks). */
+int jr_ctx_create3(const jr_grid_desc3* grid, const jr_numerics* num, int n_cases, jr_ctx** out);
 int jr_ctx_destroy(jr_ctx* ctx);
 
-/* Per-case freestream primitives q_inf[n_cases][5] (rho, u, v, p, nu_hat ratio)
+/* Per-case freestream primitives q_inf[n_cases][5] (rho, u, v, p, nu_hat ratio;
+ * 3D: q_inf[n_cases][6], rho, u, v, w, p, nu_hat ratio)
  * and mu_inf[n_cases] (NULL: inviscid) (src/jaxrans/core/state.py:176-192). */
 int jr_set_freestream(jr_ctx* ctx, const double* q_inf, const double* mu_inf);
 
@@ -164,6 +210,13 @@ int jr_set_cfl(jr_ctx* ctx, const double* cfl);
 /* Forces of the device state (force_coefficients, src/jaxrans/post/forces.py:108-161):
  * out[n_cases][8] = cl, cd, cm, cd_friction, fx, fy, 0, 0. */
 int jr_forces(jr_ctx* ctx, double length, double centre_x, double centre_y, double* out_host);
+
+/* Forces of a 3D context (force_coefficients, src/jaxrans/post/forces.py:108-161):
+ * per q_inf * area; lift normal to the freestream in the x-z plane; pitching
+ * moment about the y axis through (centre_x, centre_z), pressure and
+ * friction, per q_inf * area * length.  out[n_cases][8] = cl, cd, cm,
+ * cd_friction, fx, fy, fz, cy. */
+int jr_forces3(jr_ctx* ctx, double area, double length, double centre_x, double centre_z, double* out_host);
 
 /* Average device time of one launch of a hot kernel on the device state,
  * timed with HIP events on the context stream: 0 residual (all launches),

@@ -1,0 +1,951 @@
+// The 3D path: residual and tangent, wave speeds, line preconditioner
+// (block Thomas on 5x5 flow blocks + the scalar SA chain), relaxation,
+// accept test and forces on a 3D canvas (src/jaxrans/mesh/atlas3d.py layout:
+// flat cell (i NJ + j) NK + k, faces of family a on the cell grid with one
+// more along a, concatenated i | j | k).  Same algorithms as the 2D kernels,
+// restated for three face families and three velocity components.
+#pragma once
+#include "jr_krylov.h"
+#include "jr_physics3d.h"
+
+namespace jr {
+
+constexpr int MAX_OPS = 4;     // boundary conditions composed by one ghost cell (corners: 3)
+
+struct Plan3 {
+  int NI, NJ, NK, nc;
+  int nf[3], nft;                  // faces per family, total
+  const double* vol;               // [nc]
+  const double* S[3];              // family a: [3][nf[a]] area vectors
+  const double* cen;               // [3][(NI+2)(NJ+2)(NK+2)] centres, edge-padded by one
+  const double* dist;              // [nc]
+  const uint8_t* interior;         // [nc]
+  const uint8_t* ghost;            // [nc] written by the fill
+  const int* nbr;                  // [nc] face-neighbour map
+  int n_ghost;                     // composed fill: ghost g = ops of its root's primitives
+  const int* g_dest;
+  const int* g_root;
+  const int* g_nops;
+  const int* g_ops;                // [n_ghost][MAX_OPS] task ids
+  const int* t_kind;               // [n_tasks]
+  const double* t_norm;            // [3][n_tasks]
+  int n_tasks;
+  int n_lines, n_slots, max_len;
+  const int* l_start;              // [n_lines + 1]
+  const int* l_cell;               // [n_slots]
+  const int* l_up;                 // concatenated face index of the link to the next slot
+  const int8_t* l_ups;
+  const int* l_dn;
+  const int8_t* l_dns;
+  int n_free;
+  const int* free_cell;
+  int n_walls;
+  const int* w_cell;
+  const int* w_fam;
+  const int* w_face;
+  const double* w_into;
+  const double* w_mid;             // [3][n_walls]
+};
+
+__device__ __forceinline__ void face_vec3(const Plan3& P, int f, double* s) {
+  int a = 0;
+  if (f >= P.nf[0]) { f -= P.nf[0]; a = 1; if (f >= P.nf[1]) { f -= P.nf[1]; a = 2; } }
+  const double* S = P.S[a];
+  s[0] = S[f]; s[1] = S[(size_t)P.nf[a] + f]; s[2] = S[(size_t)2 * P.nf[a] + f];
+}
+
+// --- residual ----------------------------------------------------------------
+
+// Primitives with the ghost fill in one launch (residual.py:312-322): threads
+// [0, nc) convert the cells no stage writes; thread nc + g evaluates ghost g
+// from its root cell through its boundary conditions in order.
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_fill3(Plan3 P, double gamma, const Ctl* __restrict__ ctl,
+                                                 const double* __restrict__ U, const double* __restrict__ dU,
+                                                 double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
+  int t = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  int src, dst, g = -1;
+  if (t < nc) {
+    if (P.ghost[t]) return;
+    src = dst = t;
+  } else {
+    g = t - nc;
+    if (g >= P.n_ghost) return;
+    src = P.g_root[g];
+    dst = P.g_dest[g];
+  }
+  T u[NV], q[NV];
+  for (int k = 0; k < NV; ++k) u[k] = Io<T>::ld(U, dU, base + (size_t)k * nc + src);
+  primitive3<T, NV>(u, gamma, q);
+  if (g >= 0) {
+    const int nops = P.g_nops[g];
+    for (int o = 0; o < nops; ++o) {
+      const int task = P.g_ops[(size_t)g * MAX_OPS + o];
+      const double n[3] = {P.t_norm[task], P.t_norm[(size_t)P.n_tasks + task], P.t_norm[(size_t)2 * P.n_tasks + task]};
+      T gq[NV];
+      bc_state3<T, NV>(P.t_kind[task], q, n, ctl[cs].q_inf, gamma, gq);
+      for (int k = 0; k < NV; ++k) q[k] = gq[k];
+    }
+  }
+  for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc + dst, q[k]);
+}
+
+template <class T, int NV>
+__device__ __forceinline__ void load_q3(const double* Q, const double* dQ, size_t base, int nc, int c, T* q) {
+  for (int k = 0; k < NV; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + c);
+}
+
+// Padded-centre index of canvas cell (i, j, k).
+__device__ __forceinline__ size_t pc3(const Plan3& P, int i, int j, int k) {
+  return ((size_t)(i + 1) * (P.NJ + 2) + (j + 1)) * (P.NK + 2) + (k + 1);
+}
+
+// Mean of the 8 values round a node, halving along i, then j, then k
+// (src/jaxrans/numerics/gradients.py:31-40).
+template <class T>
+__device__ __forceinline__ T node_mean(const T* v) {   // v[(di*2 + dj)*2 + dk]
+  T a00 = 0.5 * (v[0] + v[4]), a01 = 0.5 * (v[1] + v[5]), a10 = 0.5 * (v[2] + v[6]), a11 = 0.5 * (v[3] + v[7]);
+  T b0 = 0.5 * (a00 + a10), b1 = 0.5 * (a01 + a11);
+  return 0.5 * (b0 + b1);
+}
+
+// Face fluxes of the three families, one thread per face (concatenated
+// numbering).  Faces not adjacent to an interior cell are zero.
+template <class T, int NV, bool VISC>
+__global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
+                                                    const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                    double* __restrict__ F, int check_active) {
+  int f = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (f >= P.nft) return;
+  const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
+  int a = 0, ff = f;
+  if (ff >= P.nf[0]) { ff -= P.nf[0]; a = 1; if (ff >= P.nf[1]) { ff -= P.nf[1]; a = 2; } }
+  const int dj = NJ + (a == 1), dk = NK + (a == 2);
+  const int fi = ff / (dj * dk), rem = ff - fi * dj * dk, fj = rem / dk, fk = rem - fj * dk;
+  const int idx[3] = {fi, fj, fk};
+  const int ext[3] = {NI, NJ, NK};
+  const int stride[3] = {NJ * NK, NK, 1};
+  const size_t fbase = (size_t)cs * NV * P.nft + f;
+  const bool inside = idx[a] >= 1 && idx[a] <= ext[a] - 1;
+  int cR = 0, cL = 0;
+  if (inside) {
+    cR = (fi * NJ + fj) * NK + fk;
+    cL = cR - stride[a];
+  }
+  if (!inside || !(P.interior[cL] || P.interior[cR])) {
+    for (int k = 0; k < NV; ++k) F[fbase + (size_t)k * P.nft] = 0.0;
+    return;
+  }
+  const double s[3] = {P.S[a][ff], P.S[a][(size_t)P.nf[a] + ff], P.S[a][(size_t)2 * P.nf[a] + ff]};
+  const size_t base = (size_t)cs * NV * nc;
+  const double gamma = ph.gamma;
+  const int dL = stride[a];
+  T qm[NV], qp[NV];
+  load_q3<T, NV>(Q, dQ, base, nc, cL, qm);
+  load_q3<T, NV>(Q, dQ, base, nc, cR, qp);
+  T qL[5], qR[5];
+  if (first_order) {
+    for (int k = 0; k < 5; ++k) { qL[k] = qm[k]; qR[k] = qp[k]; }
+  } else {
+    T qmm[NV], qpp[NV];
+    load_q3<T, NV>(Q, dQ, base, nc, cL - dL, qmm);
+    load_q3<T, NV>(Q, dQ, base, nc, cR + dL, qpp);
+    if (ph.kappa != 0.0) {
+      double lo = 0.25 * (1.0 - ph.kappa), hi = 0.25 * (1.0 + ph.kappa);
+      for (int k = 0; k < 5; ++k) {
+        qL[k] = qm[k] + lo * (qm[k] - qmm[k]) + hi * (qp[k] - qm[k]);
+        qR[k] = qp[k] - lo * (qpp[k] - qp[k]) - hi * (qp[k] - qm[k]);
+      }
+    } else {
+      for (int k = 0; k < 5; ++k) {
+        qL[k] = qm[k] + 0.5 * limit<T>(ph.limiter, qm[k] - qmm[k], qp[k] - qm[k]);
+        qR[k] = qp[k] - 0.5 * limit<T>(ph.limiter, qp[k] - qm[k], qpp[k] - qp[k]);
+      }
+    }
+  }
+  T Fl[NV];
+  if (ph.flux == FLUX_ROE) roe3<T>(qL, qR, s, gamma, Fl);
+  else hllc3<T>(qL, qR, s, gamma, Fl);
+  if constexpr (NV > 5) Fl[5] = lift<T>(0.0);
+
+  if (VISC) {
+    const double mu_inf = ctl[cs].mu_inf;
+    const int iL = cL / (NJ * NK), jL = (cL / NK) % NJ, kL = cL % NK;
+    // the diamond: the cell line and the face's two node diagonals; node
+    // (ni, nj, nk) is the mean of canvas cells (ni-1..ni, nj-1..nj, nk-1..nk)
+    double d1[3], d2[3], d3[3];
+    const size_t cstride = (size_t)(NI + 2) * (NJ + 2) * (NK + 2);
+    for (int c = 0; c < 3; ++c)
+      d1[c] = P.cen[c * cstride + pc3(P, fi, fj, fk)] - P.cen[c * cstride + pc3(P, iL, jL, kL)];
+    // the face's running axes in ascending order; shifts (s1, s2) along them
+    const int r1 = a == 0 ? 1 : 0, r2 = a == 2 ? 1 : 2;
+    auto node_at = [&](int s1, int s2, int* nd) {
+      nd[0] = fi; nd[1] = fj; nd[2] = fk;
+      nd[r1] += s1;
+      nd[r2] += s2;
+    };
+    auto node_centre = [&](const int* nd, int c) -> double {
+      double v[8];
+      for (int di = 0; di < 2; ++di)
+        for (int dj2 = 0; dj2 < 2; ++dj2)
+          for (int dk2 = 0; dk2 < 2; ++dk2)
+            v[(di * 2 + dj2) * 2 + dk2] = P.cen[c * cstride + pc3(P, nd[0] - 1 + di, nd[1] - 1 + dj2, nd[2] - 1 + dk2)];
+      return node_mean<double>(v);
+    };
+    int n00[3], n11[3], n10[3], n01[3];
+    node_at(0, 0, n00); node_at(1, 1, n11); node_at(1, 0, n10); node_at(0, 1, n01);
+    for (int c = 0; c < 3; ++c) {
+      d2[c] = node_centre(n11, c) - node_centre(n00, c);
+      d3[c] = node_centre(n01, c) - node_centre(n10, c);
+    }
+    double c23[3] = {d2[1] * d3[2] - d2[2] * d3[1], d2[2] * d3[0] - d2[0] * d3[2], d2[0] * d3[1] - d2[1] * d3[0]};
+    double c31[3] = {d3[1] * d1[2] - d3[2] * d1[1], d3[2] * d1[0] - d3[0] * d1[2], d3[0] * d1[1] - d3[1] * d1[0]};
+    double c12[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
+    const double det = (d1[0] * c23[0] + d1[1] * c23[1]) + d1[2] * c23[2];
+    // phi = (u, v, w, T[, nu_hat])
+    constexpr int NPHI = NV > 5 ? 5 : 4;
+    auto phi_of = [&](const T* q, T* out) {
+      out[0] = q[1]; out[1] = q[2]; out[2] = q[3]; out[3] = temperature3(q, gamma);
+      if constexpr (NPHI > 4) out[4] = q[5];
+    };
+    auto node_phi = [&](const int* nd, T* out) {
+      T v[8][NPHI];
+      for (int di = 0; di < 2; ++di)
+        for (int dj2 = 0; dj2 < 2; ++dj2)
+          for (int dk2 = 0; dk2 < 2; ++dk2) {
+            T q[NV];
+            load_q3<T, NV>(Q, dQ, base, nc, ((nd[0] - 1 + di) * NJ + nd[1] - 1 + dj2) * NK + nd[2] - 1 + dk2, q);
+            phi_of(q, v[(di * 2 + dj2) * 2 + dk2]);
+          }
+      for (int k = 0; k < NPHI; ++k) {
+        T w[8];
+        for (int e = 0; e < 8; ++e) w[e] = v[e][k];
+        out[k] = node_mean<T>(w);
+      }
+    };
+    T p00[NPHI], p11[NPHI], p10[NPHI], p01[NPHI], pl[NPHI], pr[NPHI];
+    node_phi(n00, p00); node_phi(n11, p11); node_phi(n10, p10); node_phi(n01, p01);
+    phi_of(qm, pl);
+    phi_of(qp, pr);
+    T gr[3][NPHI];                      // gr[b][k] = d phi_k / d x_b
+    for (int k = 0; k < NPHI; ++k) {
+      T dp1 = pr[k] - pl[k], dp2 = p11[k] - p00[k], dp3 = p01[k] - p10[k];
+      for (int b = 0; b < 3; ++b) gr[b][k] = (dp1 * c23[b] + dp2 * c31[b] + dp3 * c12[b]) / det;
+    }
+    T gv[3][3], gT[3];
+    for (int c = 0; c < 3; ++c) {
+      for (int b = 0; b < 3; ++b) gv[c][b] = gr[b][c];
+      gT[c] = gr[c][3];
+    }
+    T vel[3] = {0.5 * (qm[1] + qp[1]), 0.5 * (qm[2] + qp[2]), 0.5 * (qm[3] + qp[3])};
+    T Tf = 0.5 * (temperature3(qm, gamma) + temperature3(qp, gamma));
+    T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
+    T kc = mu / (PR * (gamma - 1.0));
+    T extra[3][3];
+    bool has_extra = false;
+    if constexpr (NV > 5) {
+      T face[6];
+      for (int k = 0; k < 6; ++k) face[k] = 0.5 * (qm[k] + qp[k]);
+      T muf = viscosity<T>(temperature3(face, gamma), mu_inf, ph.sutherland);
+      T mut = eddy_viscosity<T>(face[0], face[5] * mu_inf, muf / face[0]);
+      mu = mu + mut;
+      kc = kc + mut / (PR_T * (gamma - 1.0));
+      if (ph.c_cr1 != 0.0) { qcr_stress3<T>(gv, mut, ph.c_cr1, extra); has_extra = true; }
+      T up = val(Fl[0]) > 0.0 ? qm[5] : qp[5];
+      T coef = diffusion_coefficient<T>(face[0], face[5] * mu_inf, muf);
+      T gn[3] = {gr[0][4], gr[1][4], gr[2][4]};
+      Fl[5] = Fl[0] * up - coef * dot3(gn, s);
+    }
+    T fv[5];
+    viscous_flux3<T>(vel, gv, gT, s, mu, kc, has_extra ? extra : nullptr, fv);
+    for (int k = 0; k < 5; ++k) Fl[k] = Fl[k] - fv[k];
+  }
+  for (int k = 0; k < NV; ++k) {
+    if (sizeof(T) == sizeof(double)) F[fbase + (size_t)k * P.nft] = val(Fl[k]);
+    else F[fbase + (size_t)k * P.nft] = tan_of(Fl[k]);
+  }
+}
+
+// Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere;
+// tangent mode: out = vdt dU + dR.
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                    const double* __restrict__ U, const double* __restrict__ dU,
+                                                    const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                    const double* __restrict__ F, const double* __restrict__ vdt,
+                                                    double* __restrict__ out, int check_active) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ, NK = P.NK, nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  const bool tangent = sizeof(T) != sizeof(double);
+  const double vd = vdt ? vdt[(size_t)cs * nc + c] : 0.0;
+  if (!P.interior[c]) {
+    if (!tangent) {
+      double u0[6];
+      conservative3<double, NV>(ctl[cs].q_inf, ph.gamma, u0);
+      for (int k = 0; k < NV; ++k) out[base + (size_t)k * nc + c] = U[base + (size_t)k * nc + c] - u0[k];
+    } else {
+      for (int k = 0; k < NV; ++k) {
+        double d = dU[base + (size_t)k * nc + c];
+        out[base + (size_t)k * nc + c] = vdt ? vd * d + d : d;
+      }
+    }
+    return;
+  }
+  const int i = c / (NJ * NK), j = (c / NK) % NJ, k = c % NK;
+  // the cell's low and high face of each family (concatenated numbering)
+  const int lo0 = (i * NJ + j) * NK + k, hi0 = lo0 + NJ * NK;
+  const int lo1 = P.nf[0] + (i * (NJ + 1) + j) * NK + k, hi1 = lo1 + NK;
+  const int lo2 = P.nf[0] + P.nf[1] + (i * NJ + j) * (NK + 1) + k, hi2 = lo2 + 1;
+  const size_t fb = (size_t)cs * NV * P.nft;
+  double r[NV];
+  for (int v = 0; v < NV; ++v) {
+    const double* Fv = F + fb + (size_t)v * P.nft;
+    r[v] = ((Fv[hi0] - Fv[lo0]) + (Fv[hi1] - Fv[lo1])) + (Fv[hi2] - Fv[lo2]);
+  }
+  if constexpr (NV > 5) {
+    // SA source (turbulence.py:140-151): vorticity and grad nu_hat by
+    // Green-Gauss over the face averages with the six neighbours
+    const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
+    const int nb[6] = {c - NJ * NK, c + NJ * NK, c - NK, c + NK, c - 1, c + 1};
+    const int fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
+    T qc[NV];
+    load_q3<T, NV>(Q, dQ, base, nc, c, qc);
+    T fa[6][4];                       // face averages of (u, v, w, nu_hat)
+    for (int e = 0; e < 6; ++e) {
+      T qn[NV];
+      load_q3<T, NV>(Q, dQ, base, nc, nb[e], qn);
+      const bool low = (e & 1) == 0;
+      for (int m = 0; m < 3; ++m) fa[e][m] = low ? 0.5 * (qn[1 + m] + qc[1 + m]) : 0.5 * (qc[1 + m] + qn[1 + m]);
+      fa[e][3] = low ? 0.5 * (qn[5] + qc[5]) : 0.5 * (qc[5] + qn[5]);
+    }
+    double sv[6][3];
+    for (int e = 0; e < 6; ++e) face_vec3(P, fl[e], sv[e]);
+    const double V = P.vol[c];
+    auto gg = [&](int m, int b) -> T {
+      return (((fa[1][m] * sv[1][b] - fa[0][m] * sv[0][b]) + (fa[3][m] * sv[3][b] - fa[2][m] * sv[2][b])) +
+              (fa[5][m] * sv[5][b] - fa[4][m] * sv[4][b])) / V;
+    };
+    T g[3][3];                        // g[b][m] = d v_m / d x_b
+    for (int m = 0; m < 3; ++m)
+      for (int b = 0; b < 3; ++b) g[b][m] = gg(m, b);
+    T omega = vorticity3<T>(g);
+    T gn[3] = {gg(3, 0), gg(3, 1), gg(3, 2)};
+    T rho = qc[0];
+    T mu = viscosity<T>(temperature3(qc, gamma), nu_inf, ph.sutherland);
+    T prod, destr;
+    sa_source<T>(qc[5] * nu_inf, mu / rho, omega, P.dist[c], prod, destr);
+    T src = rho * ((prod - destr) / nu_inf + CB2 / SIGMA * nu_inf * dot3(gn, gn));
+    T svv = src * V;
+    r[5] = r[5] - (tangent ? tan_of(svv) : val(svv));
+  }
+  for (int v = 0; v < NV; ++v) {
+    if (tangent && vdt) r[v] = vd * dU[base + (size_t)v * nc + c] + r[v];
+    out[base + (size_t)v * nc + c] = r[v];
+  }
+}
+
+// --- wave speeds ------------------------------------------------------------
+
+template <int NV>
+__device__ __forceinline__ void prim_at3(const double* U, size_t base, int nc, int c, double gamma, double* q) {
+  double u[NV];
+  for (int k = 0; k < NV; ++k) u[k] = U[base + (size_t)k * nc + c];
+  primitive3<double, NV>(u, gamma, q);
+}
+
+// Face lambda (residual.py:169-192) from neighbour-filled face averages; SA:
+// the face mass flux and diffusion of the scalar chains.
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_face_speeds3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                        const double* __restrict__ U, double* __restrict__ lam,
+                                                        double* __restrict__ sam, double* __restrict__ sad) {
+  const int f = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (f >= P.nft) return;
+  const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
+  int a = 0, ff = f;
+  if (ff >= P.nf[0]) { ff -= P.nf[0]; a = 1; if (ff >= P.nf[1]) { ff -= P.nf[1]; a = 2; } }
+  const int dj = NJ + (a == 1), dk = NK + (a == 2);
+  int idx[3];
+  idx[0] = ff / (dj * dk);
+  const int rem = ff - idx[0] * dj * dk;
+  idx[1] = rem / dk;
+  idx[2] = rem - idx[1] * dk;
+  const int ext[3] = {NI, NJ, NK};
+  int lo[3] = {idx[0], idx[1], idx[2]}, hi[3] = {idx[0], idx[1], idx[2]};
+  lo[a] = idx[a] > 0 ? idx[a] - 1 : 0;                  // edge padding at the canvas ends
+  hi[a] = idx[a] < ext[a] ? idx[a] : ext[a] - 1;
+  const int c1 = P.nbr[(lo[0] * NJ + lo[1]) * NK + lo[2]];
+  const int c2 = P.nbr[(hi[0] * NJ + hi[1]) * NK + hi[2]];
+  double s[3];
+  face_vec3(P, f, s);
+  const size_t base = (size_t)cs * NV * nc;
+  const double gamma = ph.gamma;
+  double q1[NV], q2[NV], qf[NV];
+  prim_at3<NV>(U, base, nc, c1, gamma, q1);
+  prim_at3<NV>(U, base, nc, c2, gamma, q2);
+  for (int k = 0; k < NV; ++k) qf[k] = 0.5 * (q1[k] + q2[k]);
+  const double ss = dot3(s, s);
+  double l = fabs(dot3(qf + 1, s)) + sqrt(gamma * qf[4] / qf[0]) * sqrt(ss);
+  const size_t fo = (size_t)cs * P.nft + f;
+  if (ph.viscous) {
+    const double mu_inf = ctl[cs].mu_inf;
+    const double vol = 0.5 * (P.vol[c1] + P.vol[c2]);
+    double mu = viscosity<double>(gamma * qf[4] / qf[0], mu_inf, ph.sutherland);
+    const double mul = mu;
+    if constexpr (NV > 5) mu = mu + eddy_viscosity<double>(qf[0], qf[5] * mu_inf, mu / qf[0]);
+    l = l + 2.0 * gamma / PR * mu * ss / (qf[0] * vol);
+    if constexpr (NV > 5) {
+      const double kap = diffusion_coefficient<double>(qf[0], qf[5] * mu_inf, mul);
+      sam[fo] = qf[0] * dot3(qf + 1, s);
+      sad[fo] = kap * ss / vol;
+    }
+  }
+  lam[fo] = l;
+}
+
+// Per cell: sum of face wave speeds, V/dt, and the SA chain diagonal
+// (preconditioners.py:259-275).
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_cell_speeds3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                        const double* __restrict__ U, const double* __restrict__ lam,
+                                                        const double* __restrict__ sam, const double* __restrict__ sad,
+                                                        double* __restrict__ lsum, double* __restrict__ vdt,
+                                                        double* __restrict__ sa_diag) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ, NK = P.NK, nc = P.nc;
+  const int i = c / (NJ * NK), j = (c / NK) % NJ, k = c % NK;
+  const int lo0 = (i * NJ + j) * NK + k, hi0 = lo0 + NJ * NK;
+  const int lo1 = P.nf[0] + (i * (NJ + 1) + j) * NK + k, hi1 = lo1 + NK;
+  const int lo2 = P.nf[0] + P.nf[1] + (i * NJ + j) * (NK + 1) + k, hi2 = lo2 + 1;
+  const double* L = lam + (size_t)cs * P.nft;
+  const double s = ((((L[lo0] + L[hi0]) + L[lo1]) + L[hi1]) + L[lo2]) + L[hi2];
+  const double cfl = ctl[cs].cfl;
+  lsum[(size_t)cs * nc + c] = s;
+  vdt[(size_t)cs * nc + c] = s / cfl;
+  if constexpr (NV > 5) {
+    const double* m = sam + (size_t)cs * P.nft;
+    const double* d = sad + (size_t)cs * P.nft;
+    const double gamma = ph.gamma, mu_inf = ctl[cs].mu_inf;
+    double diag = (fmax(m[hi0], 0.0) + d[hi0]) + (fmax(-m[lo0], 0.0) + d[lo0]);
+    diag = diag + (fmax(m[hi1], 0.0) + d[hi1]) + (fmax(-m[lo1], 0.0) + d[lo1]);
+    diag = diag + (fmax(m[hi2], 0.0) + d[hi2]) + (fmax(-m[lo2], 0.0) + d[lo2]);
+    const size_t base = (size_t)cs * NV * nc;
+    double q[NV];
+    prim_at3<NV>(U, base, nc, c, gamma, q);
+    diag = diag / q[0];
+    double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (P.interior[c]) {
+      const int nb[6] = {c - NJ * NK, c + NJ * NK, c - NK, c + NK, c - 1, c + 1};
+      const int fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
+      double fa[6][3], sv[6][3];
+      for (int e = 0; e < 6; ++e) {
+        double qn[NV];
+        prim_at3<NV>(U, base, nc, P.nbr[nb[e]], gamma, qn);
+        const bool low = (e & 1) == 0;
+        for (int mm = 0; mm < 3; ++mm) fa[e][mm] = low ? 0.5 * (qn[1 + mm] + q[1 + mm]) : 0.5 * (q[1 + mm] + qn[1 + mm]);
+        face_vec3(P, fl[e], sv[e]);
+      }
+      const double V = P.vol[c];
+      for (int mm = 0; mm < 3; ++mm)
+        for (int b = 0; b < 3; ++b)
+          g[b][mm] = (((fa[1][mm] * sv[1][b] - fa[0][mm] * sv[0][b]) + (fa[3][mm] * sv[3][b] - fa[2][mm] * sv[2][b])) +
+                      (fa[5][mm] * sv[5][b] - fa[4][mm] * sv[4][b])) / V;
+    }
+    const double omega = vorticity3<double>(g);
+    const double nu = viscosity<double>(gamma * q[4] / q[0], mu_inf, ph.sutherland) / q[0];
+    const double dw = P.dist[c], nh = q[5];
+    double destr = 0.0;
+    if (nh > 0.0) {
+      const double st = s_tilde<double>(omega, nh * mu_inf, nu, dw);
+      const double fw_ = f_w<double>(r_of<double>(nh * mu_inf, st, dw));
+      destr = 2.0 * CW1 * fw_ * mu_inf * nh / (dw * dw);
+    }
+    sa_diag[(size_t)cs * nc + c] = diag + destr * P.vol[c] + s / cfl;
+  }
+}
+
+// --- line preconditioner: 5x5 block Thomas, one thread per chain -----------
+
+// d(F.S)/dU of the analytic flux by forward differentiation (euler.py:55-71).
+__device__ __forceinline__ void flux_jacobian3(const double* q, const double* s, double gamma, double A[5][5]) {
+  double U[5];
+  conservative3<double, 5>(q, gamma, U);
+  for (int k = 0; k < 5; ++k) {
+    Dual Ud[5], qd[5], f[5];
+    for (int c = 0; c < 5; ++c) Ud[c] = mkd(U[c], c == k ? 1.0 : 0.0);
+    primitive3<Dual, 5>(Ud, gamma, qd);
+    normal_flux3<Dual>(qd, s, gamma, f);
+    for (int r = 0; r < 5; ++r) A[r][k] = f[r].d;
+  }
+}
+
+// blk3[slot] = [d, Lo(25), Up(25), sa_lo, sa_d, sa_up]
+constexpr int BLKW3 = 54;
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                   const double* __restrict__ U, const double* __restrict__ lam,
+                                                   const double* __restrict__ lsum, const double* __restrict__ sam,
+                                                   const double* __restrict__ sad, const double* __restrict__ sa_diag,
+                                                   double* __restrict__ blk) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (s >= P.n_slots) return;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  const double gamma = ph.gamma, cfl = ctl[cs].cfl;
+  const double* L = lam + (size_t)cs * P.nft;
+  const int cell = P.l_cell[s];
+  double* o = blk + ((size_t)cs * P.n_slots + s) * BLKW3;
+  o[0] = lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5);
+  auto link = [&](int nb_cell, int fce, double sg, double* dst) {
+    double sv[3];
+    face_vec3(P, fce, sv);
+    for (int c = 0; c < 3; ++c) sv[c] *= sg;
+    double q[5];
+    prim_at3<5>(U, base, nc, nb_cell, gamma, q);
+    double A[5][5];
+    flux_jacobian3(q, sv, gamma, A);
+    for (int r = 0; r < 5; ++r)
+      for (int c = 0; c < 5; ++c) dst[r * 5 + c] = 0.5 * A[r][c] - (r == c ? 0.5 * L[fce] : 0.0);
+  };
+  const bool has_lo = s > 0 && P.l_dns[s - 1] != 0;
+  const bool has_up = P.l_ups[s] != 0;
+  if (has_lo) link(P.l_cell[s - 1], P.l_dn[s - 1], (double)P.l_dns[s - 1], o + 1);
+  else for (int e = 0; e < 25; ++e) o[1 + e] = 0.0;
+  if (has_up) link(P.l_cell[s + 1], P.l_up[s], (double)P.l_ups[s], o + 26);
+  else for (int e = 0; e < 25; ++e) o[26 + e] = 0.0;
+  if constexpr (NV > 5) {
+    const double* mj = sam + (size_t)cs * P.nft;
+    const double* dj = sad + (size_t)cs * P.nft;
+    double lo = 0.0, up = 0.0;
+    if (has_lo) {
+      const int fce = P.l_dn[s - 1];
+      const double sg = P.l_dns[s - 1];
+      lo = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s - 1]];
+    }
+    if (has_up) {
+      const int fce = P.l_up[s];
+      const double sg = P.l_ups[s];
+      up = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s + 1]];
+    }
+    o[51] = lo;
+    o[52] = sa_diag[(size_t)cs * nc + cell];
+    o[53] = up;
+  }
+}
+
+// Unpivoted Gauss-Jordan inverse of a 5x5 block (linalg.py:49-77).
+template <class R>
+__device__ __forceinline__ void invert5(R A[5][5], R X[5][5]) {
+  R W[5][10];
+  for (int r = 0; r < 5; ++r)
+    for (int c = 0; c < 5; ++c) { W[r][c] = A[r][c]; W[r][5 + c] = r == c ? R(1) : R(0); }
+  for (int k = 0; k < 5; ++k) {
+    const R piv = W[k][k];
+    for (int c = 0; c < 10; ++c) W[k][c] = W[k][c] / piv;
+    for (int r = 0; r < 5; ++r) {
+      if (r == k) continue;
+      const R f = W[r][k];
+      for (int c = 0; c < 10; ++c) W[r][c] = W[r][c] - f * W[k][c];
+    }
+  }
+  for (int r = 0; r < 5; ++r)
+    for (int c = 0; c < 5; ++c) X[r][c] = W[r][5 + c];
+}
+
+// Factor layout (per case): value v of chain slot position k of chain ch at
+// [(k * 75 + v) * n_lines + ch] (L 0..24, P^-1 25..49, C 50..74), so the
+// threads of a wave (consecutive chains) read consecutive addresses;
+// SA: [(k * 3 + v) * n_lines + ch] (l, 1/p, c).
+template <class R, int NV>
+__global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restrict__ blk, R* __restrict__ fac,
+                                                R* __restrict__ fac_sa) {
+  const int ch = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
+  if (ch >= P.n_lines) return;
+  const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0, nl = P.n_lines;
+  const double* B = blk + (size_t)cs * P.n_slots * BLKW3;
+  R* F = fac + (size_t)cs * P.max_len * 75 * nl;
+  R* Fs = fac_sa + (size_t)cs * P.max_len * 3 * nl;
+  R Cp[5][5];
+  for (int r = 0; r < 5; ++r)
+    for (int q = 0; q < 5; ++q) Cp[r][q] = R(0);
+  R cps = R(0);
+  for (int k = 0; k < n; ++k) {
+    const double* b = B + (size_t)(s0 + k) * BLKW3;
+    const R d = (R)b[0];
+    R Pm[5][5], Pi[5][5];
+    for (int r = 0; r < 5; ++r)
+      for (int q = 0; q < 5; ++q) {
+        R acc = R(0);
+        for (int t = 0; t < 5; ++t) acc += (R)b[1 + r * 5 + t] * Cp[t][q];
+        Pm[r][q] = (r == q ? d : R(0)) - acc;
+      }
+    invert5<R>(Pm, Pi);
+    R* o = F + (size_t)k * 75 * nl + ch;
+    for (int r = 0; r < 5; ++r)
+      for (int q = 0; q < 5; ++q) {
+        R acc = R(0);
+        for (int t = 0; t < 5; ++t) acc += Pi[r][t] * (R)b[26 + t * 5 + q];
+        o[(size_t)(r * 5 + q) * nl] = (R)b[1 + r * 5 + q];
+        o[(size_t)(25 + r * 5 + q) * nl] = Pi[r][q];
+        o[(size_t)(50 + r * 5 + q) * nl] = acc;
+      }
+    for (int r = 0; r < 5; ++r)
+      for (int q = 0; q < 5; ++q) Cp[r][q] = o[(size_t)(50 + r * 5 + q) * nl];
+    if constexpr (NV > 5) {
+      const R lo = (R)b[51], ds = (R)b[52], up = (R)b[53];
+      const R pinv = R(1) / (ds - lo * cps);
+      const R cn = pinv * up;
+      R* os = Fs + (size_t)k * 3 * nl + ch;
+      os[0] = lo;
+      os[nl] = pinv;
+      os[2 * nl] = cn;
+      cps = cn;
+    }
+  }
+}
+
+// z = M^-1 r (z += M^-1 r when accumulating), one thread per chain; the
+// forward sweep keeps d in z, the backward sweep overwrites it with x
+// (linalg.py:103-132).  With vout the input is the Arnoldi vector w/|w|,
+// also written to vout.  Workgroups past the chains copy the cells on no
+// chain (identity there).
+template <class R, int NV>
+__global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
+                                               const R* __restrict__ fac_sa, const double* __restrict__ r,
+                                               double* __restrict__ z, double* __restrict__ tmp, int accumulate,
+                                               int check_active, double* __restrict__ vout,
+                                               const double* __restrict__ p3, int nblk) {
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  const int nc = P.nc, nl = P.n_lines;
+  const size_t base = (size_t)cs * NV * nc;
+  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
+  const int chain_groups = (nl + 63) / 64;
+  if ((int)blockIdx.x >= chain_groups) {
+    const int e = ((int)blockIdx.x - chain_groups) * 64 + threadIdx.x;
+    if (e >= P.n_free) return;
+    const int cell = P.free_cell[e];
+    for (int v = 0; v < NV; ++v) {
+      const size_t o = base + (size_t)v * nc + cell;
+      const double x = apply_input(r, vout, o, den);
+      z[o] = accumulate ? z[o] + x : x;
+    }
+    return;
+  }
+  const int ch = blockIdx.x * 64 + threadIdx.x;
+  if (ch >= nl) return;
+  const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0;
+  const R* F = fac + (size_t)cs * P.max_len * 75 * nl + ch;
+  const R* Fs = fac_sa + (size_t)cs * P.max_len * 3 * nl + ch;
+  double* D = tmp + base;               // forward-sweep d, per cell of the chain
+  R dp[5] = {R(0), R(0), R(0), R(0), R(0)};
+  R dps = R(0);
+  for (int k = 0; k < n; ++k) {
+    const int cell = P.l_cell[s0 + k];
+    const R* o = F + (size_t)k * 75 * nl;
+    R rhs[5], t[5];
+    for (int v = 0; v < 5; ++v) rhs[v] = (R)apply_input(r, vout, base + (size_t)v * nc + cell, den);
+    for (int i = 0; i < 5; ++i) {
+      R acc = rhs[i];
+      for (int q = 0; q < 5; ++q) acc -= o[(size_t)(i * 5 + q) * nl] * dp[q];
+      t[i] = acc;
+    }
+    for (int i = 0; i < 5; ++i) {
+      R acc = R(0);
+      for (int q = 0; q < 5; ++q) acc += o[(size_t)(25 + i * 5 + q) * nl] * t[q];
+      dp[i] = acc;
+    }
+    for (int i = 0; i < 5; ++i) D[(size_t)i * nc + cell] = (double)dp[i];
+    if constexpr (NV > 5) {
+      const R rs = (R)apply_input(r, vout, base + (size_t)5 * nc + cell, den);
+      const R* os = Fs + (size_t)k * 3 * nl;
+      dps = os[nl] * (rs - os[0] * dps);
+      D[(size_t)5 * nc + cell] = (double)dps;
+    }
+  }
+  R xn[5] = {R(0), R(0), R(0), R(0), R(0)};
+  R xns = R(0);
+  for (int k = n - 1; k >= 0; --k) {
+    const int cell = P.l_cell[s0 + k];
+    const R* o = F + (size_t)k * 75 * nl;
+    R x[5];
+    for (int i = 0; i < 5; ++i) {
+      R acc = (R)D[(size_t)i * nc + cell];
+      for (int q = 0; q < 5; ++q) acc -= o[(size_t)(50 + i * 5 + q) * nl] * xn[q];
+      x[i] = acc;
+    }
+    for (int i = 0; i < 5; ++i) {
+      xn[i] = x[i];
+      const size_t oz = base + (size_t)i * nc + cell;
+      z[oz] = accumulate ? z[oz] + (double)x[i] : (double)x[i];
+    }
+    if constexpr (NV > 5) {
+      const R* os = Fs + (size_t)k * 3 * nl;
+      xns = (R)D[(size_t)5 * nc + cell] - os[2 * nl] * xns;
+      const size_t oz = base + (size_t)5 * nc + cell;
+      z[oz] = accumulate ? z[oz] + (double)xns : (double)xns;
+    }
+  }
+}
+
+// --- Newton update: relaxation, accept test ----------------------------------
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_relax3(Plan3 P, double gamma, Ctl* __restrict__ ctl,
+                                                  const double* __restrict__ U, const double* __restrict__ X,
+                                                  Red red, double fraction) {
+  __shared__ double sh[1][4];
+  __shared__ int is_last;
+  const int cs = blockIdx.y;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  double worst = 0.0;
+  for (int c = blockIdx.x * BLOCK + threadIdx.x; c < nc; c += gridDim.x * BLOCK) {
+    double u[NV], q[NV], d[NV];
+    for (int k = 0; k < NV; ++k) { u[k] = U[base + (size_t)k * nc + c]; d[k] = X[base + (size_t)k * nc + c]; }
+    primitive3<double, NV>(u, gamma, q);
+    double de = d[4];
+    for (int k = 0; k < 3; ++k) de = de - q[1 + k] * d[1 + k];
+    const double dp = (gamma - 1.0) * (de + 0.5 * dot3(q + 1, q + 1) * d[0]);
+    double w = fmax(fabs(d[0]) / q[0], fabs(dp) / q[4]);
+    if constexpr (NV > 5) {
+      const double nh = u[5] / q[0];
+      w = fmax(w, fabs((d[5] - nh * d[0]) / q[0]) / fmax(fabs(nh), 1.0));
+    }
+    worst = fmax(worst, w);
+  }
+  double wv = wave_max(worst);
+  if ((threadIdx.x & 63) == 0) sh[0][threadIdx.x >> 6] = wv;
+  __syncthreads();
+  double* part = red.partials + (size_t)cs * red.stride;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+    __threadfence();
+    unsigned prev = atomicAdd(red.counters + cs * 4, 1u);
+    is_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double m = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK)
+    m = fmax(m, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  m = wave_max(m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[0][threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+    ctl[cs].omega = fmin(1.0, fraction / fmax(m, 1e-300));
+    red.counters[cs * 4] = 0u;
+  }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_check3(Plan3 P, double gamma, Ctl* __restrict__ ctl,
+                                                  const double* __restrict__ U, const double* __restrict__ X,
+                                                  const double* __restrict__ Act, const double* __restrict__ R,
+                                                  Red red, Ramp ramp) {
+  __shared__ double sh[2][4];
+  const int cs = blockIdx.y;
+  const int nc = P.nc;
+  const size_t base = (size_t)cs * NV * nc;
+  const double omega = ramp.relaxed ? ctl[cs].omega : 1.0;
+  double v[2] = {0.0, 0.0};
+  for (int c = blockIdx.x * BLOCK + threadIdx.x; c < nc; c += gridDim.x * BLOCK) {
+    double cand[NV];
+    bool finite = true;
+    for (int k = 0; k < NV; ++k) {
+      const size_t o = base + (size_t)k * nc + c;
+      const double dx = omega * X[o];
+      finite = finite && isfinite(dx);
+      cand[k] = U[o] + dx;
+      const double lr = omega * Act[o] + R[o];
+      v[1] += lr * lr;
+    }
+    double q[NV];
+    primitive3<double, NV>(cand, gamma, q);
+    const bool phys = q[0] > 1e-10 && q[4] > 1e-10 && isfinite(q[0]) && isfinite(q[4]);
+    if (!(phys && finite)) v[0] += 1.0;
+  }
+  __shared__ double tot[2];
+  if (finish_sum<2>(v, red.partials + (size_t)cs * red.stride, red.counters + cs * 4, sh, tot)) {
+    if (threadIdx.x == 0) {
+      Ctl& c = ctl[cs];
+      const double lin_red = sqrt(tot[1]) / fmax(c.bnorm, 1e-300);
+      const bool accept = tot[0] == 0.0 && lin_red < 1.0;
+      const double reduction = c.res_prev / fmax(c.norm, 1e-300);
+      double nxt;
+      if (accept && lin_red < ramp.krylov_success && omega >= 1.0)
+        nxt = c.cfl * fmin(fmax(reduction, 1.0), ramp.cfl_growth);
+      else if (accept)
+        nxt = c.cfl;
+      else
+        nxt = c.cfl / ramp.cfl_cutback;
+      c.cfl = fmin(fmax(nxt, ramp.cfl_min), ramp.cfl_max);
+      if (accept) c.res_prev = c.norm;
+      c.lin_red = lin_red;
+      c.accept = accept;
+      c.steps += 1;
+      if (!accept) c.rejected += 1;
+    }
+  }
+}
+
+// --- forces (forces.py:48-161, 3D) ------------------------------------------
+
+// Laminar viscous force of wall face `fce` (concatenated numbering) from the
+// ghost-filled primitives, the 3D diamond as in k_faces3.
+template <int NV>
+__device__ void wall_shear3(const Plan3& P, const Physics& ph, const double* Q, size_t base, double mu_inf, int fce,
+                            double* out) {
+  const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
+  const double gamma = ph.gamma;
+  int a = 0, ff = fce;
+  if (ff >= P.nf[0]) { ff -= P.nf[0]; a = 1; if (ff >= P.nf[1]) { ff -= P.nf[1]; a = 2; } }
+  const int dj = NJ + (a == 1), dk = NK + (a == 2);
+  const int fi = ff / (dj * dk), rem = ff - fi * dj * dk, fj = rem / dk, fk = rem - fj * dk;
+  const int stride[3] = {NJ * NK, NK, 1};
+  const int cR = (fi * NJ + fj) * NK + fk, cL = cR - stride[a];
+  const int iL = cL / (NJ * NK), jL = (cL / NK) % NJ, kL = cL % NK;
+  double s[3];
+  face_vec3(P, fce, s);
+  const size_t cstride = (size_t)(NI + 2) * (NJ + 2) * (NK + 2);
+  double d1[3], d2[3], d3[3];
+  for (int c = 0; c < 3; ++c) d1[c] = P.cen[c * cstride + pc3(P, fi, fj, fk)] - P.cen[c * cstride + pc3(P, iL, jL, kL)];
+  const int r1 = a == 0 ? 1 : 0, r2 = a == 2 ? 1 : 2;
+  int nds[4][3];
+  const int sh[4][2] = {{0, 0}, {1, 1}, {1, 0}, {0, 1}};
+  for (int e = 0; e < 4; ++e) {
+    nds[e][0] = fi; nds[e][1] = fj; nds[e][2] = fk;
+    nds[e][r1] += sh[e][0];
+    nds[e][r2] += sh[e][1];
+  }
+  auto node_centre = [&](const int* nd, int c) -> double {
+    double v[8];
+    for (int di = 0; di < 2; ++di)
+      for (int dj2 = 0; dj2 < 2; ++dj2)
+        for (int dk2 = 0; dk2 < 2; ++dk2)
+          v[(di * 2 + dj2) * 2 + dk2] = P.cen[c * cstride + pc3(P, nd[0] - 1 + di, nd[1] - 1 + dj2, nd[2] - 1 + dk2)];
+    return node_mean<double>(v);
+  };
+  for (int c = 0; c < 3; ++c) {
+    d2[c] = node_centre(nds[1], c) - node_centre(nds[0], c);
+    d3[c] = node_centre(nds[3], c) - node_centre(nds[2], c);
+  }
+  double c23[3] = {d2[1] * d3[2] - d2[2] * d3[1], d2[2] * d3[0] - d2[0] * d3[2], d2[0] * d3[1] - d2[1] * d3[0]};
+  double c31[3] = {d3[1] * d1[2] - d3[2] * d1[1], d3[2] * d1[0] - d3[0] * d1[2], d3[0] * d1[1] - d3[1] * d1[0]};
+  double c12[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
+  const double det = (d1[0] * c23[0] + d1[1] * c23[1]) + d1[2] * c23[2];
+  auto phi_at = [&](int cell, double* o) {
+    double q[5];
+    for (int k = 0; k < 5; ++k) q[k] = Q[base + (size_t)k * nc + cell];
+    o[0] = q[1]; o[1] = q[2]; o[2] = q[3]; o[3] = gamma * q[4] / q[0];
+  };
+  double pn[4][4];
+  for (int e = 0; e < 4; ++e) {
+    double v[8][4];
+    for (int di = 0; di < 2; ++di)
+      for (int dj2 = 0; dj2 < 2; ++dj2)
+        for (int dk2 = 0; dk2 < 2; ++dk2)
+          phi_at(((nds[e][0] - 1 + di) * NJ + nds[e][1] - 1 + dj2) * NK + nds[e][2] - 1 + dk2, v[(di * 2 + dj2) * 2 + dk2]);
+    for (int k = 0; k < 4; ++k) {
+      double w[8];
+      for (int m = 0; m < 8; ++m) w[m] = v[m][k];
+      pn[e][k] = node_mean<double>(w);
+    }
+  }
+  double pl[4], pr[4];
+  phi_at(cL, pl);
+  phi_at(cR, pr);
+  double gr[3][4];
+  for (int k = 0; k < 4; ++k) {
+    const double dp1 = pr[k] - pl[k], dp2 = pn[1][k] - pn[0][k], dp3 = pn[3][k] - pn[2][k];
+    for (int b = 0; b < 3; ++b) gr[b][k] = (dp1 * c23[b] + dp2 * c31[b] + dp3 * c12[b]) / det;
+  }
+  double gv[3][3], gT[3];
+  for (int c = 0; c < 3; ++c) {
+    for (int b = 0; b < 3; ++b) gv[c][b] = gr[b][c];
+    gT[c] = gr[c][3];
+  }
+  double vel[3] = {0.5 * (pl[0] + pr[0]), 0.5 * (pl[1] + pr[1]), 0.5 * (pl[2] + pr[2])};
+  const double Tf = 0.5 * (pl[3] + pr[3]);
+  const double mu = viscosity<double>(Tf, mu_inf, ph.sutherland);
+  const double kc = mu / (PR * (gamma - 1.0));
+  double f[5];
+  viscous_flux3<double>(vel, gv, gT, s, mu, kc, nullptr, f);
+  out[0] = f[1]; out[1] = f[2]; out[2] = f[3];
+}
+
+// out[case][8] = CL, CD, CM, CD friction, Fx, Fy, Fz, CY: lift normal to the
+// freestream in the x-z plane, per q_inf * area; moment about y through
+// (cx, cz), pressure and friction, per q_inf * area * length.
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_forces3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                   const double* __restrict__ Q, double area, double length,
+                                                   double cx, double cz, double* __restrict__ out) {
+  __shared__ double sh[6][4];
+  const int cs = blockIdx.x;
+  const size_t base = (size_t)cs * NV * P.nc;
+  const double gamma = ph.gamma;
+  const double* qi = ctl[cs].q_inf;
+  const double speed = sqrt(dot3(qi + 1, qi + 1));
+  const double dr[3] = {qi[1] / speed, qi[2] / speed, qi[3] / speed};
+  const double li[3] = {-dr[2], 0.0, dr[0]};
+  const double q_area = 0.5 * qi[0] * speed * speed * area;
+  double v[6] = {0, 0, 0, 0, 0, 0};   // lift, drag, moment, shear x, y, z
+  double fs[6] = {0, 0, 0, 0, 0, 0};  // force x, y, z
+  for (int w = threadIdx.x; w < P.n_walls; w += BLOCK) {
+    const int cell = P.w_cell[w], fam = P.w_fam[w];
+    const int fce = (fam > 0 ? P.nf[0] : 0) + (fam > 1 ? P.nf[1] : 0) + P.w_face[w];
+    const double into = P.w_into[w];
+    double s[3];
+    face_vec3(P, fce, s);
+    for (int c = 0; c < 3; ++c) s[c] *= into;
+    const double p = Q[base + (size_t)4 * P.nc + cell] - 1.0 / gamma;
+    double pf[3] = {-p * s[0], -p * s[1], -p * s[2]};
+    double t[3] = {0.0, 0.0, 0.0};
+    if (ph.viscous) {
+      wall_shear3<NV>(P, ph, Q, base, ctl[cs].mu_inf, fce, t);
+      for (int c = 0; c < 3; ++c) t[c] *= into;
+    }
+    double ld[3] = {pf[0] + t[0], pf[1] + t[1], pf[2] + t[2]};
+    const double rx = P.w_mid[w] - cx, rz = P.w_mid[(size_t)2 * P.n_walls + w] - cz;
+    const double moment = rz * ld[0] - rx * ld[2];
+    v[0] += dot3(li, ld) / q_area;
+    v[1] += dot3(dr, ld) / q_area;
+    v[2] += (moment / length) / q_area;
+    v[3] += t[0]; v[4] += t[1]; v[5] += t[2];
+    fs[0] += ld[0]; fs[1] += ld[1]; fs[2] += ld[2];
+  }
+  block_sum<6>(v, sh);
+  block_sum<6>(fs, sh);
+  if (threadIdx.x == 0) {
+    double* o = out + (size_t)cs * 8;
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
+    o[3] = dot3(dr, v + 3) / q_area;
+    o[4] = fs[0];
+    o[5] = fs[1];
+    o[6] = fs[2];
+    o[7] = fs[1] / q_area;
+  }
+}
+
+}  // namespace jr

@@ -14,6 +14,7 @@
 #include "../../include/jrhip.h"
 #include "jr_krylov.h"
 #include "jr_cr.h"
+#include "jr_3d.h"
 
 using namespace jr;
 
@@ -32,6 +33,10 @@ static int fail(const std::string& msg) {
 
 struct jr_ctx {
   Plan P{};
+  int dim = 2;       // 3: the 3D path (P3; P holds nc, interior and volumes for the shared vector kernels)
+  Plan3 P3{};
+  double* F3 = nullptr;     // 3D face fluxes [case][var][faces of the three families]
+  double* dsw = nullptr;    // 3D line apply: forward-sweep workspace, one field
   Physics ph{};
   int nv = 4, n_cases = 1;
   int m = 10, restarts = 2, exact = 0, mixed = 0, thomas = 0, cr_threads = 64;
@@ -315,6 +320,124 @@ static void newton_step(jr_ctx* c, double* hist_row) {
   gmres_and_update<NV, VISC, MM, R>(c);
 }
 
+// --- the 3D path ---------------------------------------------------------------
+
+template <int NV, bool VISC>
+static void launch_residual3(jr_ctx* c, const double* U, double* out, int first_order) {
+  const Plan3& P = c->P3;
+  hipLaunchKernelGGL((k_fill3<double, NV>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
+  hipLaunchKernelGGL((k_faces3<double, NV, VISC>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
+                     c->ph, first_order, c->ctl, c->Q, nullptr, c->F3, 0);
+  hipLaunchKernelGGL((k_cells3<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, nullptr, c->Q, nullptr, c->F3, nullptr, out, 0);
+}
+
+template <int NV, bool VISC>
+static void launch_tangent3(jr_ctx* c, const double* U, const double* dU, double* out, int first_order,
+                            const double* vdt, int check_active) {
+  const Plan3& P = c->P3;
+  JR_LAUNCH(PR_FILL, (k_fill3<Dual, NV>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0, P,
+            c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
+  JR_LAUNCH(PR_FACES, (k_faces3<Dual, NV, VISC>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
+            first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, c->F3, check_active);
+  JR_LAUNCH(PR_CELLS, (k_cells3<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
+            (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->F3, vdt, out,
+            check_active);
+}
+
+template <int NV>
+static void launch_speeds3(jr_ctx* c, const double* U) {
+  const Plan3& P = c->P3;
+  hipLaunchKernelGGL((k_face_speeds3<NV>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, c->lam, c->sam, c->sad);
+  hipLaunchKernelGGL((k_cell_speeds3<NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, c->lam, c->sam, c->sad, c->lsum, c->vdt, c->sa_diag);
+}
+
+template <class R, int NV>
+static void launch_factor3(jr_ctx* c, const double* U) {
+  const Plan3& P = c->P3;
+  hipLaunchKernelGGL((k_blocks3<NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
+  JR_LAUNCH(PR_FACTOR, (k_thomas3<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
+            (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa);
+}
+
+template <class R, int NV>
+static void launch_apply3(jr_ctx* c, const double* r, double* z, int accumulate, int check_active,
+                          double* vout = nullptr) {
+  const Plan3& P = c->P3;
+  const unsigned groups = (unsigned)((P.n_lines + 63) / 64 + (P.n_free + 63) / 64);
+  JR_LAUNCH(PR_APPLY, (k_apply3<R, NV>), dim3(groups, c->n_cases), dim3(64), 0, P, (const Ctl*)c->ctl,
+            (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, accumulate, check_active, vout,
+            (const double*)c->cg.p3, c->cg.nblk);
+}
+
+template <int NV, bool VISC, int MM, class R>
+static void gmres_and_update3(jr_ctx* c) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  const size_t N = c->N, vstride = (size_t)c->n_cases * N;
+  const int fo = c->exact ? 0 : 1;
+  for (int cyc = 0; cyc < c->restarts; ++cyc) {
+    hipLaunchKernelGGL((k_cycle_begin<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->Rv, c->V, N, c->restarts,
+                       c->ktol, c->G);
+    if (cyc > 0 && !c->use_graph && !c->async_mode && !any_active(c)) break;
+    for (int j = 0; j < c->m; ++j) {
+      if (j == 0) launch_apply3<R, NV>(c, c->V, c->Z, 0, 1);
+      else launch_apply3<R, NV>(c, c->W, c->Z, 0, 1, c->V + (size_t)j * vstride);
+      launch_tangent3<NV, VISC>(c, c->U, c->Z, c->W, fo, c->vdt, 1);
+      cgs2_column<NV>(c, j);
+    }
+    hipLaunchKernelGGL((k_normalize<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->W,
+                       c->V + (size_t)c->m * vstride, N, c->cg);
+    hipLaunchKernelGGL(k_lstsq, dim3(c->n_cases), dim3(64), 0, c->stream, c->n_cases, c->ctl, c->H, c->G, c->m, c->y,
+                       c->hy, c->cg, c->hv, c->h2v);
+    hipLaunchKernelGGL((k_combine<NV, MM>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->V, vstride, N, c->y, c->hy,
+                       c->m, c->T, c->Act, c->Rv, c->R, c->red);
+    launch_apply3<R, NV>(c, c->T, c->X, 1, 1);
+  }
+  const dim3 cg(c->nblk, c->n_cases);
+  if (c->max_rel > 0.0)
+    hipLaunchKernelGGL((k_relax3<NV>), cg, dim3(BLOCK), 0, c->stream, c->P3, c->ph.gamma, c->ctl, c->U, c->X, c->red,
+                       c->max_rel);
+  Ramp ramp = c->ramp;
+  ramp.relaxed = c->max_rel > 0.0;
+  hipLaunchKernelGGL((k_check3<NV>), cg, dim3(BLOCK), 0, c->stream, c->P3, c->ph.gamma, c->ctl, c->U, c->X, c->Act,
+                     c->R, c->red, ramp);
+  hipLaunchKernelGGL((k_update<NV>), vg, dim3(BLOCK), 0, c->stream, P.nc, c->ctl, c->U, c->X, ramp.relaxed);
+}
+
+template <int NV, bool VISC, int MM, class R>
+static void newton_step3(jr_ctx* c, double* hist_row) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  launch_residual3<NV, VISC>(c, c->U, c->R, 0);
+  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
+                     hist_row, c->n_cases);
+  launch_speeds3<NV>(c, c->U);
+  launch_factor3<R, NV>(c, c->U);
+  gmres_and_update3<NV, VISC, MM, R>(c);
+}
+
+template <int NV, bool VISC>
+static void step3_m(jr_ctx* c, double* hist_row) {
+  if (c->m <= 16) {
+    if (c->mixed) newton_step3<NV, VISC, 16, float>(c, hist_row); else newton_step3<NV, VISC, 16, double>(c, hist_row);
+  } else if (c->m <= 32) {
+    if (c->mixed) newton_step3<NV, VISC, 32, float>(c, hist_row); else newton_step3<NV, VISC, 32, double>(c, hist_row);
+  } else {
+    if (c->mixed) newton_step3<NV, VISC, 64, float>(c, hist_row); else newton_step3<NV, VISC, 64, double>(c, hist_row);
+  }
+}
+
+static void step3_dispatch(jr_ctx* c, double* hist_row) {
+  if (c->nv == 6) step3_m<6, true>(c, hist_row);
+  else if (c->ph.viscous) step3_m<5, true>(c, hist_row);
+  else step3_m<5, false>(c, hist_row);
+}
+
 // --- dispatch over the compile-time variants ------------------------------------
 
 template <int NV, bool VISC, int MM>
@@ -331,36 +454,78 @@ static void step_dispatch_m(jr_ctx* c, double* hist_row) {
 }
 
 static void step_dispatch(jr_ctx* c, double* hist_row) {
+  if (c->dim == 3) return step3_dispatch(c, hist_row);
   if (c->nv == 5) step_dispatch_m<5, true>(c, hist_row);
   else if (c->ph.viscous) step_dispatch_m<4, true>(c, hist_row);
   else step_dispatch_m<4, false>(c, hist_row);
 }
 
 static void residual_dispatch(jr_ctx* c, const double* U, double* out, int fo) {
+  if (c->dim == 3) {
+    if (c->nv == 6) launch_residual3<6, true>(c, U, out, fo);
+    else if (c->ph.viscous) launch_residual3<5, true>(c, U, out, fo);
+    else launch_residual3<5, false>(c, U, out, fo);
+    return;
+  }
   if (c->nv == 5) launch_residual<5, true>(c, U, out, fo);
   else if (c->ph.viscous) launch_residual<4, true>(c, U, out, fo);
   else launch_residual<4, false>(c, U, out, fo);
 }
 
 static void tangent_dispatch(jr_ctx* c, const double* U, const double* dU, double* out, int fo, const double* vdt) {
+  if (c->dim == 3) {
+    if (c->nv == 6) launch_tangent3<6, true>(c, U, dU, out, fo, vdt, 0);
+    else if (c->ph.viscous) launch_tangent3<5, true>(c, U, dU, out, fo, vdt, 0);
+    else launch_tangent3<5, false>(c, U, dU, out, fo, vdt, 0);
+    return;
+  }
   if (c->nv == 5) launch_tangent<5, true>(c, U, dU, out, fo, vdt, 0);
   else if (c->ph.viscous) launch_tangent<4, true>(c, U, dU, out, fo, vdt, 0);
   else launch_tangent<4, false>(c, U, dU, out, fo, vdt, 0);
 }
 
 static void speeds_dispatch(jr_ctx* c, const double* U) {
+  if (c->dim == 3) {
+    if (c->nv == 6) launch_speeds3<6>(c, U); else launch_speeds3<5>(c, U);
+    return;
+  }
   if (c->nv == 5) launch_speeds<5>(c, U);
   else launch_speeds<4>(c, U);
 }
 
 static void factor_dispatch(jr_ctx* c, const double* U) {
+  if (c->dim == 3) {
+    if (c->nv == 6) { if (c->mixed) launch_factor3<float, 6>(c, U); else launch_factor3<double, 6>(c, U); }
+    else { if (c->mixed) launch_factor3<float, 5>(c, U); else launch_factor3<double, 5>(c, U); }
+    return;
+  }
   if (c->nv == 5) { if (c->mixed) launch_factor<float, 5>(c, U); else launch_factor<double, 5>(c, U); }
   else { if (c->mixed) launch_factor<float, 4>(c, U); else launch_factor<double, 4>(c, U); }
 }
 
 static void apply_dispatch(jr_ctx* c, const double* r, double* z) {
+  if (c->dim == 3) {
+    if (c->nv == 6) { if (c->mixed) launch_apply3<float, 6>(c, r, z, 0, 0); else launch_apply3<double, 6>(c, r, z, 0, 0); }
+    else { if (c->mixed) launch_apply3<float, 5>(c, r, z, 0, 0); else launch_apply3<double, 5>(c, r, z, 0, 0); }
+    return;
+  }
   if (c->nv == 5) { if (c->mixed) launch_apply<float, 5>(c, r, z, 0, 0); else launch_apply<double, 5>(c, r, z, 0, 0); }
   else { if (c->mixed) launch_apply<float, 4>(c, r, z, 0, 0); else launch_apply<double, 4>(c, r, z, 0, 0); }
+}
+
+static int forces3_dispatch(jr_ctx* c, double area, double length, double cx, double cz) {
+  const Plan3& P = c->P3;
+  if (c->nv == 6) {
+    launch_residual3<6, true>(c, c->U, c->tmpA, 0);
+    hipLaunchKernelGGL((k_forces3<6>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, area,
+                       length, cx, cz, c->coef);
+  } else {
+    if (c->ph.viscous) launch_residual3<5, true>(c, c->U, c->tmpA, 0);
+    else launch_residual3<5, false>(c, c->U, c->tmpA, 0);
+    hipLaunchKernelGGL((k_forces3<5>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, area,
+                       length, cx, cz, c->coef);
+  }
+  return 0;
 }
 
 static int forces_dispatch(jr_ctx* c, double length, double cx, double cy) {
@@ -748,6 +913,243 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   return 0;
 }
 
+int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, jr_ctx** out) {
+  if (!g || !num || !out) return fail("jr_ctx_create3: null argument");
+  if (n_cases < 1) return fail("jr_ctx_create3: n_cases must be >= 1");
+  if (g->NI < 5 || g->NJ < 5 || g->NK < 5) return fail("jr_ctx_create3: canvas too small");
+  if ((int64_t)g->NI * g->NJ * g->NK > (int64_t)1 << 30) return fail("jr_ctx_create3: canvas too large");
+  if (num->krylov_subspace < 1 || num->krylov_subspace > MMAX)
+    return fail("jr_ctx_create3: krylov_subspace must lie in [1, " + std::to_string(MMAX) + "]");
+  if (num->krylov_restarts < 1) return fail("jr_ctx_create3: krylov_restarts must be >= 1");
+  if (num->flux < 0 || num->flux > 1) return fail("jr_ctx_create3: unknown flux");
+  if (num->limiter < 0 || num->limiter > 3) return fail("jr_ctx_create3: unknown limiter");
+  if (num->sa && !num->viscous) return fail("jr_ctx_create3: SA needs a viscous flow");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("jr_ctx_create3: no HIP device");
+  jr_ctx* c = new jr_ctx();
+  c->dim = 3;
+  auto bail = [&](int) {
+    jr_ctx_destroy(c);
+    return -1;
+  };
+  Plan3& P = c->P3;
+  P.NI = g->NI; P.NJ = g->NJ; P.NK = g->NK;
+  const size_t nc = (size_t)g->NI * g->NJ * g->NK;
+  P.nc = (int)nc;
+  P.nf[0] = (g->NI + 1) * g->NJ * g->NK;
+  P.nf[1] = g->NI * (g->NJ + 1) * g->NK;
+  P.nf[2] = g->NI * g->NJ * (g->NK + 1);
+  P.nft = P.nf[0] + P.nf[1] + P.nf[2];
+  const size_t npc = (size_t)(g->NI + 2) * (g->NJ + 2) * (g->NK + 2);
+  if (upload(c, &P.vol, g->volume, nc) || upload(c, &P.S[0], g->face[0], 3 * (size_t)P.nf[0]) ||
+      upload(c, &P.S[1], g->face[1], 3 * (size_t)P.nf[1]) || upload(c, &P.S[2], g->face[2], 3 * (size_t)P.nf[2]) ||
+      upload(c, &P.cen, g->centres, 3 * npc) || upload(c, &P.dist, g->distance, nc) ||
+      upload(c, &P.interior, g->interior, nc) || upload(c, &P.nbr, g->neighbours, nc))
+    return bail(0);
+  for (size_t i = 0; i < nc; ++i)
+    if (g->neighbours[i] < 0 || g->neighbours[i] >= (int64_t)nc) {
+      fail("jr_ctx_create3: neighbour index out of range");
+      return bail(0);
+    }
+  // The composed fill: every ghost written once, from an interior root,
+  // through valid tasks.
+  std::vector<uint8_t> ghost(nc, 0);
+  std::vector<int> ops((size_t)std::max(1, g->n_ghost) * MAX_OPS, -1);
+  for (int t = 0; t < g->n_tasks; ++t)
+    if (g->task_kind[t] < 0 || g->task_kind[t] > 6) {
+      fail("jr_ctx_create3: unknown ghost-fill task kind");
+      return bail(0);
+    }
+  for (int q = 0; q < g->n_ghost; ++q) {
+    const int64_t d = g->ghost_dest[q], r = g->ghost_root[q], n = g->ghost_nops[q];
+    if (d < 0 || d >= (int64_t)nc || r < 0 || r >= (int64_t)nc || !g->interior[r] || g->interior[d] || ghost[d] ||
+        n < 0 || n > MAX_OPS) {
+      fail("jr_ctx_create3: composed ghost fill out of range");
+      return bail(0);
+    }
+    ghost[d] = 1;
+    for (int o = 0; o < n; ++o) {
+      const int64_t t = g->ghost_ops[(size_t)q * MAX_OPS + o];
+      if (t < 0 || t >= g->n_tasks) {
+        fail("jr_ctx_create3: ghost-fill task out of range");
+        return bail(0);
+      }
+      ops[(size_t)q * MAX_OPS + o] = (int)t;
+    }
+  }
+  P.n_ghost = g->n_ghost;
+  P.n_tasks = g->n_tasks;
+  if (upload(c, &P.ghost, ghost.data(), nc) || upload(c, &P.g_dest, g->ghost_dest, std::max(1, g->n_ghost)) ||
+      upload(c, &P.g_root, g->ghost_root, std::max(1, g->n_ghost)) ||
+      upload(c, &P.g_nops, g->ghost_nops, std::max(1, g->n_ghost)) || upload(c, &P.g_ops, ops.data(), ops.size()) ||
+      upload(c, &P.t_kind, g->task_kind, std::max(1, g->n_tasks)) ||
+      upload(c, &P.t_norm, g->task_normal, 3 * (size_t)std::max(1, g->n_tasks)))
+    return bail(0);
+  // Chains.
+  P.n_lines = g->n_lines;
+  P.n_slots = g->n_slots;
+  int max_len = 0;
+  std::vector<uint8_t> on_chain(nc, 0);
+  for (int l = 0; l < g->n_lines; ++l) {
+    const int64_t a = g->line_start[l], b = g->line_start[l + 1];
+    if (a < 0 || b < a || b > g->n_slots) {
+      fail("jr_ctx_create3: line starts out of range");
+      return bail(0);
+    }
+    max_len = std::max<int>(max_len, (int)(b - a));
+    for (int64_t sl = a; sl < b; ++sl) {
+      const int64_t cell = g->line_cells[sl];
+      if (cell < 0 || cell >= (int64_t)nc || on_chain[cell]) {
+        fail("jr_ctx_create3: chain cell out of range or repeated");
+        return bail(0);
+      }
+      on_chain[cell] = 1;
+      const bool up = g->line_up_sign[sl] != 0, dn = g->line_down_sign[sl] != 0;
+      if (up != dn || (up && (sl + 1 >= b || g->line_up_face[sl] < 0 || g->line_up_face[sl] >= P.nft ||
+                              g->line_down_face[sl] < 0 || g->line_down_face[sl] >= P.nft))) {
+        fail("jr_ctx_create3: chain link out of range");
+        return bail(0);
+      }
+    }
+    if (b > a && g->line_up_sign[b - 1] != 0) {
+      fail("jr_ctx_create3: the last slot of a chain must have no link");
+      return bail(0);
+    }
+  }
+  P.max_len = max_len;
+  std::vector<int> freec;
+  for (size_t i = 0; i < nc; ++i)
+    if (!on_chain[i]) freec.push_back((int)i);
+  P.n_free = (int)freec.size();
+  if (upload(c, &P.l_start, g->line_start, (size_t)g->n_lines + 1) ||
+      upload(c, &P.l_cell, g->line_cells, std::max(1, g->n_slots)) ||
+      upload(c, &P.l_up, g->line_up_face, std::max(1, g->n_slots)) ||
+      upload(c, &P.l_ups, g->line_up_sign, std::max(1, g->n_slots)) ||
+      upload(c, &P.l_dn, g->line_down_face, std::max(1, g->n_slots)) ||
+      upload(c, &P.l_dns, g->line_down_sign, std::max(1, g->n_slots)) ||
+      upload(c, &P.free_cell, freec.data(), std::max<size_t>(1, freec.size())))
+    return bail(0);
+  P.n_walls = g->n_walls;
+  for (int w = 0; w < g->n_walls; ++w) {
+    const int64_t fam = g->wall_family[w], f = g->wall_face[w], cell = g->wall_cell[w];
+    if (fam < 0 || fam > 2 || f < 0 || f >= P.nf[fam] || cell < 0 || cell >= (int64_t)nc || !g->interior[cell]) {
+      fail("jr_ctx_create3: wall face out of range");
+      return bail(0);
+    }
+  }
+  if (upload(c, &P.w_cell, g->wall_cell, std::max(1, g->n_walls)) ||
+      upload(c, &P.w_fam, g->wall_family, std::max(1, g->n_walls)) ||
+      upload(c, &P.w_face, g->wall_face, std::max(1, g->n_walls)) ||
+      upload(c, &P.w_into, g->wall_into, std::max(1, g->n_walls)) ||
+      upload(c, &P.w_mid, g->wall_mid, 3 * (size_t)std::max(1, g->n_walls)))
+    return bail(0);
+  // The shared vector kernels read the cell count, the interior mask and the
+  // cell volumes from the 2D plan.
+  c->P.nc = P.nc;
+  c->P.interior = P.interior;
+  c->P.area = P.vol;
+
+  Physics& ph = c->ph;
+  ph.gamma = num->gamma;
+  ph.kappa = num->kappa;
+  ph.flux = num->flux;
+  ph.limiter = num->limiter;
+  ph.nv = num->sa ? 6 : 5;
+  ph.viscous = num->viscous;
+  ph.sa = num->sa;
+  ph.sutherland = num->sutherland;
+  ph.c_cr1 = num->c_cr1;
+  c->nv = ph.nv;
+  c->n_cases = n_cases;
+  c->m = num->krylov_subspace;
+  c->restarts = num->krylov_restarts;
+  c->exact = num->jacobian_exact;
+  c->mixed = num->mixed;
+  c->thomas = 1;
+  c->dq_fused = false;
+  c->ktol = num->krylov_tol;
+  c->max_rel = num->max_relative_update;
+  c->ramp.cfl_growth = num->cfl_growth;
+  c->ramp.cfl_cutback = num->cfl_cutback;
+  c->ramp.cfl_min = num->cfl_min;
+  c->ramp.cfl_max = num->cfl_max;
+  c->ramp.krylov_success = num->krylov_success;
+  c->N = (size_t)c->nv * nc;
+  c->nblk = (int)std::min<size_t>(CG_BLK, std::max<size_t>(1, (c->N + BLOCK * 2 - 1) / (BLOCK * 2)));
+  const size_t F = (size_t)n_cases * c->N;
+  const size_t nf = (size_t)P.nft;
+  if (dalloc(c, &c->U, F) || dalloc(c, &c->R, F) || dalloc(c, &c->Q, F) || dalloc(c, &c->dQ, F) ||
+      dalloc(c, &c->F3, (size_t)n_cases * c->nv * nf) || dalloc(c, &c->X, F) || dalloc(c, &c->Act, F) ||
+      dalloc(c, &c->Rv, F) || dalloc(c, &c->Z, F) || dalloc(c, &c->W, F) || dalloc(c, &c->T, F) ||
+      dalloc(c, &c->V, F * (c->m + 1)) || dalloc(c, &c->tmpA, F) || dalloc(c, &c->tmpB, F) ||
+      dalloc(c, &c->dsw, F) || dalloc(c, &c->lam, (size_t)n_cases * nf) || dalloc(c, &c->sam, (size_t)n_cases * nf) ||
+      dalloc(c, &c->sad, (size_t)n_cases * nf) || dalloc(c, &c->lsum, (size_t)n_cases * nc) ||
+      dalloc(c, &c->vdt, (size_t)n_cases * nc) || dalloc(c, &c->sa_diag, (size_t)n_cases * nc))
+    return bail(0);
+  c->Fi = c->Fj = c->F3;
+  {
+    const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
+    const size_t rows = (size_t)n_cases * std::max(1, P.max_len) * std::max(1, P.n_lines);
+    void* p;
+    if (dalloc(c, (char**)&p, rows * 75 * rsz)) return bail(0);
+    c->fac = p;
+    if (dalloc(c, (char**)&p, rows * 3 * rsz)) return bail(0);
+    c->fac_sa = p;
+    c->ws = c->ws_sa = p;
+    if (dalloc(c, &c->blk, (size_t)n_cases * std::max(1, P.n_slots) * BLKW3)) return bail(0);
+  }
+  if (dalloc(c, &c->ctl, (size_t)n_cases) || dalloc(c, &c->hv, (size_t)n_cases * (MMAX + 1)) ||
+      dalloc(c, &c->h2v, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->H, (size_t)n_cases * (MMAX + 1) * MMAX) ||
+      dalloc(c, &c->G, (size_t)n_cases * GIV) || dalloc(c, &c->y, (size_t)n_cases * (MMAX + 1)) ||
+      dalloc(c, &c->hy, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->coef, (size_t)n_cases * 8) ||
+      dalloc(c, &c->gjac, 1))
+    return bail(0);
+  c->cg.nblk = c->nblk;
+  if (dalloc(c, &c->cg.p1, (size_t)n_cases * MMAX * c->nblk) ||
+      dalloc(c, &c->cg.p2, (size_t)n_cases * MMAX * c->nblk) || dalloc(c, &c->cg.p3, (size_t)n_cases * c->nblk))
+    return bail(0);
+  c->red.stride = 1024 * (MMAX + 2);
+  if (dalloc(c, &c->red.partials, (size_t)n_cases * c->red.stride) ||
+      dalloc(c, &c->red.counters, (size_t)n_cases * 4))
+    return bail(0);
+  for (const void* p : {(const void*)c->U, (const void*)c->R, (const void*)c->Q, (const void*)c->dQ,
+                        (const void*)c->F3, (const void*)c->X, (const void*)c->Act, (const void*)c->Rv,
+                        (const void*)c->Z, (const void*)c->W, (const void*)c->T, (const void*)c->V,
+                        (const void*)c->tmpA, (const void*)c->tmpB, (const void*)c->dsw, (const void*)c->lam,
+                        (const void*)c->sam, (const void*)c->sad, (const void*)c->lsum, (const void*)c->vdt,
+                        (const void*)c->sa_diag, (const void*)c->ctl, (const void*)c->hv, (const void*)c->h2v,
+                        (const void*)c->G, (const void*)c->H, (const void*)c->y, (const void*)c->hy,
+                        (const void*)c->coef, (const void*)c->red.partials, (const void*)c->red.counters,
+                        (const void*)c->cg.p1, (const void*)c->cg.p2, (const void*)c->cg.p3, (const void*)c->fac, (const void*)c->fac_sa,
+                        (const void*)c->blk})
+    if (!p) {
+      fail("jr_ctx_create3: a device buffer was not allocated");
+      return bail(0);
+    }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail("jr_ctx_create3: stream");
+    return bail(0);
+  }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    fail("jr_ctx_create3: events");
+    return bail(0);
+  }
+  std::vector<Ctl> h(n_cases);
+  for (auto& x : h) {
+    std::memset(&x, 0, sizeof(Ctl));
+    x.q_inf[0] = 1.0;
+    x.q_inf[4] = 1.0 / ph.gamma;
+    x.cfl = 1.0;
+    x.res_prev = INFINITY;
+  }
+  if (hipMemcpy(c->ctl, h.data(), sizeof(Ctl) * n_cases, hipMemcpyHostToDevice) != hipSuccess) {
+    fail("jr_ctx_create3: control upload");
+    return bail(0);
+  }
+  *out = c;
+  return 0;
+}
+
 int jr_ctx_destroy(jr_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -788,10 +1190,11 @@ int jr_set_freestream(jr_ctx* c, const double* q_inf, const double* mu_inf) {
   if (c->ph.viscous && !mu_inf) return fail("jr_set_freestream: a viscous context needs mu_inf");
   std::vector<Ctl> h;
   TRY(get_ctl(c, h));
+  const int nq = c->dim == 3 ? 6 : 5, ip = c->dim == 3 ? 4 : 3;
   for (int k = 0; k < c->n_cases; ++k) {
-    for (int v = 0; v < 5; ++v) h[k].q_inf[v] = q_inf[k * 5 + v];
+    for (int v = 0; v < nq; ++v) h[k].q_inf[v] = q_inf[k * nq + v];
     h[k].mu_inf = mu_inf ? mu_inf[k] : 0.0;
-    if (!(h[k].q_inf[0] > 0.0) || !(h[k].q_inf[3] > 0.0)) return fail("jr_set_freestream: non-physical freestream");
+    if (!(h[k].q_inf[0] > 0.0) || !(h[k].q_inf[ip] > 0.0)) return fail("jr_set_freestream: non-physical freestream");
   }
   c->has_q = true;
   return put_ctl(c, h);
@@ -853,7 +1256,7 @@ int jr_wave_speeds(jr_ctx* c, const double* U, double* lam, double* lsum) {
   HIPCHK(hipMemcpyAsync(c->tmpA, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
   speeds_dispatch(c, c->tmpA);
   TRY(check_launch());
-  const size_t nf = (size_t)c->P.nfi + c->P.nfj;
+  const size_t nf = c->dim == 3 ? (size_t)c->P3.nft : (size_t)c->P.nfi + c->P.nfj;
   if (lam) HIPCHK(hipMemcpyAsync(lam, c->lam, (size_t)c->n_cases * nf * 8, hipMemcpyDeviceToHost, c->stream));
   if (lsum) HIPCHK(hipMemcpyAsync(lsum, c->lsum, (size_t)c->n_cases * c->P.nc * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -953,6 +1356,7 @@ int jr_newton_block_async(jr_ctx* c, int n_steps, const double* force_ref) {
   if (!c->has_q) return fail("jr_newton_block_async: set the freestream first");
   if (c->pending_steps) return fail("jr_newton_block_async: a block is already pending on this context");
   if (c->use_graph) return fail("jr_newton_block_async: graphs are for jr_newton_block");
+  if (force_ref && c->dim == 3) return fail("jr_newton_block_async: forces of a 3D context come from jr_forces3");
   if (n_steps > c->hist_cap) {
     HIPCHK(hipStreamSynchronize(c->stream));
     drop_graph(c);
@@ -1090,7 +1494,20 @@ int jr_get_scalars(jr_ctx* c, double* out) {
 int jr_forces(jr_ctx* c, double length, double cx, double cy, double* out) {
   if (!c || !out) return fail("jr_forces: null argument");
   if (!c->has_q) return fail("jr_forces: set the freestream first");
+  if (c->dim == 3) return fail("jr_forces: a 3D context needs jr_forces3 (a reference area)");
   forces_dispatch(c, length, cx, cy);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_forces3(jr_ctx* c, double area, double length, double cx, double cz, double* out) {
+  if (!c || !out) return fail("jr_forces3: null argument");
+  if (!c->has_q) return fail("jr_forces3: set the freestream first");
+  if (c->dim != 3) return fail("jr_forces3: not a 3D context");
+  if (!(area > 0.0) || !(length > 0.0)) return fail("jr_forces3: area and length must be positive");
+  forces3_dispatch(c, area, length, cx, cz);
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1150,6 +1567,7 @@ static void profile_dispatch(jr_ctx* c, int which) {
 extern "C" int jr_profile(jr_ctx* c, int which, int reps, float* ms) {
   if (!c || !ms) return fail("jr_profile: null argument");
   if (which < 0 || which > 7 || reps < 1) return fail("jr_profile: bad kernel id or reps");
+  if (c->dim == 3) return fail("jr_profile: the per-kernel profile covers the 2D path; use jr_probe_arm");
   if (!c->has_q) return fail("jr_profile: set the freestream first");
   // inputs: the residual at the device state (fills Q), wave speeds, factor,
   // a Krylov-like vector in Z and Rv, V filled with it

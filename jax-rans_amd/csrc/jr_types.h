@@ -73,7 +73,7 @@ struct Physics {
 
 // Per-case solver control, resident on the device.
 struct Ctl {
-  double q_inf[5];
+  double q_inf[6];     // rho, u, v, p, nu_hat ratio (3D: rho, u, v, w, p, nu_hat ratio)
   double mu_inf;
   double cfl;
   double res_prev;     // |R| of the last accepted step (inf at start)

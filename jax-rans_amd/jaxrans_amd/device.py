@@ -25,7 +25,8 @@ EXPORTS = (
     "jr_newton_init", "jr_newton_block", "jr_get_scalars", "jr_set_cfl", "jr_forces", "jr_timer_start",
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
-    "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_last_error",
+    "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
+    "jr_last_error",
 )
 
 
@@ -48,6 +49,21 @@ class GridDesc(C.Structure):
         ("interior", _u8), ("neighbours", _i64),
         ("n_task", C.c_int32 * 2), ("task_dest", _i64 * 2), ("task_src", _i64 * 2), ("task_kind", _i32 * 2),
         ("task_normal", _d * 2),
+        ("n_lines", C.c_int32), ("n_slots", C.c_int32),
+        ("line_start", _i64), ("line_cells", _i64), ("line_up_face", _i64), ("line_up_sign", _i64),
+        ("line_down_face", _i64), ("line_down_sign", _i64),
+        ("n_walls", C.c_int32),
+        ("wall_cell", _i64), ("wall_family", _i64), ("wall_face", _i64), ("wall_into", _d), ("wall_mid", _d),
+    ]
+
+
+class GridDesc3(C.Structure):
+    _fields_ = [
+        ("NI", C.c_int32), ("NJ", C.c_int32), ("NK", C.c_int32),
+        ("volume", _d), ("face", _d * 3), ("centres", _d), ("distance", _d),
+        ("interior", _u8), ("neighbours", _i64),
+        ("n_ghost", C.c_int32), ("ghost_dest", _i64), ("ghost_root", _i64), ("ghost_nops", _i64),
+        ("ghost_ops", _i64), ("n_tasks", C.c_int32), ("task_kind", _i32), ("task_normal", _d),
         ("n_lines", C.c_int32), ("n_slots", C.c_int32),
         ("line_start", _i64), ("line_cells", _i64), ("line_up_face", _i64), ("line_up_sign", _i64),
         ("line_down_face", _i64), ("line_down_sign", _i64),
@@ -124,48 +140,18 @@ class Context:
         if device_count() < 1:
             raise DeviceUnavailable("no HIP device is visible")
         self.atlas, self.n_cases = atlas, int(n_cases)
-        self.nvar = 5 if sa else 4
+        self.dim = getattr(atlas, "dim", 2)
+        self.nvar = (5 if sa else 4) + (self.dim == 3)
         self.viscous = bool(viscous)
         self.shape = (self.n_cases, self.nvar) + tuple(atlas.shape)
         self._keep = []
-        g = GridDesc()
-        NI, NJ = atlas.shape
-        g.NI, g.NJ = NI, NJ
 
         def arr(a, dtype):
             a = np.ascontiguousarray(a, dtype=dtype)
             self._keep.append(a)
             return a
 
-        g.area = _ptr(arr(atlas.area, np.float64), C.c_double)
-        g.face_i = _ptr(arr(atlas.face_i.reshape(2, -1), np.float64), C.c_double)
-        g.face_j = _ptr(arr(atlas.face_j.reshape(2, -1), np.float64), C.c_double)
-        g.centres = _ptr(arr(atlas.centres, np.float64), C.c_double)
-        g.distance = _ptr(arr(np.where(np.isfinite(atlas.distance), atlas.distance, 1e300), np.float64), C.c_double)
-        g.interior = _ptr(arr(atlas.interior, np.uint8), C.c_uint8)
-        g.neighbours = _ptr(arr(atlas.neighbours, np.int64), C.c_int64)
-        for s, st in enumerate(atlas.stages):
-            g.n_task[s] = st.dest.size
-            g.task_dest[s] = _ptr(arr(st.dest, np.int64), C.c_int64)
-            g.task_src[s] = _ptr(arr(st.src, np.int64), C.c_int64)
-            g.task_kind[s] = _ptr(arr(st.kind, np.int32), C.c_int32)
-            g.task_normal[s] = _ptr(arr(st.normal, np.float64), C.c_double)
-        L = atlas.lines
-        g.n_lines, g.n_slots = L.n_lines, L.cells.size
-        g.line_start = _ptr(arr(L.start, np.int64), C.c_int64)
-        g.line_cells = _ptr(arr(L.cells, np.int64), C.c_int64)
-        g.line_up_face = _ptr(arr(L.up_face, np.int64), C.c_int64)
-        g.line_up_sign = _ptr(arr(L.up_sign, np.int64), C.c_int64)
-        g.line_down_face = _ptr(arr(L.down_face, np.int64), C.c_int64)
-        g.line_down_sign = _ptr(arr(L.down_sign, np.int64), C.c_int64)
-        W = atlas.walls
-        g.n_walls = W.n
-        g.wall_cell = _ptr(arr(W.cell, np.int64), C.c_int64)
-        g.wall_family = _ptr(arr(W.family, np.int64), C.c_int64)
-        g.wall_face = _ptr(arr(W.face, np.int64), C.c_int64)
-        g.wall_into = _ptr(arr(W.into_fluid, np.float64), C.c_double)
-        g.wall_mid = _ptr(arr(W.mid.T, np.float64), C.c_double)
-
+        g = self._desc3(atlas, arr) if self.dim == 3 else self._desc2(atlas, arr)
         from .core.config import ImplicitConfig
         cfg = implicit if implicit is not None else ImplicitConfig()
         n = Numerics()
@@ -185,9 +171,71 @@ class Context:
             raise DeviceError("only the line_j preconditioner is implemented on the device")
         self.config = cfg
         h = C.c_void_p()
-        _check(lib.jr_ctx_create(C.byref(g), C.byref(n), self.n_cases, C.byref(h)))
+        create = lib.jr_ctx_create3 if self.dim == 3 else lib.jr_ctx_create
+        _check(create(C.byref(g), C.byref(n), self.n_cases, C.byref(h)))
         self._h = h
         self._lib = lib
+
+    @staticmethod
+    def _desc2(atlas, arr) -> GridDesc:
+        g = GridDesc()
+        g.NI, g.NJ = atlas.shape
+        g.area = _ptr(arr(atlas.area, np.float64), C.c_double)
+        g.face_i = _ptr(arr(atlas.face_i.reshape(2, -1), np.float64), C.c_double)
+        g.face_j = _ptr(arr(atlas.face_j.reshape(2, -1), np.float64), C.c_double)
+        g.centres = _ptr(arr(atlas.centres, np.float64), C.c_double)
+        g.distance = _ptr(arr(np.where(np.isfinite(atlas.distance), atlas.distance, 1e300), np.float64), C.c_double)
+        g.interior = _ptr(arr(atlas.interior, np.uint8), C.c_uint8)
+        g.neighbours = _ptr(arr(atlas.neighbours, np.int64), C.c_int64)
+        for s, st in enumerate(atlas.stages):
+            g.n_task[s] = st.dest.size
+            g.task_dest[s] = _ptr(arr(st.dest, np.int64), C.c_int64)
+            g.task_src[s] = _ptr(arr(st.src, np.int64), C.c_int64)
+            g.task_kind[s] = _ptr(arr(st.kind, np.int32), C.c_int32)
+            g.task_normal[s] = _ptr(arr(st.normal, np.float64), C.c_double)
+        Context._lines_walls(g, atlas, arr)
+        return g
+
+    @staticmethod
+    def _desc3(atlas, arr) -> GridDesc3:
+        g = GridDesc3()
+        g.NI, g.NJ, g.NK = atlas.shape
+        g.volume = _ptr(arr(atlas.volume, np.float64), C.c_double)
+        for a in range(3):
+            g.face[a] = _ptr(arr(atlas.faces[a].reshape(3, -1), np.float64), C.c_double)
+        g.centres = _ptr(arr(atlas.centres, np.float64), C.c_double)
+        g.distance = _ptr(arr(np.where(np.isfinite(atlas.distance), atlas.distance, 1e300), np.float64), C.c_double)
+        g.interior = _ptr(arr(atlas.interior, np.uint8), C.c_uint8)
+        g.neighbours = _ptr(arr(atlas.neighbours, np.int64), C.c_int64)
+        ch = atlas.chains
+        g.n_ghost = ch.dest.size
+        g.ghost_dest = _ptr(arr(ch.dest, np.int64), C.c_int64)
+        g.ghost_root = _ptr(arr(ch.root, np.int64), C.c_int64)
+        g.ghost_nops = _ptr(arr(ch.n_ops, np.int64), C.c_int64)
+        g.ghost_ops = _ptr(arr(ch.ops, np.int64), C.c_int64)
+        g.n_tasks = ch.kind.size
+        g.task_kind = _ptr(arr(ch.kind, np.int32), C.c_int32)
+        g.task_normal = _ptr(arr(ch.normal, np.float64), C.c_double)
+        Context._lines_walls(g, atlas, arr)
+        return g
+
+    @staticmethod
+    def _lines_walls(g, atlas, arr) -> None:
+        L = atlas.lines
+        g.n_lines, g.n_slots = L.n_lines, L.cells.size
+        g.line_start = _ptr(arr(L.start, np.int64), C.c_int64)
+        g.line_cells = _ptr(arr(L.cells, np.int64), C.c_int64)
+        g.line_up_face = _ptr(arr(L.up_face, np.int64), C.c_int64)
+        g.line_up_sign = _ptr(arr(L.up_sign, np.int64), C.c_int64)
+        g.line_down_face = _ptr(arr(L.down_face, np.int64), C.c_int64)
+        g.line_down_sign = _ptr(arr(L.down_sign, np.int64), C.c_int64)
+        W = atlas.walls
+        g.n_walls = W.n
+        g.wall_cell = _ptr(arr(W.cell, np.int64), C.c_int64)
+        g.wall_family = _ptr(arr(W.family, np.int64), C.c_int64)
+        g.wall_face = _ptr(arr(W.face, np.int64), C.c_int64)
+        g.wall_into = _ptr(arr(W.into_fluid, np.float64), C.c_double)
+        g.wall_mid = _ptr(arr(W.mid.T, np.float64), C.c_double)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -202,7 +250,8 @@ class Context:
 
     # --- set-up ---------------------------------------------------------------
     def set_freestream(self, q_inf: np.ndarray, mu_inf=None) -> None:
-        q = np.zeros((self.n_cases, 5))
+        """Freestream primitives per case: (rho, u, v, p[, nu ratio]), 3D (rho, u, v, w, p[, nu ratio])."""
+        q = np.zeros((self.n_cases, 6 if self.dim == 3 else 5))
         qi = np.atleast_2d(np.asarray(q_inf, np.float64))
         q[:, :qi.shape[1]] = qi
         mu = None if mu_inf is None else np.ascontiguousarray(np.broadcast_to(mu_inf, (self.n_cases,)), np.float64)
@@ -235,9 +284,13 @@ class Context:
 
     def wave_speeds(self, U):
         U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
-        NI, NJ = self.atlas.shape
-        lam = np.empty((self.n_cases, (NI + 1) * NJ + NI * (NJ + 1)))
-        lsum = np.empty((self.n_cases, NI, NJ))
+        if self.dim == 3:
+            nf = sum(int(np.prod(f.shape[1:])) for f in self.atlas.faces)
+        else:
+            NI, NJ = self.atlas.shape
+            nf = (NI + 1) * NJ + NI * (NJ + 1)
+        lam = np.empty((self.n_cases, nf))
+        lsum = np.empty((self.n_cases,) + tuple(self.atlas.shape))
         _check(self._lib.jr_wave_speeds(self._h, _ptr(U, C.c_double), _ptr(lam, C.c_double), _ptr(lsum, C.c_double)))
         return lam, lsum
 
@@ -326,6 +379,13 @@ class Context:
     def scalars(self) -> np.ndarray:
         out = np.empty((self.n_cases, 9))
         _check(self._lib.jr_get_scalars(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def forces3(self, area: float, length: float = 1.0, centre=(0.25, 0.0, 0.0)) -> np.ndarray:
+        """3D: (cl, cd, cm, cd_friction, fx, fy, fz, cy) per case; moment about y through ``centre``."""
+        out = np.empty((self.n_cases, 8))
+        _check(self._lib.jr_forces3(self._h, C.c_double(area), C.c_double(length), C.c_double(centre[0]),
+                                    C.c_double(centre[2]), _ptr(out, C.c_double)))
         return out
 
     def forces(self, length: float = 1.0, centre=(0.25, 0.0)) -> np.ndarray:
