@@ -6,6 +6,7 @@ Albada, GMRES(30) x 2 with the line preconditioner, FP64).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
     python bench.py --converge [CASE_DIR]
     python bench.py --serve N [--rate R] [--forces-tol T] [--in-flight K]
+    python bench.py --wing [--steps K] [--warmup W]
     python bench.py --calibrate-copy
 
 Prints one JSON line.  A step is one full Newton step of the case (residual,
@@ -217,6 +218,58 @@ def run_steps(case, atlas, precision: str, steps: int, warmup: int, batch: int =
     return ctx, dt, hist
 
 
+WING = {"mach": 0.8, "alpha_deg": 2.0, "reynolds": 1e7}
+
+
+def wing_bench(steps: int, warmup: int) -> dict:
+    """The 3D case: SA RANS on the synthetic swept, tapered wing (NACA
+    thickness law t = 0.10, taper 0.55, LE sweep 28 deg, AR 7) in four
+    point-matched blocks of 192 x 49 x 29 cells (1,091,328), M 0.8, alpha 2,
+    Re 1e7, HLLC + minmod, GMRES(30) x 2, line preconditioner (block Thomas,
+    5x5 flow blocks), FP64 and mixed; Newton steps from the freestream."""
+    from jaxrans_amd.core.config import FlowConditions, ImplicitConfig, NumericsConfig
+    from jaxrans_amd.core.state import q_infinity, uniform_canvas
+    from jaxrans_amd.device import context_for
+    from jaxrans_amd.mesh.atlas3d import build_atlas_3d
+    from jaxrans_amd.mesh.wing import wing_blocks, wing_reference
+
+    t0 = time.perf_counter()
+    kinds = {"wall": "no_slip_wall", "farfield": "far_field", "symmetry": "symmetry"}
+    atlas = build_atlas_3d(tuple(b.with_sides(b.sides.with_conditions(kinds)) for b in wing_blocks()))
+    print(f"[wing] atlas {atlas.shape}, {atlas.cells} cells, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    flow = FlowConditions(**WING, turbulence="sa_noft2")
+    num = NumericsConfig(flux="hllc", limiter="minmod", solver="implicit")
+    ref = wing_reference()
+    out = {"cells": atlas.cells}
+    for precision in ("fp64", "mixed"):
+        cfg = ImplicitConfig(krylov_subspace=30, max_relative_update=0.2, precision=precision)
+        ctx = context_for(atlas, flow, num, cfg)
+        q = q_infinity(flow, dim=3)
+        ctx.set_freestream(q, flow.mu_inf)
+        ctx.set_state(uniform_canvas(q, atlas.shape, flow.gamma))
+        ctx.newton_init(cfg.cfl)
+        if warmup:
+            ctx.newton_block(warmup)
+        ctx.synchronize()
+        t = time.perf_counter()
+        hist = ctx.newton_block(steps)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        probe = ctx.probe_block(2)
+        f = ctx.forces3(ref["area"], ref["length"], ref["centre"])[0]
+        out[precision] = {"value": round(atlas.cells * steps / dt / 1e6, 4), "ms_per_step": round(1e3 * dt / steps, 3),
+                          "residual_first": float(hist[0, 0]), "residual_last": float(hist[-1, 0]),
+                          "cl": float(f[0]), "cd": float(f[1]),
+                          "probe_us": {k: round(v["ms"] * 1e3, 1) for k, v in probe.items() if v["launches"]},
+                          "probe_launches_per_step": {k: v["launches"] / 2 for k, v in probe.items()
+                                                      if v["launches"]},
+                          "device_bytes": ctx.device_bytes}
+        print(f"[wing] {precision}: {json.dumps(out[precision])}", file=sys.stderr, flush=True)
+        ctx.close()
+    return out
+
+
 def converge(case_dir: Path, grid=None) -> dict:
     """The case run to convergence through run() in FP64 and in mixed precision:
     iterations, wall time and the force differences (mixed - FP64)."""
@@ -325,6 +378,8 @@ def main(argv=None) -> int:
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
+    ap.add_argument("--wing", action="store_true",
+                    help="instead of the 2D case: Newton steps of the 3D SA wing (four blocks, 1.09M cells)")
     ap.add_argument("--serve", type=int, metavar="N", default=0,
                     help="instead of the step benchmark: the served stream, first N of its 256 requests")
     ap.add_argument("--rate", type=float, default=0.5, help="--serve: Poisson arrival rate, requests per second")
@@ -354,6 +409,18 @@ def main(argv=None) -> int:
         print(json.dumps({"calibration_copy_bytes": n * 8, "checked": ok, "evict_sum": float(evict[:8].sum())}),
               flush=True)
         return 0 if ok else 1
+    if args.wing:
+        w = wing_bench(args.steps, args.warmup)
+        line = {"metric": "Newton iters x Mcells/s", "value": w["fp64"]["value"], "unit": "Mcell-steps/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": w["fp64"]["ms_per_step"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                "config": {"workload": "3D SA RANS, swept tapered wing (t 0.10, taper 0.55, LE sweep 28, AR 7), "
+                                       "4 blocks of 192x49x29 cells, M 0.8, alpha 2, Re 1e7, HLLC + minmod, "
+                                       "GMRES(30)x2, line_j (block Thomas, 5x5), FP64",
+                           "cells": w["cells"]},
+                "mixed_value": w["mixed"]["value"], "detail": w}
+        print(json.dumps(line), flush=True)
+        return 0
     if args.serve:
         line = {"metric": "solves/min + p50/p95/p99 latency, served stream", "n_gpus": 1,
                 "config": {"workload": f"served stream: first {args.serve} of 256 requests, Poisson {args.rate}/s, "

@@ -115,17 +115,60 @@ __device__ __forceinline__ T node_mean(const T* v) {   // v[(di*2 + dj)*2 + dk]
 
 // Face fluxes of the three families, one thread per face (concatenated
 // numbering).  Faces not adjacent to an interior cell are zero.
-template <class T, int NV, bool VISC>
-__global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
-                                                    const double* __restrict__ Q, const double* __restrict__ dQ,
-                                                    double* __restrict__ F, int check_active) {
-  int f = blockIdx.x * BLOCK + threadIdx.x;
+// phi = (u, v, w, T[, nu_hat]) at the nodes of the diamond gradients: node
+// (ni, nj, nk) is the mean of canvas cells (ni-1..ni, nj-1..nj, nk-1..nk),
+// halving along i, then j, then k (gradients.py:31-40).  One thread per node
+// of the node grid (NI+1)(NJ+1)(NK+1); nodes on the canvas edge are never
+// read and are skipped.  Computing the nodes once per evaluation (instead of
+// 4 x 8 cell loads in every face thread) keeps the face kernel in registers.
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_nodes3(Plan3 P, double gamma, const Ctl* __restrict__ ctl,
+                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                  double* __restrict__ Nd, double* __restrict__ dNd,
+                                                  int check_active) {
+  constexpr int NPHI = NV > 5 ? 5 : 4;
+  const int n = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
-  if (f >= P.nft) return;
   const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
-  int a = 0, ff = f;
-  if (ff >= P.nf[0]) { ff -= P.nf[0]; a = 1; if (ff >= P.nf[1]) { ff -= P.nf[1]; a = 2; } }
+  const int nn = (NI + 1) * (NJ + 1) * (NK + 1);
+  if (n >= nn) return;
+  const int ni = n / ((NJ + 1) * (NK + 1)), rem = n - ni * (NJ + 1) * (NK + 1);
+  const int nj = rem / (NK + 1), nk = rem - nj * (NK + 1);
+  if (ni < 1 || nj < 1 || nk < 1 || ni > NI - 1 || nj > NJ - 1 || nk > NK - 1) return;
+  const size_t base = (size_t)cs * NV * nc;
+  auto phi_at = [&](int i, int j, int k, T* out) {
+    T q[NV];
+    load_q3<T, NV>(Q, dQ, base, nc, (i * NJ + j) * NK + k, q);
+    out[0] = q[1]; out[1] = q[2]; out[2] = q[3]; out[3] = temperature3(q, gamma);
+    if constexpr (NPHI > 4) out[4] = q[5];
+  };
+  T b[2][NPHI];
+  for (int dk2 = 0; dk2 < 2; ++dk2) {
+    T a0[NPHI];
+    for (int dj2 = 0; dj2 < 2; ++dj2) {
+      T p0[NPHI], p1[NPHI];
+      phi_at(ni - 1, nj - 1 + dj2, nk - 1 + dk2, p0);
+      phi_at(ni, nj - 1 + dj2, nk - 1 + dk2, p1);
+      for (int k = 0; k < NPHI; ++k) {
+        const T pair = 0.5 * (p0[k] + p1[k]);
+        if (dj2 == 0) a0[k] = pair;
+        else b[dk2][k] = 0.5 * (a0[k] + pair);
+      }
+    }
+  }
+  const size_t nb = (size_t)cs * NPHI * nn + n;
+  for (int k = 0; k < NPHI; ++k) Io<T>::st(Nd, dNd, nb + (size_t)k * nn, 0.5 * (b[0][k] + b[1][k]));
+}
+
+// The family a is a template parameter so that every index below is known
+// at compile time (a runtime family index sent the index arrays to scratch).
+template <class T, int NV, bool VISC, int a>
+__device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int first_order, const Ctl* __restrict__ ctl,
+                                      const double* __restrict__ Q, const double* __restrict__ dQ,
+                                      const double* __restrict__ Nd, const double* __restrict__ dNd,
+                                      double* __restrict__ F, int f, int ff, int cs) {
+  const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
   const int dj = NJ + (a == 1), dk = NK + (a == 2);
   const int fi = ff / (dj * dk), rem = ff - fi * dj * dk, fj = rem / dk, fk = rem - fj * dk;
   const int idx[3] = {fi, fj, fk};
@@ -208,36 +251,39 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int fi
     double c31[3] = {d3[1] * d1[2] - d3[2] * d1[1], d3[2] * d1[0] - d3[0] * d1[2], d3[0] * d1[1] - d3[1] * d1[0]};
     double c12[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
     const double det = (d1[0] * c23[0] + d1[1] * c23[1]) + d1[2] * c23[2];
-    // phi = (u, v, w, T[, nu_hat])
+    // phi = (u, v, w, T[, nu_hat]); node values streamed in the halving
+    // order of node_mean (pairs along i, then j, then k) so that no more than
+    // two cells and two partial sums are live at once
     constexpr int NPHI = NV > 5 ? 5 : 4;
     auto phi_of = [&](const T* q, T* out) {
       out[0] = q[1]; out[1] = q[2]; out[2] = q[3]; out[3] = temperature3(q, gamma);
       if constexpr (NPHI > 4) out[4] = q[5];
     };
+    const int nn = (NI + 1) * (NJ + 1) * (NK + 1);
+    const size_t nbase = (size_t)cs * NPHI * nn;
     auto node_phi = [&](const int* nd, T* out) {
-      T v[8][NPHI];
-      for (int di = 0; di < 2; ++di)
-        for (int dj2 = 0; dj2 < 2; ++dj2)
-          for (int dk2 = 0; dk2 < 2; ++dk2) {
-            T q[NV];
-            load_q3<T, NV>(Q, dQ, base, nc, ((nd[0] - 1 + di) * NJ + nd[1] - 1 + dj2) * NK + nd[2] - 1 + dk2, q);
-            phi_of(q, v[(di * 2 + dj2) * 2 + dk2]);
-          }
-      for (int k = 0; k < NPHI; ++k) {
-        T w[8];
-        for (int e = 0; e < 8; ++e) w[e] = v[e][k];
-        out[k] = node_mean<T>(w);
-      }
+      const size_t o = nbase + ((size_t)nd[0] * (NJ + 1) + nd[1]) * (NK + 1) + nd[2];
+      for (int k = 0; k < NPHI; ++k) out[k] = Io<T>::ld(Nd, dNd, o + (size_t)k * nn);
     };
-    T p00[NPHI], p11[NPHI], p10[NPHI], p01[NPHI], pl[NPHI], pr[NPHI];
-    node_phi(n00, p00); node_phi(n11, p11); node_phi(n10, p10); node_phi(n01, p01);
-    phi_of(qm, pl);
-    phi_of(qp, pr);
-    T gr[3][NPHI];                      // gr[b][k] = d phi_k / d x_b
-    for (int k = 0; k < NPHI; ++k) {
-      T dp1 = pr[k] - pl[k], dp2 = p11[k] - p00[k], dp3 = p01[k] - p10[k];
-      for (int b = 0; b < 3; ++b) gr[b][k] = (dp1 * c23[b] + dp2 * c31[b] + dp3 * c12[b]) / det;
+    T dp1[NPHI], dp2[NPHI], dp3[NPHI];
+    {
+      T pl[NPHI], pr[NPHI];
+      phi_of(qm, pl);
+      phi_of(qp, pr);
+      for (int k = 0; k < NPHI; ++k) dp1[k] = pr[k] - pl[k];
     }
+    {
+      T pa[NPHI], pb[NPHI];
+      node_phi(n00, pa);
+      node_phi(n11, pb);
+      for (int k = 0; k < NPHI; ++k) dp2[k] = pb[k] - pa[k];
+      node_phi(n10, pa);
+      node_phi(n01, pb);
+      for (int k = 0; k < NPHI; ++k) dp3[k] = pb[k] - pa[k];
+    }
+    T gr[3][NPHI];                      // gr[b][k] = d phi_k / d x_b
+    for (int k = 0; k < NPHI; ++k)
+      for (int bb = 0; bb < 3; ++bb) gr[bb][k] = (dp1[k] * c23[bb] + dp2[k] * c31[bb] + dp3[k] * c12[bb]) / det;
     T gv[3][3], gT[3];
     for (int c = 0; c < 3; ++c) {
       for (int b = 0; b < 3; ++b) gv[c][b] = gr[b][c];
@@ -270,6 +316,21 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int fi
     if (sizeof(T) == sizeof(double)) F[fbase + (size_t)k * P.nft] = val(Fl[k]);
     else F[fbase + (size_t)k * P.nft] = tan_of(Fl[k]);
   }
+}
+
+template <class T, int NV, bool VISC>
+__global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
+                                                    const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                    const double* __restrict__ Nd, const double* __restrict__ dNd,
+                                                    double* __restrict__ F, int check_active) {
+  const int f = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  if (f >= P.nft) return;
+  if (f < P.nf[0]) face3<T, NV, VISC, 0>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f, cs);
+  else if (f < P.nf[0] + P.nf[1])
+    face3<T, NV, VISC, 1>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f - P.nf[0], cs);
+  else face3<T, NV, VISC, 2>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f - P.nf[0] - P.nf[1], cs);
 }
 
 // Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere;

@@ -37,6 +37,7 @@ struct jr_ctx {
   Plan3 P3{};
   double* F3 = nullptr;     // 3D face fluxes [case][var][faces of the three families]
   double* dsw = nullptr;    // 3D line apply: forward-sweep workspace, one field
+  double *Nd = nullptr, *dNd = nullptr;   // 3D viscous: node values of (u, v, w, T[, nu_hat]) and their tangents
   Physics ph{};
   int nv = 4, n_cases = 1;
   int m = 10, restarts = 2, exact = 0, mixed = 0, thomas = 0, cr_threads = 64;
@@ -327,8 +328,12 @@ static void launch_residual3(jr_ctx* c, const double* U, double* out, int first_
   const Plan3& P = c->P3;
   hipLaunchKernelGGL((k_fill3<double, NV>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0,
                      c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
+  const size_t nn = (size_t)(P.NI + 1) * (P.NJ + 1) * (P.NK + 1);
+  if (VISC)
+    hipLaunchKernelGGL((k_nodes3<double, NV>), grid1(nn, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
+                       c->ph.gamma, c->ctl, c->Q, nullptr, c->Nd, nullptr, 0);
   hipLaunchKernelGGL((k_faces3<double, NV, VISC>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
-                     c->ph, first_order, c->ctl, c->Q, nullptr, c->F3, 0);
+                     c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->F3, 0);
   hipLaunchKernelGGL((k_cells3<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
                      c->ctl, U, nullptr, c->Q, nullptr, c->F3, nullptr, out, 0);
 }
@@ -339,8 +344,14 @@ static void launch_tangent3(jr_ctx* c, const double* U, const double* dU, double
   const Plan3& P = c->P3;
   JR_LAUNCH(PR_FILL, (k_fill3<Dual, NV>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0, P,
             c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
+  const size_t nn = (size_t)(P.NI + 1) * (P.NJ + 1) * (P.NK + 1);
+  if (VISC)
+    hipLaunchKernelGGL((k_nodes3<Dual, NV>), grid1(nn, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
+                       (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, (double*)nullptr, c->dNd,
+                       check_active);
   JR_LAUNCH(PR_FACES, (k_faces3<Dual, NV, VISC>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
-            first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, c->F3, check_active);
+            first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Nd,
+            (const double*)c->dNd, c->F3, check_active);
   JR_LAUNCH(PR_CELLS, (k_cells3<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
             (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->F3, vdt, out,
             check_active);
@@ -1082,7 +1093,9 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
       dalloc(c, &c->F3, (size_t)n_cases * c->nv * nf) || dalloc(c, &c->X, F) || dalloc(c, &c->Act, F) ||
       dalloc(c, &c->Rv, F) || dalloc(c, &c->Z, F) || dalloc(c, &c->W, F) || dalloc(c, &c->T, F) ||
       dalloc(c, &c->V, F * (c->m + 1)) || dalloc(c, &c->tmpA, F) || dalloc(c, &c->tmpB, F) ||
-      dalloc(c, &c->dsw, F) || dalloc(c, &c->lam, (size_t)n_cases * nf) || dalloc(c, &c->sam, (size_t)n_cases * nf) ||
+      dalloc(c, &c->dsw, F) || dalloc(c, &c->Nd, (size_t)n_cases * 5 * (g->NI + 1) * (g->NJ + 1) * (g->NK + 1)) ||
+      dalloc(c, &c->dNd, (size_t)n_cases * 5 * (g->NI + 1) * (g->NJ + 1) * (g->NK + 1)) ||
+      dalloc(c, &c->lam, (size_t)n_cases * nf) || dalloc(c, &c->sam, (size_t)n_cases * nf) ||
       dalloc(c, &c->sad, (size_t)n_cases * nf) || dalloc(c, &c->lsum, (size_t)n_cases * nc) ||
       dalloc(c, &c->vdt, (size_t)n_cases * nc) || dalloc(c, &c->sa_diag, (size_t)n_cases * nc))
     return bail(0);
@@ -1115,7 +1128,8 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
   for (const void* p : {(const void*)c->U, (const void*)c->R, (const void*)c->Q, (const void*)c->dQ,
                         (const void*)c->F3, (const void*)c->X, (const void*)c->Act, (const void*)c->Rv,
                         (const void*)c->Z, (const void*)c->W, (const void*)c->T, (const void*)c->V,
-                        (const void*)c->tmpA, (const void*)c->tmpB, (const void*)c->dsw, (const void*)c->lam,
+                        (const void*)c->tmpA, (const void*)c->tmpB, (const void*)c->dsw, (const void*)c->Nd,
+                        (const void*)c->dNd, (const void*)c->lam,
                         (const void*)c->sam, (const void*)c->sad, (const void*)c->lsum, (const void*)c->vdt,
                         (const void*)c->sa_diag, (const void*)c->ctl, (const void*)c->hv, (const void*)c->h2v,
                         (const void*)c->G, (const void*)c->H, (const void*)c->y, (const void*)c->hy,

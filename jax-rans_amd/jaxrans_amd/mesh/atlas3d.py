@@ -28,6 +28,7 @@ tasks applied on the way, in order -- the form the device fill evaluates.
 
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -473,24 +474,37 @@ def patch_distance(p: np.ndarray, quads: np.ndarray, iterations: int = 12) -> np
     return best
 
 
-def wall_distance_3d(mesh, walls: Walls3D, candidates: int = 32, chunk: int = 8192) -> list[np.ndarray]:
+def _nearest_patches(args):
+    pts, mids, corners, k = args
+    from scipy.spatial import cKDTree
+    _, idx = cKDTree(mids).query(pts, k=k)
+    return patch_distance(pts[:, None, :], corners[idx.reshape(len(pts), k)]).min(axis=1)
+
+
+def wall_distance_3d(mesh, walls: Walls3D, candidates: int = 32, chunk: int = 8192,
+                     workers: int | None = None) -> list[np.ndarray]:
     """Distance from every cell centre to the nearest wall quad: the exact
     patch distance over the ``candidates`` quads whose corner-mean is
     nearest (a k-d tree); the reference ranks candidates by the distance to
-    each quad's two triangles instead (``wall3d.py:140-158``)."""
-    from scipy.spatial import cKDTree
+    each quad's two triangles instead (``wall3d.py:140-158``).  Large grids
+    spread the chunks over a process pool (``workers``: default the CPU
+    count, at most 16)."""
     if walls.n == 0:
         return [np.full(b.shape, np.inf) for b in mesh]
-    tree = cKDTree(walls.mid)
     k = min(candidates, walls.n)
-    out = []
-    for blk in mesh:
-        pts = np.moveaxis(blk.centres, 0, -1).reshape(-1, 3)
-        d = np.empty(len(pts))
-        for s in range(0, len(pts), chunk):
-            p = pts[s:s + chunk]
-            _, idx = tree.query(p, k=k)
-            idx = idx.reshape(len(p), k)
-            d[s:s + chunk] = patch_distance(p[:, None, :], walls.corners[idx]).min(axis=1)
-        out.append(d.reshape(blk.shape))
+    pts = np.concatenate([np.moveaxis(b.centres, 0, -1).reshape(-1, 3) for b in mesh])
+    jobs = [(pts[s:s + chunk], walls.mid, walls.corners, k) for s in range(0, len(pts), chunk)]
+    n = min(16, os.cpu_count() or 1) if workers is None else workers
+    if n > 1 and len(jobs) > 4:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(n) as pool:
+            parts = pool.map(_nearest_patches, jobs, chunksize=1)
+    else:
+        parts = [_nearest_patches(j) for j in jobs]
+    d = np.concatenate(parts)
+    out, s = [], 0
+    for b in mesh:
+        n_b = int(np.prod(b.shape))
+        out.append(d[s:s + n_b].reshape(b.shape))
+        s += n_b
     return out
