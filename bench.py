@@ -308,7 +308,7 @@ SERVE_MODES = {"fp64_serial": ("fp64", False), "fp64_served": ("fp64", True), "m
 
 
 def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iters: int, driver: str = "threads",
-                modes=("fp64_serial", "fp64_served", "mixed_served")) -> dict:
+                modes=("fp64_serial", "fp64_served", "mixed_served"), slots: int = 4) -> dict:
     """The first ``n`` requests of the 256-request stream (Poisson arrivals at
     ``rate`` per second, numpy default_rng(3)) served three ways: the plain
     FP64 path one solve at a time; served FP64; served mixed, ``in_flight``
@@ -322,7 +322,7 @@ def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iter
     from jaxrans_amd import serve
     reqs = serve.make_requests(256, rate)[:n]
     cfg = serve.default_config("fp64", max_iters=max_iters, forces_tol=forces_tol, max_in_flight=in_flight,
-                               driver=driver)
+                               driver=driver, slots=slots)
     t0 = time.perf_counter()
     workers = max(1, min(16, os.cpu_count() or 1, n))
     with mp.get_context("fork").Pool(workers) as pool:
@@ -386,8 +386,10 @@ def main(argv=None) -> int:
     ap.add_argument("--forces-tol", type=float, default=1e-5, help="--serve: settle tolerance on CL, CD, CM")
     ap.add_argument("--in-flight", type=int, default=8, help="--serve: solves on the device together")
     ap.add_argument("--max-iters", type=int, default=4000, help="--serve: Newton budget per solve")
-    ap.add_argument("--driver", choices=("threads", "async"), default="threads",
-                    help="--serve: a host thread per solve, or one thread polling asynchronous blocks")
+    ap.add_argument("--driver", choices=("threads", "async", "slots"), default="threads",
+                    help="--serve: a host thread per solve, one thread polling asynchronous blocks, or one batched "
+                         "context per grid size with the requests in its slots (shared launches)")
+    ap.add_argument("--slots", type=int, default=4, help="--serve --driver slots: slots per grid size")
     ap.add_argument("--serve-modes", default="fp64_serial,fp64_served,mixed_served",
                     help=f"--serve: comma-separated runs of {sorted(SERVE_MODES)}; the first is the reference "
                          f"for the force differences")
@@ -428,9 +430,9 @@ def main(argv=None) -> int:
                                        "{257x65, 513x129, 769x193, 1025x257} nodes, M U[0.3,0.8], alpha U[-2,6], "
                                        "Re log-U[1e6,2e7], SA, Roe + van_albada_smooth, GMRES(30)x2, line_j",
                            "forces_tol": args.forces_tol, "max_iters": args.max_iters, "in_flight": args.in_flight,
-                           "driver": args.driver},
+                           "driver": args.driver, "slots": args.slots},
                 **serve_bench(args.serve, args.rate, args.forces_tol, args.in_flight, args.max_iters, args.driver,
-                              tuple(args.serve_modes.split(",")))}
+                              tuple(args.serve_modes.split(",")), args.slots)}
         print(json.dumps(line), flush=True)
         return 0
     if args.converge:

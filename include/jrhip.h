@@ -132,9 +132,20 @@ int jr_ctx_destroy(jr_ctx* ctx);
  * and mu_inf[n_cases] (NULL: inviscid) (src/jaxrans/core/state.py:176-192). */
 int jr_set_freestream(jr_ctx* ctx, const double* q_inf, const double* mu_inf);
 
-/* State on the device, host copies in/out: [n_cases][nvar][NI][NJ]. */
+/* State on the device, host copies in/out: [n_cases][nvar][NI][NJ];
+ * jr_set_state_case writes one case's [nvar][NI][NJ]. */
 int jr_set_state(jr_ctx* ctx, const double* U_host);
+int jr_set_state_case(jr_ctx* ctx, int case_index, const double* U_host);
 int jr_get_state(jr_ctx* ctx, double* U_host);
+
+/* A different grid of the same topology (identical index plans: stages,
+ * chains, walls, neighbours; only coordinates differ) for one case of a 2D
+ * context: its areas, face vectors, centres, wall distance, boundary normals
+ * and wall midpoints.  The first call gives every case its own geometry.
+ * One batch then steps different geometries in shared launches, as the
+ * reference's slot scheduler steps different freestreams on one grid
+ * (src/jaxrans/sweep.py:172-327).  Refuses a grid of another topology. */
+int jr_set_geometry(jr_ctx* ctx, int case_index, const jr_grid_desc* grid);
 
 /* R(U) (ResidualOperator.residual, src/jaxrans/numerics/residual.py:324-327);
  * first_order selects first_order_faces (reconstruction.py:71-80). */
@@ -161,6 +172,8 @@ int jr_precond_apply(jr_ctx* ctx, const double* r_host, double* z_host);
 /* Newton control: start every case at cfl0[n_cases] with |R|_prev = inf
  * (ImplicitState, src/jaxrans/solvers/implicit.py:35-51). */
 int jr_newton_init(jr_ctx* ctx, const double* cfl0);
+/* The same for one case (a slot refilled between blocks, sweep.py:229-264). */
+int jr_newton_init_case(jr_ctx* ctx, int case_index, double cfl0);
 
 /* n_steps pseudo-transient Newton-Krylov steps on the device state
  * (one block of run_to_steady, src/jaxrans/solvers/driver.py:87-104);

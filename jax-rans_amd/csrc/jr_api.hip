@@ -83,7 +83,40 @@ struct jr_ctx {
   std::vector<int> probe_id;      // kernel id of each recorded pair
   int probe_used = 0;
   bool has_q = false;
+  bool per_case_geo = false;   // geometry arrays per case (jr_set_geometry)
+  uint64_t topo_hash = 0;      // hash of the index plans: grids of one topology share them
 };
+
+// FNV-1a over the index plans of a grid (stages, chains, walls, neighbours,
+// interior): two grids with the same hash share every index plan, so one
+// batch can hold both with per-case geometry.
+static uint64_t topology_hash(const jr_grid_desc* g) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t bytes) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  const size_t nc = (size_t)g->NI * g->NJ;
+  mix(&g->NI, sizeof(g->NI));
+  mix(&g->NJ, sizeof(g->NJ));
+  mix(g->interior, nc);
+  mix(g->neighbours, nc * sizeof(int64_t));
+  for (int s = 0; s < 2; ++s) {
+    mix(&g->n_task[s], sizeof(int32_t));
+    mix(g->task_dest[s], g->n_task[s] * sizeof(int64_t));
+    mix(g->task_src[s], g->n_task[s] * sizeof(int64_t));
+    mix(g->task_kind[s], g->n_task[s] * sizeof(int32_t));
+  }
+  mix(&g->n_lines, sizeof(int32_t));
+  mix(&g->n_slots, sizeof(int32_t));
+  mix(g->line_start, ((size_t)g->n_lines + 1) * sizeof(int64_t));
+  for (const int64_t* a : {g->line_cells, g->line_up_face, g->line_up_sign, g->line_down_face, g->line_down_sign})
+    mix(a, (size_t)g->n_slots * sizeof(int64_t));
+  mix(&g->n_walls, sizeof(int32_t));
+  for (const int64_t* a : {g->wall_cell, g->wall_family, g->wall_face}) mix(a, (size_t)g->n_walls * sizeof(int64_t));
+  mix(g->wall_into, (size_t)g->n_walls * sizeof(double));
+  return h;
+}
 
 template <class T>
 static int dalloc(jr_ctx* c, T** p, size_t count) {
@@ -617,6 +650,7 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   P.nfi = (g->NI + 1) * g->NJ;
   P.nfj = g->NI * (g->NJ + 1);
   const size_t nc = P.nc;
+  c->topo_hash = topology_hash(g);
   if (upload(c, &P.area, g->area, nc) || upload(c, &P.fi, g->face_i, 2 * (size_t)P.nfi) ||
       upload(c, &P.fj, g->face_j, 2 * (size_t)P.nfj) ||
       upload(c, &P.cen, g->centres, 2 * (size_t)(g->NI + 2) * (g->NJ + 2)) ||
@@ -1221,6 +1255,67 @@ int jr_set_state(jr_ctx* c, const double* U) {
   return 0;
 }
 
+int jr_set_state_case(jr_ctx* c, int k, const double* U) {
+  if (!c || !U) return fail("jr_set_state_case: null argument");
+  if (k < 0 || k >= c->n_cases) return fail("jr_set_state_case: case out of range");
+  HIPCHK(hipMemcpyAsync(c->U + (size_t)k * c->N, U, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Geometry of case k from a grid of the context's topology.  The first call
+// gives every case its own copy of the geometry arrays (filled with the
+// context's grid), so the other cases keep theirs.
+int jr_set_geometry(jr_ctx* c, int k, const jr_grid_desc* g) {
+  if (!c || !g) return fail("jr_set_geometry: null argument");
+  if (c->dim != 2) return fail("jr_set_geometry: 2D contexts only");
+  if (k < 0 || k >= c->n_cases) return fail("jr_set_geometry: case out of range");
+  if (g->NI != c->P.NI || g->NJ != c->P.NJ || topology_hash(g) != c->topo_hash)
+    return fail("jr_set_geometry: the grid's index plans differ from the context's (another topology)");
+  if (c->pending_steps) return fail("jr_set_geometry: an asynchronous block is pending");
+  Plan& P = c->P;
+  const size_t nc = P.nc;
+  const size_t n_area = nc, n_fi = 2 * (size_t)P.nfi, n_fj = 2 * (size_t)P.nfj,
+               n_cen = 2 * (size_t)(P.NI + 2) * (P.NJ + 2), n_tn0 = 2 * (size_t)std::max(1, P.n_task[0]),
+               n_tn1 = 2 * (size_t)std::max(1, P.n_task[1]), n_wm = 2 * (size_t)std::max(1, P.n_walls);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->per_case_geo) {
+    // one copy per case, each starting as the shared geometry
+    auto widen = [&](const double** field, size_t count, size_t* stride) -> int {
+      double* d = nullptr;
+      if (dalloc(c, &d, count * c->n_cases)) return -1;
+      for (int q = 0; q < c->n_cases; ++q)
+        HIPCHK(hipMemcpy(d + q * count, *field, count * sizeof(double), hipMemcpyDeviceToDevice));
+      *field = d;
+      *stride = count;
+      return 0;
+    };
+    if (widen(&P.area, n_area, &P.gs_area) || widen(&P.fi, n_fi, &P.gs_fi) || widen(&P.fj, n_fj, &P.gs_fj) ||
+        widen(&P.cen, n_cen, &P.gs_cen) || widen(&P.dist, n_area, &P.gs_dist) ||
+        widen(&P.t_norm[0], n_tn0, &P.gs_tn0) || widen(&P.t_norm[1], n_tn1, &P.gs_tn1) ||
+        widen(&P.w_mid, n_wm, &P.gs_wmid))
+      return -1;
+    c->per_case_geo = true;
+    drop_graph(c);
+  }
+  std::vector<double> dist(nc);
+  for (size_t i = 0; i < nc; ++i) dist[i] = g->distance[i];
+  auto put = [&](const double* field, size_t stride, const double* src, size_t count) -> int {
+    if (!src) return fail("jr_set_geometry: a geometry array is null");
+    HIPCHK(hipMemcpy((double*)field + (size_t)k * stride, src, count * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  };
+  TRY(put(P.area, P.gs_area, g->area, n_area));
+  TRY(put(P.fi, P.gs_fi, g->face_i, n_fi));
+  TRY(put(P.fj, P.gs_fj, g->face_j, n_fj));
+  TRY(put(P.cen, P.gs_cen, g->centres, n_cen));
+  TRY(put(P.dist, P.gs_dist, dist.data(), n_area));
+  if (P.n_task[0]) TRY(put(P.t_norm[0], P.gs_tn0, g->task_normal[0], 2 * (size_t)P.n_task[0]));
+  if (P.n_task[1]) TRY(put(P.t_norm[1], P.gs_tn1, g->task_normal[1], 2 * (size_t)P.n_task[1]));
+  if (P.n_walls) TRY(put(P.w_mid, P.gs_wmid, g->wall_mid, 2 * (size_t)P.n_walls));
+  return 0;
+}
+
 int jr_get_state(jr_ctx* c, double* U) {
   if (!c || !U) return fail("jr_get_state: null argument");
   HIPCHK(hipMemcpyAsync(U, c->U, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
@@ -1311,6 +1406,21 @@ int jr_newton_init(jr_ctx* c, const double* cfl0) {
     h[k].accept = 0;
     h[k].omega = 1.0;
   }
+  return put_ctl(c, h);
+}
+
+int jr_newton_init_case(jr_ctx* c, int k, double cfl0) {
+  if (!c) return fail("jr_newton_init_case: null context");
+  if (k < 0 || k >= c->n_cases) return fail("jr_newton_init_case: case out of range");
+  if (!(cfl0 > 0.0)) return fail("jr_newton_init_case: cfl must be positive");
+  std::vector<Ctl> h;
+  TRY(get_ctl(c, h));
+  h[k].cfl = cfl0;
+  h[k].res_prev = INFINITY;
+  h[k].steps = 0;
+  h[k].rejected = 0;
+  h[k].accept = 0;
+  h[k].omega = 1.0;
   return put_ctl(c, h);
 }
 

@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, 
                                                  double* __restrict__ X, double* __restrict__ Act,
                                                  double* __restrict__ Rv, Red red, double* __restrict__ hist,
                                                  int n_cases) {
+  bind_case(P, blockIdx.y);
   __shared__ double sh[3][4];
   const int cs = blockIdx.y;
   const int nc = P.nc;
@@ -639,6 +640,7 @@ template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_forces(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                   const double* __restrict__ Q, double length, double cx, double cy,
                                                   double* __restrict__ out) {
+  bind_case(P, blockIdx.x);
   __shared__ double sh[6][4];
   const int cs = blockIdx.x;
   const size_t base = (size_t)cs * NV * P.nc;

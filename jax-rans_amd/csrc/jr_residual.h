@@ -55,6 +55,7 @@ template <class T, int NV>
 __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl* __restrict__ ctl,
                                                 const double* __restrict__ U, const double* __restrict__ dU,
                                                 double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
+  bind_case(P, blockIdx.y);
   int t = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
@@ -101,6 +102,7 @@ __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl*
 template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_ghost_jac(Plan P, double gamma, const Ctl* __restrict__ ctl,
                                                      const double* __restrict__ Q, double* __restrict__ jac) {
+  bind_case(P, blockIdx.y);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
   if (q >= P.n_final) return;
@@ -143,6 +145,7 @@ template <class T, int NV, bool VISC>
 __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
                                                  double* __restrict__ Fi, double* __restrict__ Fj, int check_active) {
+  bind_case(P, blockIdx.y);
   int f = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
@@ -297,6 +300,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
                                                  const double* __restrict__ Fi, const double* __restrict__ Fj,
                                                  const double* __restrict__ vdt, double* __restrict__ out,
                                                  int check_active) {
+  bind_case(P, blockIdx.y);
   int c = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;

@@ -97,6 +97,7 @@ template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_face_speeds(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ U, double* __restrict__ lam,
                                                        double* __restrict__ sam, double* __restrict__ sad) {
+  bind_case(P, blockIdx.y);
   int f = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
   const int nf = P.nfi + P.nfj;
@@ -150,6 +151,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_speeds(Plan P, Physics ph, const
                                                        const double* __restrict__ sam, const double* __restrict__ sad,
                                                        double* __restrict__ lsum, double* __restrict__ vdt,
                                                        double* __restrict__ sa_diag) {
+  bind_case(P, blockIdx.y);
   int c = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
   if (c >= P.nc) return;
@@ -270,6 +272,7 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
                                                   const double* __restrict__ lsum, const double* __restrict__ sam,
                                                   const double* __restrict__ sad, const double* __restrict__ sa_diag,
                                                   double* __restrict__ blk) {
+  bind_case(P, blockIdx.y);
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
   if (s >= P.n_slots) return;

@@ -2,6 +2,7 @@
 // the C-ABI layer.  Plain structs of device pointers: a plan is uploaded once
 // per grid, a case block per solve.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -57,7 +58,23 @@ struct Plan {
   const int* w_face;
   const double* w_into;
   const double* w_mid;             // [2][n_walls]
+  // Geometry per case: case cs reads area + cs * gs_area, and so on (all 0
+  // when the cases share the canvas geometry).  Per-case geometry lets one
+  // batch hold different grids of one topology (the same index plans).
+  size_t gs_area, gs_fi, gs_fj, gs_cen, gs_dist, gs_tn0, gs_tn1, gs_wmid;
 };
+
+// The geometry of case cs (identity when the cases share it).
+__device__ __forceinline__ void bind_case(Plan& P, int cs) {
+  P.area += cs * P.gs_area;
+  P.fi += cs * P.gs_fi;
+  P.fj += cs * P.gs_fj;
+  P.cen += cs * P.gs_cen;
+  P.dist += cs * P.gs_dist;
+  P.t_norm[0] += cs * P.gs_tn0;
+  P.t_norm[1] += cs * P.gs_tn1;
+  P.w_mid += cs * P.gs_wmid;
+}
 
 struct Physics {
   double gamma;

@@ -26,7 +26,7 @@ EXPORTS = (
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
-    "jr_last_error",
+    "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_last_error",
 )
 
 
@@ -257,6 +257,26 @@ class Context:
         mu = None if mu_inf is None else np.ascontiguousarray(np.broadcast_to(mu_inf, (self.n_cases,)), np.float64)
         _check(self._lib.jr_set_freestream(self._h, _ptr(q, C.c_double),
                                            None if mu is None else _ptr(mu, C.c_double)))
+
+    def set_geometry(self, k: int, atlas) -> None:
+        """Case ``k`` steps on ``atlas``: another grid of this context's
+        topology (same index plans, other coordinates)."""
+        keep = []
+
+        def arr(a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a
+
+        g = self._desc2(atlas, arr)
+        _check(self._lib.jr_set_geometry(self._h, int(k), C.byref(g)))
+
+    def set_state_case(self, k: int, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape[1:]), np.float64)
+        _check(self._lib.jr_set_state_case(self._h, int(k), _ptr(U, C.c_double)))
+
+    def newton_init_case(self, k: int, cfl0: float) -> None:
+        _check(self._lib.jr_newton_init_case(self._h, int(k), C.c_double(cfl0)))
 
     def set_state(self, U: np.ndarray) -> None:
         U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)

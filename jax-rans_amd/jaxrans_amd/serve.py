@@ -97,11 +97,13 @@ class ServeConfig:
     forces_tol: float | None = 1e-5
     max_in_flight: int = 8
     reserve_bytes: int = 8 << 30          # device memory kept free
-    driver: str = "threads"               # "threads": a host thread per solve; "async": one thread polls all
+    driver: str = "threads"               # "threads": a host thread per solve; "async": one thread polls all;
+    #                                       "slots": one batched context per grid size, requests in its slots
+    slots: int = 4                        # "slots": cases per batched context (at most)
 
     def __post_init__(self) -> None:
-        if self.driver not in ("threads", "async"):
-            raise ValueError(f"driver must be 'threads' or 'async', got {self.driver!r}")
+        if self.driver not in ("threads", "async", "slots"):
+            raise ValueError(f"driver must be 'threads', 'async' or 'slots', got {self.driver!r}")
 
 
 def request_grid(req: Request, cfg: ServeConfig):
@@ -228,9 +230,16 @@ def serve(requests: list[Request], atlases: dict, cfg: ServeConfig, clock=time.p
     thread through blocking blocks (the library call releases the
     interpreter lock, so the threads enqueue and wait concurrently, each on
     its context's stream); with ``driver="async"`` this thread enqueues
-    every solve's blocks asynchronously and polls them."""
+    every solve's blocks asynchronously and polls them.  With
+    ``driver="slots"`` the solves share launches: requests of one grid size
+    (one topology) are packed into the slots of one batched context, each
+    slot with its own geometry, freestream and Newton control, refilled
+    between blocks as slots free up (the reference's slot scheduler,
+    ``src/jaxrans/sweep.py:172-327``, with per-slot grids)."""
     if cfg.driver == "threads":
         return _serve_threads(requests, atlases, cfg, clock, on_done)
+    if cfg.driver == "slots":
+        return _serve_slots(requests, atlases, cfg, clock, on_done)
     t0 = clock()
     waiting = sorted(requests, key=lambda r: r.arrival)
     running: list[Solve] = []
@@ -326,6 +335,99 @@ def _serve_threads(requests, atlases, cfg: ServeConfig, clock, on_done) -> list[
                 on_done(out)
     for t in threads:
         t.join()
+    return done
+
+
+def _serve_slots(requests, atlases, cfg: ServeConfig, clock, on_done) -> list[Outcome]:
+    """One batched context per grid size, ``cfg.slots`` slots each (fewer if
+    the free device memory less the reserve does not hold them), marched by
+    one host thread per size; a slot is refilled from the requests of its
+    size that have arrived, between blocks."""
+    t0 = clock()
+    by_size: dict = {}
+    for r in sorted(requests, key=lambda r: r.arrival):
+        by_size.setdefault(r.cells, []).append(r)
+    lock = threading.Lock()
+    done: list[Outcome] = []
+    errors: list = []
+
+    def run_size(queue: list) -> None:
+        first = queue[0]
+        flow0 = request_flow(first, cfg)
+        probe = context_for(atlases[first.rid], flow0, cfg.numerics, cfg.implicit, n_cases=1)
+        per_case = probe.device_bytes
+        probe.close()
+        free, _ = mem_info()
+        with lock:
+            n = int(max(1, min(cfg.slots, len(queue), (free - cfg.reserve_bytes) // max(per_case, 1))))
+        ctx = context_for(atlases[first.rid], flow0, cfg.numerics, cfg.implicit, n_cases=n)
+        nq = 5
+        qs, mus = np.zeros((n, nq)), np.full(n, flow0.mu_inf if not flow0.inviscid else 0.0)
+        q0 = q_infinity(flow0)
+        qs[:, :q0.size] = q0
+        slot: list = [None] * n
+        waiting = list(queue)
+        ctx.set_freestream(qs, None if flow0.inviscid else mus)
+        shape = atlases[first.rid].shape
+        ctx.set_state(np.broadcast_to(uniform_canvas(q0, shape, flow0.gamma), (n,) + ctx.shape[1:]))
+        ctx.newton_init(np.full(n, cfg.implicit.cfl))
+        try:
+            while waiting or any(s is not None for s in slot):
+                now = clock() - t0
+                changed = False
+                for k in range(n):
+                    if slot[k] is None and waiting and waiting[0].arrival <= now:
+                        req = waiting.pop(0)
+                        flow = request_flow(req, cfg)
+                        q = q_infinity(flow)
+                        qs[k] = 0.0
+                        qs[k, :q.size] = q
+                        mus[k] = 0.0 if flow.inviscid else flow.mu_inf
+                        ctx.set_geometry(k, atlases[req.rid])
+                        ctx.set_state_case(k, uniform_canvas(q, shape, flow.gamma))
+                        ctx.newton_init_case(k, cfg.implicit.cfl)
+                        slot[k] = Solve(req, ctx, now)
+                        changed = True
+                if changed:
+                    ctx.set_freestream(qs, None if flow0.inviscid else mus)
+                if not any(s is not None for s in slot):
+                    time.sleep(max(0.0, min(0.01, waiting[0].arrival - (clock() - t0))))
+                    continue
+                res = ctx.newton_block(cfg.block_iters)
+                forces = ctx.forces(FORCE_REF[0], FORCE_REF[1])
+                for k in range(n):
+                    s = slot[k]
+                    if s is None:
+                        continue
+                    if s.block_done(res[:, k:k + 1], forces[k:k + 1], cfg):
+                        cl, cd, cm = s.coefficients[-1]
+                        orders = (float(np.log10(s.peak / s.last_residual))
+                                  if s.peak > 0 and s.last_residual > 0 else 0.0)
+                        out = Outcome(s.req.rid, s.req.arrival, s.started, clock() - t0, s.iters, s.reason, cl, cd,
+                                      cm, int(atlases[s.req.rid].cells), orders)
+                        slot[k] = None
+                        with lock:
+                            done.append(out)
+                            if on_done is not None:
+                                on_done(out)
+        finally:
+            ctx.close()
+
+    def work(queue):
+        try:
+            run_size(queue)
+        except BaseException as e:          # reported by the serving thread
+            with lock:
+                errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(q,), name=f"slots-{size}", daemon=True)
+               for size, q in by_size.items()]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
     return done
 
 

@@ -63,7 +63,7 @@ def _small_stream():
     return reqs, cfg, {r.rid: serve.request_grid(r, cfg) for r in reqs}
 
 
-@pytest.mark.parametrize("driver", ["threads", "async"])
+@pytest.mark.parametrize("driver", ["threads", "async", "slots"])
 def test_served_results_are_the_same_alone_and_in_the_mix(driver):
     """Five requests of two grid sizes, all arriving at once, served five
     in flight (their blocks overlap on the device, each on its own stream)
@@ -83,3 +83,45 @@ def test_served_results_are_the_same_alone_and_in_the_mix(driver):
         assert (a.iterations, a.reason, a.cells) == (m.iterations, m.reason, m.cells), rid
         assert (a.cl, a.cd, a.cm) == (m.cl, m.cd, m.cm), (rid, a, m)
         assert np.isfinite(a.cl) and a.iterations == cfg.max_iters
+
+
+def test_batched_geometries_equal_each_grid_alone():
+    """Three different sections on C-grids of one topology in one context
+    (per-case geometry, jr_set_geometry) give, case by case, bit for bit the
+    residual histories, states and forces of each grid in its own context."""
+    print("[gpu] batched geometries", flush=True)
+    reqs = serve.make_requests(3, 0.0, seed=21, sizes=((96, 24),))
+    cfg = serve.default_config("fp64", wall_spacing=1e-3, far_field=15.0)
+    atlases = [serve.request_grid(r, cfg) for r in reqs]
+    flows = [serve.request_flow(r, cfg) for r in reqs]
+    from jaxrans_amd.core.state import q_infinity, uniform_canvas
+    imp = ImplicitConfig(krylov_subspace=12, krylov_restarts=2, max_relative_update=0.2)
+    batch = context_for(atlases[0], flows[0], cfg.numerics, imp, n_cases=3)
+    qs = np.zeros((3, 5))
+    for k, (a, f) in enumerate(zip(atlases, flows)):
+        q = q_infinity(f)
+        qs[k, :q.size] = q
+        batch.set_geometry(k, a)
+    batch.set_freestream(qs, np.array([f.mu_inf for f in flows]))
+    U0 = [uniform_canvas(q_infinity(f), a.shape, f.gamma) for a, f in zip(atlases, flows)]
+    for k in range(3):
+        batch.set_state_case(k, U0[k])
+        batch.newton_init_case(k, imp.cfl)
+    hb = batch.newton_block(6)
+    Fb = batch.forces()
+    Ub = batch.get_state()
+    other = serve.request_grid(serve.make_requests(1, 0.0, seed=5, sizes=((128, 32),))[0], cfg)
+    from jaxrans_amd.device import DeviceError
+    with pytest.raises(DeviceError, match="topology"):
+        batch.set_geometry(0, other)
+    batch.close()
+    for k, (a, f) in enumerate(zip(atlases, flows)):
+        alone = context_for(a, f, cfg.numerics, imp)
+        alone.set_freestream(qs[k], f.mu_inf)
+        alone.set_state(U0[k])
+        alone.newton_init(imp.cfl)
+        ha = alone.newton_block(6)
+        assert np.array_equal(ha[:, 0], hb[:, k]), (k, ha[:, 0], hb[:, k])
+        assert np.array_equal(alone.get_state()[0], Ub[k]), k
+        assert np.array_equal(alone.forces()[0], Fb[k]), k
+        alone.close()
