@@ -218,32 +218,38 @@ def run_steps(case, atlas, precision: str, steps: int, warmup: int, batch: int =
     return ctx, dt, hist
 
 
-WING = {"mach": 0.8, "alpha_deg": 2.0, "reynolds": 1e7}
-
 
 def wing_bench(steps: int, warmup: int) -> dict:
     """The 3D case: SA RANS on the synthetic swept, tapered wing (NACA
     thickness law t = 0.10, taper 0.55, LE sweep 28 deg, AR 7) in four
     point-matched blocks of 192 x 49 x 29 cells (1,091,328), M 0.8, alpha 2,
     Re 1e7, HLLC + minmod, GMRES(30) x 2, line preconditioner (block Thomas,
-    5x5 flow blocks), FP64 and mixed; Newton steps from the freestream."""
-    from jaxrans_amd.core.config import FlowConditions, ImplicitConfig, NumericsConfig
+    5x5 flow blocks), FP64 and mixed; Newton steps from the freestream.  The
+    grid goes through the case path: written as multi-block Plot3D, read back
+    by load_case (domain.grid "file") and assembled."""
+    import shutil
+    import tempfile
+    from dataclasses import replace
+
+    from jaxrans_amd import load_case
     from jaxrans_amd.core.state import q_infinity, uniform_canvas
     from jaxrans_amd.device import context_for
-    from jaxrans_amd.mesh.atlas3d import build_atlas_3d
-    from jaxrans_amd.mesh.wing import wing_blocks, wing_reference
+    from jaxrans_amd.mesh.wing import write_wing_grid
 
     t0 = time.perf_counter()
-    kinds = {"wall": "no_slip_wall", "farfield": "far_field", "symmetry": "symmetry"}
-    atlas = build_atlas_3d(tuple(b.with_sides(b.sides.with_conditions(kinds)) for b in wing_blocks()))
-    print(f"[wing] atlas {atlas.shape}, {atlas.cells} cells, built in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
-    flow = FlowConditions(**WING, turbulence="sa_noft2")
-    num = NumericsConfig(flux="hllc", limiter="minmod", solver="implicit")
-    ref = wing_reference()
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in ("case.json", "numerical.json"):
+            shutil.copy(ROOT / "cases" / "wing_sa" / name, tmp)
+        case = load_case(tmp)
+        write_wing_grid(Path(tmp) / case.grid_spec.file)
+        atlas = case.grid()
+    print(f"[wing] grid written, read back from Plot3D and assembled: atlas {atlas.shape}, {atlas.cells} cells, "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    flow, num, spec = case.flow, case.numerics, case.grid_spec
+    ref = {"area": spec.reference_area, "length": spec.reference_length, "centre": tuple(spec.moment_centre)}
     out = {"cells": atlas.cells}
     for precision in ("fp64", "mixed"):
-        cfg = ImplicitConfig(krylov_subspace=30, max_relative_update=0.2, precision=precision)
+        cfg = replace(case.implicit, precision=precision)
         ctx = context_for(atlas, flow, num, cfg)
         q = q_infinity(flow, dim=3)
         ctx.set_freestream(q, flow.mu_inf)

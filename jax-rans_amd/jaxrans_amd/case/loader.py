@@ -3,7 +3,9 @@
 Keys, defaults, cross-key rules and error messages follow the reference
 loader (``src/jaxrans/case/loader.py:31-373``, ``case/domain.py:404-538``), so
 ``load_case`` is a drop-in.  Grids are generated here (``o_grid``,
-``c_grid``); the ``file`` and ``multi_element`` sources are refused.
+``c_grid``) or read from a 3D Plot3D file (``file``, ``case/domain.py:148-154``:
+patches named by ``domain.patches`` box rules); 2D grid files and the
+``multi_element`` source are refused.
 """
 
 from __future__ import annotations
@@ -20,12 +22,15 @@ from ..core import constants
 from ..core.config import FlowConditions, ImplicitConfig, NumericsConfig
 from ..geometry.airfoil import Airfoil, airfoil_source
 from ..mesh.assemble import assemble
+from ..mesh.assemble3d import PatchRule, is_3d, load_grid_3d
 from ..mesh.atlas import Atlas, assign_conditions, build_atlas, patches
+from ..mesh.atlas3d import build_atlas_3d
 from ..mesh.block import folded
 from ..mesh.generate import c_grid, o_grid
 from . import schema
 
 CASE_FILE, NUMERICAL_FILE = "case.json", "numerical.json"
+PATCH_RULE_KEYS = ("name", "x_min", "x_max", "y_min", "y_max", "z_min", "z_max")
 
 
 class CaseError(ValueError):
@@ -66,10 +71,37 @@ class GridSpec:
             foil = foil.closed()
         return foil
 
+    def rules(self) -> tuple:
+        out = []
+        for k, rule in enumerate(self.patches):
+            if not isinstance(rule, dict) or "name" not in rule:
+                raise CaseError(f"'domain.patches[{k}]' must be an object with a 'name'")
+            unknown = sorted(set(rule) - set(PATCH_RULE_KEYS) - {"notes"})
+            if unknown:
+                raise CaseError(f"unknown key(s) {unknown} in 'domain.patches[{k}]'; allowed: {list(PATCH_RULE_KEYS)}")
+            out.append(PatchRule(**{key: v for key, v in rule.items() if key != "notes"}))
+        return tuple(out)
+
+    def resolve(self, base: Path) -> Path:
+        """The grid file, relative to the case directory or the working directory."""
+        for root in (Path(base), Path.cwd()):
+            path = Path(self.file) if Path(self.file).is_absolute() else root / self.file
+            if path.is_file():
+                return path
+        raise FileNotFoundError(f"{self.file!r} not found in {base} or {Path.cwd()}")
+
     def build(self, base: Path):
+        if self.grid == "file":
+            path = self.resolve(base)
+            if not is_3d(path, self.format):
+                raise CaseError(f"{path.name}: 2D grid files are not read by this implementation (3D Plot3D only)")
+            try:
+                return load_grid_3d(path, self.format, self.coordinate_scale, self.rules())
+            except ValueError as error:
+                raise CaseError(f"{path.name}: {error}") from None
         if self.grid not in ("o_grid", "c_grid"):
             raise CaseError(f"domain.grid {self.grid!r} is not built by this implementation "
-                            f"(generated o_grid and c_grid only)")
+                            f"(generated o_grid and c_grid, 3D grid files)")
         foil = self.airfoil_shape(base)
         common = dict(cells_normal=self.cells_normal, wall_spacing=self.wall_spacing, far_field=self.far_field,
                       le=self.leading_edge_spacing, te=self.trailing_edge_spacing)
@@ -84,6 +116,8 @@ class GridSpec:
         return assemble(raw)
 
     def describe(self) -> str:
+        if self.grid == "file":
+            return f"file {Path(self.file).name}"
         wake = f" + 2x{self.cells_wake} wake" if self.grid == "c_grid" else ""
         return (f"{self.grid} round {self.airfoil}: {self.cells_airfoil}{wake} x {self.cells_normal} cells, "
                 f"wall spacing {self.wall_spacing:g}, far field {self.far_field:g}")
@@ -113,6 +147,10 @@ class Case:
         mesh = self.grid_spec.build(self.directory)
         kinds = resolve_conditions(self.conditions, patches(mesh), self.flow)
         mesh = assign_conditions(mesh, kinds)
+        if mesh[0].dim == 3:
+            if self.grid_spec.reference_area is None:
+                raise CaseError("a 3D case needs domain.reference_area")
+            return build_atlas_3d(mesh, self.numerics.n_ghost)
         if any(folded(b) for b in mesh):
             raise CaseError("the grid has folded cells")
         return build_atlas(mesh, self.numerics.n_ghost)

@@ -90,3 +90,43 @@ def wing_reference(aspect_ratio: float = 7.0, taper: float = 0.55) -> dict:
     semi, area = wing_planform(aspect_ratio, taper)
     mac = 2.0 / 3.0 * (1.0 + taper + taper ** 2) / (1.0 + taper)
     return {"area": area, "length": mac, "centre": (0.25, 0.0, 0.0), "semi_span": semi}
+
+
+def wing_patch_rules(aspect_ratio: float = 7.0, taper: float = 0.55) -> tuple:
+    """Box rules naming the wing grid's boundary faces when it is read back
+    from a grid file: the root and tip planes ``symmetry``, the section's
+    surface ``wall`` (every unmatched face within a chord of it), the rest
+    ``farfield``."""
+    from .assemble3d import PatchRule
+
+    semi, _ = wing_planform(aspect_ratio, taper)
+    reach = semi * np.tan(np.deg2rad(40.0)) + 2.0
+    return (PatchRule("symmetry", y_max=1e-9), PatchRule("symmetry", y_min=semi - 1e-9),
+            PatchRule("wall", x_min=-1.0, x_max=reach, z_min=-1.0, z_max=1.0), PatchRule("farfield"))
+
+
+def write_wing_grid(path, **kw):
+    """The wing grid's blocks as a multi-block Plot3D file."""
+    from .plot3d import write_plot3d_3d
+
+    return write_plot3d_3d(path, [g.nodes for g in wing_blocks(**kw)])
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="write the synthetic wing grid as multi-block Plot3D")
+    ap.add_argument("path")
+    ap.add_argument("--cells-airfoil", type=int, default=128)
+    ap.add_argument("--cells-wake", type=int, default=32)
+    ap.add_argument("--cells-normal", type=int, default=49)
+    ap.add_argument("--cells-span", type=int, default=29)
+    ap.add_argument("--blocks", type=int, default=4)
+    ap.add_argument("--wall-spacing", type=float, default=1e-6)
+    ap.add_argument("--far-field", type=float, default=50.0)
+    a = ap.parse_args()
+    out = write_wing_grid(a.path, cells_airfoil=a.cells_airfoil, cells_wake=a.cells_wake,
+                          cells_normal=a.cells_normal, cells_span=a.cells_span, blocks=a.blocks,
+                          wall_spacing=a.wall_spacing, far_field=a.far_field)
+    print(f"wrote {out} ({a.blocks} blocks of {a.cells_airfoil + 2 * a.cells_wake} x {a.cells_normal} x "
+          f"{a.cells_span} cells)")

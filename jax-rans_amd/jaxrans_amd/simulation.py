@@ -90,8 +90,24 @@ class Forces:
         return f"CL = {self.cl:+.6f}   CD = {self.cd:+.6f}{friction}   CM = {self.cm:+.6f}"
 
 
-def _forces(row: np.ndarray) -> Forces:
+def _forces(row: np.ndarray, dim: int = 2) -> Forces:
+    if dim == 3:
+        return Forces(float(row[0]), float(row[1]), float(row[2]), float(row[4]), float(row[5]), float(row[3]),
+                      fz=float(row[6]), cy=float(row[7]))
     return Forces(float(row[0]), float(row[1]), float(row[2]), float(row[4]), float(row[5]), float(row[3]))
+
+
+def force_rows(ctx: Context, case: Case) -> np.ndarray:
+    """The coefficient rows of every case on the device, by the case's force
+    reference (``post/forces.py:98-135``): per unit span about ``(x, y)`` in
+    2D; per ``reference_area`` with the moment about the ``y`` axis through
+    ``moment_centre`` in 3D."""
+    spec = case.grid_spec
+    if ctx.dim == 3:
+        if spec.reference_area is None:
+            raise CaseError("a 3D case needs domain.reference_area")
+        return ctx.forces3(spec.reference_area, spec.reference_length, tuple(spec.moment_centre))
+    return ctx.forces(spec.reference_length, tuple(spec.moment_centre[:2]))
 
 
 @dataclass(frozen=True)
@@ -172,8 +188,8 @@ def make_context(case: Case, grid: Atlas, n_cases: int = 1) -> Context:
     return context_for(grid, case.flow, case.numerics, case.implicit, n_cases=n_cases)
 
 
-def freestream_of(flow: FlowConditions) -> tuple[np.ndarray, float | None]:
-    return q_infinity(flow), (None if flow.inviscid else flow.mu_inf)
+def freestream_of(flow: FlowConditions, dim: int = 2) -> tuple[np.ndarray, float | None]:
+    return q_infinity(flow, dim), (None if flow.inviscid else flow.mu_inf)
 
 
 def run_to_steady(ctx: Context, *, max_iters: int, tol: float, block_iters: int,
@@ -238,7 +254,7 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) 
     flow, numerics = case.flow, case.numerics
     spec = case.grid_spec
     ctx = make_context(case, grid)
-    q, mu = freestream_of(flow)
+    q, mu = freestream_of(flow, grid.dim)
     ctx.set_freestream(q, mu)
     U0 = uniform_canvas(q, grid.shape, flow.gamma)
     if initial is not None:
@@ -249,13 +265,12 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) 
     ctx.set_state(U0)
     ctx.newton_init(case.implicit.cfl)
     cl_history, coefficients, peak = [], [], [0.0]
-    centre = tuple(spec.moment_centre[:2])
 
     def on_block(index: int, residuals: np.ndarray) -> bool:
         finite = residuals[np.isfinite(residuals)]
         if finite.size:
             peak[0] = max(peak[0], float(finite.max()))
-        f = _forces(ctx.forces(spec.reference_length, centre)[0])
+        f = _forces(force_rows(ctx, case)[0], grid.dim)
         cl_history.append(f.cl)
         coefficients.append((f.cl, f.cd, f.cm))
         if progress is not None:
@@ -268,7 +283,7 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) 
     history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
                             on_block=on_block)
     U = ctx.get_state()[0]
-    forces = _forces(ctx.forces(spec.reference_length, centre)[0])
+    forces = _forces(force_rows(ctx, case)[0], grid.dim)
     timing = Timing(history.seconds, history.first_block_seconds, history.n_iters, case.block_iters, grid.cells)
     result = Result(case=case, grid=grid, U=U, forces=forces, history=history, cl_history=np.asarray(cl_history),
                     timing=timing, extra={"scalars": ctx.scalars()[0]})
@@ -292,33 +307,33 @@ def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None 
         re = case.flow.reynolds if reynolds is None else float(reynolds)
         flows.append(replace(case.flow, alpha_deg=float(a), mach=float(m), reynolds=re))
     ctx = make_context(case, grid, n_cases=n)
-    qs = np.stack([q_infinity(f) for f in flows])
-    q5 = np.zeros((n, 5))
+    qs = np.stack([q_infinity(f, grid.dim) for f in flows])
+    q5 = np.zeros((n, 5 if grid.dim == 2 else 6))
     q5[:, :qs.shape[1]] = qs
     mus = None if case.flow.inviscid else np.array([f.mu_inf for f in flows])
     ctx.set_freestream(q5, mus)
     U0 = np.stack([uniform_canvas(q, grid.shape, f.gamma) for q, f in zip(qs, flows)])
     ctx.set_state(U0)
     ctx.newton_init(np.full(n, case.implicit.cfl))
-    spec, numerics = case.grid_spec, case.numerics
-    centre = tuple(spec.moment_centre[:2])
+    numerics = case.numerics
     cl_hist = []
 
     def on_block(index, residuals):
-        cl_hist.append(ctx.forces(spec.reference_length, centre)[:, 0].copy())
+        cl_hist.append(force_rows(ctx, case)[:, 0].copy())
         return False
 
     history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
                             on_block=on_block)
     U = ctx.get_state()
-    F = ctx.forces(spec.reference_length, centre)
+    F = force_rows(ctx, case)
     cl_hist = np.asarray(cl_hist)
     timing = Timing(history.seconds, history.first_block_seconds, history.n_iters, case.block_iters, grid.cells,
                     cases=n)
     out = []
     for k in range(n):
         h = history.case(k) if n > 1 else history
-        out.append(Result(case=replace(case, flow=flows[k]), grid=grid, U=U[k], forces=_forces(F[k]), history=h,
+        out.append(Result(case=replace(case, flow=flows[k]), grid=grid, U=U[k], forces=_forces(F[k], grid.dim),
+                          history=h,
                           cl_history=cl_hist[:, k] if cl_hist.size else cl_hist, timing=timing))
     ctx.close()
     return out

@@ -163,3 +163,34 @@ def test_forces_3d_match_oracle(name, kind, flowf, flux, lim):
         assert got[k] == pytest.approx(ref[key], rel=1e-11, abs=1e-14), key
     assert got[7] == pytest.approx(ref["cy"], rel=1e-11, abs=1e-14)
     ctx.close()
+
+
+def test_run_on_the_m6_file_case_matches_oracle_newton_steps():
+    """``run`` on the reference's ONERA M6 example read from its Plot3D file
+    (a pole, an O seam, Sutherland viscosity, SA): four Newton steps in two
+    blocks against the oracle's, and the forces by the case's reference area,
+    length and moment centre."""
+    from pathlib import Path
+
+    from jaxrans_amd.case.loader import load_case
+    from jaxrans_amd.simulation import run
+
+    print("[gpu] run m6 file case", flush=True)
+    directory = Path(__file__).resolve().parent / "golden" / "cases" / "onera_m6_sa"
+    case = load_case(directory, ("numerical.convergence.max_iters=4", "numerical.convergence.block_iters=2"))
+    result = run(case)
+    st = O3.setup_for(result.grid, case.flow, case.numerics)
+    U0 = O3.U0_of(st)
+    s = OS.NewtonState(U0, case.implicit.cfl)
+    norms = []
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(4):
+            s, n, _ = S3.newton_step(s, st, case.implicit, "thomas")
+            norms.append(n)
+    assert result.history.n_iters == 4
+    assert np.allclose(result.history.residual, norms, rtol=1e-9, atol=0.0), (result.history.residual, norms)
+    assert rel(result.U - U0, s.U - U0) < 1e-8
+    spec = case.grid_spec
+    ref = S3.forces(s.U, st, spec.reference_area, spec.reference_length, tuple(spec.moment_centre))
+    for key in ("cl", "cd", "cm", "cd_friction", "cy"):
+        assert getattr(result.forces, key) == pytest.approx(ref[key], rel=1e-8, abs=1e-12), key
