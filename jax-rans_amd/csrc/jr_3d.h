@@ -680,17 +680,130 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
   }
 }
 
-// z = M^-1 r (z += M^-1 r when accumulating), one thread per chain; the
-// forward sweep keeps d in z, the backward sweep overwrites it with x
-// (linalg.py:103-132).  With vout the input is the Arnoldi vector w/|w|,
-// also written to vout.  Workgroups past the chains copy the cells on no
-// chain (identity there).
+// z = M^-1 r (z += M^-1 r when accumulating), one thread per chain
+// (linalg.py:103-132): the forward sweep writes d to tmp, the backward sweep
+// reads it back and writes x.  With vout the input is the Arnoldi vector
+// w/|w|, also written to vout.  Workgroups past the chains copy the cells on
+// no chain (identity there).
+//
+// A chain is a dependent sequence, so each step's time is its load latency
+// unless the loads run ahead.  Two register stages alternate: while one
+// step computes from its stage, the loads of the next step (its factor
+// blocks, the right-hand side of its cell, the SA scalars) are in flight
+// into the other, and the cell index one step further is loaded too.  A
+// scheduling barrier after each stage's loads keeps the compiler from
+// moving the previous stage's arithmetic (and its waits) above them.
 template <class R, int NV>
+struct FwdStage {
+  R L[25], Pm[25];
+  double rhs[NV];
+  R ls, ps;
+  int cell;
+};
+
+template <class R, int NV>
+struct BwdStage {
+  R C[25];
+  double d[NV], z[NV];
+  R cs;
+  int cell;
+};
+
+template <class R, int NV>
+__device__ __forceinline__ void fwd_load(FwdStage<R, NV>& s, const R* __restrict__ o, const R* __restrict__ os,
+                                         const double* __restrict__ rb, int cell, int nl, int nc) {
+#pragma unroll
+  for (int e = 0; e < 25; ++e) {
+    s.L[e] = o[(size_t)e * nl];
+    s.Pm[e] = o[(size_t)(25 + e) * nl];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s.rhs[v] = rb[(size_t)v * nc + cell];
+  if constexpr (NV > 5) {
+    s.ls = os[0];
+    s.ps = os[nl];
+  }
+  s.cell = cell;
+}
+
+// t = r_k - L_k d_{k-1}, d_k = P_k^-1 t (and the SA scalar), d_k to D.
+template <class R, int NV>
+__device__ __forceinline__ void fwd_step(FwdStage<R, NV>& s, R (&dp)[5], R& dps, double* __restrict__ D,
+                                         double* __restrict__ vout, size_t base, double den, int nc) {
+  if (vout) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double x = s.rhs[v] / den;
+      vout[base + (size_t)v * nc + s.cell] = x;
+      s.rhs[v] = x;
+    }
+  }
+  R t[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    R acc = (R)s.rhs[i];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc -= s.L[i * 5 + q] * dp[q];
+    t[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    R acc = R(0);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc += s.Pm[i * 5 + q] * t[q];
+    dp[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) D[(size_t)i * nc + s.cell] = (double)dp[i];
+  if constexpr (NV > 5) {
+    dps = s.ps * ((R)s.rhs[5] - s.ls * dps);
+    D[(size_t)5 * nc + s.cell] = (double)dps;
+  }
+}
+
+template <bool ACC, class R, int NV>
+__device__ __forceinline__ void bwd_load(BwdStage<R, NV>& s, const R* __restrict__ o, const R* __restrict__ os,
+                                         const double* __restrict__ D, const double* __restrict__ zb, int cell,
+                                         int nl, int nc) {
+#pragma unroll
+  for (int e = 0; e < 25; ++e) s.C[e] = o[(size_t)e * nl];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    s.d[v] = D[(size_t)v * nc + cell];
+    s.z[v] = ACC ? zb[(size_t)v * nc + cell] : 0.0;
+  }
+  if constexpr (NV > 5) s.cs = os[2 * nl];
+  s.cell = cell;
+}
+
+// x_k = d_k - C_k x_{k+1}, written (or added) to z.
+template <bool ACC, class R, int NV>
+__device__ __forceinline__ void bwd_step(const BwdStage<R, NV>& s, R (&xn)[5], R& xns, double* __restrict__ zb,
+                                         int nc) {
+  R x[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    R acc = (R)s.d[i];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc -= s.C[i * 5 + q] * xn[q];
+    x[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    xn[i] = x[i];
+    zb[(size_t)i * nc + s.cell] = ACC ? s.z[i] + (double)x[i] : (double)x[i];
+  }
+  if constexpr (NV > 5) {
+    xns = (R)s.d[5] - s.cs * xns;
+    zb[(size_t)5 * nc + s.cell] = ACC ? s.z[5] + (double)xns : (double)xns;
+  }
+}
+
+template <class R, int NV, bool ACC>
 __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
                                                const R* __restrict__ fac_sa, const double* __restrict__ r,
-                                               double* __restrict__ z, double* __restrict__ tmp, int accumulate,
-                                               int check_active, double* __restrict__ vout,
-                                               const double* __restrict__ p3, int nblk) {
+                                               double* __restrict__ z, double* __restrict__ tmp, int check_active,
+                                               double* __restrict__ vout, const double* __restrict__ p3, int nblk) {
   const int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc, nl = P.n_lines;
@@ -704,63 +817,65 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
     for (int v = 0; v < NV; ++v) {
       const size_t o = base + (size_t)v * nc + cell;
       const double x = apply_input(r, vout, o, den);
-      z[o] = accumulate ? z[o] + x : x;
+      z[o] = ACC ? z[o] + x : x;
     }
     return;
   }
   const int ch = blockIdx.x * 64 + threadIdx.x;
   if (ch >= nl) return;
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0;
+  if (n <= 0) return;
+  const int* __restrict__ lc = P.l_cell + s0;
   const R* F = fac + (size_t)cs * P.max_len * 75 * nl + ch;
   const R* Fs = fac_sa + (size_t)cs * P.max_len * 3 * nl + ch;
-  double* D = tmp + base;               // forward-sweep d, per cell of the chain
+  const double* rb = r + base;
+  double* D = tmp + base;
+  double* zb = z + base;
+  const size_t step = (size_t)75 * nl, sstep = (size_t)3 * nl;
+
   R dp[5] = {R(0), R(0), R(0), R(0), R(0)};
   R dps = R(0);
-  for (int k = 0; k < n; ++k) {
-    const int cell = P.l_cell[s0 + k];
-    const R* o = F + (size_t)k * 75 * nl;
-    R rhs[5], t[5];
-    for (int v = 0; v < 5; ++v) rhs[v] = (R)apply_input(r, vout, base + (size_t)v * nc + cell, den);
-    for (int i = 0; i < 5; ++i) {
-      R acc = rhs[i];
-      for (int q = 0; q < 5; ++q) acc -= o[(size_t)(i * 5 + q) * nl] * dp[q];
-      t[i] = acc;
+  // Loads are unconditional (indices clamped to the chain) so that every
+  // stage is loaded on every path and the next step's loads stay in flight
+  // across the current step's arithmetic.
+  {
+    FwdStage<R, NV> A, B;
+    fwd_load(A, F, Fs, rb, lc[0], nl, nc);
+    int nxt = lc[min(1, n - 1)];
+    int k = 0;
+    for (; k + 1 < n; k += 2) {
+      const int nxt2 = lc[min(k + 2, n - 1)];
+      fwd_load(B, F + (size_t)(k + 1) * step, Fs + (size_t)(k + 1) * sstep, rb, nxt, nl, nc);
+      __builtin_amdgcn_sched_barrier(0);
+      fwd_step(A, dp, dps, D, vout, base, den, nc);
+      const int k2 = min(k + 2, n - 1);
+      nxt = lc[min(k + 3, n - 1)];
+      fwd_load(A, F + (size_t)k2 * step, Fs + (size_t)k2 * sstep, rb, nxt2, nl, nc);
+      __builtin_amdgcn_sched_barrier(0);
+      fwd_step(B, dp, dps, D, vout, base, den, nc);
     }
-    for (int i = 0; i < 5; ++i) {
-      R acc = R(0);
-      for (int q = 0; q < 5; ++q) acc += o[(size_t)(25 + i * 5 + q) * nl] * t[q];
-      dp[i] = acc;
-    }
-    for (int i = 0; i < 5; ++i) D[(size_t)i * nc + cell] = (double)dp[i];
-    if constexpr (NV > 5) {
-      const R rs = (R)apply_input(r, vout, base + (size_t)5 * nc + cell, den);
-      const R* os = Fs + (size_t)k * 3 * nl;
-      dps = os[nl] * (rs - os[0] * dps);
-      D[(size_t)5 * nc + cell] = (double)dps;
-    }
+    if (k < n) fwd_step(A, dp, dps, D, vout, base, den, nc);
   }
   R xn[5] = {R(0), R(0), R(0), R(0), R(0)};
   R xns = R(0);
-  for (int k = n - 1; k >= 0; --k) {
-    const int cell = P.l_cell[s0 + k];
-    const R* o = F + (size_t)k * 75 * nl;
-    R x[5];
-    for (int i = 0; i < 5; ++i) {
-      R acc = (R)D[(size_t)i * nc + cell];
-      for (int q = 0; q < 5; ++q) acc -= o[(size_t)(50 + i * 5 + q) * nl] * xn[q];
-      x[i] = acc;
+  {
+    BwdStage<R, NV> A, B;
+    const R* C0 = F + (size_t)50 * nl;
+    bwd_load<ACC>(A, C0 + (size_t)(n - 1) * step, Fs + (size_t)(n - 1) * sstep, D, zb, lc[n - 1], nl, nc);
+    int nxt = lc[max(n - 2, 0)];
+    int k = n - 1;
+    for (; k > 0; k -= 2) {
+      const int nxt2 = lc[max(k - 2, 0)];
+      bwd_load<ACC>(B, C0 + (size_t)(k - 1) * step, Fs + (size_t)(k - 1) * sstep, D, zb, nxt, nl, nc);
+      __builtin_amdgcn_sched_barrier(0);
+      bwd_step<ACC>(A, xn, xns, zb, nc);
+      const int k2 = max(k - 2, 0);
+      nxt = lc[max(k - 3, 0)];
+      bwd_load<ACC>(A, C0 + (size_t)k2 * step, Fs + (size_t)k2 * sstep, D, zb, nxt2, nl, nc);
+      __builtin_amdgcn_sched_barrier(0);
+      bwd_step<ACC>(B, xn, xns, zb, nc);
     }
-    for (int i = 0; i < 5; ++i) {
-      xn[i] = x[i];
-      const size_t oz = base + (size_t)i * nc + cell;
-      z[oz] = accumulate ? z[oz] + (double)x[i] : (double)x[i];
-    }
-    if constexpr (NV > 5) {
-      const R* os = Fs + (size_t)k * 3 * nl;
-      xns = (R)D[(size_t)5 * nc + cell] - os[2 * nl] * xns;
-      const size_t oz = base + (size_t)5 * nc + cell;
-      z[oz] = accumulate ? z[oz] + (double)xns : (double)xns;
-    }
+    if (k == 0) bwd_step<ACC>(A, xn, xns, zb, nc);
   }
 }
 

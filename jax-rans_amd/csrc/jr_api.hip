@@ -413,9 +413,14 @@ static void launch_apply3(jr_ctx* c, const double* r, double* z, int accumulate,
                           double* vout = nullptr) {
   const Plan3& P = c->P3;
   const unsigned groups = (unsigned)((P.n_lines + 63) / 64 + (P.n_free + 63) / 64);
-  JR_LAUNCH(PR_APPLY, (k_apply3<R, NV>), dim3(groups, c->n_cases), dim3(64), 0, P, (const Ctl*)c->ctl,
-            (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, accumulate, check_active, vout,
-            (const double*)c->cg.p3, c->cg.nblk);
+  if (accumulate)
+    JR_LAUNCH(PR_APPLY, (k_apply3<R, NV, true>), dim3(groups, c->n_cases), dim3(64), 0, P, (const Ctl*)c->ctl,
+              (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, check_active, vout, (const double*)c->cg.p3,
+              c->cg.nblk);
+  else
+    JR_LAUNCH(PR_APPLY, (k_apply3<R, NV, false>), dim3(groups, c->n_cases), dim3(64), 0, P, (const Ctl*)c->ctl,
+              (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, check_active, vout, (const double*)c->cg.p3,
+              c->cg.nblk);
 }
 
 template <int NV, bool VISC, int MM, class R>
