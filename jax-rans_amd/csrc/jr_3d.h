@@ -45,6 +45,7 @@ struct Plan3 {
   const int* w_face;
   const double* w_into;
   const double* w_mid;             // [3][n_walls]
+  const double* dg;                // [10][nft] each face's diamond: d2 x d3, d3 x d1, d1 x d2, det (viscous)
 };
 
 __device__ __forceinline__ void face_vec3(const Plan3& P, int f, double* s) {
@@ -161,6 +162,76 @@ __global__ void __launch_bounds__(BLOCK) k_nodes3(Plan3 P, double gamma, const C
   for (int k = 0; k < NPHI; ++k) Io<T>::st(Nd, dNd, nb + (size_t)k * nn, 0.5 * (b[0][k] + b[1][k]));
 }
 
+// The diamond of one face of family a (gradients.py:59-95): the cell line
+// d1 from the left to the right cell centre and the node diagonals d2 =
+// node(1,1) - node(0,0), d3 = node(0,1) - node(1,0) over the face's running
+// axes, node (ni, nj, nk) the mean of canvas centres (ni-1..ni, nj-1..nj,
+// nk-1..nk); out = d2 x d3, d3 x d1, d1 x d2 and det = d1 . (d2 x d3).
+template <int a>
+__device__ __forceinline__ void diamond3(const Plan3& P, int fi, int fj, int fk, int cL, double* out) {
+  const int NJ = P.NJ, NK = P.NK, NI = P.NI;
+  const int iL = cL / (NJ * NK), jL = (cL / NK) % NJ, kL = cL % NK;
+  double d1[3], d2[3], d3[3];
+  const size_t cstride = (size_t)(NI + 2) * (NJ + 2) * (NK + 2);
+  for (int c = 0; c < 3; ++c)
+    d1[c] = P.cen[c * cstride + pc3(P, fi, fj, fk)] - P.cen[c * cstride + pc3(P, iL, jL, kL)];
+  const int r1 = a == 0 ? 1 : 0, r2 = a == 2 ? 1 : 2;
+  auto node_at = [&](int s1, int s2, int* nd) {
+    nd[0] = fi; nd[1] = fj; nd[2] = fk;
+    nd[r1] += s1;
+    nd[r2] += s2;
+  };
+  auto node_centre = [&](const int* nd, int c) -> double {
+    double v[8];
+    for (int di = 0; di < 2; ++di)
+      for (int dj2 = 0; dj2 < 2; ++dj2)
+        for (int dk2 = 0; dk2 < 2; ++dk2)
+          v[(di * 2 + dj2) * 2 + dk2] = P.cen[c * cstride + pc3(P, nd[0] - 1 + di, nd[1] - 1 + dj2, nd[2] - 1 + dk2)];
+    return node_mean<double>(v);
+  };
+  int n00[3], n11[3], n10[3], n01[3];
+  node_at(0, 0, n00); node_at(1, 1, n11); node_at(1, 0, n10); node_at(0, 1, n01);
+  for (int c = 0; c < 3; ++c) {
+    d2[c] = node_centre(n11, c) - node_centre(n00, c);
+    d3[c] = node_centre(n01, c) - node_centre(n10, c);
+  }
+  out[0] = d2[1] * d3[2] - d2[2] * d3[1];
+  out[1] = d2[2] * d3[0] - d2[0] * d3[2];
+  out[2] = d2[0] * d3[1] - d2[1] * d3[0];
+  out[3] = d3[1] * d1[2] - d3[2] * d1[1];
+  out[4] = d3[2] * d1[0] - d3[0] * d1[2];
+  out[5] = d3[0] * d1[1] - d3[1] * d1[0];
+  out[6] = d1[1] * d2[2] - d1[2] * d2[1];
+  out[7] = d1[2] * d2[0] - d1[0] * d2[2];
+  out[8] = d1[0] * d2[1] - d1[1] * d2[0];
+  out[9] = (d1[0] * out[0] + d1[1] * out[1]) + d1[2] * out[2];
+}
+
+template <int a>
+__device__ __forceinline__ void diamond3_face(const Plan3& P, double* __restrict__ dg, int f, int ff) {
+  const int NI = P.NI, NJ = P.NJ, NK = P.NK;
+  const int dj = NJ + (a == 1), dk = NK + (a == 2);
+  const int fi = ff / (dj * dk), rem = ff - fi * dj * dk, fj = rem / dk, fk = rem - fj * dk;
+  const int idx[3] = {fi, fj, fk};
+  const int ext[3] = {NI, NJ, NK};
+  const int stride[3] = {NJ * NK, NK, 1};
+  double out[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (idx[a] >= 1 && idx[a] <= ext[a] - 1) {
+    const int cR = (fi * NJ + fj) * NK + fk, cL = cR - stride[a];
+    if (P.interior[cL] || P.interior[cR]) diamond3<a>(P, fi, fj, fk, cL, out);
+  }
+  for (int k = 0; k < 10; ++k) dg[(size_t)k * P.nft + f] = out[k];
+}
+
+// The diamonds of every face, once per context (the geometry is fixed).
+__global__ void __launch_bounds__(BLOCK) k_diamond3(Plan3 P, double* __restrict__ dg) {
+  const int f = blockIdx.x * BLOCK + threadIdx.x;
+  if (f >= P.nft) return;
+  if (f < P.nf[0]) diamond3_face<0>(P, dg, f, f);
+  else if (f < P.nf[0] + P.nf[1]) diamond3_face<1>(P, dg, f, f - P.nf[0]);
+  else diamond3_face<2>(P, dg, f, f - P.nf[0] - P.nf[1]);
+}
+
 // The family a is a template parameter so that every index below is known
 // at compile time (a runtime family index sent the index arrays to scratch).
 template <class T, int NV, bool VISC, int a>
@@ -219,13 +290,15 @@ __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int fir
 
   if (VISC) {
     const double mu_inf = ctl[cs].mu_inf;
-    const int iL = cL / (NJ * NK), jL = (cL / NK) % NJ, kL = cL % NK;
-    // the diamond: the cell line and the face's two node diagonals; node
-    // (ni, nj, nk) is the mean of canvas cells (ni-1..ni, nj-1..nj, nk-1..nk)
-    double d1[3], d2[3], d3[3];
-    const size_t cstride = (size_t)(NI + 2) * (NJ + 2) * (NK + 2);
-    for (int c = 0; c < 3; ++c)
-      d1[c] = P.cen[c * cstride + pc3(P, fi, fj, fk)] - P.cen[c * cstride + pc3(P, iL, jL, kL)];
+    // the diamond's geometry (state independent, k_diamond3): the cross
+    // products of the cell line d1 and the two node diagonals d2, d3
+    double c23[3], c31[3], c12[3];
+    for (int b = 0; b < 3; ++b) {
+      c23[b] = P.dg[(size_t)b * P.nft + f];
+      c31[b] = P.dg[(size_t)(3 + b) * P.nft + f];
+      c12[b] = P.dg[(size_t)(6 + b) * P.nft + f];
+    }
+    const double det = P.dg[(size_t)9 * P.nft + f];
     // the face's running axes in ascending order; shifts (s1, s2) along them
     const int r1 = a == 0 ? 1 : 0, r2 = a == 2 ? 1 : 2;
     auto node_at = [&](int s1, int s2, int* nd) {
@@ -233,24 +306,8 @@ __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int fir
       nd[r1] += s1;
       nd[r2] += s2;
     };
-    auto node_centre = [&](const int* nd, int c) -> double {
-      double v[8];
-      for (int di = 0; di < 2; ++di)
-        for (int dj2 = 0; dj2 < 2; ++dj2)
-          for (int dk2 = 0; dk2 < 2; ++dk2)
-            v[(di * 2 + dj2) * 2 + dk2] = P.cen[c * cstride + pc3(P, nd[0] - 1 + di, nd[1] - 1 + dj2, nd[2] - 1 + dk2)];
-      return node_mean<double>(v);
-    };
     int n00[3], n11[3], n10[3], n01[3];
     node_at(0, 0, n00); node_at(1, 1, n11); node_at(1, 0, n10); node_at(0, 1, n01);
-    for (int c = 0; c < 3; ++c) {
-      d2[c] = node_centre(n11, c) - node_centre(n00, c);
-      d3[c] = node_centre(n01, c) - node_centre(n10, c);
-    }
-    double c23[3] = {d2[1] * d3[2] - d2[2] * d3[1], d2[2] * d3[0] - d2[0] * d3[2], d2[0] * d3[1] - d2[1] * d3[0]};
-    double c31[3] = {d3[1] * d1[2] - d3[2] * d1[1], d3[2] * d1[0] - d3[0] * d1[2], d3[0] * d1[1] - d3[1] * d1[0]};
-    double c12[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
-    const double det = (d1[0] * c23[0] + d1[1] * c23[1]) + d1[2] * c23[2];
     // phi = (u, v, w, T[, nu_hat]); node values streamed in the halving
     // order of node_mean (pairs along i, then j, then k) so that no more than
     // two cells and two partial sums are live at once
@@ -294,7 +351,7 @@ __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int fir
     T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
     T kc = mu / (PR * (gamma - 1.0));
     T extra[3][3];
-    bool has_extra = false;
+    bool has_extra = false;      // two inlined calls below keep extra in registers
     if constexpr (NV > 5) {
       T face[6];
       for (int k = 0; k < 6; ++k) face[k] = 0.5 * (qm[k] + qp[k]);
@@ -309,7 +366,8 @@ __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int fir
       Fl[5] = Fl[0] * up - coef * dot3(gn, s);
     }
     T fv[5];
-    viscous_flux3<T>(vel, gv, gT, s, mu, kc, has_extra ? extra : nullptr, fv);
+    if (has_extra) viscous_flux3<T>(vel, gv, gT, s, mu, kc, extra, fv);
+    else viscous_flux3<T>(vel, gv, gT, s, mu, kc, nullptr, fv);
     for (int k = 0; k < 5; ++k) Fl[k] = Fl[k] - fv[k];
   }
   for (int k = 0; k < NV; ++k) {

@@ -959,6 +959,12 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
     fail("jr_ctx_create: control upload");
     return bail(0);
   }
+  // every zero fill and upload (null stream) is complete before the first
+  // launch on the context's non-blocking stream
+  if (hipDeviceSynchronize() != hipSuccess) {
+    fail("jr_ctx_create: synchronize");
+    return bail(0);
+  }
   *out = c;
   return 0;
 }
@@ -1187,6 +1193,23 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
     fail("jr_ctx_create3: events");
     return bail(0);
   }
+  if (num->viscous) {
+    // the diamonds of the viscous face gradients depend on the grid only
+    double* dg = nullptr;
+    if (dalloc(c, &dg, (size_t)10 * P.nft)) return bail(0);
+    P.dg = dg;
+    // the zero fills of dalloc run on the null stream, which the context's
+    // non-blocking stream does not wait for
+    if (hipDeviceSynchronize() != hipSuccess) {
+      fail("jr_ctx_create3: synchronize");
+      return bail(0);
+    }
+    hipLaunchKernelGGL(k_diamond3, dim3((P.nft + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, P, dg);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+      fail("jr_ctx_create3: face diamonds");
+      return bail(0);
+    }
+  }
   std::vector<Ctl> h(n_cases);
   for (auto& x : h) {
     std::memset(&x, 0, sizeof(Ctl));
@@ -1197,6 +1220,12 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
   }
   if (hipMemcpy(c->ctl, h.data(), sizeof(Ctl) * n_cases, hipMemcpyHostToDevice) != hipSuccess) {
     fail("jr_ctx_create3: control upload");
+    return bail(0);
+  }
+  // every zero fill and upload (null stream) is complete before the first
+  // launch on the context's non-blocking stream
+  if (hipDeviceSynchronize() != hipSuccess) {
+    fail("jr_ctx_create: synchronize");
     return bail(0);
   }
   *out = c;
@@ -1318,6 +1347,9 @@ int jr_set_geometry(jr_ctx* c, int k, const jr_grid_desc* g) {
   if (P.n_task[0]) TRY(put(P.t_norm[0], P.gs_tn0, g->task_normal[0], 2 * (size_t)P.n_task[0]));
   if (P.n_task[1]) TRY(put(P.t_norm[1], P.gs_tn1, g->task_normal[1], 2 * (size_t)P.n_task[1]));
   if (P.n_walls) TRY(put(P.w_mid, P.gs_wmid, g->wall_mid, 2 * (size_t)P.n_walls));
+  // the fills and copies above ran on the null stream; the context's stream
+  // does not wait for it
+  HIPCHK(hipDeviceSynchronize());
   return 0;
 }
 
