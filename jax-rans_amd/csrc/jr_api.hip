@@ -182,8 +182,11 @@ static void launch_residual(jr_ctx* c, const double* U, double* out, int first_o
   const Plan& P = c->P;
   hipLaunchKernelGGL((k_fill<double, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
                      dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
+  if (VISC)
+    hipLaunchKernelGGL((k_nodes<double, NV>), grid1((size_t)(P.NI + 1) * (P.NJ + 1), BLOCK, c->n_cases), dim3(BLOCK),
+                       0, c->stream, P, c->ph.gamma, c->ctl, c->Q, nullptr, c->Nd, nullptr, 0);
   hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
-                     c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Fi, c->Fj, 0);
+                     c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->Fi, c->Fj, 0);
   hipLaunchKernelGGL((k_cells<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
                      c->ctl, U, nullptr, c->Q, nullptr, c->Fi, c->Fj, nullptr, out, 0);
 }
@@ -198,8 +201,13 @@ static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double*
   if (!dq_ready)
     JR_LAUNCH(PR_FILL, (k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
               dim3(BLOCK), 0, P, c->ph.gamma, c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
+  if (VISC)
+    hipLaunchKernelGGL((k_nodes<Dual, NV>), grid1((size_t)(P.NI + 1) * (P.NJ + 1), BLOCK, c->n_cases), dim3(BLOCK), 0,
+                       c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ,
+                       (double*)nullptr, c->dNd, check_active);
   JR_LAUNCH(PR_FACES, (k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
-            first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, c->Fi, c->Fj, check_active);
+            first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Nd,
+            (const double*)c->dNd, c->Fi, c->Fj, check_active);
   JR_LAUNCH(PR_CELLS, (k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
             (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Fi,
             (const double*)c->Fj, vdt, out, check_active);
@@ -882,6 +890,8 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   const size_t nf = (size_t)P.nfi + P.nfj;
   if (dalloc(c, &c->U, F) || dalloc(c, &c->R, F) || dalloc(c, &c->Q, F) || dalloc(c, &c->dQ, F) ||
       dalloc(c, &c->Fi, (size_t)n_cases * c->nv * P.nfi) || dalloc(c, &c->Fj, (size_t)n_cases * c->nv * P.nfj) ||
+      dalloc(c, &c->Nd, (size_t)n_cases * 4 * (P.NI + 1) * (P.NJ + 1)) ||
+      dalloc(c, &c->dNd, (size_t)n_cases * 4 * (P.NI + 1) * (P.NJ + 1)) ||
       dalloc(c, &c->X, F) || dalloc(c, &c->Act, F) || dalloc(c, &c->Rv, F) || dalloc(c, &c->Z, F) ||
       dalloc(c, &c->W, F) || dalloc(c, &c->T, F) || dalloc(c, &c->V, F * (c->m + 1)) || dalloc(c, &c->tmpA, F) ||
       dalloc(c, &c->tmpB, F) || dalloc(c, &c->lam, (size_t)n_cases * nf) || dalloc(c, &c->sam, (size_t)n_cases * nf) ||
@@ -1688,11 +1698,11 @@ static void profile_launch(jr_ctx* c, int which) {
     case 2: launch_apply<R, NV>(c, c->Rv, c->Z, 0, 0); break;
     case 3:
       hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
-                         c->stream, P, c->ph, 0, c->ctl, c->Q, nullptr, c->Fi, c->Fj, 0);
+                         c->stream, P, c->ph, 0, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->Fi, c->Fj, 0);
       break;
     case 4:
       hipLaunchKernelGGL((k_faces<Dual, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
-                         c->stream, P, c->ph, c->exact ? 0 : 1, c->ctl, c->Q, c->dQ, c->Fi, c->Fj, 0);
+                         c->stream, P, c->ph, c->exact ? 0 : 1, c->ctl, c->Q, c->dQ, c->Nd, c->dNd, c->Fi, c->Fj, 0);
       break;
     case 5: {
       const int j = c->prof_rep++ % c->m;   // cycles through the Arnoldi columns

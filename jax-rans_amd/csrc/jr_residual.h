@@ -141,9 +141,45 @@ __device__ __forceinline__ void load_q(const double* Q, const double* dQ, size_t
 // cells (fi-1, j) and (fi, j); else j-face (i, fj) between (i, fj-1) and (i, fj).
 // Faces not adjacent to an interior cell are zero.  Rows 0..3 hold the
 // inviscid minus the viscous flux, row 4 (SA) the nu_hat face flux.
+// Node values phi = (u, v, T[, nu_hat]) of the 2D diamond: node (ni, nj) is
+// the mean of canvas cells (ni-1..ni, nj-1..nj), ((p00 + p10) + p01) + p11
+// (gradients.py:31-40).  One thread per node of the (NI+1)(NJ+1) node grid;
+// nodes on the canvas edge are never read and are skipped.  Computing each
+// node once per evaluation, instead of 2 x 4 cell loads in every face
+// thread, keeps the face kernel in registers.  Tangent mode writes dNd
+// only: the values are those of the residual at the same state.
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_nodes(Plan P, double gamma, const Ctl* __restrict__ ctl,
+                                                 const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                 double* __restrict__ Nd, double* __restrict__ dNd,
+                                                 int check_active) {
+  constexpr int NPHI = NV > 4 ? 4 : 3;
+  const int n = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  const int NI = P.NI, NJ = P.NJ, nc = P.nc;
+  const int nn = (NI + 1) * (NJ + 1);
+  if (n >= nn) return;
+  const int ni = n / (NJ + 1), nj = n - ni * (NJ + 1);
+  if (ni < 1 || nj < 1 || ni > NI - 1 || nj > NJ - 1) return;
+  const size_t base = (size_t)cs * NV * nc;
+  auto phi_at = [&](int i, int j, T* out) {
+    T q[NV];
+    load_q<T, NV>(Q, dQ, base, nc, i * NJ + j, q);
+    out[0] = q[1]; out[1] = q[2]; out[2] = temperature(q, gamma);
+    if constexpr (NPHI > 3) out[3] = q[NV - 1];
+  };
+  T p00[4], p10[4], p01[4], p11[4];
+  phi_at(ni - 1, nj - 1, p00); phi_at(ni, nj - 1, p10); phi_at(ni - 1, nj, p01); phi_at(ni, nj, p11);
+  const size_t o = (size_t)cs * NPHI * nn + n;
+  for (int k = 0; k < NPHI; ++k)
+    Io<T>::st(Nd, dNd, o + (size_t)k * nn, 0.25 * (((p00[k] + p10[k]) + p01[k]) + p11[k]));
+}
+
 template <class T, int NV, bool VISC>
 __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                 const double* __restrict__ Nd, const double* __restrict__ dNd,
                                                  double* __restrict__ Fi, double* __restrict__ Fj, int check_active) {
   bind_case(P, blockIdx.y);
   int f = blockIdx.x * BLOCK + threadIdx.x;
@@ -232,19 +268,13 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
     double d2x = node_c(n1i, n1j, 0) - node_c(n0i, n0j, 0);
     double d2y = node_c(n1i, n1j, 1) - node_c(n0i, n0j, 1);
     double det = d1x * d2y - d2x * d1y;
-    // phi = (u, v, T[, nu_hat]) at the four cells round each node: p1 node (ni, nj)
-    // averages canvas cells (ni-1, nj-1), (ni, nj-1), (ni-1, nj), (ni, nj).
+    // phi = (u, v, T[, nu_hat]) at the face's two nodes (k_nodes): node
+    // (ni, nj) averages canvas cells (ni-1, nj-1), (ni, nj-1), (ni-1, nj), (ni, nj).
     const int NPHI = NV > 4 ? 4 : 3;
-    auto phi_at = [&](int i, int j, T* out) {
-      T q[NV];
-      load_q<T, NV>(Q, dQ, base, nc, i * NJ + j, q);
-      out[0] = q[1]; out[1] = q[2]; out[2] = temperature(q, gamma);
-      if (NPHI > 3) out[3] = q[NV - 1];
-    };
+    const int nn = (P.NI + 1) * (NJ + 1);
     auto node_phi = [&](int ni, int nj, T* out) {
-      T p00[4], p10[4], p01[4], p11[4];
-      phi_at(ni - 1, nj - 1, p00); phi_at(ni, nj - 1, p10); phi_at(ni - 1, nj, p01); phi_at(ni, nj, p11);
-      for (int k = 0; k < NPHI; ++k) out[k] = 0.25 * (((p00[k] + p10[k]) + p01[k]) + p11[k]);
+      const size_t o = (size_t)cs * NPHI * nn + (size_t)ni * (NJ + 1) + nj;
+      for (int k = 0; k < NPHI; ++k) out[k] = Io<T>::ld(Nd, dNd, o + (size_t)k * nn);
     };
     T pn0[4], pn1[4], pl[4], pr[4];
     node_phi(n0i, n0j, pn0);
@@ -282,7 +312,9 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
       Fl[4] = Fl[0] * up - coef * (gx[3] * sx + gy[3] * sy);
     }
     T fv[4];
-    viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, has_extra ? extra : nullptr, fv);
+    // two inlined calls keep extra in registers (a selected pointer sent it to scratch)
+    if (has_extra) viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, extra, fv);
+    else viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, nullptr, fv);
     for (int k = 0; k < 4; ++k) Fl[k] = Fl[k] - fv[k];
   }
   for (int k = 0; k < NV; ++k) {
