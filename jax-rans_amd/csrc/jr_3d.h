@@ -393,19 +393,97 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int fi
 
 // Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere;
 // tangent mode: out = vdt dU + dR.
+// Partials of the SA source at the state, once per residual (the tangent
+// of every Krylov step then needs no dual-number source): src depends on
+// the cell's density, pressure and nu_hat, on |curl v| and on grad nu_hat
+// (turbulence.py:140-151).  Per cell, SALIN values: d src / d(rho, p,
+// nu_hat); d src / d curl_i (through |curl v|, zero where curl v = 0, as
+// the dual-number square root's guard); d src / d (grad nu_hat)_b.
+constexpr int SALIN = 9;
+
+__global__ void __launch_bounds__(BLOCK) k_sa_lin3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                   const double* __restrict__ Q, double* __restrict__ salin) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ, NK = P.NK, nc = P.nc;
+  double* L = salin + (size_t)cs * SALIN * nc + c;
+  if (!P.interior[c]) {
+    for (int k = 0; k < SALIN; ++k) L[(size_t)k * nc] = 0.0;
+    return;
+  }
+  const size_t base = (size_t)cs * 6 * nc;
+  const int i = c / (NJ * NK), j = (c / NK) % NJ, k = c % NK;
+  const int lo0 = (i * NJ + j) * NK + k, hi0 = lo0 + NJ * NK;
+  const int lo1 = P.nf[0] + (i * (NJ + 1) + j) * NK + k, hi1 = lo1 + NK;
+  const int lo2 = P.nf[0] + P.nf[1] + (i * NJ + j) * (NK + 1) + k, hi2 = lo2 + 1;
+  const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
+  const int nb[6] = {c - NJ * NK, c + NJ * NK, c - NK, c + NK, c - 1, c + 1};
+  const int fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
+  double qc[6];
+  load_q3<double, 6>(Q, nullptr, base, nc, c, qc);
+  double fa[6][4];
+  for (int e = 0; e < 6; ++e) {
+    double qn[6];
+    load_q3<double, 6>(Q, nullptr, base, nc, nb[e], qn);
+    const bool low = (e & 1) == 0;
+    for (int m = 0; m < 3; ++m) fa[e][m] = low ? 0.5 * (qn[1 + m] + qc[1 + m]) : 0.5 * (qc[1 + m] + qn[1 + m]);
+    fa[e][3] = low ? 0.5 * (qn[5] + qc[5]) : 0.5 * (qc[5] + qn[5]);
+  }
+  double sv[6][3];
+  for (int e = 0; e < 6; ++e) face_vec3(P, fl[e], sv[e]);
+  const double V = P.vol[c];
+  auto gg = [&](int m, int b) -> double {
+    return (((fa[1][m] * sv[1][b] - fa[0][m] * sv[0][b]) + (fa[3][m] * sv[3][b] - fa[2][m] * sv[2][b])) +
+            (fa[5][m] * sv[5][b] - fa[4][m] * sv[4][b])) / V;
+  };
+  double g[3][3];
+  for (int m = 0; m < 3; ++m)
+    for (int b = 0; b < 3; ++b) g[b][m] = gg(m, b);
+  const double curl[3] = {g[1][2] - g[2][1], g[2][0] - g[0][2], g[0][1] - g[1][0]};
+  const double sq = dot3(curl, curl);
+  const double omega = sq > 0.0 ? sqrt(sq) : 0.0;
+  const double gn[3] = {gg(3, 0), gg(3, 1), gg(3, 2)};
+  const double gn2 = dot3(gn, gn);
+  // seeds: density, pressure, nu_hat, |curl v|
+  double part[4];
+  for (int sd = 0; sd < 4; ++sd) {
+    Dual q[6];
+    for (int v = 0; v < 6; ++v) q[v] = mkd(qc[v], 0.0);
+    if (sd == 0) q[0].d = 1.0;
+    if (sd == 1) q[4].d = 1.0;
+    if (sd == 2) q[5].d = 1.0;
+    const Dual om = mkd(omega, sd == 3 ? 1.0 : 0.0);
+    const Dual rho = q[0];
+    const Dual mu = viscosity<Dual>(temperature3(q, gamma), nu_inf, ph.sutherland);
+    Dual prod, destr;
+    sa_source<Dual>(q[5] * nu_inf, mu / rho, om, P.dist[c], prod, destr);
+    const Dual src = rho * ((prod - destr) / nu_inf + CB2 / SIGMA * nu_inf * gn2);
+    part[sd] = src.d;
+  }
+  L[0] = part[0];
+  L[(size_t)nc] = part[1];
+  L[(size_t)2 * nc] = part[2];
+  const double w = sq > 0.0 ? part[3] / omega : 0.0;   // d|curl| / d curl_i = curl_i / |curl|
+  for (int a = 0; a < 3; ++a) L[(size_t)(3 + a) * nc] = w * curl[a];
+  const double rg = qc[0] * (CB2 / SIGMA * nu_inf);
+  for (int b = 0; b < 3; ++b) L[(size_t)(6 + b) * nc] = rg * (2.0 * gn[b]);
+}
+
 template <class T, int NV>
 __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
                                                     const double* __restrict__ U, const double* __restrict__ dU,
                                                     const double* __restrict__ Q, const double* __restrict__ dQ,
                                                     const double* __restrict__ F, const double* __restrict__ vdt,
-                                                    double* __restrict__ out, int check_active) {
+                                                    const double* __restrict__ salin, double* __restrict__ out,
+                                                    int check_active) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
   if (c >= P.nc) return;
   const int NJ = P.NJ, NK = P.NK, nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  const bool tangent = sizeof(T) != sizeof(double);
+  constexpr bool tangent = sizeof(T) != sizeof(double);
   const double vd = vdt ? vdt[(size_t)cs * nc + c] : 0.0;
   if (!P.interior[c]) {
     if (!tangent) {
@@ -432,6 +510,41 @@ __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const 
     r[v] = ((Fv[hi0] - Fv[lo0]) + (Fv[hi1] - Fv[lo1])) + (Fv[hi2] - Fv[lo2]);
   }
   if constexpr (NV > 5) {
+   if constexpr (tangent) {
+    // the source's tangent from its partials at the state (k_sa_lin3): the
+    // face-average gradients are linear in dq, the rest is a dot product
+    const int nb[6] = {c - NJ * NK, c + NJ * NK, c - NK, c + NK, c - 1, c + 1};
+    const int fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
+    const double* dq = dQ + base;
+    double dqc[6];
+    for (int v = 0; v < 6; ++v) dqc[v] = dq[(size_t)v * nc + c];
+    double fa[6][4];
+    for (int e = 0; e < 6; ++e) {
+      const bool low = (e & 1) == 0;
+      for (int m = 0; m < 3; ++m) {
+        const double qn = dq[(size_t)(1 + m) * nc + nb[e]];
+        fa[e][m] = low ? 0.5 * (qn + dqc[1 + m]) : 0.5 * (dqc[1 + m] + qn);
+      }
+      const double qn = dq[(size_t)5 * nc + nb[e]];
+      fa[e][3] = low ? 0.5 * (qn + dqc[5]) : 0.5 * (dqc[5] + qn);
+    }
+    double sv[6][3];
+    for (int e = 0; e < 6; ++e) face_vec3(P, fl[e], sv[e]);
+    const double V = P.vol[c];
+    auto gg = [&](int m, int b) -> double {
+      return (((fa[1][m] * sv[1][b] - fa[0][m] * sv[0][b]) + (fa[3][m] * sv[3][b] - fa[2][m] * sv[2][b])) +
+              (fa[5][m] * sv[5][b] - fa[4][m] * sv[4][b])) / V;
+    };
+    double g[3][3];
+    for (int m = 0; m < 3; ++m)
+      for (int b = 0; b < 3; ++b) g[b][m] = gg(m, b);
+    const double dcurl[3] = {g[1][2] - g[2][1], g[2][0] - g[0][2], g[0][1] - g[1][0]};
+    const double* L = salin + (size_t)cs * SALIN * nc + c;
+    double ds = (L[0] * dqc[0] + L[(size_t)nc] * dqc[4]) + L[(size_t)2 * nc] * dqc[5];
+    for (int i = 0; i < 3; ++i) ds += L[(size_t)(3 + i) * nc] * dcurl[i];
+    for (int b = 0; b < 3; ++b) ds += L[(size_t)(6 + b) * nc] * gg(3, b);
+    r[5] = r[5] - ds * V;
+   } else {
     // SA source (turbulence.py:140-151): vorticity and grad nu_hat by
     // Green-Gauss over the face averages with the six neighbours
     const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
@@ -466,6 +579,7 @@ __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const 
     T src = rho * ((prod - destr) / nu_inf + CB2 / SIGMA * nu_inf * dot3(gn, gn));
     T svv = src * V;
     r[5] = r[5] - (tangent ? tan_of(svv) : val(svv));
+   }
   }
   for (int v = 0; v < NV; ++v) {
     if (tangent && vdt) r[v] = vd * dU[base + (size_t)v * nc + c] + r[v];

@@ -37,6 +37,7 @@ struct jr_ctx {
   Plan3 P3{};
   double* F3 = nullptr;     // 3D face fluxes [case][var][faces of the three families]
   double* dsw = nullptr;    // 3D line apply: forward-sweep workspace, one field
+  double* salin = nullptr;  // 3D SA: partials of the source at the state, SALIN fields
   double *Nd = nullptr, *dNd = nullptr;   // 3D viscous: node values of (u, v, w, T[, nu_hat]) and their tangents
   Physics ph{};
   int nv = 4, n_cases = 1;
@@ -376,7 +377,10 @@ static void launch_residual3(jr_ctx* c, const double* U, double* out, int first_
   hipLaunchKernelGGL((k_faces3<double, NV, VISC>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
                      c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->F3, 0);
   hipLaunchKernelGGL((k_cells3<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     c->ctl, U, nullptr, c->Q, nullptr, c->F3, nullptr, out, 0);
+                     c->ctl, U, nullptr, c->Q, nullptr, c->F3, nullptr, nullptr, out, 0);
+  if (NV > 5)   // the SA source's partials at this state, for the tangents that follow
+    hipLaunchKernelGGL(k_sa_lin3, grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q,
+                       c->salin);
 }
 
 template <int NV, bool VISC>
@@ -394,8 +398,8 @@ static void launch_tangent3(jr_ctx* c, const double* U, const double* dU, double
             first_order, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Nd,
             (const double*)c->dNd, c->F3, check_active);
   JR_LAUNCH(PR_CELLS, (k_cells3<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
-            (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->F3, vdt, out,
-            check_active);
+            (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->F3, vdt,
+            (const double*)c->salin, out, check_active);
 }
 
 template <int NV>
@@ -1148,7 +1152,8 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
       dalloc(c, &c->F3, (size_t)n_cases * c->nv * nf) || dalloc(c, &c->X, F) || dalloc(c, &c->Act, F) ||
       dalloc(c, &c->Rv, F) || dalloc(c, &c->Z, F) || dalloc(c, &c->W, F) || dalloc(c, &c->T, F) ||
       dalloc(c, &c->V, F * (c->m + 1)) || dalloc(c, &c->tmpA, F) || dalloc(c, &c->tmpB, F) ||
-      dalloc(c, &c->dsw, F) || dalloc(c, &c->Nd, (size_t)n_cases * 5 * (g->NI + 1) * (g->NJ + 1) * (g->NK + 1)) ||
+      dalloc(c, &c->dsw, F) || dalloc(c, &c->salin, (size_t)n_cases * (num->sa ? SALIN : 1) * nc) ||
+      dalloc(c, &c->Nd, (size_t)n_cases * 5 * (g->NI + 1) * (g->NJ + 1) * (g->NK + 1)) ||
       dalloc(c, &c->dNd, (size_t)n_cases * 5 * (g->NI + 1) * (g->NJ + 1) * (g->NK + 1)) ||
       dalloc(c, &c->lam, (size_t)n_cases * nf) || dalloc(c, &c->sam, (size_t)n_cases * nf) ||
       dalloc(c, &c->sad, (size_t)n_cases * nf) || dalloc(c, &c->lsum, (size_t)n_cases * nc) ||

@@ -378,14 +378,24 @@ __global__ void k_lstsq(int n_cases, const Ctl* __restrict__ ctl, double* __rest
   const double* Gc = G + (size_t)cs * GIV;
   const double* Rr = Gc;
   const double* g = Gc + (size_t)(MMAX + 1) * MMAX + 2 * MMAX;
+  // the triangle and g staged in LDS by the whole workgroup, so the serial
+  // back substitution of thread 0 reads no global memory
+  __shared__ double Rs[MMAX * MMAX];
+  __shared__ double gs[MMAX];
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int row = e / m, col = e - row * m;
+    Rs[row * MMAX + col] = Rr[row * MMAX + col];
+  }
+  for (int jj = threadIdx.x; jj < m; jj += blockDim.x) gs[jj] = g[jj];
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double largest = fabs(Rr[0]);
-    for (int jj = 1; jj < m; ++jj) largest = fmax(largest, fabs(Rr[jj * MMAX + jj]));
+    double largest = fabs(Rs[0]);
+    for (int jj = 1; jj < m; ++jj) largest = fmax(largest, fabs(Rs[jj * MMAX + jj]));
     const double rcond = 2.220446049250313e-16 * (m + 1);
     for (int jj = m - 1; jj >= 0; --jj) {
-      double rest = g[jj];
-      for (int k = jj + 1; k < m; ++k) rest = rest - Rr[jj * MMAX + k] * ys[k];
-      const double piv = Rr[jj * MMAX + jj];
+      double rest = gs[jj];
+      for (int k = jj + 1; k < m; ++k) rest = rest - Rs[jj * MMAX + k] * ys[k];
+      const double piv = Rs[jj * MMAX + jj];
       ys[jj] = fabs(piv) > rcond * largest ? rest / piv : 0.0;
     }
   }

@@ -200,3 +200,31 @@ def test_run_on_the_m6_file_case_matches_oracle_newton_steps():
     ref = S3.forces(s.U, st, spec.reference_area, spec.reference_length, tuple(spec.moment_centre))
     for key in ("cl", "cd", "cm", "cd_friction", "cy"):
         assert getattr(result.forces, key) == pytest.approx(ref[key], rel=1e-8, abs=1e-12), key
+
+
+@pytest.mark.parametrize("which", ["euler_o48", "onera_m6_sa"])
+def test_run_polar_equals_each_run_alone(which):
+    """``run_polar`` (the reference's batched polar, ``src/jaxrans/polar.py:80-121``)
+    on a 2D generated grid and on the 3D M6 file grid: every case of the batch
+    has, bit for bit, the residual history and forces of ``run`` on that
+    freestream alone."""
+    from dataclasses import replace
+    from pathlib import Path
+
+    from jaxrans_amd.case.loader import load_case
+    from jaxrans_amd.simulation import run, run_polar
+
+    print(f"[gpu] run_polar {which}", flush=True)
+    directory = Path(__file__).resolve().parent / "golden" / "cases" / which
+    case = load_case(directory, ("numerical.convergence.max_iters=4", "numerical.convergence.block_iters=2",
+                                 "numerical.convergence.forces_tol=null"))
+    grid = case.grid()
+    alphas, machs = [1.0, 3.5], [0.5, 0.7]
+    batch = run_polar(case, alphas, machs, grid=grid)
+    assert len(batch) == 2
+    for k, (a, m) in enumerate(zip(alphas, machs)):
+        alone = run(replace(case, flow=replace(case.flow, alpha_deg=a, mach=m)), grid)
+        assert np.array_equal(alone.history.residual, batch[k].history.residual), (k, alone.history.residual,
+                                                                                  batch[k].history.residual)
+        for key in ("cl", "cd", "cm", "cd_friction", "cy"):
+            assert getattr(alone.forces, key) == getattr(batch[k].forces, key), (k, key)
