@@ -734,15 +734,25 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
                                                    const double* __restrict__ lsum, const double* __restrict__ sam,
                                                    const double* __restrict__ sad, const double* __restrict__ sa_diag,
                                                    double* __restrict__ blk) {
-  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  // thread t: step k = t / n_lines of chain ch = t mod n_lines, so that the
+  // block components are written (and read by k_thomas3) chain-minor
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (s >= P.n_slots) return;
+  const int nl = P.n_lines;
+  if (t >= P.max_len * nl) return;
+  const int k = t / nl, ch = t - k * nl;
+  const int s0 = P.l_start[ch];
+  if (k >= P.l_start[ch + 1] - s0) return;
+  const int s = s0 + k;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   const double gamma = ph.gamma, cfl = ctl[cs].cfl;
   const double* L = lam + (size_t)cs * P.nft;
   const int cell = P.l_cell[s];
-  double* o = blk + ((size_t)cs * P.n_slots + s) * BLKW3;
+  // component e of this step at o[e * cstride]
+  const size_t cstride = (size_t)P.max_len * nl;
+  double* o = blk + (size_t)cs * BLKW3 * cstride + (size_t)k * nl + ch;
+  double blk_lo[25], blk_up[25];
   o[0] = lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5);
   auto link = [&](int nb_cell, int fce, double sg, double* dst) {
     double sv[3];
@@ -757,10 +767,14 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
   };
   const bool has_lo = s > 0 && P.l_dns[s - 1] != 0;
   const bool has_up = P.l_ups[s] != 0;
-  if (has_lo) link(P.l_cell[s - 1], P.l_dn[s - 1], (double)P.l_dns[s - 1], o + 1);
-  else for (int e = 0; e < 25; ++e) o[1 + e] = 0.0;
-  if (has_up) link(P.l_cell[s + 1], P.l_up[s], (double)P.l_ups[s], o + 26);
-  else for (int e = 0; e < 25; ++e) o[26 + e] = 0.0;
+  if (has_lo) link(P.l_cell[s - 1], P.l_dn[s - 1], (double)P.l_dns[s - 1], blk_lo);
+  else for (int e = 0; e < 25; ++e) blk_lo[e] = 0.0;
+  if (has_up) link(P.l_cell[s + 1], P.l_up[s], (double)P.l_ups[s], blk_up);
+  else for (int e = 0; e < 25; ++e) blk_up[e] = 0.0;
+  for (int e = 0; e < 25; ++e) {
+    o[(1 + e) * cstride] = blk_lo[e];
+    o[(26 + e) * cstride] = blk_up[e];
+  }
   if constexpr (NV > 5) {
     const double* mj = sam + (size_t)cs * P.nft;
     const double* dj = sad + (size_t)cs * P.nft;
@@ -775,9 +789,9 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
       const double sg = P.l_ups[s];
       up = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s + 1]];
     }
-    o[51] = lo;
-    o[52] = sa_diag[(size_t)cs * nc + cell];
-    o[53] = up;
+    o[51 * cstride] = lo;
+    o[52 * cstride] = sa_diag[(size_t)cs * nc + cell];
+    o[53 * cstride] = up;
   }
 }
 
@@ -810,7 +824,8 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
   const int ch = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
   if (ch >= P.n_lines) return;
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0, nl = P.n_lines;
-  const double* B = blk + (size_t)cs * P.n_slots * BLKW3;
+  const size_t cstride = (size_t)P.max_len * nl;
+  const double* B = blk + (size_t)cs * BLKW3 * cstride + ch;   // component e of step k at B[e cstride + k nl]
   R* F = fac + (size_t)cs * P.max_len * 75 * nl;
   R* Fs = fac_sa + (size_t)cs * P.max_len * 3 * nl;
   R Cp[5][5];
@@ -818,13 +833,18 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
     for (int q = 0; q < 5; ++q) Cp[r][q] = R(0);
   R cps = R(0);
   for (int k = 0; k < n; ++k) {
-    const double* b = B + (size_t)(s0 + k) * BLKW3;
-    const R d = (R)b[0];
+    // the step's 54 block values, all loaded before the first factor store
+    const double* b0 = B + (size_t)k * nl;
+    double bk[BLKW3];
+#pragma unroll
+    for (int e = 0; e < BLKW3; ++e) bk[e] = b0[(size_t)e * cstride];
+    auto b = [&](int e) { return bk[e]; };
+    const R d = (R)b(0);
     R Pm[5][5], Pi[5][5];
     for (int r = 0; r < 5; ++r)
       for (int q = 0; q < 5; ++q) {
         R acc = R(0);
-        for (int t = 0; t < 5; ++t) acc += (R)b[1 + r * 5 + t] * Cp[t][q];
+        for (int t = 0; t < 5; ++t) acc += (R)b(1 + r * 5 + t) * Cp[t][q];
         Pm[r][q] = (r == q ? d : R(0)) - acc;
       }
     invert5<R>(Pm, Pi);
@@ -832,15 +852,15 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
     for (int r = 0; r < 5; ++r)
       for (int q = 0; q < 5; ++q) {
         R acc = R(0);
-        for (int t = 0; t < 5; ++t) acc += Pi[r][t] * (R)b[26 + t * 5 + q];
-        o[(size_t)(r * 5 + q) * nl] = (R)b[1 + r * 5 + q];
+        for (int t = 0; t < 5; ++t) acc += Pi[r][t] * (R)b(26 + t * 5 + q);
+        o[(size_t)(r * 5 + q) * nl] = (R)b(1 + r * 5 + q);
         o[(size_t)(25 + r * 5 + q) * nl] = Pi[r][q];
         o[(size_t)(50 + r * 5 + q) * nl] = acc;
       }
     for (int r = 0; r < 5; ++r)
       for (int q = 0; q < 5; ++q) Cp[r][q] = o[(size_t)(50 + r * 5 + q) * nl];
     if constexpr (NV > 5) {
-      const R lo = (R)b[51], ds = (R)b[52], up = (R)b[53];
+      const R lo = (R)b(51), ds = (R)b(52), up = (R)b(53);
       const R pinv = R(1) / (ds - lo * cps);
       const R cn = pinv * up;
       R* os = Fs + (size_t)k * 3 * nl + ch;

@@ -414,8 +414,8 @@ static void launch_speeds3(jr_ctx* c, const double* U) {
 template <class R, int NV>
 static void launch_factor3(jr_ctx* c, const double* U) {
   const Plan3& P = c->P3;
-  hipLaunchKernelGGL((k_blocks3<NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
+  hipLaunchKernelGGL((k_blocks3<NV>), grid1((size_t)P.max_len * P.n_lines, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph, c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
   JR_LAUNCH(PR_FACTOR, (k_thomas3<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
             (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa);
 }
@@ -1169,7 +1169,9 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
     if (dalloc(c, (char**)&p, rows * 3 * rsz)) return bail(0);
     c->fac_sa = p;
     c->ws = c->ws_sa = p;
-    if (dalloc(c, &c->blk, (size_t)n_cases * std::max(1, P.n_slots) * BLKW3)) return bail(0);
+    // chain blocks component-major, chain-minor: [case][BLKW3][max_len][n_lines]
+    if (dalloc(c, &c->blk, (size_t)n_cases * std::max<size_t>(1, (size_t)P.max_len * P.n_lines) * BLKW3))
+      return bail(0);
   }
   if (dalloc(c, &c->ctl, (size_t)n_cases) || dalloc(c, &c->hv, (size_t)n_cases * (MMAX + 1)) ||
       dalloc(c, &c->h2v, (size_t)n_cases * (MMAX + 1)) || dalloc(c, &c->H, (size_t)n_cases * (MMAX + 1) * MMAX) ||
