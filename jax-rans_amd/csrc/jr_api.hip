@@ -37,7 +37,7 @@ struct jr_ctx {
   Plan3 P3{};
   double* F3 = nullptr;     // 3D face fluxes [case][var][faces of the three families]
   double* dsw = nullptr;    // 3D line apply: forward-sweep workspace, one field
-  double* salin = nullptr;  // 3D SA: partials of the source at the state, SALIN fields
+  double* salin = nullptr;  // SA: partials of the source at the state (SALIN2 / SALIN fields)
   double *Nd = nullptr, *dNd = nullptr;   // 3D viscous: node values of (u, v, w, T[, nu_hat]) and their tangents
   Physics ph{};
   int nv = 4, n_cases = 1;
@@ -189,7 +189,10 @@ static void launch_residual(jr_ctx* c, const double* U, double* out, int first_o
   hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
                      c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->Fi, c->Fj, 0);
   hipLaunchKernelGGL((k_cells<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     c->ctl, U, nullptr, c->Q, nullptr, c->Fi, c->Fj, nullptr, out, 0);
+                     c->ctl, U, nullptr, c->Q, nullptr, c->Fi, c->Fj, nullptr, nullptr, out, 0);
+  if (NV > 4)   // the SA source's partials at this state, for the tangents that follow
+    hipLaunchKernelGGL(k_sa_lin, grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q,
+                       c->salin);
 }
 
 // Requires c->Q (primitives with ghosts) for the same U.
@@ -211,7 +214,7 @@ static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double*
             (const double*)c->dNd, c->Fi, c->Fj, check_active);
   JR_LAUNCH(PR_CELLS, (k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, P, c->ph,
             (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Fi,
-            (const double*)c->Fj, vdt, out, check_active);
+            (const double*)c->Fj, vdt, (const double*)c->salin, out, check_active);
 }
 
 template <int NV>
@@ -895,6 +898,7 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   if (dalloc(c, &c->U, F) || dalloc(c, &c->R, F) || dalloc(c, &c->Q, F) || dalloc(c, &c->dQ, F) ||
       dalloc(c, &c->Fi, (size_t)n_cases * c->nv * P.nfi) || dalloc(c, &c->Fj, (size_t)n_cases * c->nv * P.nfj) ||
       dalloc(c, &c->Nd, (size_t)n_cases * 4 * (P.NI + 1) * (P.NJ + 1)) ||
+      dalloc(c, &c->salin, (size_t)n_cases * (num->sa ? SALIN2 : 1) * nc) ||
       dalloc(c, &c->dNd, (size_t)n_cases * 4 * (P.NI + 1) * (P.NJ + 1)) ||
       dalloc(c, &c->X, F) || dalloc(c, &c->Act, F) || dalloc(c, &c->Rv, F) || dalloc(c, &c->Z, F) ||
       dalloc(c, &c->W, F) || dalloc(c, &c->T, F) || dalloc(c, &c->V, F * (c->m + 1)) || dalloc(c, &c->tmpA, F) ||
@@ -1719,7 +1723,7 @@ static void profile_launch(jr_ctx* c, int which) {
     case 6: launch_factor<R, NV>(c, c->U); break;
     case 7:
       hipLaunchKernelGGL((k_cells<Dual, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                         c->ctl, c->U, c->Z, c->Q, c->dQ, c->Fi, c->Fj, c->vdt, c->W, 0);
+                         c->ctl, c->U, c->Z, c->Q, c->dQ, c->Fi, c->Fj, c->vdt, c->salin, c->W, 0);
       break;
     default: break;
   }

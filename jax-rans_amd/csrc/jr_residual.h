@@ -325,13 +325,80 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
 
 // Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere.
 // Tangent mode writes  out = vdt * dU + dR  (vdt may be null: no pseudo-time term).
+// Partials of the 2D SA source at the state, once per residual, for the
+// tangents of the Krylov steps: d src / d(rho, p, nu_hat), d src / d(dv/dx -
+// du/dy) (through |.|, the dual-number abs's sign), d src / d(grad nu_hat).
+constexpr int SALIN2 = 6;
+
+__global__ void __launch_bounds__(BLOCK) k_sa_lin(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                  const double* __restrict__ Q, double* __restrict__ salin) {
+  bind_case(P, blockIdx.y);
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ, nc = P.nc;
+  double* L = salin + (size_t)cs * SALIN2 * nc + c;
+  if (!P.interior[c]) {
+    for (int k = 0; k < SALIN2; ++k) L[(size_t)k * nc] = 0.0;
+    return;
+  }
+  const size_t base = (size_t)cs * 5 * nc;
+  const int i = c / NJ, j = c - i * NJ;
+  const int fi0 = i * NJ + j, fi1 = fi0 + NJ;
+  const int fj0 = i * (NJ + 1) + j, fj1 = fj0 + 1;
+  const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
+  double qc[5], qw[5], qe[5], qs[5], qn[5];
+  load_q<double, 5>(Q, nullptr, base, nc, c, qc);
+  load_q<double, 5>(Q, nullptr, base, nc, c - NJ, qw);
+  load_q<double, 5>(Q, nullptr, base, nc, c + NJ, qe);
+  load_q<double, 5>(Q, nullptr, base, nc, c - 1, qs);
+  load_q<double, 5>(Q, nullptr, base, nc, c + 1, qn);
+  const double sxi0 = P.fi[fi0], syi0 = P.fi[P.nfi + fi0], sxi1 = P.fi[fi1], syi1 = P.fi[P.nfi + fi1];
+  const double sxj0 = P.fj[fj0], syj0 = P.fj[P.nfj + fj0], sxj1 = P.fj[fj1], syj1 = P.fj[P.nfj + fj1];
+  const double V = P.area[c];
+  auto gg = [&](int k, double& gxo, double& gyo) {
+    const double fw = 0.5 * (qw[k] + qc[k]), fe = 0.5 * (qc[k] + qe[k]);
+    const double fs = 0.5 * (qs[k] + qc[k]), fn = 0.5 * (qc[k] + qn[k]);
+    gxo = ((fe * sxi1 - fw * sxi0) + (fn * sxj1 - fs * sxj0)) / V;
+    gyo = ((fe * syi1 - fw * syi0) + (fn * syj1 - fs * syj0)) / V;
+  };
+  double gux, guy, gvx, gvy, gnx, gny;
+  gg(1, gux, guy);
+  gg(2, gvx, gvy);
+  gg(4, gnx, gny);
+  const double w = gvx - guy;
+  const double omega = fabs(w);
+  double part[4];
+  for (int sd = 0; sd < 4; ++sd) {
+    Dual q[5];
+    for (int v = 0; v < 5; ++v) q[v] = mkd(qc[v], 0.0);
+    if (sd == 0) q[0].d = 1.0;
+    if (sd == 1) q[3].d = 1.0;
+    if (sd == 2) q[4].d = 1.0;
+    const Dual om = mkd(omega, sd == 3 ? 1.0 : 0.0);
+    const Dual rho = q[0];
+    const Dual mu = viscosity<Dual>(temperature(q, gamma), nu_inf, ph.sutherland);
+    Dual prod, destr;
+    sa_source<Dual>(q[4] * nu_inf, mu / rho, om, P.dist[c], prod, destr);
+    const Dual src = rho * ((prod - destr) / nu_inf + CB2 / SIGMA * nu_inf * (gnx * gnx + gny * gny));
+    part[sd] = src.d;
+  }
+  L[0] = part[0];
+  L[(size_t)nc] = part[1];
+  L[(size_t)2 * nc] = part[2];
+  L[(size_t)3 * nc] = w >= 0.0 ? part[3] : -part[3];     // d|w| / dw
+  const double rg = qc[0] * (CB2 / SIGMA * nu_inf);
+  L[(size_t)4 * nc] = rg * (2.0 * gnx);
+  L[(size_t)5 * nc] = rg * (2.0 * gny);
+}
+
 template <class T, int NV>
 __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ U, const double* __restrict__ dU,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
                                                  const double* __restrict__ Fi, const double* __restrict__ Fj,
-                                                 const double* __restrict__ vdt, double* __restrict__ out,
-                                                 int check_active) {
+                                                 const double* __restrict__ vdt, const double* __restrict__ salin,
+                                                 double* __restrict__ out, int check_active) {
   bind_case(P, blockIdx.y);
   int c = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
@@ -339,7 +406,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
   if (c >= P.nc) return;
   const int NJ = P.NJ, nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  const bool tangent = sizeof(T) != sizeof(double);
+  constexpr bool tangent = sizeof(T) != sizeof(double);
   double vd = vdt ? vdt[(size_t)cs * nc + c] : 0.0;
   if (!P.interior[c]) {
     if (!tangent) {
@@ -363,7 +430,31 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
     r[k] = (Fi[bi + (size_t)k * P.nfi + fi1] - Fi[bi + (size_t)k * P.nfi + fi0]) +
            (Fj[bj + (size_t)k * P.nfj + fj1] - Fj[bj + (size_t)k * P.nfj + fj0]);
   }
-  if constexpr (NV > 4) {
+  if constexpr (NV > 4 && tangent) {
+    // the source's tangent from its partials at the state (k_sa_lin): the
+    // face-average gradients are linear in dq, the rest is a dot product
+    const double* dq = dQ + base;
+    auto d_at = [&](int cell, int k) { return dq[(size_t)k * nc + cell]; };
+    const double sxi0 = P.fi[fi0], syi0 = P.fi[P.nfi + fi0], sxi1 = P.fi[fi1], syi1 = P.fi[P.nfi + fi1];
+    const double sxj0 = P.fj[fj0], syj0 = P.fj[P.nfj + fj0], sxj1 = P.fj[fj1], syj1 = P.fj[P.nfj + fj1];
+    const double V = P.area[c];
+    auto gg = [&](int k, double& gxo, double& gyo) {
+      const double qc = d_at(c, k);
+      const double fw = 0.5 * (d_at(c - NJ, k) + qc), fe = 0.5 * (qc + d_at(c + NJ, k));
+      const double fs = 0.5 * (d_at(c - 1, k) + qc), fn = 0.5 * (qc + d_at(c + 1, k));
+      gxo = ((fe * sxi1 - fw * sxi0) + (fn * sxj1 - fs * sxj0)) / V;
+      gyo = ((fe * syi1 - fw * syi0) + (fn * syj1 - fs * syj0)) / V;
+    };
+    double gux, guy, gvx, gvy, gnx, gny;
+    gg(1, gux, guy);
+    gg(2, gvx, gvy);
+    gg(4, gnx, gny);
+    const double* L = salin + (size_t)cs * SALIN2 * nc + c;
+    double ds = (L[0] * d_at(c, 0) + L[(size_t)nc] * d_at(c, 3)) + L[(size_t)2 * nc] * d_at(c, 4);
+    ds += L[(size_t)3 * nc] * (gvx - guy);
+    ds += L[(size_t)4 * nc] * gnx + L[(size_t)5 * nc] * gny;
+    r[4] = r[4] - ds * V;
+  } else if constexpr (NV > 4) {
     // SA source at the cell (turbulence.py:140-151): vorticity and grad nu_hat by
     // Green-Gauss over the face-averaged neighbours.
     const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
