@@ -273,20 +273,35 @@ static bool any_active(jr_ctx* c) {
 
 // --- CGS2 of Arnoldi column j, instantiated for j / KC + 1 basis chunks --------
 
-template <int NV, int NCH>
-static void cgs2_passes(jr_ctx* c, int j, int passes) {
+template <int NV, int NCH, int PW>
+static void cgs2_passes_pw(jr_ctx* c, int j, int passes) {
   const Plan& P = c->P;
   const dim3 vg(c->nblk, c->n_cases);
   const size_t N = c->N, vstride = (size_t)c->n_cases * N;
   if (passes & 1)
-    JR_LAUNCH(PR_DOTS, (k_dots<NV, NCH>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl, (const double*)c->V, vstride,
-              N, (const double*)c->W, j, c->cg, (const double*)c->hv, (const double*)c->h2v, c->H, c->G, c->m);
+    JR_LAUNCH(PR_DOTS, (k_dots<NV, NCH, PW>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl, (const double*)c->V,
+              vstride, N, (const double*)c->W, j, c->cg, (const double*)c->hv, (const double*)c->h2v, c->H, c->G,
+              c->m);
   if (passes & 2)
-    JR_LAUNCH(PR_UPDATE_DOTS, (k_update_dots<NV, NCH>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl,
+    JR_LAUNCH(PR_UPDATE_DOTS, (k_update_dots<NV, NCH, PW>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl,
               (const double*)c->V, vstride, N, c->W, j, c->hv, c->cg);
   if (passes & 4)
-    JR_LAUNCH(PR_UPDATE_NORM, (k_update_norm<NV, NCH>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl,
+    JR_LAUNCH(PR_UPDATE_NORM, (k_update_norm<NV, NCH, PW>), vg, dim3(BLOCK), 0, P.nc, (const Ctl*)c->ctl,
               (const double*)c->V, vstride, N, c->W, j, c->h2v, c->cg);
+}
+
+// Two entries per thread and loop step (16-byte loads) when the field length
+// is even (every case offset then keeps the alignment) and the batch of basis
+// values fits the registers twice (j < 32).
+template <int NV, int NCH>
+static void cgs2_passes(jr_ctx* c, int j, int passes) {
+  if constexpr (NCH <= 4) {
+    if (c->N % 2 == 0) {
+      cgs2_passes_pw<NV, NCH, 2>(c, j, passes);
+      return;
+    }
+  }
+  cgs2_passes_pw<NV, NCH, 1>(c, j, passes);
 }
 
 template <int NV>

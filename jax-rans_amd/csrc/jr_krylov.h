@@ -113,6 +113,49 @@ __device__ __forceinline__ void basis_batch(const double* __restrict__ Vc, size_
   __builtin_amdgcn_sched_barrier(0);   // keep the batch ahead of its first use
 }
 
+// PW consecutive entries per thread and loop step (PW = 2: 16-byte loads,
+// half the steps of the grid-stride loop; the vectors' case strides and
+// lengths must then be even).  v[k][p] = V_min(k0 + k, j)[e + p].
+template <int NB, int PW>
+__device__ __forceinline__ void basis_batch_pw(const double* __restrict__ Vc, size_t vstride, size_t e, int k0, int j,
+                                               double (&v)[NB][PW]) {
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const double* p = Vc + (size_t)min(k0 + k, j) * vstride + e;
+    if constexpr (PW == 2) {
+      const double2 t = *reinterpret_cast<const double2*>(p);
+      v[k][0] = t.x;
+      v[k][1] = t.y;
+    } else {
+      v[k][0] = *p;
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);   // keep the batch ahead of its first use
+}
+
+template <int PW>
+__device__ __forceinline__ void load_pw(const double* __restrict__ a, size_t o, double (&w)[PW]) {
+  if constexpr (PW == 2) {
+    const double2 t = *reinterpret_cast<const double2*>(a + o);
+    w[0] = t.x;
+    w[1] = t.y;
+  } else {
+    w[0] = a[o];
+  }
+}
+
+template <int PW>
+__device__ __forceinline__ void store_pw(double* __restrict__ a, size_t o, const double (&w)[PW]) {
+  if constexpr (PW == 2) {
+    *reinterpret_cast<double2*>(a + o) = make_double2(w[0], w[1]);
+  } else {
+    a[o] = w[0];
+  }
+}
+
+#define JR_VEC_LOOP_PW(N, PW) \
+  for (size_t e = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * (PW); e < (N); e += (size_t)gridDim.x * BLOCK * (PW))
+
 template <int H, int O, int MM>
 __device__ __forceinline__ void halve(double (&v)[MM], int lane) {
   if constexpr (H >= 1) {
@@ -243,7 +286,7 @@ __device__ void column_finish(int cs, int j, int m, const double* __restrict__ p
 
 // CGS2 pass 1: h_k = V_k . w, k <= j.  Workgroup 0 first finishes column
 // j - 1 (its pass 3 ended the previous kernel sequence).
-template <int NV, int NCH>
+template <int NV, int NCH, int PW>
 __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ V,
                                                 size_t vstride, size_t vcase, const double* __restrict__ W, int j,
                                                 Cg cg, const double* __restrict__ hv,
@@ -259,15 +302,18 @@ __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ 
   double acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) acc[k] = 0.0;
-  JR_VEC_LOOP(N) {
-    const double w = W[base + e];
+  JR_VEC_LOOP_PW(N, PW) {
+    double w[PW];
+    load_pw<PW>(W, base + e, w);
 #pragma unroll
     for (int g = 0; g < Cgs<NCH>::NG; ++g) {
       constexpr int GS = Cgs<NCH>::GS;
-      double v[GS];
-      basis_batch<GS>(Vc, vstride, e, g * GS, j, v);
+      double v[GS][PW];
+      basis_batch_pw<GS, PW>(Vc, vstride, e, g * GS, j, v);
 #pragma unroll
-      for (int k = 0; k < GS; ++k) acc[g * GS + k] += v[k] * w;
+      for (int k = 0; k < GS; ++k)
+#pragma unroll
+        for (int p = 0; p < PW; ++p) acc[g * GS + k] += v[k][p] * w[p];
     }
   }
   put_partials<NP>(acc, cg.p1 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j + 1);
@@ -275,7 +321,7 @@ __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ 
 
 // CGS2 pass 2: h = the pass-1 totals; w <- w - sum_{k<=j} h_k V_k, then
 // h2_k = V_k . w.  Workgroup 0 records h.
-template <int NV, int NCH>
+template <int NV, int NCH, int PW>
 __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
                                                        double* __restrict__ W, int j, double* __restrict__ hv,
@@ -294,18 +340,24 @@ __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __rest
   double acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) acc[k] = 0.0;
-  JR_VEC_LOOP(N) {
-    double vk[NK];
-    double w = W[base + e];
-    basis_batch<NK>(Vc, vstride, e, 0, j, vk);
-    double s = 0.0;
+  JR_VEC_LOOP_PW(N, PW) {
+    double vk[NK][PW];
+    double w[PW];
+    load_pw<PW>(W, base + e, w);
+    basis_batch_pw<NK, PW>(Vc, vstride, e, 0, j, vk);
+#pragma unroll
+    for (int p = 0; p < PW; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        if (k <= j) s += h[k] * vk[k][p];
+      w[p] = w[p] - s;
+    }
+    store_pw<PW>(W, base + e, w);
 #pragma unroll
     for (int k = 0; k < NK; ++k)
-      if (k <= j) s += h[k] * vk[k];
-    w = w - s;
-    W[base + e] = w;
 #pragma unroll
-    for (int k = 0; k < NK; ++k) acc[k] += vk[k] * w;
+      for (int p = 0; p < PW; ++p) acc[k] += vk[k][p] * w[p];
   }
   put_partials<NP>(acc, cg.p2 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j + 1);
 }
@@ -313,7 +365,7 @@ __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __rest
 // CGS2 pass 3: h2 = the pass-2 totals; w <- w - sum h2_k V_k and |w|^2.
 // Workgroup 0 records h2.  The column is finished (H, Givens) by the next
 // column's pass 1, or by the cycle's least squares for the last column.
-template <int NV, int NCH>
+template <int NV, int NCH, int PW>
 __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
                                                        double* __restrict__ W, int j, double* __restrict__ h2v,
@@ -331,20 +383,27 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __rest
 #pragma unroll
   for (int k = 0; k < NKG; ++k) h2[k] = k <= j ? tot[min(k, NP - 1)] : 0.0;
   double acc[1] = {0.0};
-  JR_VEC_LOOP(N) {
-    double w = W[base + e];
-    double s = 0.0;
+  JR_VEC_LOOP_PW(N, PW) {
+    double w[PW];
+    load_pw<PW>(W, base + e, w);
+    double s[PW];
+#pragma unroll
+    for (int p = 0; p < PW; ++p) s[p] = 0.0;
 #pragma unroll
     for (int g = 0; g < Cgs<NCH>::NG; ++g) {
-      double v[GS];
-      basis_batch<GS>(Vc, vstride, e, g * GS, j, v);
+      double v[GS][PW];
+      basis_batch_pw<GS, PW>(Vc, vstride, e, g * GS, j, v);
 #pragma unroll
-      for (int k = 0; k < GS; ++k)
-        if (g * GS + k <= j) s += h2[g * GS + k] * v[k];
+      for (int p = 0; p < PW; ++p)
+#pragma unroll
+        for (int k = 0; k < GS; ++k)
+          if (g * GS + k <= j) s[p] += h2[g * GS + k] * v[k][p];
     }
-    w = w - s;
-    W[base + e] = w;
-    acc[0] += w * w;
+#pragma unroll
+    for (int p = 0; p < PW; ++p) w[p] = w[p] - s[p];
+    store_pw<PW>(W, base + e, w);
+#pragma unroll
+    for (int p = 0; p < PW; ++p) acc[0] += w[p] * w[p];
   }
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) cg.p3[(size_t)cs * cg.nblk + blockIdx.x] = acc[0];
