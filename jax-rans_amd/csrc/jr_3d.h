@@ -826,8 +826,10 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0, nl = P.n_lines;
   const size_t cstride = (size_t)P.max_len * nl;
   const double* B = blk + (size_t)cs * BLKW3 * cstride + ch;   // component e of step k at B[e cstride + k nl]
-  R* F = fac + (size_t)cs * P.max_len * 75 * nl;
-  R* Fs = fac_sa + (size_t)cs * P.max_len * 3 * nl;
+  // factor tiles of 64 chains: [case][chain / 64][step][75][64] (and [3][64])
+  const size_t ngrp = ((size_t)nl + 63) / 64;
+  R* F = fac + ((size_t)cs * ngrp + blockIdx.x) * P.max_len * 75 * 64 + threadIdx.x;
+  R* Fs = fac_sa + ((size_t)cs * ngrp + blockIdx.x) * P.max_len * 3 * 64 + threadIdx.x;
   R Cp[5][5];
   for (int r = 0; r < 5; ++r)
     for (int q = 0; q < 5; ++q) Cp[r][q] = R(0);
@@ -848,25 +850,25 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
         Pm[r][q] = (r == q ? d : R(0)) - acc;
       }
     invert5<R>(Pm, Pi);
-    R* o = F + (size_t)k * 75 * nl + ch;
+    R* o = F + (size_t)k * 75 * 64;
     for (int r = 0; r < 5; ++r)
       for (int q = 0; q < 5; ++q) {
         R acc = R(0);
         for (int t = 0; t < 5; ++t) acc += Pi[r][t] * (R)b(26 + t * 5 + q);
-        o[(size_t)(r * 5 + q) * nl] = (R)b(1 + r * 5 + q);
-        o[(size_t)(25 + r * 5 + q) * nl] = Pi[r][q];
-        o[(size_t)(50 + r * 5 + q) * nl] = acc;
+        o[(size_t)(r * 5 + q) * 64] = (R)b(1 + r * 5 + q);
+        o[(size_t)(25 + r * 5 + q) * 64] = Pi[r][q];
+        o[(size_t)(50 + r * 5 + q) * 64] = acc;
       }
     for (int r = 0; r < 5; ++r)
-      for (int q = 0; q < 5; ++q) Cp[r][q] = o[(size_t)(50 + r * 5 + q) * nl];
+      for (int q = 0; q < 5; ++q) Cp[r][q] = o[(size_t)(50 + r * 5 + q) * 64];
     if constexpr (NV > 5) {
       const R lo = (R)b(51), ds = (R)b(52), up = (R)b(53);
       const R pinv = R(1) / (ds - lo * cps);
       const R cn = pinv * up;
-      R* os = Fs + (size_t)k * 3 * nl + ch;
+      R* os = Fs + (size_t)k * 3 * 64;
       os[0] = lo;
-      os[nl] = pinv;
-      os[2 * nl] = cn;
+      os[64] = pinv;
+      os[2 * 64] = cn;
       cps = cn;
     }
   }
@@ -903,17 +905,17 @@ struct BwdStage {
 
 template <class R, int NV>
 __device__ __forceinline__ void fwd_load(FwdStage<R, NV>& s, const R* __restrict__ o, const R* __restrict__ os,
-                                         const double* __restrict__ rb, int cell, int nl, int nc) {
+                                         const double* __restrict__ rb, int cell, int vs, int nc) {
 #pragma unroll
   for (int e = 0; e < 25; ++e) {
-    s.L[e] = o[(size_t)e * nl];
-    s.Pm[e] = o[(size_t)(25 + e) * nl];
+    s.L[e] = o[(size_t)e * vs];
+    s.Pm[e] = o[(size_t)(25 + e) * vs];
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) s.rhs[v] = rb[(size_t)v * nc + cell];
   if constexpr (NV > 5) {
     s.ls = os[0];
-    s.ps = os[nl];
+    s.ps = os[vs];
   }
   s.cell = cell;
 }
@@ -956,15 +958,15 @@ __device__ __forceinline__ void fwd_step(FwdStage<R, NV>& s, R (&dp)[5], R& dps,
 template <bool ACC, class R, int NV>
 __device__ __forceinline__ void bwd_load(BwdStage<R, NV>& s, const R* __restrict__ o, const R* __restrict__ os,
                                          const double* __restrict__ D, const double* __restrict__ zb, int cell,
-                                         int nl, int nc) {
+                                         int vs, int nc) {
 #pragma unroll
-  for (int e = 0; e < 25; ++e) s.C[e] = o[(size_t)e * nl];
+  for (int e = 0; e < 25; ++e) s.C[e] = o[(size_t)e * vs];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     s.d[v] = D[(size_t)v * nc + cell];
     s.z[v] = ACC ? zb[(size_t)v * nc + cell] : 0.0;
   }
-  if constexpr (NV > 5) s.cs = os[2 * nl];
+  if constexpr (NV > 5) s.cs = os[2 * vs];
   s.cell = cell;
 }
 
@@ -1018,12 +1020,15 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0;
   if (n <= 0) return;
   const int* __restrict__ lc = P.l_cell + s0;
-  const R* F = fac + (size_t)cs * P.max_len * 75 * nl + ch;
-  const R* Fs = fac_sa + (size_t)cs * P.max_len * 3 * nl + ch;
+  // factor tiles of 64 chains (k_thomas3): one step of a wave's 64 chains is
+  // 75 x 64 consecutive values, read at value stride 64
+  const size_t ngrp = ((size_t)nl + 63) / 64;
+  const R* F = fac + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 75 * 64 + (ch & 63);
+  const R* Fs = fac_sa + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 3 * 64 + (ch & 63);
   const double* rb = r + base;
   double* D = tmp + base;
   double* zb = z + base;
-  const size_t step = (size_t)75 * nl, sstep = (size_t)3 * nl;
+  const size_t step = (size_t)75 * 64, sstep = (size_t)3 * 64;
 
   R dp[5] = {R(0), R(0), R(0), R(0), R(0)};
   R dps = R(0);
@@ -1032,17 +1037,17 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
   // across the current step's arithmetic.
   {
     FwdStage<R, NV> A, B;
-    fwd_load(A, F, Fs, rb, lc[0], nl, nc);
+    fwd_load(A, F, Fs, rb, lc[0], 64, nc);
     int nxt = lc[min(1, n - 1)];
     int k = 0;
     for (; k + 1 < n; k += 2) {
       const int nxt2 = lc[min(k + 2, n - 1)];
-      fwd_load(B, F + (size_t)(k + 1) * step, Fs + (size_t)(k + 1) * sstep, rb, nxt, nl, nc);
+      fwd_load(B, F + (size_t)(k + 1) * step, Fs + (size_t)(k + 1) * sstep, rb, nxt, 64, nc);
       __builtin_amdgcn_sched_barrier(0);
       fwd_step(A, dp, dps, D, vout, base, den, nc);
       const int k2 = min(k + 2, n - 1);
       nxt = lc[min(k + 3, n - 1)];
-      fwd_load(A, F + (size_t)k2 * step, Fs + (size_t)k2 * sstep, rb, nxt2, nl, nc);
+      fwd_load(A, F + (size_t)k2 * step, Fs + (size_t)k2 * sstep, rb, nxt2, 64, nc);
       __builtin_amdgcn_sched_barrier(0);
       fwd_step(B, dp, dps, D, vout, base, den, nc);
     }
@@ -1052,22 +1057,123 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
   R xns = R(0);
   {
     BwdStage<R, NV> A, B;
-    const R* C0 = F + (size_t)50 * nl;
-    bwd_load<ACC>(A, C0 + (size_t)(n - 1) * step, Fs + (size_t)(n - 1) * sstep, D, zb, lc[n - 1], nl, nc);
+    const R* C0 = F + (size_t)50 * 64;
+    bwd_load<ACC>(A, C0 + (size_t)(n - 1) * step, Fs + (size_t)(n - 1) * sstep, D, zb, lc[n - 1], 64, nc);
     int nxt = lc[max(n - 2, 0)];
     int k = n - 1;
     for (; k > 0; k -= 2) {
       const int nxt2 = lc[max(k - 2, 0)];
-      bwd_load<ACC>(B, C0 + (size_t)(k - 1) * step, Fs + (size_t)(k - 1) * sstep, D, zb, nxt, nl, nc);
+      bwd_load<ACC>(B, C0 + (size_t)(k - 1) * step, Fs + (size_t)(k - 1) * sstep, D, zb, nxt, 64, nc);
       __builtin_amdgcn_sched_barrier(0);
       bwd_step<ACC>(A, xn, xns, zb, nc);
       const int k2 = max(k - 2, 0);
       nxt = lc[max(k - 3, 0)];
-      bwd_load<ACC>(A, C0 + (size_t)k2 * step, Fs + (size_t)k2 * sstep, D, zb, nxt2, nl, nc);
+      bwd_load<ACC>(A, C0 + (size_t)k2 * step, Fs + (size_t)k2 * sstep, D, zb, nxt2, 64, nc);
       __builtin_amdgcn_sched_barrier(0);
       bwd_step<ACC>(B, xn, xns, zb, nc);
     }
     if (k == 0) bwd_step<ACC>(A, xn, xns, zb, nc);
+  }
+}
+
+// The same solve with three register stages, so two steps' loads are in
+// flight while one computes: a chain's wave is latency-bound (one wave per
+// 64 chains, about one wave per CU on the wing), and the extra stage doubles
+// the bytes it keeps in flight.  The chain's cell indices are staged in LDS
+// first (64 x max_len ints), so that no index load sits behind a stage's
+// loads in the in-order memory counter.  For chains of at most
+// APPLY3_LDS_LEN cells.
+constexpr int APPLY3_LDS_LEN = 192;
+
+template <class R, int NV, bool ACC>
+__global__ void __launch_bounds__(64) k_apply3_s3(Plan3 P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
+                                                  const R* __restrict__ fac_sa, const double* __restrict__ r,
+                                                  double* __restrict__ z, double* __restrict__ tmp, int check_active,
+                                                  double* __restrict__ vout, const double* __restrict__ p3,
+                                                  int nblk) {
+  extern __shared__ int cl[];          // cl[k * 64 + lane]: cell of step k of this lane's chain
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  const int nc = P.nc, nl = P.n_lines;
+  const size_t base = (size_t)cs * NV * nc;
+  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
+  const int chain_groups = (nl + 63) / 64;
+  if ((int)blockIdx.x >= chain_groups) {
+    const int e = ((int)blockIdx.x - chain_groups) * 64 + threadIdx.x;
+    if (e >= P.n_free) return;
+    const int cell = P.free_cell[e];
+    for (int v = 0; v < NV; ++v) {
+      const size_t o = base + (size_t)v * nc + cell;
+      const double x = apply_input(r, vout, o, den);
+      z[o] = ACC ? z[o] + x : x;
+    }
+    return;
+  }
+  const int lane = threadIdx.x;
+  const int ch = blockIdx.x * 64 + lane;
+  if (ch >= nl) return;
+  const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0;
+  if (n <= 0) return;
+  for (int k = 0; k < n; ++k) cl[k * 64 + lane] = P.l_cell[s0 + k];
+  auto cell_at = [&](int k) { return cl[k * 64 + lane]; };
+  // factor tiles of 64 chains (k_thomas3): one step of a wave's 64 chains is
+  // 75 x 64 consecutive values, read at value stride 64
+  const size_t ngrp = ((size_t)nl + 63) / 64;
+  const R* F = fac + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 75 * 64 + (ch & 63);
+  const R* Fs = fac_sa + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 3 * 64 + (ch & 63);
+  const double* rb = r + base;
+  double* D = tmp + base;
+  double* zb = z + base;
+  const size_t step = (size_t)75 * 64, sstep = (size_t)3 * 64;
+
+  R dp[5] = {R(0), R(0), R(0), R(0), R(0)};
+  R dps = R(0);
+  {
+    FwdStage<R, NV> A, B, C;
+    auto load = [&](FwdStage<R, NV>& S, int k) {
+      fwd_load(S, F + (size_t)k * step, Fs + (size_t)k * sstep, rb, cell_at(k), 64, nc);
+    };
+    load(A, 0);
+    load(B, min(1, n - 1));
+    int k = 0;
+    for (; k + 2 < n; k += 3) {
+      load(C, k + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      fwd_step(A, dp, dps, D, vout, base, den, nc);
+      load(A, min(k + 3, n - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      fwd_step(B, dp, dps, D, vout, base, den, nc);
+      load(B, min(k + 4, n - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      fwd_step(C, dp, dps, D, vout, base, den, nc);
+    }
+    if (k < n) fwd_step(A, dp, dps, D, vout, base, den, nc);
+    if (k + 1 < n) fwd_step(B, dp, dps, D, vout, base, den, nc);
+  }
+  R xn[5] = {R(0), R(0), R(0), R(0), R(0)};
+  R xns = R(0);
+  {
+    BwdStage<R, NV> A, B, C;
+    const R* C0 = F + (size_t)50 * 64;
+    auto load = [&](BwdStage<R, NV>& S, int k) {
+      bwd_load<ACC>(S, C0 + (size_t)k * step, Fs + (size_t)k * sstep, D, zb, cell_at(k), 64, nc);
+    };
+    load(A, n - 1);
+    load(B, max(n - 2, 0));
+    int k = n - 1;
+    for (; k - 2 >= 0; k -= 3) {
+      load(C, k - 2);
+      __builtin_amdgcn_sched_barrier(0);
+      bwd_step<ACC>(A, xn, xns, zb, nc);
+      load(A, max(k - 3, 0));
+      __builtin_amdgcn_sched_barrier(0);
+      bwd_step<ACC>(B, xn, xns, zb, nc);
+      load(B, max(k - 4, 0));
+      __builtin_amdgcn_sched_barrier(0);
+      bwd_step<ACC>(C, xn, xns, zb, nc);
+    }
+    if (k >= 0) bwd_step<ACC>(A, xn, xns, zb, nc);
+    if (k - 1 >= 0) bwd_step<ACC>(B, xn, xns, zb, nc);
   }
 }
 

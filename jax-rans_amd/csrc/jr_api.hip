@@ -443,6 +443,18 @@ static void launch_apply3(jr_ctx* c, const double* r, double* z, int accumulate,
                           double* vout = nullptr) {
   const Plan3& P = c->P3;
   const unsigned groups = (unsigned)((P.n_lines + 63) / 64 + (P.n_free + 63) / 64);
+  if (P.max_len <= APPLY3_LDS_LEN) {   // three stages, the chains' cells in LDS
+    const size_t lds = (size_t)64 * std::max(1, P.max_len) * sizeof(int);
+    if (accumulate)
+      JR_LAUNCH(PR_APPLY, (k_apply3_s3<R, NV, true>), dim3(groups, c->n_cases), dim3(64), lds, P,
+                (const Ctl*)c->ctl, (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, check_active, vout,
+                (const double*)c->cg.p3, c->cg.nblk);
+    else
+      JR_LAUNCH(PR_APPLY, (k_apply3_s3<R, NV, false>), dim3(groups, c->n_cases), dim3(64), lds, P,
+                (const Ctl*)c->ctl, (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, check_active, vout,
+                (const double*)c->cg.p3, c->cg.nblk);
+    return;
+  }
   if (accumulate)
     JR_LAUNCH(PR_APPLY, (k_apply3<R, NV, true>), dim3(groups, c->n_cases), dim3(64), 0, P, (const Ctl*)c->ctl,
               (const R*)c->fac, (const R*)c->fac_sa, r, z, c->dsw, check_active, vout, (const double*)c->cg.p3,
@@ -1181,7 +1193,8 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
   c->Fi = c->Fj = c->F3;
   {
     const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
-    const size_t rows = (size_t)n_cases * std::max(1, P.max_len) * std::max(1, P.n_lines);
+    // factor tiles of 64 chains (k_thomas3, k_apply3): chains rounded up to whole tiles
+    const size_t rows = (size_t)n_cases * std::max(1, P.max_len) * (((size_t)std::max(1, P.n_lines) + 63) / 64 * 64);
     void* p;
     if (dalloc(c, (char**)&p, rows * 75 * rsz)) return bail(0);
     c->fac = p;
