@@ -818,6 +818,42 @@ __device__ __forceinline__ void invert5(R A[5][5], R X[5][5]) {
 // [(k * 75 + v) * n_lines + ch] (L 0..24, P^-1 25..49, C 50..74), so the
 // threads of a wave (consecutive chains) read consecutive addresses;
 // SA: [(k * 3 + v) * n_lines + ch] (l, 1/p, c).
+// The 3D factor in tiles of 64 chains, each chain's 75 values of a step
+// packed W to a 16-byte vector (W = 2 doubles or 4 floats, padded to 76
+// values): value v of step k of lane l in its tile at
+// [(k * (76 / W) + v / W) * 64 + l] * W + v mod W.  A wave then reads a
+// step with 16-byte loads, 1 KB per instruction, so that the 63 loads a
+// wait can name keep twice the bytes in flight.
+constexpr int FAC3_VALUES = 76;
+
+template <class R> struct Vec16;
+template <> struct Vec16<double> {
+  using T = double2;
+  static constexpr int W = 2;
+  static __device__ __forceinline__ void get(const T& t, double* o) { o[0] = t.x; o[1] = t.y; }
+};
+template <> struct Vec16<float> {
+  using T = float4;
+  static constexpr int W = 4;
+  static __device__ __forceinline__ void get(const T& t, float* o) { o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w; }
+};
+
+// values [v0, v0 + NVAL) of one step into out (v0 a multiple of W is not
+// required: the covering vectors are loaded and the range picked out)
+template <class R, int V0, int NVAL>
+__device__ __forceinline__ void fac3_values(const R* __restrict__ step_base, R* out) {
+  constexpr int W = Vec16<R>::W;
+  constexpr int Q0 = V0 / W, Q1 = (V0 + NVAL - 1) / W;
+  R buf[(Q1 - Q0 + 1) * W];
+#pragma unroll
+  for (int q = Q0; q <= Q1; ++q) {
+    const typename Vec16<R>::T t = *reinterpret_cast<const typename Vec16<R>::T*>(step_base + (size_t)q * 64 * W);
+    Vec16<R>::get(t, buf + (q - Q0) * W);
+  }
+#pragma unroll
+  for (int e = 0; e < NVAL; ++e) out[e] = buf[V0 - Q0 * W + e];
+}
+
 template <class R, int NV>
 __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restrict__ blk, R* __restrict__ fac,
                                                 R* __restrict__ fac_sa) {
@@ -826,10 +862,13 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0, nl = P.n_lines;
   const size_t cstride = (size_t)P.max_len * nl;
   const double* B = blk + (size_t)cs * BLKW3 * cstride + ch;   // component e of step k at B[e cstride + k nl]
-  // factor tiles of 64 chains: [case][chain / 64][step][75][64] (and [3][64])
+  // factor tiles of 64 chains, values packed W to a vector (FAC3_VALUES);
+  // SA [case][chain / 64][step][3][64]
+  constexpr int W = Vec16<R>::W;
   const size_t ngrp = ((size_t)nl + 63) / 64;
-  R* F = fac + ((size_t)cs * ngrp + blockIdx.x) * P.max_len * 75 * 64 + threadIdx.x;
+  R* F = fac + (((size_t)cs * ngrp + blockIdx.x) * P.max_len * FAC3_VALUES * 64) + (size_t)threadIdx.x * W;
   R* Fs = fac_sa + ((size_t)cs * ngrp + blockIdx.x) * P.max_len * 3 * 64 + threadIdx.x;
+  auto slot = [&](int v) { return (size_t)(v / W) * 64 * W + v % W; };
   R Cp[5][5];
   for (int r = 0; r < 5; ++r)
     for (int q = 0; q < 5; ++q) Cp[r][q] = R(0);
@@ -850,17 +889,16 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
         Pm[r][q] = (r == q ? d : R(0)) - acc;
       }
     invert5<R>(Pm, Pi);
-    R* o = F + (size_t)k * 75 * 64;
+    R* o = F + (size_t)k * FAC3_VALUES * 64;
     for (int r = 0; r < 5; ++r)
       for (int q = 0; q < 5; ++q) {
         R acc = R(0);
         for (int t = 0; t < 5; ++t) acc += Pi[r][t] * (R)b(26 + t * 5 + q);
-        o[(size_t)(r * 5 + q) * 64] = (R)b(1 + r * 5 + q);
-        o[(size_t)(25 + r * 5 + q) * 64] = Pi[r][q];
-        o[(size_t)(50 + r * 5 + q) * 64] = acc;
+        o[slot(r * 5 + q)] = (R)b(1 + r * 5 + q);
+        o[slot(25 + r * 5 + q)] = Pi[r][q];
+        o[slot(50 + r * 5 + q)] = acc;
+        Cp[r][q] = acc;
       }
-    for (int r = 0; r < 5; ++r)
-      for (int q = 0; q < 5; ++q) Cp[r][q] = o[(size_t)(50 + r * 5 + q) * 64];
     if constexpr (NV > 5) {
       const R lo = (R)b(51), ds = (R)b(52), up = (R)b(53);
       const R pinv = R(1) / (ds - lo * cps);
@@ -906,10 +944,12 @@ struct BwdStage {
 template <class R, int NV>
 __device__ __forceinline__ void fwd_load(FwdStage<R, NV>& s, const R* __restrict__ o, const R* __restrict__ os,
                                          const double* __restrict__ rb, int cell, int vs, int nc) {
+  R lp[50];
+  fac3_values<R, 0, 50>(o, lp);
 #pragma unroll
   for (int e = 0; e < 25; ++e) {
-    s.L[e] = o[(size_t)e * vs];
-    s.Pm[e] = o[(size_t)(25 + e) * vs];
+    s.L[e] = lp[e];
+    s.Pm[e] = lp[25 + e];
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) s.rhs[v] = rb[(size_t)v * nc + cell];
@@ -959,8 +999,7 @@ template <bool ACC, class R, int NV>
 __device__ __forceinline__ void bwd_load(BwdStage<R, NV>& s, const R* __restrict__ o, const R* __restrict__ os,
                                          const double* __restrict__ D, const double* __restrict__ zb, int cell,
                                          int vs, int nc) {
-#pragma unroll
-  for (int e = 0; e < 25; ++e) s.C[e] = o[(size_t)e * vs];
+  fac3_values<R, 50, 25>(o, s.C);
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     s.d[v] = D[(size_t)v * nc + cell];
@@ -1023,12 +1062,12 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
   // factor tiles of 64 chains (k_thomas3): one step of a wave's 64 chains is
   // 75 x 64 consecutive values, read at value stride 64
   const size_t ngrp = ((size_t)nl + 63) / 64;
-  const R* F = fac + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 75 * 64 + (ch & 63);
+  const R* F = fac + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * FAC3_VALUES * 64 + (size_t)(ch & 63) * Vec16<R>::W;
   const R* Fs = fac_sa + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 3 * 64 + (ch & 63);
   const double* rb = r + base;
   double* D = tmp + base;
   double* zb = z + base;
-  const size_t step = (size_t)75 * 64, sstep = (size_t)3 * 64;
+  const size_t step = (size_t)FAC3_VALUES * 64, sstep = (size_t)3 * 64;
 
   R dp[5] = {R(0), R(0), R(0), R(0), R(0)};
   R dps = R(0);
@@ -1057,7 +1096,7 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
   R xns = R(0);
   {
     BwdStage<R, NV> A, B;
-    const R* C0 = F + (size_t)50 * 64;
+    const R* C0 = F;   // the C values (50..74) are picked from the step's vectors
     bwd_load<ACC>(A, C0 + (size_t)(n - 1) * step, Fs + (size_t)(n - 1) * sstep, D, zb, lc[n - 1], 64, nc);
     int nxt = lc[max(n - 2, 0)];
     int k = n - 1;
@@ -1119,12 +1158,12 @@ __global__ void __launch_bounds__(64) k_apply3_s3(Plan3 P, const Ctl* __restrict
   // factor tiles of 64 chains (k_thomas3): one step of a wave's 64 chains is
   // 75 x 64 consecutive values, read at value stride 64
   const size_t ngrp = ((size_t)nl + 63) / 64;
-  const R* F = fac + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 75 * 64 + (ch & 63);
+  const R* F = fac + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * FAC3_VALUES * 64 + (size_t)(ch & 63) * Vec16<R>::W;
   const R* Fs = fac_sa + ((size_t)cs * ngrp + (ch >> 6)) * P.max_len * 3 * 64 + (ch & 63);
   const double* rb = r + base;
   double* D = tmp + base;
   double* zb = z + base;
-  const size_t step = (size_t)75 * 64, sstep = (size_t)3 * 64;
+  const size_t step = (size_t)FAC3_VALUES * 64, sstep = (size_t)3 * 64;
 
   R dp[5] = {R(0), R(0), R(0), R(0), R(0)};
   R dps = R(0);
@@ -1154,7 +1193,7 @@ __global__ void __launch_bounds__(64) k_apply3_s3(Plan3 P, const Ctl* __restrict
   R xns = R(0);
   {
     BwdStage<R, NV> A, B, C;
-    const R* C0 = F + (size_t)50 * 64;
+    const R* C0 = F;   // the C values (50..74) are picked from the step's vectors
     auto load = [&](BwdStage<R, NV>& S, int k) {
       bwd_load<ACC>(S, C0 + (size_t)k * step, Fs + (size_t)k * sstep, D, zb, cell_at(k), 64, nc);
     };

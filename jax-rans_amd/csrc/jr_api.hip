@@ -1196,7 +1196,7 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
     // factor tiles of 64 chains (k_thomas3, k_apply3): chains rounded up to whole tiles
     const size_t rows = (size_t)n_cases * std::max(1, P.max_len) * (((size_t)std::max(1, P.n_lines) + 63) / 64 * 64);
     void* p;
-    if (dalloc(c, (char**)&p, rows * 75 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, rows * FAC3_VALUES * rsz)) return bail(0);
     c->fac = p;
     if (dalloc(c, (char**)&p, rows * 3 * rsz)) return bail(0);
     c->fac_sa = p;
