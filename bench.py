@@ -276,13 +276,14 @@ def wing_bench(steps: int, warmup: int) -> dict:
     return out
 
 
-def converge(case_dir: Path, grid=None) -> dict:
+def converge(case_dir: Path, grid=None, overrides=()) -> dict:
     """The case run to convergence through run() in FP64 and in mixed precision:
-    iterations, wall time and the force differences (mixed - FP64)."""
+    iterations, wall time and the force differences (mixed - FP64).
+    ``overrides``: case overrides (file.section.key=value) applied to both."""
     from jaxrans_amd import load_case, run
     out = {}
     for precision in ("fp64", "mixed"):
-        case = load_case(case_dir, (f"numerical.implicit.precision={json.dumps(precision)}",))
+        case = load_case(case_dir, tuple(overrides) + (f"numerical.implicit.precision={json.dumps(precision)}",))
         grid = case.grid() if grid is None else grid
         t0 = time.perf_counter()
 
@@ -384,6 +385,8 @@ def main(argv=None) -> int:
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
+    ap.add_argument("--set", action="append", default=[], metavar="FILE.SECTION.KEY=VALUE",
+                    help="--converge: a case override, as load_case takes it (repeatable)")
     ap.add_argument("--wing", action="store_true",
                     help="instead of the 2D case: Newton steps of the 3D SA wing (four blocks, 1.09M cells)")
     ap.add_argument("--serve", type=int, metavar="N", default=0,
@@ -443,7 +446,7 @@ def main(argv=None) -> int:
         return 0
     if args.converge:
         line = {"metric": "mixed vs FP64 at convergence", "case": args.converge, "n_gpus": 1,
-                **converge(Path(args.converge))}
+                "overrides": list(args.set), **converge(Path(args.converge), overrides=args.set)}
         print(json.dumps(line), flush=True)
         return 0
     from jaxrans_amd import load_case
