@@ -249,25 +249,36 @@ __device__ void column_finish(int cs, int j, int m, const double* __restrict__ p
                               const double* __restrict__ hv, const double* __restrict__ h2v, double* __restrict__ H,
                               double* __restrict__ G) {
   __shared__ double gcol[MMAX + 1];
+  __shared__ double gcs[MMAX], gsn[MMAX];
   const double nw = column_norm(p3, nblk);
   double* Hc = H + (size_t)cs * (MMAX + 1) * MMAX;   // H[row][col], row-major (MMAX+1) x MMAX
-  for (int k = threadIdx.x; k <= j; k += blockDim.x)
-    Hc[k * MMAX + j] = hv[(size_t)cs * (MMAX + 1) + k] + h2v[(size_t)cs * (MMAX + 1) + k];
+  double* Gc = G + (size_t)cs * GIV;
+  double* Rr = Gc;                                     // rotated columns
+  double* cs_ = Gc + (size_t)(MMAX + 1) * MMAX;
+  double* sn = cs_ + MMAX;
+  double* g = sn + MMAX;
+  // the column and the previous rotations staged in LDS by the whole
+  // workgroup, so that the serial rotation sweep of thread 0 reads no global
+  // memory
+  for (int k = threadIdx.x; k <= j; k += blockDim.x) {
+    const double h = hv[(size_t)cs * (MMAX + 1) + k] + h2v[(size_t)cs * (MMAX + 1) + k];
+    Hc[k * MMAX + j] = h;
+    gcol[k] = h;
+  }
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    gcs[i] = cs_[i];
+    gsn[i] = sn[i];
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     Hc[(j + 1) * MMAX + j] = nw;
     for (int k = j + 2; k <= m; ++k) Hc[k * MMAX + j] = 0.0;
-    double* Gc = G + (size_t)cs * GIV;
-    double* Rr = Gc;                                   // rotated columns
-    double* cs_ = Gc + (size_t)(MMAX + 1) * MMAX;
-    double* sn = cs_ + MMAX;
-    double* g = sn + MMAX;
     double* col = gcol;
-    for (int k = 0; k <= j + 1; ++k) col[k] = Hc[k * MMAX + j];
+    col[j + 1] = nw;
     for (int i = 0; i < j; ++i) {
       double a = col[i], c = col[i + 1];
-      col[i] = cs_[i] * a + sn[i] * c;
-      col[i + 1] = cs_[i] * c - sn[i] * a;
+      col[i] = gcs[i] * a + gsn[i] * c;
+      col[i + 1] = gcs[i] * c - gsn[i] * a;
     }
     double a = col[j], c = col[j + 1];
     double r = sqrt(a * a + c * c);
