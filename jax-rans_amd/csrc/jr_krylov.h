@@ -61,6 +61,13 @@ __global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, 
   }
 }
 
+// R <- -b: a linear solve for the right-hand side b runs the Newton step's
+// GMRES, whose start is r = -R (k_norms).
+__global__ void __launch_bounds__(BLOCK) k_negate(size_t n, const double* __restrict__ b, double* __restrict__ R) {
+  const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e < n) R[e] = -b[e];
+}
+
 // Cycle start: active = cycles < maxiter and |r| > tol |b| (krylov.py:85-87);
 // V0 = r / max(beta, tiny).
 template <int NV>
@@ -152,9 +159,6 @@ __device__ __forceinline__ void store_pw(double* __restrict__ a, size_t o, const
     a[o] = w[0];
   }
 }
-
-#define JR_VEC_LOOP_PW(N, PW) \
-  for (size_t e = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * (PW); e < (N); e += (size_t)gridDim.x * BLOCK * (PW))
 
 template <int H, int O, int MM>
 __device__ __forceinline__ void halve(double (&v)[MM], int lane) {
@@ -295,30 +299,39 @@ __device__ void column_finish(int cs, int j, int m, const double* __restrict__ p
   __syncthreads();
 }
 
-// CGS2 pass 1: h_k = V_k . w, k <= j.  Workgroup 0 first finishes column
-// j - 1 (its pass 3 ended the previous kernel sequence).
+// CGS2 pass 1: h_k = V_k . w, k <= j.  The last workgroup (the fewest
+// grid-stride steps) first finishes column j - 1 (its pass 3 ended the
+// previous kernel sequence), with its first step's loads already in flight.
 template <int NV, int NCH, int PW>
 __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ ctl, const double* __restrict__ V,
                                                 size_t vstride, size_t vcase, const double* __restrict__ W, int j,
                                                 Cg cg, const double* __restrict__ hv,
                                                 const double* __restrict__ h2v, double* __restrict__ H,
                                                 double* __restrict__ G, int m) {
-  constexpr int NP = Cgs<NCH>::NP;
+  constexpr int NP = Cgs<NCH>::NP, GS = Cgs<NCH>::GS;
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
-  if (blockIdx.x == 0 && j > 0)
-    column_finish(cs, j - 1, m, cg.p3 + (size_t)cs * cg.nblk, cg.nblk, hv, h2v, H, G);
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   const double* Vc = V + (size_t)cs * vcase;
+  const size_t step = (size_t)gridDim.x * BLOCK * PW;
+  size_t e = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * PW;
+  double w[PW], v0[GS][PW];
+  if (e < N) {
+    load_pw<PW>(W, base + e, w);
+    basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
+  }
+  if (blockIdx.x == gridDim.x - 1 && j > 0)
+    column_finish(cs, j - 1, m, cg.p3 + (size_t)cs * cg.nblk, cg.nblk, hv, h2v, H, G);
   double acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) acc[k] = 0.0;
-  JR_VEC_LOOP_PW(N, PW) {
-    double w[PW];
-    load_pw<PW>(W, base + e, w);
+  while (e < N) {
 #pragma unroll
-    for (int g = 0; g < Cgs<NCH>::NG; ++g) {
-      constexpr int GS = Cgs<NCH>::GS;
+    for (int k = 0; k < GS; ++k)
+#pragma unroll
+      for (int p = 0; p < PW; ++p) acc[k] += v0[k][p] * w[p];
+#pragma unroll
+    for (int g = 1; g < Cgs<NCH>::NG; ++g) {
       double v[GS][PW];
       basis_batch_pw<GS, PW>(Vc, vstride, e, g * GS, j, v);
 #pragma unroll
@@ -326,12 +339,20 @@ __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ 
 #pragma unroll
         for (int p = 0; p < PW; ++p) acc[g * GS + k] += v[k][p] * w[p];
     }
+    e += step;
+    __builtin_amdgcn_sched_barrier(0);   // the next step's loads reuse this step's registers
+    if (e < N) {
+      load_pw<PW>(W, base + e, w);
+      basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
+    }
   }
   put_partials<NP>(acc, cg.p1 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j + 1);
 }
 
 // CGS2 pass 2: h = the pass-1 totals; w <- w - sum_{k<=j} h_k V_k, then
-// h2_k = V_k . w.  Workgroup 0 records h.
+// h2_k = V_k . w.  Workgroup 0 records h.  (Loading the first grid-stride
+// step ahead of the totals, as the other two passes do, needs a second set
+// of NK basis registers here and spills at NK = 32.)
 template <int NV, int NCH, int PW>
 __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
@@ -351,7 +372,7 @@ __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __rest
   double acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) acc[k] = 0.0;
-  JR_VEC_LOOP_PW(N, PW) {
+  for (size_t e = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * PW; e < N; e += (size_t)gridDim.x * BLOCK * PW) {
     double vk[NK][PW];
     double w[PW];
     load_pw<PW>(W, base + e, w);
@@ -376,6 +397,7 @@ __global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __rest
 // CGS2 pass 3: h2 = the pass-2 totals; w <- w - sum h2_k V_k and |w|^2.
 // Workgroup 0 records h2.  The column is finished (H, Givens) by the next
 // column's pass 1, or by the cycle's least squares for the last column.
+// Each thread's first grid-stride step is loaded before the totals are added.
 template <int NV, int NCH, int PW>
 __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
@@ -385,23 +407,33 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __rest
   __shared__ double sh[1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
+  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
+  const double* Vc = V + (size_t)cs * vcase;
+  const size_t step = (size_t)gridDim.x * BLOCK * PW;
+  size_t e = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * PW;
+  double w[PW], v0[GS][PW];
+  if (e < N) {
+    load_pw<PW>(W, base + e, w);
+    basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
+  }
   __shared__ double tot[NP];
   get_totals<NP>(cg.p2 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j, tot);
   if (blockIdx.x == 0 && threadIdx.x <= j) h2v[(size_t)cs * (MMAX + 1) + threadIdx.x] = tot[threadIdx.x];
-  const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
-  const double* Vc = V + (size_t)cs * vcase;
   double h2[NKG];
 #pragma unroll
   for (int k = 0; k < NKG; ++k) h2[k] = k <= j ? tot[min(k, NP - 1)] : 0.0;
   double acc[1] = {0.0};
-  JR_VEC_LOOP_PW(N, PW) {
-    double w[PW];
-    load_pw<PW>(W, base + e, w);
+  while (e < N) {
     double s[PW];
 #pragma unroll
     for (int p = 0; p < PW; ++p) s[p] = 0.0;
 #pragma unroll
-    for (int g = 0; g < Cgs<NCH>::NG; ++g) {
+    for (int p = 0; p < PW; ++p)
+#pragma unroll
+      for (int k = 0; k < GS; ++k)
+        if (k <= j) s[p] += h2[k] * v0[k][p];
+#pragma unroll
+    for (int g = 1; g < Cgs<NCH>::NG; ++g) {
       double v[GS][PW];
       basis_batch_pw<GS, PW>(Vc, vstride, e, g * GS, j, v);
 #pragma unroll
@@ -415,6 +447,12 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __rest
     store_pw<PW>(W, base + e, w);
 #pragma unroll
     for (int p = 0; p < PW; ++p) acc[0] += w[p] * w[p];
+    e += step;
+    __builtin_amdgcn_sched_barrier(0);   // the next step's loads reuse this step's registers
+    if (e < N) {
+      load_pw<PW>(W, base + e, w);
+      basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
+    }
   }
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) cg.p3[(size_t)cs * cg.nblk + blockIdx.x] = acc[0];
