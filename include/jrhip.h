@@ -180,6 +180,16 @@ int jr_newton_init_case(jr_ctx* ctx, int case_index, double cfl0);
  * hist_host[n_steps][n_cases] receives |R(U)| before each update. */
 int jr_newton_block(jr_ctx* ctx, int n_steps, double* hist_host);
 
+/* One solve of the Newton step's linear system, (V/dt + dR~/dU) x = b, at
+ * the device state and the current CFL (set it with jr_set_cfl): GMRES from
+ * zero with the context's subspace, restarts and tolerance, right-
+ * preconditioned by the line solve; R~ is the first-order residual unless the
+ * context takes the exact Jacobian.  b_host and x_host are fields
+ * [n_cases][nv][cells]; the state is not changed.  The approximate inverse
+ * of the sensitivity's defect correction: newton_inverse,
+ * src/jaxrans/derivatives.py:94-120. */
+int jr_linear_solve(jr_ctx* ctx, const double* b_host, double* x_host);
+
 /* Execution option: replay a block of Newton steps from a HIP graph captured
  * on the first block of that length (1), or launch every kernel on the
  * context stream (0, the default: faster on MI355X with ROCm 7).  Results are

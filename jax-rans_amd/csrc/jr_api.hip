@@ -84,6 +84,10 @@ struct jr_ctx {
   std::vector<int> probe_id;      // kernel id of each recorded pair
   int probe_used = 0;
   bool has_q = false;
+  // Linear solve with a given right-hand side (jr_linear_solve): while set,
+  // a Newton step solves its system for b = solve_b and stops before the update.
+  const double* solve_b = nullptr;
+  double* solve_buf = nullptr;
   bool per_case_geo = false;   // geometry arrays per case (jr_set_geometry)
   uint64_t topo_hash = 0;      // hash of the index plans: grids of one topology share them
 };
@@ -355,6 +359,7 @@ static void gmres_and_update(jr_ctx* c) {
                        c->m, c->T, c->Act, c->Rv, c->R, c->red);
     launch_apply<R, NV>(c, c->T, c->X, 1, 1);
   }
+  if (c->solve_b) return;   // a linear solve: X is the answer, U stays
   const dim3 cg(c->nblk, c->n_cases);
   if (c->max_rel > 0.0)
     hipLaunchKernelGGL((k_relax<NV>), cg, dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, c->U, c->X, c->red,
@@ -374,6 +379,9 @@ static void newton_step(jr_ctx* c, double* hist_row) {
   if (c->dq_fused)
     hipLaunchKernelGGL((k_ghost_jac<NV>), grid1(P.n_final, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
                        c->ph.gamma, c->ctl, c->Q, c->gjac);
+  if (c->solve_b)   // the GMRES start r = -R is then b
+    hipLaunchKernelGGL(k_negate, grid1((size_t)c->n_cases * c->N, BLOCK, 1), dim3(BLOCK), 0, c->stream,
+                       (size_t)c->n_cases * c->N, c->solve_b, c->R);
   hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
                      hist_row, c->n_cases);
   launch_speeds<NV>(c, c->U);
@@ -489,6 +497,7 @@ static void gmres_and_update3(jr_ctx* c) {
                        c->m, c->T, c->Act, c->Rv, c->R, c->red);
     launch_apply3<R, NV>(c, c->T, c->X, 1, 1);
   }
+  if (c->solve_b) return;   // a linear solve: X is the answer, U stays
   const dim3 cg(c->nblk, c->n_cases);
   if (c->max_rel > 0.0)
     hipLaunchKernelGGL((k_relax3<NV>), cg, dim3(BLOCK), 0, c->stream, c->P3, c->ph.gamma, c->ctl, c->U, c->X, c->red,
@@ -505,6 +514,9 @@ static void newton_step3(jr_ctx* c, double* hist_row) {
   const Plan& P = c->P;
   const dim3 vg(c->nblk, c->n_cases);
   launch_residual3<NV, VISC>(c, c->U, c->R, 0);
+  if (c->solve_b)   // the GMRES start r = -R is then b
+    hipLaunchKernelGGL(k_negate, grid1((size_t)c->n_cases * c->N, BLOCK, 1), dim3(BLOCK), 0, c->stream,
+                       (size_t)c->n_cases * c->N, c->solve_b, c->R);
   hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
                      hist_row, c->n_cases);
   launch_speeds3<NV>(c, c->U);
@@ -1698,6 +1710,29 @@ int jr_get_scalars(jr_ctx* c, double* out) {
     o[7] = h[k].rejected;
     o[8] = h[k].cycles_total;
   }
+  return 0;
+}
+
+// One solve of the Newton step's linear system at the context's state and
+// CFL, for the right-hand side b: GMRES (the context's subspace, restarts and
+// tolerance, line-preconditioned) on V/dt + dR~/dU, started from zero.  The
+// state is left unchanged.  The step's residual, wave speeds and line factor
+// are recomputed at the state first, as a Newton step does.
+int jr_linear_solve(jr_ctx* c, const double* b, double* x) {
+  if (!c || !b || !x) return fail("jr_linear_solve: null argument");
+  if (c->pending_steps) return fail("jr_linear_solve: an asynchronous block is pending: wait for it first");
+  if (!c->has_q) return fail("jr_linear_solve: set the freestream first");
+  if (!c->solve_buf) {
+    TRY(dalloc(c, &c->solve_buf, (size_t)c->n_cases * c->N));
+    HIPCHK(hipDeviceSynchronize());   // dalloc clears on the null stream
+  }
+  HIPCHK(hipMemcpyAsync(c->solve_buf, b, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  c->solve_b = c->solve_buf;
+  step_dispatch(c, nullptr);
+  c->solve_b = nullptr;
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(x, c->X, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -1,0 +1,153 @@
+"""Derivatives of a converged result's force coefficients with respect to the
+flow parameters, on the HIP library: the reference's ``sensitivity``
+(``src/jaxrans/derivatives.py:123-147``) and its tangent solve
+(``src/jaxrans/solvers/sensitivity.py:34-64``).
+
+    result = run(load_case("cases/naca0012_o_euler"))
+    s = sensitivity(result, outputs=("cl", "cd"), parameters=("alpha", "mach"))
+    s.derivatives["cl"]["alpha"]            # dCL/dalpha, per degree
+
+At a converged state ``R(U*(p), p) = 0``, so ``dR/dU dU/dp = -dR/dp`` and
+``dJ/dp = dJ/dp|_U + dJ/dU dU/dp``: one linear solve per parameter.  The
+solve is the reference's defect correction ``x <- x + G (b - A x)`` from
+zero until ``|b - A x| <= tol |b|``, with ``A x`` the exact (second-order)
+Jacobian-vector product of the residual (``jr_tangent``) and ``G`` the linear
+solve of the case's own Newton step at the end of its CFL ramp
+(``jr_linear_solve``: GMRES on ``V/dt + dR~/dU`` at ``cfl_max``,
+line-preconditioned).
+
+The parameters reach the residual and the force coefficients only through
+the freestream primitives and ``mu_inf = M / Re`` (``flow_inputs``, as
+``derivatives.py:65-70``).  The two parameter products that the reference
+takes by forward-mode differentiation of its traced functions -- ``dR/dp`` at
+fixed ``U`` and the output tangent ``dJ/dp|_U + dJ/dU dU`` -- are formed here
+from device evaluations of the residual and the force integration at
+perturbed freestreams and states, by Richardson's fourth-order central
+difference ``(4 D(h/2) - D(h)) / 3`` of ``D(h) = (f(h) - f(-h)) / 2h``, with
+``h`` = 1e-4 degree for ``alpha`` and 1e-5 for ``mach``: truncation
+``O(h^4)`` and round-off near ``1e-16 / h`` relative.  The steps are small so
+that no far-field face changes between inflow and outflow within them (the
+characteristic state switches on the sign of the boundary normal velocity).
+The derivative exists where the discretisation is differentiable
+(``minmod`` and ``van_albada`` have kinks: use ``van_albada_smooth`` or no
+limiter) and is as accurate as the result is converged.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from .core.config import FlowConditions
+from .simulation import Result, _forces, force_rows, freestream_of, make_context
+
+PARAMETERS: tuple[str, ...] = ("alpha", "mach")
+OUTPUTS: tuple[str, ...] = ("cl", "cd", "cm", "cd_friction")
+STEPS = {"alpha": 1e-4, "mach": 1e-5}   # central-difference steps h of the parameter products
+
+
+@dataclass(frozen=True)
+class Sensitivity:
+    """``derivatives[output][parameter]`` (per degree for ``alpha``);
+    ``linear_residuals``: each parameter's linear solve, relative residual;
+    ``iterations``: its defect-correction steps; ``seconds``: wall clock of
+    the solves; ``compile_seconds`` is 0 (the library is built ahead)."""
+
+    derivatives: dict[str, dict[str, float]]
+    linear_residuals: dict[str, float]
+    seconds: float
+    compile_seconds: float = 0.0
+    iterations: dict[str, int] | None = None
+
+    def summary(self) -> str:
+        lines = [f"d{out.upper()}/d{name} = {value:+.8e}"
+                 for out, row in self.derivatives.items() for name, value in row.items()]
+        worst = max(self.linear_residuals.values())
+        return "\n".join(lines + [f"{len(self.linear_residuals)} linear solve(s), "
+                                  f"largest relative residual {worst:.1e}, {self.seconds:.1f} s"])
+
+
+def flow_inputs(flow: FlowConditions, dim: int, p: dict) -> tuple[np.ndarray, float | None]:
+    """Freestream primitives and ``mu_inf`` (``None`` if inviscid) at
+    parameters ``p`` (``alpha`` in degrees, ``mach``), the Reynolds number
+    held (``derivatives.py:65-70``)."""
+    return freestream_of(replace(flow, alpha_deg=float(p["alpha"]), mach=float(p["mach"])), dim)
+
+
+def defect_correction(matvec, b: np.ndarray, approximate_inverse, tol: float, maxiter: int):
+    """``x <- x + G (b - A x)`` from zero until ``|b - A x| <= tol |b|`` or
+    ``maxiter`` steps (``solvers/sensitivity.py:34-50``); returns ``x``, its
+    relative residual and the step count."""
+    b_norm = max(float(np.linalg.norm(b)), np.finfo(np.float64).tiny)
+    x = np.zeros_like(b)
+    r = b
+    k = 0
+    while k < maxiter and float(np.linalg.norm(r)) > tol * b_norm:
+        x = x + approximate_inverse(r)
+        r = b - matvec(x)
+        k += 1
+    return x, float(np.linalg.norm(r)) / b_norm, k
+
+
+def _richardson(f, h: float):
+    """``(4 D(h/2) - D(h)) / 3``, ``D(s) = (f(s) - f(-s)) / 2s``: the
+    derivative of ``f`` at 0 to ``O(h^4)``."""
+    d_h = (f(h) - f(-h)) / (2.0 * h)
+    d_half = (f(0.5 * h) - f(-0.5 * h)) / h
+    return (4.0 * d_half - d_h) / 3.0
+
+
+def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), parameters: tuple[str, ...] = ("alpha",),
+                tol: float = 1e-10, maxiter: int = 10_000) -> Sensitivity:
+    """Derivatives of ``outputs`` (``OUTPUTS``) with respect to ``parameters``
+    (``PARAMETERS``) at ``result``'s state, one linear solve per parameter
+    (``derivatives.py:123-147``)."""
+    for kind, given, known in (("output", outputs, OUTPUTS), ("parameter", parameters, PARAMETERS)):
+        unknown = [g for g in given if g not in known]
+        if unknown or not given:
+            raise ValueError(f"unknown {kind}(s) {unknown or given}; available: {known}")
+    case, grid = result.case, result.grid
+    flow, dim = case.flow, grid.dim
+    p0 = {"alpha": float(flow.alpha_deg), "mach": float(flow.mach)}
+    U = np.asarray(result.U, np.float64)
+    started = time.perf_counter()
+    ctx = make_context(case, grid)
+    try:
+        def at(p: dict, state: np.ndarray | None = None) -> None:
+            q, mu = flow_inputs(flow, dim, p)
+            ctx.set_freestream(q, mu)
+            if state is not None:
+                ctx.set_state(state)
+
+        def shifted(name: str, t: float) -> dict:
+            return {k: v + (t if k == name else 0.0) for k, v in p0.items()}
+
+        def residual_at(name: str, t: float) -> np.ndarray:
+            at(shifted(name, t))
+            return ctx.residual(U)[0]
+
+        def coefficients_at(name: str, dU: np.ndarray, t: float) -> np.ndarray:
+            at(shifted(name, t), U + t * dU)
+            f = _forces(force_rows(ctx, case)[0], dim)
+            return np.array([getattr(f, out) for out in outputs])
+
+        rhs = {name: _richardson(lambda t, n=name: residual_at(n, t), STEPS[name]) for name in parameters}
+
+        def matvec(x: np.ndarray) -> np.ndarray:
+            return ctx.tangent(U, x, first_order=False)[0]
+
+        def inverse(r: np.ndarray) -> np.ndarray:
+            return ctx.linear_solve(r)[0]
+
+        columns, errors, iterations = {}, {}, {}
+        for name in parameters:
+            at(p0, U)   # the output differences below move the freestream and the state
+            ctx.newton_init(case.implicit.cfl_max)
+            dU, errors[name], iterations[name] = defect_correction(matvec, -rhs[name], inverse, tol, maxiter)
+            columns[name] = _richardson(lambda t, n=name, d=dU: coefficients_at(n, d, t), STEPS[name])
+    finally:
+        ctx.close()
+    derivatives = {out: {name: float(columns[name][i]) for name in parameters} for i, out in enumerate(outputs)}
+    return Sensitivity(derivatives, errors, time.perf_counter() - started, 0.0, iterations)

@@ -26,7 +26,7 @@ EXPORTS = (
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
-    "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_last_error",
+    "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_last_error",
 )
 
 
@@ -332,6 +332,14 @@ class Context:
     def set_cfl(self, cfl) -> None:
         c = np.ascontiguousarray(np.broadcast_to(cfl, (self.n_cases,)), np.float64)
         _check(self._lib.jr_set_cfl(self._h, _ptr(c, C.c_double)))
+
+    def linear_solve(self, b: np.ndarray) -> np.ndarray:
+        """x with (V/dt + dR~/dU) x = b at the device state and CFL: the
+        Newton step's GMRES solve, the state left as it is."""
+        b = np.ascontiguousarray(np.broadcast_to(b, self.shape), np.float64)
+        out = np.empty(self.shape)
+        _check(self._lib.jr_linear_solve(self._h, _ptr(b, C.c_double), _ptr(out, C.c_double)))
+        return out
 
     def newton_block(self, n_steps: int) -> np.ndarray:
         hist = np.empty((n_steps, self.n_cases))

@@ -1,0 +1,68 @@
+"""Host logic of ``sensitivity()``: the defect correction
+(``src/jaxrans/solvers/sensitivity.py:34-50``), the fourth-order central
+difference of the parameter products, and the refusal of unknown names
+(``tests/unit/test_sensitivity.py:72-76``).  No GPU."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from jaxrans_amd.derivatives import OUTPUTS, PARAMETERS, _richardson, defect_correction, flow_inputs, sensitivity
+from jaxrans_amd.core.config import FlowConditions
+
+
+def test_defect_correction_solves_with_an_approximate_inverse():
+    rng = np.random.default_rng(3)
+    n = 40
+    A = np.eye(n) * 4.0 + rng.standard_normal((n, n)) * 0.3
+    G = np.linalg.inv(np.diag(np.diag(A)) + np.tril(A, -1))   # Gauss-Seidel: an approximate inverse
+    b = rng.standard_normal(n)
+    x, rel, k = defect_correction(lambda v: A @ v, b, lambda r: G @ r, 1e-12, 500)
+    assert rel <= 1e-12 and 0 < k < 500
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=0, atol=1e-10)
+
+
+def test_defect_correction_stops_at_maxiter_and_reports_the_residual():
+    A = np.array([[2.0, 1.0], [1.0, 3.0]])
+    b = np.array([1.0, 2.0])
+    x, rel, k = defect_correction(lambda v: A @ v, b, lambda r: 0.1 * r, 1e-14, 3)
+    assert k == 3
+    assert rel == pytest.approx(np.linalg.norm(b - A @ x) / np.linalg.norm(b), rel=1e-14)
+
+
+def test_defect_correction_of_a_zero_right_hand_side_takes_no_step():
+    x, rel, k = defect_correction(lambda v: v, np.zeros(5), lambda r: r, 1e-10, 10)
+    assert k == 0 and rel == 0.0 and not x.any()
+
+
+def test_fourth_order_difference():
+    # sin: the error of (4 D(h/2) - D(h)) / 3 is h^4 cos(x) / 480 to leading order
+    x0, h = 0.3, 1e-2
+    got = _richardson(lambda t: np.sin(x0 + t), h)
+    assert abs(got - np.cos(x0)) < 2e-11
+    # a cubic: the fourth-order difference is exact up to round-off
+    got = _richardson(lambda t: np.array([(1.0 + t) ** 3, t ** 2]), 0.1)
+    assert np.allclose(got, [3.0, 0.0], rtol=0, atol=1e-13)
+
+
+def test_flow_inputs_at_the_case_values_are_the_freestream():
+    from jaxrans_amd.simulation import freestream_of
+    flow = FlowConditions(mach=0.6, alpha_deg=2.5, reynolds=5e3)
+    q, mu = flow_inputs(flow, 2, {"alpha": 2.5, "mach": 0.6})
+    q0, mu0 = freestream_of(flow, 2)
+    assert np.array_equal(q, q0) and mu == mu0
+    q1, mu1 = flow_inputs(flow, 2, {"alpha": 2.5, "mach": 0.72})
+    assert mu1 == pytest.approx(0.72 / 5e3, rel=1e-15)   # the Reynolds number held
+    assert np.hypot(q1[1], q1[2]) == pytest.approx(0.72, rel=1e-15)
+    assert flow_inputs(FlowConditions(mach=0.5), 2, {"alpha": 0.0, "mach": 0.5})[1] is None
+
+
+def test_unknown_names_are_refused():
+    with pytest.raises(ValueError, match="unknown parameter"):
+        sensitivity(None, parameters=("reynolds",))
+    with pytest.raises(ValueError, match="unknown output"):
+        sensitivity(None, outputs=("lift",))
+    with pytest.raises(ValueError, match="unknown parameter"):
+        sensitivity(None, parameters=())
+    assert PARAMETERS == ("alpha", "mach") and OUTPUTS == ("cl", "cd", "cm", "cd_friction")
