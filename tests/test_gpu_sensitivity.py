@@ -111,3 +111,31 @@ def test_linear_solve_is_the_newton_step_solve(base):
         assert lin < 0.5
     finally:
         ctx.close()
+
+
+def test_linear_solve_3d_is_the_newton_step_solve():
+    """The 3D context's jr_linear_solve is its Newton step's update too (the
+    reference's ONERA M6 grid, SA, relaxation off so the step is x itself)."""
+    from jaxrans_amd import load_case
+    from jaxrans_amd.core.state import uniform_canvas
+    from jaxrans_amd.simulation import freestream_of, make_context
+    case = load_case(CASE.parent / "onera_m6_sa", ("numerical.implicit.max_relative_update=null",))
+    grid = case.grid()
+    ctx = make_context(case, grid)
+    try:
+        q, mu = freestream_of(case.flow, grid.dim)
+        ctx.set_freestream(q, mu)
+        rng = np.random.default_rng(11)
+        U0 = uniform_canvas(q, grid.shape, case.flow.gamma)
+        U1 = U0 * (1.0 + 1e-3 * rng.standard_normal(U0.shape) * grid.interior[None])
+        ctx.set_state(U1)
+        ctx.newton_init(case.implicit.cfl)
+        x = ctx.linear_solve(-ctx.residual(U1))[0]
+        assert np.array_equal(ctx.get_state()[0], U1)
+        ctx.newton_block(1)
+        step = ctx.get_state()[0] - U1
+        rel = np.linalg.norm(step - x) / np.linalg.norm(x)
+        print(f"[gpu] 3D linear solve vs Newton step: {rel:.3e}", flush=True)
+        assert rel < 1e-12
+    finally:
+        ctx.close()
