@@ -190,6 +190,14 @@ int jr_newton_block(jr_ctx* ctx, int n_steps, double* hist_host);
  * src/jaxrans/derivatives.py:94-120. */
 int jr_linear_solve(jr_ctx* ctx, const double* b_host, double* x_host);
 
+/* n_steps explicit multistage Runge-Kutta steps on the device state, 2D
+ * contexts: U^(k) = U^n - alphas[k] dt/V R(U^(k-1)) with the local time step
+ * dt frozen at U^n (CFL: each case's control value, set by jr_newton_init or
+ * jr_set_cfl); hist_host[n_steps][n_cases] receives |R(U^n)| of each step.
+ * make_rk_step, src/jaxrans/solvers/explicit.py:48-71, with local_time_step,
+ * src/jaxrans/numerics/residual.py:142-166. */
+int jr_rk_block(jr_ctx* ctx, int n_steps, const double* alphas, int n_alphas, double* hist_host);
+
 /* Execution option: replay a block of Newton steps from a HIP graph captured
  * on the first block of that length (1), or launch every kernel on the
  * context stream (0, the default: faster on MI355X with ROCm 7).  Results are

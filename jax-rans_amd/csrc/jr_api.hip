@@ -389,6 +389,33 @@ static void newton_step(jr_ctx* c, double* hist_row) {
   gmres_and_update<NV, VISC, MM, R>(c);
 }
 
+// One explicit multistage step (solvers/explicit.py:48-71): dt / V frozen at
+// U^n, the first stage's residual reported, U^(k) = U^n - alpha_k dt/V
+// R(U^(k-1)); the intermediate stages live in W, the last writes U.
+template <int NV, bool VISC>
+static void rk_step(jr_ctx* c, double* hist_row, const double* alphas, int n) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  const dim3 eg((unsigned)(((size_t)NV * P.nc + BLOCK - 1) / BLOCK), (unsigned)c->n_cases);
+  hipLaunchKernelGGL((k_local_dt<NV, VISC>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, c->U, c->vdt);
+  launch_residual<NV, VISC>(c, c->U, c->R, 0);
+  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
+                     hist_row, c->n_cases);
+  for (int k = 0; k < n; ++k) {
+    if (k > 0) launch_residual<NV, VISC>(c, c->W, c->R, 0);
+    double* out = k + 1 < n ? c->W : c->U;
+    hipLaunchKernelGGL((k_rk_stage<NV>), eg, dim3(BLOCK), 0, c->stream, P.nc, (const double*)c->U,
+                       (const double*)c->vdt, (const double*)c->R, alphas[k], out);
+  }
+}
+
+static void rk_dispatch(jr_ctx* c, double* hist_row, const double* alphas, int n) {
+  if (c->nv == 5) rk_step<5, true>(c, hist_row, alphas, n);
+  else if (c->ph.viscous) rk_step<4, true>(c, hist_row, alphas, n);
+  else rk_step<4, false>(c, hist_row, alphas, n);
+}
+
 // --- the 3D path ---------------------------------------------------------------
 
 template <int NV, bool VISC>
@@ -1732,6 +1759,36 @@ int jr_linear_solve(jr_ctx* c, const double* b, double* x) {
   c->solve_b = nullptr;
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(x, c->X, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// n_steps explicit multistage steps on the device state (2D contexts):
+// alphas[n_alphas] the stage coefficients, the CFL each case's control
+// value (jr_newton_init / jr_set_cfl); hist_host[n_steps][n_cases] receives
+// |R(U)| before each step.
+int jr_rk_block(jr_ctx* c, int n_steps, const double* alphas, int n_alphas, double* hist_host) {
+  if (!c || !alphas) return fail("jr_rk_block: null argument");
+  if (c->dim == 3) return fail("jr_rk_block: the explicit scheme is built for 2D contexts only");
+  if (c->pending_steps) return fail("jr_rk_block: an asynchronous block is pending: wait for it first");
+  if (n_steps < 1) return fail("jr_rk_block: n_steps must be >= 1");
+  if (n_alphas < 1 || n_alphas > 16) return fail("jr_rk_block: between 1 and 16 stage coefficients");
+  if (!c->has_q) return fail("jr_rk_block: set the freestream first");
+  if (n_steps > c->hist_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);   // the captured launches hold the old history pointer
+    if (c->hist) (void)hipFree(c->hist);
+    c->hist = nullptr;
+    HIPCHK(hipMalloc(&c->hist, (size_t)n_steps * c->n_cases * sizeof(double)));
+    c->hist_cap = n_steps;
+  }
+  for (int s = 0; s < n_steps; ++s) {
+    rk_dispatch(c, c->hist + (size_t)s * c->n_cases, alphas, n_alphas);
+    TRY(check_launch());
+  }
+  if (hist_host)
+    HIPCHK(hipMemcpyAsync(hist_host, c->hist, (size_t)n_steps * c->n_cases * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -208,6 +208,66 @@ __global__ void __launch_bounds__(BLOCK) k_cell_speeds(Plan P, Physics ph, const
   }
 }
 
+// --- explicit multistage Runge-Kutta (solvers/explicit.py:48-71) --------------
+// Per canvas cell dt / V with dt = CFL V / max(total, 1e-300), total the
+// spectral radii |v . S| + a |S| at the cell's own state over its lower and
+// upper i-faces, then its lower and upper j-faces, plus, when viscous, the
+// diffusive limit 4 (gamma / Pr) (mu / rho) (S_i^2 + S_j^2) / V with S_a the
+// mean area of the two faces normal to a (numerics/residual.py:142-166).
+// CFL: the case's control value.
+template <int NV, bool VISC>
+__global__ void __launch_bounds__(BLOCK) k_local_dt(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                    const double* __restrict__ U, double* __restrict__ fac) {
+  bind_case(P, blockIdx.y);
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ;
+  const int i = c / NJ, j = c - i * NJ;
+  const int fi0 = i * NJ + j, fi1 = fi0 + NJ;
+  const int fj0 = i * (NJ + 1) + j, fj1 = fj0 + 1;
+  const double gamma = ph.gamma;
+  double q[NV];
+  prim_at<NV>(U, (size_t)cs * NV * P.nc, P.nc, c, gamma, q);
+  const double a = sqrt(gamma * q[3] / q[0]);
+  const double* Si0 = P.fi;
+  const double* Si1 = P.fi + P.nfi;
+  const double* Sj0 = P.fj;
+  const double* Sj1 = P.fj + P.nfj;
+  const double mi0 = hypot_j(Si0[fi0], Si1[fi0]), mi1 = hypot_j(Si0[fi1], Si1[fi1]);
+  const double mj0 = hypot_j(Sj0[fj0], Sj1[fj0]), mj1 = hypot_j(Sj0[fj1], Sj1[fj1]);
+  const double li0 = fabs(q[1] * Si0[fi0] + q[2] * Si1[fi0]) + a * mi0;
+  const double li1 = fabs(q[1] * Si0[fi1] + q[2] * Si1[fi1]) + a * mi1;
+  const double lj0 = fabs(q[1] * Sj0[fj0] + q[2] * Sj1[fj0]) + a * mj0;
+  const double lj1 = fabs(q[1] * Sj0[fj1] + q[2] * Sj1[fj1]) + a * mj1;
+  double total = ((li0 + li1) + lj0) + lj1;
+  const double V = P.area[c];
+  if constexpr (VISC) {
+    const double si = 0.5 * (mi0 + mi1), sj = 0.5 * (mj0 + mj1);
+    const double squares = si * si + sj * sj;
+    const double mu_inf = ctl[cs].mu_inf;
+    double mu = viscosity<double>(gamma * q[3] / q[0], mu_inf, ph.sutherland);
+    if constexpr (NV > 4) mu = mu + eddy_viscosity<double>(q[0], q[4] * mu_inf, mu / q[0]);
+    const double factor = gamma / PR * mu / q[0];
+    total = total + 4.0 * factor * squares / V;
+  }
+  const double dt = ctl[cs].cfl * V / fmax(total, 1e-300);
+  fac[(size_t)cs * P.nc + c] = dt / V;
+}
+
+// One stage: out = U0 - alpha (dt / V) R, every row of every cell.  out may
+// be U0 itself (each entry is read before it is written).
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_rk_stage(int nc, const double* U0, const double* __restrict__ fac,
+                                                    const double* __restrict__ R, double alpha, double* out) {
+  const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (e >= (size_t)NV * nc) return;
+  const size_t o = (size_t)cs * NV * nc + e;
+  const int c = (int)(e % (size_t)nc);
+  out[o] = U0[o] - alpha * fac[(size_t)cs * nc + c] * R[o];
+}
+
 // ---------------------------------------------------------------------------
 // Line preconditioner: one thread per chain, block Thomas along it.
 //   diag  = sum lambda (1/CFL + 1/2) I

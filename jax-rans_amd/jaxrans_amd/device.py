@@ -26,7 +26,7 @@ EXPORTS = (
     "jr_timer_stop", "jr_synchronize", "jr_device_alloc", "jr_device_free", "jr_copy_to_device", "jr_copy_to_host",
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
-    "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_last_error",
+    "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
 )
 
 
@@ -340,6 +340,16 @@ class Context:
         out = np.empty(self.shape)
         _check(self._lib.jr_linear_solve(self._h, _ptr(b, C.c_double), _ptr(out, C.c_double)))
         return out
+
+    def rk_block(self, n_steps: int, alphas) -> np.ndarray:
+        """``n_steps`` explicit multistage steps (stage coefficients
+        ``alphas``, the CFL of ``newton_init`` / ``set_cfl``); returns the
+        residual history ``(n_steps, n_cases)``."""
+        a = np.ascontiguousarray(alphas, np.float64)
+        hist = np.empty((n_steps, self.n_cases))
+        _check(self._lib.jr_rk_block(self._h, int(n_steps), _ptr(a, C.c_double), int(a.size),
+                                     _ptr(hist, C.c_double)))
+        return hist
 
     def newton_block(self, n_steps: int) -> np.ndarray:
         hist = np.empty((n_steps, self.n_cases))

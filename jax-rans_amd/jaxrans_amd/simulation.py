@@ -176,10 +176,17 @@ class Result:
         return self.grid.cells
 
 
-def _check_case(case: Case) -> None:
-    if case.numerics.solver != "implicit":
-        raise CaseError("only the implicit (Newton-Krylov) solver runs on the device; set "
-                        "numerical.time_integration.solver = \"implicit\"")
+RK_SCHEMES = {"euler": (1.0,), "rk4_jameson": (0.25, 1.0 / 3.0, 0.5, 1.0),
+              "rk5_jameson": (0.25, 1.0 / 6.0, 0.375, 0.5, 1.0)}   # solvers/explicit.py:23-27
+
+
+def _check_case(case: Case, dim: int | None = None) -> None:
+    if case.numerics.solver == "explicit":
+        if case.rk_scheme not in RK_SCHEMES:
+            raise CaseError(f"unknown RK scheme {case.rk_scheme!r}; available: {sorted(RK_SCHEMES)}")
+        if dim == 3:
+            raise CaseError("the explicit Runge-Kutta solver is built for 2D cases only; set "
+                            "numerical.time_integration.solver = \"implicit\"")
     if case.flow.swept:
         raise CaseError("swept (2.5D) flow is not built by this implementation")
 
@@ -193,7 +200,8 @@ def freestream_of(flow: FlowConditions, dim: int = 2) -> tuple[np.ndarray, float
 
 
 def run_to_steady(ctx: Context, *, max_iters: int, tol: float, block_iters: int,
-                  on_block: Callable[[int, np.ndarray], bool] | None = None):
+                  on_block: Callable[[int, np.ndarray], bool] | None = None,
+                  block: Callable[[int], np.ndarray] | None = None):
     """Blocks of Newton steps until every case converges (``last <= tol *
     running peak``), stalls, goes non-finite, settles or hits ``max_iters``
     (driver.py:123-201, the same tests in the same order)."""
@@ -203,7 +211,7 @@ def run_to_steady(ctx: Context, *, max_iters: int, tol: float, block_iters: int,
     started = time.perf_counter()
     first = float("nan")
     for b in range(n_blocks):
-        res = ctx.newton_block(block_iters)
+        res = ctx.newton_block(block_iters) if block is None else block(block_iters)
         if b == 0:
             first = time.perf_counter() - started
         blocks.append(res)
@@ -263,7 +271,14 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) 
             raise ValueError(f"initial field must have shape {U0.shape}, got {initial.shape}")
         U0 = np.where(grid.interior[None], initial, U0)
     ctx.set_state(U0)
-    ctx.newton_init(case.implicit.cfl)
+    block = None
+    if numerics.solver == "explicit":   # solvers/explicit.py:48-71, CFL numerics.cfl
+        _check_case(case, grid.dim)
+        ctx.newton_init(numerics.cfl)
+        alphas = RK_SCHEMES[case.rk_scheme]
+        block = lambda n: ctx.rk_block(n, alphas)   # noqa: E731
+    else:
+        ctx.newton_init(case.implicit.cfl)
     cl_history, coefficients, peak = [], [], [0.0]
 
     def on_block(index: int, residuals: np.ndarray) -> bool:
@@ -281,7 +296,7 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) 
             cl_spread(h) < numerics.forces_tol for h in np.transpose(coefficients))
 
     history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
-                            on_block=on_block)
+                            on_block=on_block, block=block)
     U = ctx.get_state()[0]
     forces = _forces(force_rows(ctx, case)[0], grid.dim)
     timing = Timing(history.seconds, history.first_block_seconds, history.n_iters, case.block_iters, grid.cells)
@@ -314,8 +329,15 @@ def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None 
     ctx.set_freestream(q5, mus)
     U0 = np.stack([uniform_canvas(q, grid.shape, f.gamma) for q, f in zip(qs, flows)])
     ctx.set_state(U0)
-    ctx.newton_init(np.full(n, case.implicit.cfl))
     numerics = case.numerics
+    block = None
+    if numerics.solver == "explicit":
+        _check_case(case, grid.dim)
+        ctx.newton_init(np.full(n, numerics.cfl))
+        rk = RK_SCHEMES[case.rk_scheme]
+        block = lambda k: ctx.rk_block(k, rk)   # noqa: E731
+    else:
+        ctx.newton_init(np.full(n, case.implicit.cfl))
     cl_hist = []
 
     def on_block(index, residuals):
@@ -323,7 +345,7 @@ def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None 
         return False
 
     history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
-                            on_block=on_block)
+                            on_block=on_block, block=block)
     U = ctx.get_state()
     F = force_rows(ctx, case)
     cl_hist = np.asarray(cl_hist)
