@@ -26,6 +26,7 @@ from ..mesh.assemble3d import PatchRule, is_3d, load_grid_3d
 from ..mesh.atlas import Atlas, assign_conditions, build_atlas, patches
 from ..mesh.atlas3d import build_atlas_3d
 from ..mesh.block import folded
+from ..mesh.coarsen import coarsen as coarsen_mesh
 from ..mesh.generate import c_grid, o_grid
 from . import schema
 
@@ -140,13 +141,21 @@ class Case:
     sequencing: int = 0
 
     def grid(self, coarsen: int = 0) -> Atlas:
-        if coarsen:
-            raise CaseError("grid sequencing is not built by this implementation")
+        """The case's grid, ``coarsen`` times coarsened by every other node
+        (grid sequencing, ``src/jaxrans/mesh/coarsen.py:58-72``)."""
         if self.flow.swept:
             raise CaseError("'freestream.sweep_deg' (2.5D) is not built by this implementation")
         mesh = self.grid_spec.build(self.directory)
         kinds = resolve_conditions(self.conditions, patches(mesh), self.flow)
         mesh = assign_conditions(mesh, kinds)
+        if coarsen:
+            if mesh[0].dim == 3:
+                raise CaseError("grid sequencing is built for 2D grids only")
+            try:
+                for _ in range(coarsen):
+                    mesh = coarsen_mesh(mesh)
+            except ValueError as error:
+                raise CaseError(str(error)) from None
         if mesh[0].dim == 3:
             if self.grid_spec.reference_area is None:
                 raise CaseError("a 3D case needs domain.reference_area")

@@ -21,6 +21,7 @@ from .core.config import FlowConditions
 from .core.state import q_infinity, uniform_canvas
 from .device import Context, context_for
 from .mesh.atlas import Atlas
+from .mesh.coarsen import prolong
 
 Progress = Callable[[int, float, float, "Forces"], None]
 
@@ -251,14 +252,27 @@ def run(case: Case, grid: Atlas | None = None, progress: Progress | None = None,
     """March ``case`` to steady state on the device (simulation.py:213-240)."""
     if devices is not None and len(devices) > 1:
         raise CaseError("this implementation runs one case on one GPU; multi-device slabs are not built")
-    if case.sequencing:
-        raise CaseError("grid sequencing is not built by this implementation")
     _check_case(case)
     grid = case.grid() if grid is None else grid
-    return _march(case, grid, progress, initial=initial)
+    if initial is not None:
+        return _march(case, grid, progress, initial=initial)
+    # grid sequencing (simulation.py:230-240): the coarser grids first, each
+    # started from the one before by injection
+    levels, start = [], None
+    for level in range(case.sequencing, 0, -1):
+        coarse = case.grid(coarsen=level)
+        if on_level is not None:
+            on_level(level, coarse)
+        r = _march(case, coarse, progress, start=start)
+        levels.append({"cells": coarse.cells, "iterations": r.history.n_iters, "seconds": r.seconds,
+                       "exit_reason": r.history.reason, "cl": r.forces.cl, "cd": r.forces.cd})
+        start = (r.U, coarse)
+    if levels and on_level is not None:
+        on_level(0, grid)
+    return replace(_march(case, grid, progress, start=start), levels=tuple(levels))
 
 
-def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) -> Result:
+def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, start=None) -> Result:
     flow, numerics = case.flow, case.numerics
     spec = case.grid_spec
     ctx = make_context(case, grid)
@@ -270,6 +284,8 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None) 
         if initial.shape != U0.shape:
             raise ValueError(f"initial field must have shape {U0.shape}, got {initial.shape}")
         U0 = np.where(grid.interior[None], initial, U0)
+    elif start is not None:
+        U0 = prolong(start[0], start[1], grid, U0)
     ctx.set_state(U0)
     block = None
     if numerics.solver == "explicit":   # solvers/explicit.py:48-71, CFL numerics.cfl
