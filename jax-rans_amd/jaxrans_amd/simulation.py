@@ -176,6 +176,12 @@ class Result:
     def cells(self) -> int:
         return self.grid.cells
 
+    def write(self, directory=None, extra: dict | None = None):
+        """Metadata, residual and lift histories and fields under the
+        reference's file names (``simulation.py:135-160``); returns the directory."""
+        from .output import write_result
+        return write_result(self, directory, extra)
+
 
 RK_SCHEMES = {"euler": (1.0,), "rk4_jameson": (0.25, 1.0 / 3.0, 0.5, 1.0),
               "rk5_jameson": (0.25, 1.0 / 6.0, 0.375, 0.5, 1.0)}   # solvers/explicit.py:23-27

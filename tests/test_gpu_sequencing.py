@@ -26,3 +26,26 @@ def test_sequenced_run_converges_to_the_unsequenced_forces():
     assert seq.history.reason == "converged" and plain.history.reason == "converged"
     for out in ("cl", "cd", "cm"):
         assert abs(getattr(seq.forces, out) - getattr(plain.forces, out)) < 1e-8, out
+
+
+def test_result_write_files(tmp_path):
+    """Result.write: the reference's file names (io/output.py:80-135) with
+    the run's own numbers in them."""
+    import json
+
+    import numpy as np
+
+    from jaxrans_amd import load_case, run
+    result = run(load_case(CASE, ("numerical.convergence.max_iters=40",)))
+    out = result.write(tmp_path / "out")
+    name = result.case.name
+    meta = json.loads((out / f"{name}_metadata.json").read_text())
+    assert meta["config"]["flow"]["mach"] == result.case.flow.mach and meta["extra"]["cells"] == result.cells
+    res = np.loadtxt(out / f"{name}_residual.dat")
+    assert res.shape == (40, 2) and np.allclose(res[:, 1], result.history.residual, rtol=1e-11, atol=0)
+    lift = np.loadtxt(out / f"{name}_lift.dat", ndmin=2)
+    assert np.allclose(lift[:, 1], result.cl_history, rtol=1e-11, atol=0)
+    with np.load(out / f"{name}_fields.npz") as f:
+        assert int(f["n_blocks"]) == 1
+        assert np.array_equal(f["U_0"], result.grid.unpack(result.U)[0])
+        assert f["x_node_0"].shape == (f["U_0"].shape[1] + 1, f["U_0"].shape[2] + 1)
