@@ -252,7 +252,13 @@ __global__ void __launch_bounds__(BLOCK) k_local_dt(Plan P, Physics ph, const Ct
     total = total + 4.0 * factor * squares / V;
   }
   const double dt = ctl[cs].cfl * V / fmax(total, 1e-300);
-  fac[(size_t)cs * P.nc + c] = dt / V;
+  // Gap and ghost rows carry R = U - U0, which the reference's march keeps at
+  // exactly zero from a freestream start.  U0 is recomputed here (contracted
+  // to FMAs), so it can sit an ulp from the host's U0, and dt/V on those
+  // degenerate rows reaches ~80: an ulp seed would grow by ~1e7 per RK5
+  // step.  Their update factor is zero, which keeps them at their start
+  // value; interior cells never read them (ghosts are filled from sources).
+  fac[(size_t)cs * P.nc + c] = P.interior[c] ? dt / V : 0.0;
 }
 
 // One stage: out = U0 - alpha (dt / V) R, every row of every cell.  out may

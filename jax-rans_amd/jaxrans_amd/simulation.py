@@ -280,7 +280,7 @@ def run(case: Case, grid: Atlas | None = None, progress: Progress | None = None,
 
 def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, start=None) -> Result:
     flow, numerics = case.flow, case.numerics
-    spec = case.grid_spec
+    _check_case(case, grid.dim)   # before the device context exists: a refusal leaves nothing open
     ctx = make_context(case, grid)
     q, mu = freestream_of(flow, grid.dim)
     ctx.set_freestream(q, mu)
@@ -295,7 +295,6 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, 
     ctx.set_state(U0)
     block = None
     if numerics.solver == "explicit":   # solvers/explicit.py:48-71, CFL numerics.cfl
-        _check_case(case, grid.dim)
         ctx.newton_init(numerics.cfl)
         alphas = RK_SCHEMES[case.rk_scheme]
         block = lambda n: ctx.rk_block(n, alphas)   # noqa: E731
@@ -328,14 +327,63 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, 
     return result
 
 
-def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None = None,
-              progress: Progress | None = None):
-    """A batch of freestreams on one grid in one device context (the reference's
-    ``run_polar``, ``src/jaxrans/polar.py:80-121``, with per-case Mach and
-    Reynolds as its slot scheduler carries them, ``sweep.py:196-201``).
-    Returns one ``Result`` per case."""
-    _check_case(case)
+@dataclass(frozen=True)
+class Polar:
+    """A batch of freestreams on one grid (``src/jaxrans/polar.py:43-75``):
+    one ``Result`` per case, in the order of ``alphas``, and the batch timing."""
+
+    case: Case
+    alphas: np.ndarray
+    results: list
+    timing: Timing
+
+    def table(self) -> np.ndarray:
+        """One row per case: alpha, CL, CD, CD friction, CM, orders dropped,
+        iterations, CL spread over the last quarter of blocks, settled (1.0 when
+        that spread is within the settled threshold)."""
+        rows = []
+        for alpha, r in zip(self.alphas, self.results):
+            spread = r.cl_spread
+            rows.append([alpha, r.forces.cl, r.forces.cd, r.forces.cd_friction, r.forces.cm,
+                         r.history.orders_dropped, r.history.n_iters, spread, float(spread <= SETTLED_CL_SPREAD)])
+        return np.asarray(rows, dtype=np.float64)
+
+    def write(self, directory) -> "Path":
+        """``polar.dat`` (the table), ``polar_metadata.json`` (case name and
+        set-up, alphas, timing) and each case's result files in
+        ``alpha_{alpha:+.2f}/``; returns the directory."""
+        import json
+        from pathlib import Path
+
+        from .output import _jsonable
+        out = Path(directory)
+        out.mkdir(parents=True, exist_ok=True)
+        np.savetxt(out / "polar.dat", self.table(),
+                   header="alpha_deg cl cd cd_friction cm orders_dropped iterations cl_spread settled",
+                   fmt=["%.4f", "%.8f", "%.8f", "%.8f", "%.8f", "%.3f", "%d", "%.3e", "%d"])
+        meta = {"case": self.case.name, "case_setup": _jsonable(self.case.raw),
+                "alphas": [float(a) for a in self.alphas], "timing": self.timing.record(),
+                "settled_cl_spread": SETTLED_CL_SPREAD}
+        (out / "polar_metadata.json").write_text(json.dumps(meta, indent=2, default=float))
+        for alpha, r in zip(self.alphas, self.results):
+            r.write(out / f"alpha_{alpha:+.2f}")
+        return out
+
+
+def run_polar(case: Case, alphas, grid: Atlas | None = None, devices=None, progress=None, *,
+              machs=None, reynolds=None) -> Polar:
+    """``case`` at every angle of ``alphas`` (degrees) in one device context,
+    every kernel launch shared by the whole batch (the reference's
+    ``run_polar``, ``src/jaxrans/polar.py:80-121``; same positional order).
+    ``progress(iteration, CL per case)`` is called after each block.  The
+    keyword-only ``machs`` and ``reynolds`` give each case its own Mach
+    number and Reynolds number, as the reference's slot scheduler carries
+    them per slot (``sweep.py:196-201``); by default every case has the
+    case's own.  One GPU: ``devices`` above 1 is refused."""
+    if devices is not None and int(devices) > 1:
+        raise CaseError("this implementation runs a polar on one GPU; splitting it over devices is not built")
     grid = case.grid() if grid is None else grid
+    _check_case(case, grid.dim)
     alphas = np.atleast_1d(np.asarray(alphas, np.float64))
     n = alphas.size
     machs = np.full(n, case.flow.mach) if machs is None else np.broadcast_to(machs, (n,)).astype(np.float64)
@@ -354,7 +402,6 @@ def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None 
     numerics = case.numerics
     block = None
     if numerics.solver == "explicit":
-        _check_case(case, grid.dim)
         ctx.newton_init(np.full(n, numerics.cfl))
         rk = RK_SCHEMES[case.rk_scheme]
         block = lambda k: ctx.rk_block(k, rk)   # noqa: E731
@@ -364,6 +411,8 @@ def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None 
 
     def on_block(index, residuals):
         cl_hist.append(force_rows(ctx, case)[:, 0].copy())
+        if progress is not None:
+            progress((index + 1) * case.block_iters, cl_hist[-1].copy())
         return False
 
     history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
@@ -376,8 +425,8 @@ def run_polar(case: Case, alphas, machs=None, reynolds=None, grid: Atlas | None 
     out = []
     for k in range(n):
         h = history.case(k) if n > 1 else history
-        out.append(Result(case=replace(case, flow=flows[k]), grid=grid, U=U[k], forces=_forces(F[k], grid.dim),
-                          history=h,
+        sub = replace(case, flow=flows[k], name=f"{case.name}_a{alphas[k]:+.2f}")
+        out.append(Result(case=sub, grid=grid, U=U[k], forces=_forces(F[k], grid.dim), history=h,
                           cl_history=cl_hist[:, k] if cl_hist.size else cl_hist, timing=timing))
     ctx.close()
-    return out
+    return Polar(case=case, alphas=alphas, results=out, timing=timing)

@@ -206,39 +206,50 @@ def net_flux(q, setup: Setup, first_order: bool):
     sa_row = None
     if setup.viscous:
         p1 = [r[g - 1:r.shape[0] - (g - 1), g - 1:r.shape[1] - (g - 1)] for r in pad]
-        T1 = P.temperature(p1, setup.gamma)
-        phi = [p1[1], p1[2], T1] + ([p1[4]] if setup.sa else [])
-        grads = face_gradients(phi, at.centres)
-        visc = []
-        mass = []
-        for a in range(2):
-            left, right = _lr(p1, a)
-            gx, gy = grads[a]
-            gv = [[gx[0], gy[0]], [gx[1], gy[1]]]
-            gT = [gx[2], gy[2]]
-            vel = [0.5 * (left[1] + right[1]), 0.5 * (left[2] + right[2])]
-            T = 0.5 * (P.temperature(left, setup.gamma) + P.temperature(right, setup.gamma))
-            mu = P.viscosity(T, setup.mu_inf, setup.sutherland)
-            k = mu / (P.PR * (setup.gamma - 1.0))
-            extra = None
-            if setup.sa:
-                face = [0.5 * (left[c] + right[c]) for c in range(5)]
-                Tf = P.temperature(face, setup.gamma)
-                muf = P.viscosity(Tf, setup.mu_inf, setup.sutherland)
-                mut = P.eddy_viscosity(face[0], face[4] * setup.mu_inf, muf / face[0])
-                mu = mu + mut
-                k = k + mut / (P.PR_T * (setup.gamma - 1.0))
-                if setup.c_cr1:
-                    extra = P.qcr_stress(gv, mut, setup.c_cr1)
-            visc.append(P.viscous_flux(vel, gv, gT, S[a], mu, k, extra))
-            mass.append(fluxes[a][0])
+        visc, grads = viscous_face_fluxes(p1, setup)
         if setup.sa:
+            mass = [fluxes[a][0] for a in range(2)]
             sa_row = _sa_row(p1, mass, [(grads[a][0][3], grads[a][1][3]) for a in range(2)], setup)
         fluxes = [[fluxes[a][c] - visc[a][c] for c in range(4)] for a in range(2)]
     out = [divergence((fluxes[0][c], fluxes[1][c])) for c in range(4)]
     if sa_row is not None:
         out.append(sa_row)
     return out
+
+
+def viscous_face_fluxes(p1, setup: Setup):
+    """``F_v . S`` rows of both face families from primitive rows ``p1`` on the
+    canvas padded by one layer (``(NI+2, NJ+2)``), and the diamond gradients
+    they used (src/jaxrans/numerics/residual.py:94-124 with
+    physics/navier_stokes.py:86-116; the SA eddy viscosity and QCR stress of
+    turbulence.py:78-90 and spalart_allmaras.py:197-216)."""
+    at = setup.atlas
+    S = (at.face_i, at.face_j)
+    T1 = P.temperature(p1, setup.gamma)
+    phi = [p1[1], p1[2], T1] + ([p1[4]] if setup.sa else [])
+    grads = face_gradients(phi, at.centres)
+    visc = []
+    for a in range(2):
+        left, right = _lr(p1, a)
+        gx, gy = grads[a]
+        gv = [[gx[0], gy[0]], [gx[1], gy[1]]]
+        gT = [gx[2], gy[2]]
+        vel = [0.5 * (left[1] + right[1]), 0.5 * (left[2] + right[2])]
+        T = 0.5 * (P.temperature(left, setup.gamma) + P.temperature(right, setup.gamma))
+        mu = P.viscosity(T, setup.mu_inf, setup.sutherland)
+        k = mu / (P.PR * (setup.gamma - 1.0))
+        extra = None
+        if setup.sa:
+            face = [0.5 * (left[c] + right[c]) for c in range(5)]
+            Tf = P.temperature(face, setup.gamma)
+            muf = P.viscosity(Tf, setup.mu_inf, setup.sutherland)
+            mut = P.eddy_viscosity(face[0], face[4] * setup.mu_inf, muf / face[0])
+            mu = mu + mut
+            k = k + mut / (P.PR_T * (setup.gamma - 1.0))
+            if setup.c_cr1:
+                extra = P.qcr_stress(gv, mut, setup.c_cr1)
+        visc.append(P.viscous_flux(vel, gv, gT, S[a], mu, k, extra))
+    return visc, grads
 
 
 def _sa_row(p1, mass, grad_nu, setup):

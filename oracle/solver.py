@@ -338,6 +338,21 @@ def relax(dU, U, gamma, fraction):
     return omega * dU, omega
 
 
+def next_cfl(cfl, accept, lin_red, omega, reduction, cfg):
+    """Switched evolution relaxation (implicit.py:152-157): grow by the
+    residual reduction, at most ``cfl_growth``, only after an accepted,
+    unrelaxed step whose linear solve reached ``krylov_success``; hold after
+    other accepted steps; cut back by ``cfl_cutback`` after a rejection;
+    clipped to ``[cfl_min, cfl_max]``."""
+    if accept and lin_red < cfg.krylov_success and omega >= 1.0:
+        nxt = cfl * min(max(reduction, 1.0), cfg.cfl_growth)
+    elif accept:
+        nxt = cfl
+    else:
+        nxt = cfl / cfg.cfl_cutback
+    return min(max(nxt, cfg.cfl_min), cfg.cfl_max)
+
+
 class NewtonState:
     def __init__(self, U, cfl, residual_norm=np.inf):
         self.U, self.cfl, self.residual_norm = U, cfl, residual_norm
@@ -367,13 +382,7 @@ def newton_step(state: NewtonState, setup: Setup, cfg, method: str | None = None
     physical = bool(np.all(is_physical(cand, setup.gamma))) and bool(np.all(np.isfinite(dU)))
     accept = physical and lin_red < 1.0
     reduction = state.residual_norm / max(nrm, 1e-300)
-    if accept and lin_red < cfg.krylov_success and omega >= 1.0:
-        nxt = cfl * min(max(reduction, 1.0), cfg.cfl_growth)
-    elif accept:
-        nxt = cfl
-    else:
-        nxt = cfl / cfg.cfl_cutback
-    new = NewtonState(cand if accept else U, min(max(nxt, cfg.cfl_min), cfg.cfl_max),
+    new = NewtonState(cand if accept else U, next_cfl(cfl, accept, lin_red, omega, reduction, cfg),
                       nrm if accept else state.residual_norm)
     return new, nrm, {"accept": accept, "lin_red": lin_red, "omega": omega}
 

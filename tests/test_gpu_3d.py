@@ -220,8 +220,10 @@ def test_run_polar_equals_each_run_alone(which):
                                  "numerical.convergence.forces_tol=null"))
     grid = case.grid()
     alphas, machs = [1.0, 3.5], [0.5, 0.7]
-    batch = run_polar(case, alphas, machs, grid=grid)
-    assert len(batch) == 2
+    polar = run_polar(case, alphas, grid, None, None, machs=machs)   # the reference's positional order
+    batch = polar.results
+    assert len(batch) == 2 and polar.table().shape == (2, 9)
+    assert np.array_equal(polar.table()[:, 0], alphas)
     for k, (a, m) in enumerate(zip(alphas, machs)):
         alone = run(replace(case, flow=replace(case.flow, alpha_deg=a, mach=m)), grid)
         assert np.array_equal(alone.history.residual, batch[k].history.residual), (k, alone.history.residual,

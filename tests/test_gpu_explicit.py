@@ -66,3 +66,34 @@ def test_explicit_run_marches_with_the_oracle():
     print(f"[gpu] explicit run: {result.history.summary()}", flush=True)
     assert np.allclose(result.history.residual, norms, rtol=1e-10, atol=0.0)
     assert rel(result.U, U) < 1e-11
+
+
+def test_explicit_march_from_freestream_where_host_and_device_round_differently():
+    """M 0.33, alpha 0.75 deg: a freestream whose conservative energy the
+    device's fused multiply-adds round one ulp away from NumPy's.  The gap and
+    ghost rows (R = U - U0) must not amplify that seed: 20 RK5 steps from the
+    host freestream stay finite and equal the oracle's steps."""
+    from jaxrans_amd.core.config import FlowConditions, NumericsConfig
+    from jaxrans_amd.device import context_for
+    from conftest import o_atlas
+    print("[gpu] explicit march M0.33 a0.75", flush=True)
+    atlas = o_atlas()
+    flow = FlowConditions(mach=0.33, alpha_deg=0.75)
+    num = NumericsConfig(flux="hllc", limiter="minmod", solver="explicit")
+    st = OR.setup_for(atlas, flow, num)
+    ctx = context_for(atlas, flow, num)
+    ctx.set_freestream(st.q_inf, None)
+    alphas = OX.RK_SCHEMES["rk5_jameson"]
+    U = OR.U0_of(st)
+    ctx.set_state(U)
+    ctx.newton_init(1.5)
+    hist = ctx.rk_block(20, alphas)[:, 0]
+    ref, norms = U, []
+    for _ in range(20):
+        ref, n = OX.rk_step(ref, st, alphas, 1.5)
+        norms.append(n)
+    got = ctx.get_state()[0]
+    ctx.close()
+    assert np.all(np.isfinite(got)) and np.all(np.isfinite(hist))
+    assert rel(got, ref) < 1e-11, rel(got, ref)
+    assert np.allclose(hist, norms, rtol=1e-9, atol=1e-15), (hist, norms)
