@@ -1,20 +1,26 @@
 """Benchmark: Newton iterations x Mcells/s of the pseudo-transient Newton-Krylov
-step on one MI355X, on the 2D SA RANS case (Karman-Trefftz 10 deg section,
-C-grid 512 x 128 cells, M 0.73, alpha 2.5 deg, Re 6.5e6, Roe + smooth van
-Albada, GMRES(30) x 2 with the line preconditioner, FP64).
+step on one MI355X.  Default workload: the batched polar -- 32 cases of the 2D
+SA RANS case (Karman-Trefftz 10 deg section, C-grid 512 x 128 cells, Re 6e6,
+alpha U[-4, 12] deg, Mach U[0.3, 0.78], Roe + smooth van Albada, GMRES(30) x 2
+with the line preconditioner) in one context, every launch shared by the
+batch: 2.1M cells, FP64 (mixed precision reported beside it).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+    python bench.py --single            (the 512 x 128 case alone, M 0.73, alpha 2.5, Re 6.5e6)
+    python bench.py --polar B           (B cases instead of 32)
     python bench.py --converge [CASE_DIR]
     python bench.py --serve N [--rate R] [--forces-tol T] [--in-flight K]
     python bench.py --wing [--steps K] [--warmup W]
     python bench.py --calibrate-copy
 
-Prints one JSON line.  A step is one full Newton step of the case (residual,
+Prints one JSON line.  A step is one full Newton step of every case (residual,
 wave speeds, line factor, GMRES, update) on the device; the state starts at
 the freestream and W steps run untimed first.  The timed region is bracketed
 by device synchronisation.  The roofline entry times the dominant kernel with
-HIP events on the library's stream.  The CPU baseline times the FP64 oracle
-(NumPy, one thread) on one Newton step of the same case in a child process.
+HIP events on the library's stream and prices it with the algorithmic bytes
+per interior cell of the path it belongs to (residual 8 (2 nv + d^2 + (d+1)
+viscous + 1 SA), line apply 3 m^2 (+3 SA) block entries + 16 nv, ...).  The
+CPU baseline times the FP64 oracle on a bounded sample of the same work.
 """
 
 from __future__ import annotations
@@ -119,6 +125,56 @@ def path_bytes_per_cell(ctx) -> dict:
             "precond_factor": 8 * (nv + d + d * d + (2 if sa else 0)) + blocks * fb}
 
 
+def survey_bytes(ctx) -> dict:
+    """Algorithmic bytes per interior cell of each hot path, every array
+    touched once (nv variables, d dimensions, m x m flow blocks): residual
+    ``8 (2 nv + d^2 + (d + 1) viscous + 1 SA)``; tangent ``+ 8 (nv + 1)``;
+    wave speeds ``8 (nv + d^2 + d + 1 [+ 1 SA])``; line factor ``8 (nv + d +
+    d^2 [+ 2 SA]) + (3 m^2 [+ 3 SA]) b``; line apply ``(3 m^2 [+ 3 SA]) b + 16
+    nv`` (b = 8 FP64, 4 FP32 factor); one GMRES(k) cycle of fused three-pass
+    CGS2 ``8 nv (1.5 k (k - 1) + 7 k)``."""
+    nv, d = ctx.nvar, ctx.atlas.dim
+    visc, sa = ctx.viscous, ctx.nvar > d + 2
+    m = d + 2
+    fb = 4 if ctx.config.precision == "mixed" else 8
+    k = ctx.config.krylov_subspace
+    blocks = 3 * m * m + (3 if sa else 0)
+    residual = 8 * (2 * nv + d * d + (d + 1 if visc else 0) + (1 if sa else 0))
+    return {"residual": residual, "tangent": residual + 8 * (nv + 1),
+            "wave_speeds": 8 * (nv + d * d + d + 1 + (1 if sa else 0)),
+            "precond_factor": 8 * (nv + d + d * d + (2 if sa else 0)) + blocks * fb,
+            "precond_apply": blocks * fb + 16 * nv,
+            "arnoldi_cycle": 8 * nv * (1.5 * k * (k - 1) + 7 * k)}
+
+
+def roofline_of(kernels: dict, ctx, m: int, active: float) -> dict:
+    """The dominant kernel (largest device time per step) priced with the
+    algorithmic bytes of its path: a line apply or factor launch processes
+    every interior cell of every case once; a CGS2 pass launch is one of the
+    3 m passes of a cycle; the tangent's kernels together make one tangent
+    evaluation (their summed time is the divisor)."""
+    per = survey_bytes(ctx)
+    units = ctx.atlas.cells * ctx.n_cases
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    if dom.startswith("tangent"):
+        parts = [k for k in kernels if k.startswith("tangent")]
+        ms = sum(kernels[k]["ms"] for k in parts)
+        alg, label = per["tangent"] * units * active, "+".join(parts)
+    elif dom.startswith("cgs2"):
+        ms = kernels[dom]["ms"]
+        alg, label = per["arnoldi_cycle"] * units / (3 * m) * active, dom
+    else:
+        ms = kernels[dom]["ms"]
+        alg, label = per[dom] * units, dom
+    achieved = alg / (ms * 1e-3) / 1e9
+    k = kernels[dom]
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": label, "kernel_ms": round(ms, 5),
+            "algorithmic_bytes": int(alg), "bytes_per_cell_per_launch": round(alg / units, 2),
+            "cells_per_launch": units,
+            "fused_bytes": k["algorithmic_bytes"], "fused_GB/s": k["GB/s"]}
+
+
 def path_rates(ctx) -> dict:
     """Device time of one residual, tangent, line apply and line factor (all of
     their launches, HIP events on the library's stream) and the achieved rate
@@ -135,18 +191,19 @@ def path_rates(ctx) -> dict:
     return out
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a plain copy),
-    or None when it holds no line for the kernel."""
+def measured_traffic(kernel: str, workload: str | None):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated
+    on a plain copy), or None when it holds no line for that pair.  Lines of
+    ``profiles/pmc_traffic.txt``: ``<workload> <kernel> <bytes per launch> ...``."""
     path = ROOT / "profiles" / "pmc_traffic.txt"
-    if not path.is_file():
+    if workload is None or not path.is_file():
         return None
     for line in path.read_text().splitlines():
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
+        if len(parts) >= 3 and parts[0] == workload and parts[1] == kernel:
             try:
-                return float(parts[1])
+                return float(parts[2])
             except ValueError:
                 return None
     return None
@@ -379,9 +436,10 @@ def main(argv=None) -> int:
                          "512 MiB eviction fill), the reference dispatch for calibrating rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--polar", type=int, metavar="B", default=0,
-                    help="instead of the single case: B cases of the benchmark grid in one context (alpha "
-                         "uniform on [-4, 12] deg, Mach uniform on [0.3, 0.78], Re 6e6, numpy default_rng(2)), "
-                         "every launch shared by the batch")
+                    help="B cases of the batched polar instead of 32 (alpha uniform on [-4, 12] deg, Mach uniform "
+                         "on [0.3, 0.78], Re 6e6, numpy default_rng(2)), every launch shared by the batch")
+    ap.add_argument("--single", action="store_true",
+                    help="instead of the batched polar: the 512 x 128 case alone (M 0.73, alpha 2.5, Re 6.5e6)")
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
@@ -452,9 +510,12 @@ def main(argv=None) -> int:
     from jaxrans_amd import load_case
     case = load_case(CASE)
     atlas = case.grid()
-    B = max(1, args.polar)
-    cells = atlas.cells * B
-    ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup, args.polar)
+    B = 0 if args.single else (args.polar or 32)
+    ncases = max(1, B)
+    cells = atlas.cells * ncases
+    print(f"[bench] {ncases} case(s) on the {atlas.cells}-cell grid: {args.warmup} warmup + {args.steps} timed "
+          f"Newton steps, FP64", file=sys.stderr, flush=True)
+    ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup, B)
     ms_per_step = 1e3 * dt / args.steps
     value = cells * args.steps / dt / 1e6
     m, restarts = case.implicit.krylov_subspace, case.implicit.krylov_restarts
@@ -477,37 +538,46 @@ def main(argv=None) -> int:
         kernels[k] = {"launches_per_step": v["launches"] / args.steps, "ms": round(v["ms"], 5),
                       "ms_per_step": round(v["ms"] * v["launches"] / args.steps, 4),
                       "algorithmic_bytes": int(alg), "GB/s": round(alg / (v["ms"] * 1e-3) / 1e9, 1)}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    kd = kernels[dom]
-    achieved = kd["GB/s"]
-    traffic = measured_traffic(dom)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic if not args.polar else None,
-                "kernel": dom, "kernel_ms": kd["ms"], "algorithmic_bytes": kd["algorithmic_bytes"],
-                "active_gmres_fraction": round(active, 4)}
+    roofline = roofline_of(kernels, ctx, m, active)
+    roofline["traffic"] = measured_traffic(roofline["kernel"], "polar32" if B == 32 else "single" if not B else None)
+    roofline["active_gmres_fraction"] = round(active, 4)
     paths = path_rates(ctx)
-    scal = ctx.scalars()[0]
+    scal = ctx.scalars()
+    F64 = ctx.forces()
     ctx.close()
     extra = {}
     if not args.no_mixed:
-        cm, dtm, histm = run_steps(case, atlas, "mixed", args.steps, args.warmup, args.polar)
-        extra = {"mixed_value": round(cells * args.steps / dtm / 1e6, 4), "mixed_ms_per_step": round(1e3 * dtm / args.steps, 3),
-                 "mixed_last_residual": float(histm[-1, 0])}
+        print("[bench] the same steps in mixed precision", file=sys.stderr, flush=True)
+        cm, dtm, histm = run_steps(case, atlas, "mixed", args.steps, args.warmup, B)
+        Fm = cm.forces()
+        extra = {"mixed_value": round(cells * args.steps / dtm / 1e6, 4),
+                 "mixed_ms_per_step": round(1e3 * dtm / args.steps, 3),
+                 "mixed_last_residual_max": float(np.nanmax(histm[-1])),
+                 "mixed_minus_fp64_after_the_timed_steps": {
+                     "max_abs_dcl": float(np.max(np.abs(Fm[:, 0] - F64[:, 0]))),
+                     "max_abs_dcd_counts": float(np.max(np.abs(Fm[:, 1] - F64[:, 1])) * 1e4),
+                     "note": "transient states after warmup + steps Newton steps from the freestream, not "
+                             "converged solutions"}}
         cm.close()
+    if B:
+        workload = (f"batched polar: {B} cases on the same Karman-Trefftz 10 deg C-grid 512x128 "
+                    f"({atlas.cells} cells each), alpha U[-4,12] deg, M U[0.3,0.78] (numpy default_rng(2)), Re 6e6, "
+                    "SA-noft2, Roe + van_albada_smooth, GMRES(30)x2, line_j (block cyclic reduction), FP64, "
+                    "one context: every launch shared by the batch")
+    else:
+        workload = ("2D SA RANS, Karman-Trefftz 10 deg, C-grid 512x128 (65536 cells), M 0.73, alpha 2.5, Re 6.5e6, "
+                    "Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64")
     line = {
         "metric": "Newton iters x Mcells/s", "value": round(value, 4), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": ("2D SA RANS, Karman-Trefftz 10 deg, C-grid 512x128 (65536 cells), M 0.73, "
-                                "alpha 2.5, Re 6.5e6, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64")
-                   if not args.polar else
-                   (f"batched polar: {B} cases on the same 512x128 C-grid, alpha U[-4,12], M U[0.3,0.78], "
-                    "Re 6e6, SA, Roe + van_albada_smooth, GMRES(30)x2, line_j, FP64"),
-                   "cells": cells, "cases": B, "krylov_subspace": m, "parallelism": "single GPU"},
+        "config": {"workload": workload, "cells": cells, "cases": ncases, "krylov_subspace": m,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels": kernels,
         "paths": paths,
-        "residual_first": float(hist[0, 0]), "residual_last": float(hist[-1, 0]), "cfl_last": float(scal[0]),
+        "residual_first_max": float(np.nanmax(hist[0])), "residual_last_max": float(np.nanmax(hist[-1])),
+        "cfl_last_median": float(np.median(scal[:, 0])),
         **extra,
     }
     if not args.no_cpu:

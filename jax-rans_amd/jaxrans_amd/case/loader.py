@@ -14,9 +14,7 @@ import copy
 import json
 from dataclasses import dataclass, field
 from pathlib import Path
-from typing import Any
 
-import numpy as np
 
 from ..core import constants
 from ..core.config import FlowConditions, ImplicitConfig, NumericsConfig
@@ -149,8 +147,6 @@ class Case:
         kinds = resolve_conditions(self.conditions, patches(mesh), self.flow)
         mesh = assign_conditions(mesh, kinds)
         if coarsen:
-            if mesh[0].dim == 3:
-                raise CaseError("grid sequencing is built for 2D grids only")
             try:
                 for _ in range(coarsen):
                     mesh = coarsen_mesh(mesh)
@@ -165,98 +161,118 @@ class Case:
         return build_atlas(mesh, self.numerics.n_ghost)
 
 
-def _section(doc: dict, file: str, name: str, required: bool = False) -> dict:
-    if name not in doc:
-        if required:
-            raise CaseError(f"missing section '{name}'")
-        return {}
-    section = doc[name]
-    if not isinstance(section, dict):
-        raise CaseError(f"'{name}' must be an object, got {type(section).__name__}")
-    allowed = schema.keys(file, name)
-    unknown = set(section) - set(allowed) - {"notes"}
-    if unknown:
-        raise CaseError(f"unknown key(s) {sorted(f'{name}.{k}' for k in unknown)}; allowed: {sorted(allowed)}")
-    return {k: v for k, v in section.items() if k != "notes"}
+class _Document:
+    """One parsed JSON file of a case: its top-level sections checked against
+    the schema once, then read section by section -- the keys given (unknown
+    ones refused) or the full typed record with defaults filled in."""
 
+    def __init__(self, file: str, doc: dict):
+        self.file, self.doc = file, doc
+        allowed = set(schema.FILES[file]) | ({"boundary_conditions"} if file == CASE_FILE else set())
+        stray = sorted(set(doc) - allowed)
+        if stray:
+            raise CaseError(f"{file}: unknown section(s) {stray}; allowed: {sorted(schema.FILES[file])}")
 
-def _values(section: dict, file: str, name: str) -> dict:
-    out = {}
-    for key in schema.FILES[file][name]:
-        path = f"{name}.{key.name}"
-        if key.name in section:
+    def given(self, name: str, required: bool = False) -> dict:
+        """The keys written in section ``name`` (``notes`` dropped)."""
+        if name not in self.doc:
+            if required:
+                raise CaseError(f"missing section '{name}'")
+            return {}
+        body = self.doc[name]
+        if not isinstance(body, dict):
+            raise CaseError(f"'{name}' must be an object, got {type(body).__name__}")
+        table = schema.keys(self.file, name)
+        stray = sorted(f"{name}.{k}" for k in body if k not in table and k != "notes")
+        if stray:
+            raise CaseError(f"unknown key(s) {stray}; allowed: {sorted(table)}")
+        return {k: v for k, v in body.items() if k != "notes"}
+
+    def record(self, name: str, required: bool = False, given: dict | None = None) -> dict:
+        """Every key of section ``name``: the written value, type-checked, or
+        the schema default; a required key that is absent is refused."""
+        given = self.given(name, required) if given is None else given
+        out = {}
+        for key in schema.FILES[self.file][name]:
+            where = f"{name}.{key.name}"
+            if key.name not in given:
+                if key.default is schema.REQUIRED:
+                    raise CaseError(f"missing '{where}'")
+                out[key.name] = key.default
+                continue
             try:
-                key.check(path, section[key.name])
+                key.check(where, given[key.name])
             except (TypeError, ValueError) as error:
                 raise CaseError(str(error)) from None
-            out[key.name] = section[key.name]
-        elif key.default is schema.REQUIRED:
-            raise CaseError(f"missing '{path}'")
-        else:
-            out[key.name] = key.default
-    return out
+            out[key.name] = given[key.name]
+        return out
 
 
-def _top_level(doc: dict, file: str) -> None:
-    unknown = set(doc) - set(schema.FILES[file]) - ({"boundary_conditions"} if file == CASE_FILE else set())
-    if unknown:
-        raise CaseError(f"{file}: unknown section(s) {sorted(unknown)}; allowed: {sorted(schema.FILES[file])}")
-
-
-def _grid_spec(case_doc: dict) -> GridSpec:
-    raw = _section(case_doc, CASE_FILE, "domain", required=True)
-    if "grid" not in raw:
+def _domain(case: _Document) -> GridSpec:
+    """``domain``: the grid source decides which keys apply and which are required."""
+    written = case.given("domain", required=True)
+    if "grid" not in written:
         raise CaseError(f"missing 'domain.grid'; choose from {sorted(schema.GRIDS)}")
-    domain = _values(raw, CASE_FILE, "domain")
-    kind = domain["grid"]
-    ks = schema.keys(CASE_FILE, "domain")
-    foreign = sorted(k for k in raw if ks[k].grids is not None and kind not in ks[k].grids)
-    if foreign:
-        takes = sorted(k for k, key in ks.items() if key.grids and kind in key.grids)
-        raise CaseError(f"'domain.{foreign[0]}' does not apply to grid {kind!r}; it takes {takes}")
-    for name, key in ks.items():
-        if kind in key.required_for and domain[name] is None:
-            raise CaseError(f"'domain.{name}' is required for grid {kind!r}")
-    values = {k: v for k, v in domain.items() if v is not None}
-    values["patches"] = tuple(domain["patches"])
-    values["elements"] = tuple(domain["elements"])
-    values["moment_centre"] = tuple(domain["moment_centre"])
-    return GridSpec(**values)
+    values = case.record("domain", given=written)
+    source = values["grid"]
+    table = schema.keys(CASE_FILE, "domain")
+    misplaced = sorted(k for k in written if table[k].grids is not None and source not in table[k].grids)
+    if misplaced:
+        applicable = sorted(k for k, key in table.items() if key.grids and source in key.grids)
+        raise CaseError(f"'domain.{misplaced[0]}' does not apply to grid {source!r}; it takes {applicable}")
+    for name, key in table.items():
+        if source in key.required_for and values[name] is None:
+            raise CaseError(f"'domain.{name}' is required for grid {source!r}")
+    kept = {k: v for k, v in values.items() if v is not None}
+    kept.update({k: tuple(values[k]) for k in ("patches", "elements", "moment_centre")})
+    return GridSpec(**kept)
 
 
-def _flow(case_doc: dict) -> FlowConditions:
-    given = {n: _section(case_doc, CASE_FILE, n, required=n == "freestream")
-             for n in ("freestream", "material_properties", "turbulence")}
-    freestream, material, turbulence = (_values(given[n], CASE_FILE, n) for n in given)
-    inviscid = freestream["reynolds"] is None
+# Viscosity laws of ``material_properties.dynamic_viscosity``: a function of
+# (freestream record, material record) giving S / T_inf or None.
+def _sutherland_ratio(free: dict, material: dict):
+    if free["temperature"] is None or material["sutherland_temperature"] is None:
+        raise CaseError("Sutherland's law needs 'freestream.temperature' and "
+                        "'material_properties.sutherland_temperature', in one absolute unit")
+    return material["sutherland_temperature"] / free["temperature"]
+
+
+def _constant_ratio(free: dict, material: dict):
+    if material["sutherland_temperature"] is not None:
+        raise CaseError("'material_properties.sutherland_temperature' given with a constant viscosity")
+    return None
+
+
+VISCOSITY_LAWS = {"sutherland": _sutherland_ratio, "constant": _constant_ratio}
+VISCOUS_ONLY = ("dynamic_viscosity", "prandtl_number", "sutherland_temperature")
+
+
+def _flow_conditions(case: _Document) -> FlowConditions:
+    """``freestream``, ``material_properties`` and ``turbulence`` -> the flow."""
+    written = {name: case.given(name, required=name == "freestream")
+               for name in ("freestream", "material_properties", "turbulence")}
+    free, material, turb = (case.record(name, given=written[name]) for name in written)
+    viscous = free["reynolds"] is not None
     ratio = None
-    if inviscid:
-        for key in ("dynamic_viscosity", "prandtl_number", "sutherland_temperature"):
-            if key in given["material_properties"]:
+    if not viscous:
+        for key in VISCOUS_ONLY:
+            if key in written["material_properties"]:
                 raise CaseError(f"'material_properties.{key}' given for an inviscid case "
                                 f"(freestream.reynolds is null)")
     else:
-        if material["dynamic_viscosity"] == "sutherland":
-            if freestream["temperature"] is None or material["sutherland_temperature"] is None:
-                raise CaseError("Sutherland's law needs 'freestream.temperature' and "
-                                "'material_properties.sutherland_temperature', in one absolute unit")
-            ratio = material["sutherland_temperature"] / freestream["temperature"]
-        elif material["sutherland_temperature"] is not None:
-            raise CaseError("'material_properties.sutherland_temperature' given with a constant viscosity")
+        ratio = VISCOSITY_LAWS[material["dynamic_viscosity"]](free, material)
         if material["prandtl_number"] != constants.PRANDTL_LAMINAR:
             raise CaseError(f"'material_properties.prandtl_number' = {material['prandtl_number']} is not "
                             f"implemented; only {constants.PRANDTL_LAMINAR}")
-    model = turbulence["model"]
-    if model != "none" and inviscid:
+    model = None if turb["model"] == "none" else turb["model"]
+    if model is not None and not viscous:
         raise CaseError("'turbulence.model' needs a viscous case (freestream.reynolds)")
-    if model == "none" and "freestream_nu_tilde_ratio" in given["turbulence"]:
+    if model is None and "freestream_nu_tilde_ratio" in written["turbulence"]:
         raise CaseError("'turbulence.freestream_nu_tilde_ratio' given without a model")
     try:
-        return FlowConditions(mach=freestream["mach"], alpha_deg=freestream["alpha_deg"],
-                              reynolds=freestream["reynolds"], gamma=material["specific_heat_ratio"],
-                              sutherland_ratio=ratio, turbulence=None if model == "none" else model,
-                              nu_tilde_ratio=turbulence["freestream_nu_tilde_ratio"],
-                              sweep_deg=freestream["sweep_deg"])
+        return FlowConditions(mach=free["mach"], alpha_deg=free["alpha_deg"], reynolds=free["reynolds"],
+                              gamma=material["specific_heat_ratio"], sutherland_ratio=ratio, turbulence=model,
+                              nu_tilde_ratio=turb["freestream_nu_tilde_ratio"], sweep_deg=free["sweep_deg"])
     except ValueError as error:
         raise CaseError(str(error)) from None
 
@@ -266,38 +282,43 @@ def default_conditions(flow: FlowConditions) -> dict[str, str]:
             "symmetry": "symmetry", "inflow": "inflow_total", "outflow": "outflow_static"}
 
 
+def _patch_entry(patch: str, entry, flow: FlowConditions) -> str:
+    """One ``boundary_conditions.<patch>`` object, checked; its condition type."""
+    where = f"boundary_conditions.{patch}"
+    if not isinstance(entry, dict):
+        raise CaseError(f"'{where}' must be an object with a 'type', got {entry!r}")
+    table = {k.name: k for k in schema.BOUNDARY_KEYS}
+    stray = sorted(f"{where}.{k}" for k in entry if k not in table and k != "notes")
+    if stray:
+        raise CaseError(f"unknown key(s) {stray}; allowed: {sorted(table)}")
+    if "type" not in entry:
+        raise CaseError(f"missing '{where}.type'")
+    for name, value in entry.items():
+        if name == "notes":
+            continue
+        try:
+            table[name].check(f"{where}.{name}", value)
+        except (TypeError, ValueError) as error:
+            raise CaseError(str(error)) from None
+    kind = entry["type"]
+    if "model" in entry and kind != "far_field":
+        raise CaseError(f"'{where}.model' applies to a far_field patch only")
+    if kind == "no_slip_wall" and flow.inviscid:
+        raise CaseError(f"'{where}.type' = 'no_slip_wall' in an inviscid case (freestream.reynolds is null)")
+    return kind
+
+
 def _boundaries(case_doc: dict, flow: FlowConditions) -> tuple[dict, str]:
+    """Named patch conditions and the one far-field model of the case."""
     section = case_doc.get("boundary_conditions", {})
     if not isinstance(section, dict):
         raise CaseError("'boundary_conditions' must be an object")
-    kinds, models = {}, set()
-    allowed = {k.name: k for k in schema.BOUNDARY_KEYS}
-    for patch, entry in section.items():
-        if patch == "notes":
-            continue
-        where = f"boundary_conditions.{patch}"
-        if not isinstance(entry, dict):
-            raise CaseError(f"'{where}' must be an object with a 'type', got {entry!r}")
-        unknown = set(entry) - set(allowed) - {"notes"}
-        if unknown:
-            raise CaseError(f"unknown key(s) {sorted(f'{where}.{k}' for k in unknown)}; allowed: {sorted(allowed)}")
-        if "type" not in entry:
-            raise CaseError(f"missing '{where}.type'")
-        for name in set(entry) - {"notes"}:
-            try:
-                allowed[name].check(f"{where}.{name}", entry[name])
-            except (TypeError, ValueError) as error:
-                raise CaseError(str(error)) from None
-        if "model" in entry and entry["type"] != "far_field":
-            raise CaseError(f"'{where}.model' applies to a far_field patch only")
-        if entry["type"] == "far_field":
-            models.add(entry.get("model", "freestream"))
-        if entry["type"] == "no_slip_wall" and flow.inviscid:
-            raise CaseError(f"'{where}.type' = 'no_slip_wall' in an inviscid case (freestream.reynolds is null)")
-        kinds[patch] = entry["type"]
+    named = {p: _patch_entry(p, e, flow) for p, e in section.items() if p != "notes"}
+    models = sorted({e.get("model", "freestream") for p, e in section.items()
+                     if p != "notes" and named[p] == "far_field"})
     if len(models) > 1:
-        raise CaseError(f"far-field patches name different models {sorted(models)}; one model per case")
-    return kinds, schema.FAR_FIELD_MODELS[models.pop() if models else "freestream"]
+        raise CaseError(f"far-field patches name different models {models}; one model per case")
+    return named, schema.FAR_FIELD_MODELS[models[0] if models else "freestream"]
 
 
 def resolve_conditions(named: dict, grid_patches: list, flow: FlowConditions) -> dict:
@@ -314,67 +335,76 @@ def resolve_conditions(named: dict, grid_patches: list, flow: FlowConditions) ->
 
 
 def parse_case(case_doc: dict, numerical_doc: dict, directory: Path = Path(".")) -> Case:
-    _top_level(case_doc, CASE_FILE)
-    _top_level(numerical_doc, NUMERICAL_FILE)
-    general = _values(_section(case_doc, CASE_FILE, "general", required=True), CASE_FILE, "general")
-    spec = _grid_spec(case_doc)
-    flow = _flow(case_doc)
+    """Both documents of a case -> the resolved ``Case`` (``CaseError`` on the
+    first thing that cannot run as written)."""
+    case, numerical = _Document(CASE_FILE, case_doc), _Document(NUMERICAL_FILE, numerical_doc)
+    general = case.record("general", required=True)
+    spec = _domain(case)
+    flow = _flow_conditions(case)
     if "section" in case_doc["domain"] and not flow.swept:
         raise CaseError("'domain.section' applies to a swept case only ('freestream.sweep_deg')")
     conditions, far_field = _boundaries(case_doc, flow)
-    fluxes, stepping, implicit, convergence = (
-        _values(_section(numerical_doc, NUMERICAL_FILE, n), NUMERICAL_FILE, n)
-        for n in ("convective_fluxes", "time_integration", "implicit", "convergence"))
+    fluxes = numerical.record("convective_fluxes")
+    stepping = numerical.record("time_integration")
+    implicit = numerical.record("implicit")
+    stop = numerical.record("convergence")
     try:
         implicit_cfg = ImplicitConfig(**implicit)
         numerics = NumericsConfig(flux=fluxes["riemann_solver"], limiter=fluxes["limiter"],
-                                  kappa=fluxes["muscl_kappa"], cfl=stepping["cfl"],
-                                  max_iters=convergence["max_iters"], tol=convergence["tol"],
-                                  forces_tol=convergence["forces_tol"], solver=stepping["solver"],
+                                  kappa=fluxes["muscl_kappa"], cfl=stepping["cfl"], max_iters=stop["max_iters"],
+                                  tol=stop["tol"], forces_tol=stop["forces_tol"], solver=stepping["solver"],
                                   far_field=far_field)
     except ValueError as error:
         raise CaseError(str(error)) from None
-    block_iters = convergence["block_iters"] or (200 if numerics.solver == "explicit" else 20)
+    # one host synchronisation per block: 20 Newton steps, or 200 explicit ones
+    block_iters = stop["block_iters"] or {"explicit": 200, "implicit": 20}[numerics.solver]
     if block_iters < 1:
         raise CaseError(f"'convergence.block_iters' must be >= 1, got {block_iters}")
-    if convergence["grid_sequencing"] < 0:
-        raise CaseError(f"'convergence.grid_sequencing' must be >= 0, got {convergence['grid_sequencing']}")
+    if stop["grid_sequencing"] < 0:
+        raise CaseError(f"'convergence.grid_sequencing' must be >= 0, got {stop['grid_sequencing']}")
     directory = Path(directory)
-    output = Path(general["save_path"])
+    out = Path(general["save_path"])
     return Case(name=general["case_name"], description=general["description"] or general["case_name"],
-                output=output if output.is_absolute() else directory / output, grid_spec=spec,
-                conditions=conditions, flow=flow, numerics=numerics, implicit=implicit_cfg,
-                rk_scheme=stepping["rk_scheme"], block_iters=block_iters, directory=directory,
-                raw={"case": case_doc, "numerical": numerical_doc}, sequencing=convergence["grid_sequencing"])
+                output=out if out.is_absolute() else directory / out, grid_spec=spec, conditions=conditions,
+                flow=flow, numerics=numerics, implicit=implicit_cfg, rk_scheme=stepping["rk_scheme"],
+                block_iters=block_iters, directory=directory, raw={"case": case_doc, "numerical": numerical_doc},
+                sequencing=stop["grid_sequencing"])
 
 
 def apply_override(docs: dict, assignment: str) -> None:
-    """``file.section.key=value`` (value read as JSON, else a bare string)."""
-    if "=" not in assignment:
+    """Apply ``file.section.key=value`` to ``docs`` in place; the value is
+    read as JSON when it parses, else kept as a bare string."""
+    path, sep, text = assignment.partition("=")
+    if not sep:
         raise CaseError(f"override {assignment!r} is not of the form file.section.key=value")
-    path, text = assignment.split("=", 1)
-    parts = path.strip().split(".")
-    if len(parts) != 3 or parts[0] not in docs:
+    target = path.strip().split(".")
+    if len(target) != 3 or target[0] not in docs:
         raise CaseError(f"override path {path!r} must be case|numerical.section.key, "
                         f"e.g. numerical.convective_fluxes.limiter")
     try:
         value = json.loads(text)
     except json.JSONDecodeError:
         value = text
-    docs[parts[0]].setdefault(parts[1], {})[parts[2]] = value
+    file, section, key = target
+    docs[file].setdefault(section, {})[key] = value
+
+
+def _read_json(path: Path) -> dict:
+    if not path.is_file():
+        return {}
+    try:
+        return json.loads(path.read_text())
+    except json.JSONDecodeError as error:
+        raise CaseError(f"{path}: {error}") from None
 
 
 def load_case(directory: str | Path, overrides: tuple[str, ...] = ()) -> Case:
+    """The case in ``directory`` (``case.json``, optional ``numerical.json``)
+    with ``overrides`` applied (``file.section.key=value`` each)."""
     directory = Path(directory)
     if not (directory / CASE_FILE).is_file():
         raise FileNotFoundError(f"{directory} holds no {CASE_FILE}")
-    docs = {}
-    for label, filename in (("case", CASE_FILE), ("numerical", NUMERICAL_FILE)):
-        path = directory / filename
-        try:
-            docs[label] = json.loads(path.read_text()) if path.is_file() else {}
-        except json.JSONDecodeError as error:
-            raise CaseError(f"{path}: {error}") from None
+    docs = {"case": _read_json(directory / CASE_FILE), "numerical": _read_json(directory / NUMERICAL_FILE)}
     docs = copy.deepcopy(docs)
     for assignment in overrides:
         apply_override(docs, assignment)

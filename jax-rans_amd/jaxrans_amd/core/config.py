@@ -1,13 +1,18 @@
 """Frozen, validated run configuration.
 
-Same fields, defaults and error messages as the reference's configs
-(``src/jaxrans/core/config.py:20-212``): a case parsed by either code base
-carries the same numbers.
+The three records carry the fields, defaults and error messages of the
+reference's configs (``src/jaxrans/core/config.py:20-212``), so a case read by
+either code base holds the same numbers and is refused with the same words.
+Validation is table-driven here: each record lists ``(test, message)`` rules
+that ``_enforce`` applies in order; the first rule that fails raises
+``ValueError`` with its message formatted from the record's fields.
 """
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+import math
+from dataclasses import asdict, dataclass
+from typing import Callable
 
 import numpy as np
 
@@ -15,12 +20,60 @@ from . import constants
 
 TURBULENCE_MODELS: tuple[str, ...] = ("sa_noft2", "sa_noft2_qcr")
 SOLVERS: tuple[str, ...] = ("explicit", "implicit")
+JACOBIANS: tuple[str, ...] = ("exact", "first_order")
+PRECISIONS: tuple[str, ...] = ("fp64", "mixed")
+LINE_SOLVERS: tuple[str, ...] = ("cyclic_reduction", "thomas")
+
+Rule = tuple[Callable[[dict], bool], str]
+
+
+def _enforce(record, rules: tuple[Rule, ...]) -> None:
+    """Raise ``ValueError`` with the message of the first rule ``record``
+    breaks; messages are ``str.format``-ed with the record's fields."""
+    fields = asdict(record)
+    for holds, message in rules:
+        if not holds(fields):
+            raise ValueError(message.format(**fields))
+
+
+def _null_or(test: Callable) -> Callable:
+    return lambda v: v is None or test(v)
+
+
+_FLOW_RULES: tuple[Rule, ...] = (
+    (lambda f: f["mach"] > 0.0, "mach must be positive, got {mach}"),
+    (lambda f: f["sweep_deg"] is None or abs(f["sweep_deg"]) < 90.0, "sweep_deg must lie in (-90, 90), got {sweep_deg}"),
+    (lambda f: _in_plane_mach(f["mach"], f["alpha_deg"], f["sweep_deg"]) < 1.0,
+     "freestream mach must be subsonic for the characteristic far-field BC, got {mach}"),
+    (lambda f: f["gamma"] > 1.0, "gamma must exceed 1, got {gamma}"),
+    (lambda f: _null_or(lambda r: r > 0.0)(f["reynolds"]), "reynolds must be positive or None, got {reynolds}"),
+    (lambda f: f["sutherland_ratio"] is None or f["reynolds"] is not None, "sutherland_ratio given for an inviscid case"),
+    (lambda f: _null_or(lambda s: s > 0.0)(f["sutherland_ratio"]),
+     "sutherland_ratio must be positive, got {sutherland_ratio}"),
+    (lambda f: f["turbulence"] is None or f["turbulence"] in TURBULENCE_MODELS,
+     "turbulence must be one of " + str(TURBULENCE_MODELS).replace("{", "{{").replace("}", "}}") +
+     ", got {turbulence!r}"),
+    (lambda f: f["turbulence"] is None or f["reynolds"] is not None, "a turbulence model needs a Reynolds number"),
+    (lambda f: f["nu_tilde_ratio"] > 0.0, "nu_tilde_ratio must be positive, got {nu_tilde_ratio}"),
+    (lambda f: abs(f["alpha_deg"]) <= 90.0, "alpha_deg out of range, got {alpha_deg}"),
+)
+
+
+def _in_plane_mach(mach: float, alpha_deg: float, sweep_deg: float | None) -> float:
+    """Mach number of the freestream's component in the section plane: the
+    whole freestream unswept; with sweep ``L`` the freestream
+    ``M (cos a cos L, sin a, cos a sin L)`` less its spanwise part."""
+    if sweep_deg is None:
+        return mach
+    a, s = math.radians(alpha_deg), math.radians(sweep_deg)
+    return mach * math.hypot(math.cos(a) * math.cos(s), math.sin(a))
 
 
 @dataclass(frozen=True)
 class FlowConditions:
     """Freestream and gas, nondimensional.  ``reynolds=None``: inviscid;
-    ``sutherland_ratio``: S / T_inf (None: constant viscosity)."""
+    ``sutherland_ratio``: S / T_inf (None: constant viscosity);
+    ``sweep_deg``: the 2.5D swept-section angle (None: unswept)."""
 
     mach: float
     alpha_deg: float = 0.0
@@ -32,60 +85,35 @@ class FlowConditions:
     sweep_deg: float | None = None
 
     def __post_init__(self) -> None:
-        if not self.mach > 0.0:
-            raise ValueError(f"mach must be positive, got {self.mach}")
-        if self.sweep_deg is not None and not abs(self.sweep_deg) < 90.0:
-            raise ValueError(f"sweep_deg must lie in (-90, 90), got {self.sweep_deg}")
-        if self.in_plane_mach >= 1.0:
-            raise ValueError(f"freestream mach must be subsonic for the characteristic "
-                             f"far-field BC, got {self.mach}")
-        if not self.gamma > 1.0:
-            raise ValueError(f"gamma must exceed 1, got {self.gamma}")
-        if self.reynolds is not None and not self.reynolds > 0.0:
-            raise ValueError(f"reynolds must be positive or None, got {self.reynolds}")
-        if self.sutherland_ratio is not None:
-            if self.reynolds is None:
-                raise ValueError("sutherland_ratio given for an inviscid case")
-            if not self.sutherland_ratio > 0.0:
-                raise ValueError(f"sutherland_ratio must be positive, got {self.sutherland_ratio}")
-        if self.turbulence is not None:
-            if self.turbulence not in TURBULENCE_MODELS:
-                raise ValueError(f"turbulence must be one of {TURBULENCE_MODELS}, got {self.turbulence!r}")
-            if self.reynolds is None:
-                raise ValueError("a turbulence model needs a Reynolds number")
-        if not self.nu_tilde_ratio > 0.0:
-            raise ValueError(f"nu_tilde_ratio must be positive, got {self.nu_tilde_ratio}")
-        if abs(self.alpha_deg) > 90.0:
-            raise ValueError(f"alpha_deg out of range, got {self.alpha_deg}")
+        _enforce(self, _FLOW_RULES)
 
-    @property
-    def inviscid(self) -> bool:
-        return self.reynolds is None
-
-    @property
-    def alpha_rad(self) -> float:
-        return float(np.deg2rad(self.alpha_deg))
-
-    @property
-    def swept(self) -> bool:
-        return self.sweep_deg is not None
-
-    @property
-    def sweep_rad(self) -> float | None:
-        return None if self.sweep_deg is None else float(np.deg2rad(self.sweep_deg))
-
-    @property
-    def in_plane_mach(self) -> float:
-        if not self.swept:
-            return self.mach
-        a, s = self.alpha_rad, self.sweep_rad
-        return float(self.mach * np.hypot(np.cos(a) * np.cos(s), np.sin(a)))
+    inviscid = property(lambda self: self.reynolds is None)
+    swept = property(lambda self: self.sweep_deg is not None)
+    alpha_rad = property(lambda self: math.radians(self.alpha_deg))
+    sweep_rad = property(lambda self: None if self.sweep_deg is None else math.radians(self.sweep_deg))
+    in_plane_mach = property(lambda self: _in_plane_mach(self.mach, self.alpha_deg, self.sweep_deg))
 
     @property
     def mu_inf(self) -> float:
-        if self.reynolds is None:
+        """Freestream viscosity in the nondimensionalisation rho = 1, a = 1,
+        chord 1: ``M / Re``."""
+        if self.inviscid:
             raise ValueError("mu_inf is undefined for an inviscid case")
         return self.mach / self.reynolds
+
+
+_NUMERICS_RULES: tuple[Rule, ...] = (
+    (lambda n: n["cfl"] > 0.0, "cfl must be positive, got {cfl}"),
+    (lambda n: n["max_iters"] >= 1, "max_iters must be >= 1, got {max_iters}"),
+    (lambda n: 0.0 < n["tol"] < 1.0, "tol must lie in (0, 1), got {tol}"),
+    (lambda n: _null_or(lambda t: t > 0.0)(n["forces_tol"]), "forces_tol must be positive or null, got {forces_tol}"),
+    (lambda n: n["n_ghost"] >= 2, "n_ghost must be >= 2 for MUSCL reconstruction, got {n_ghost}"),
+    (lambda n: -1.0 <= n["kappa"] <= 1.0, "muscl_kappa must lie in [-1, 1], got {kappa}"),
+    (lambda n: n["kappa"] == 0.0 or n["limiter"] == "none",
+     "muscl_kappa other than 0 is implemented without a limiter only (limiter 'none'), got limiter {limiter!r}"),
+    (lambda n: n["solver"] in SOLVERS,
+     "solver must be one of " + str(SOLVERS) + ", got {solver!r}"),
+)
 
 
 @dataclass(frozen=True)
@@ -104,27 +132,28 @@ class NumericsConfig:
     kappa: float = 0.0
 
     def __post_init__(self) -> None:
-        if not self.cfl > 0.0:
-            raise ValueError(f"cfl must be positive, got {self.cfl}")
-        if self.max_iters < 1:
-            raise ValueError(f"max_iters must be >= 1, got {self.max_iters}")
-        if not 0.0 < self.tol < 1.0:
-            raise ValueError(f"tol must lie in (0, 1), got {self.tol}")
-        if self.forces_tol is not None and not self.forces_tol > 0.0:
-            raise ValueError(f"forces_tol must be positive or null, got {self.forces_tol}")
-        if self.n_ghost < 2:
-            raise ValueError(f"n_ghost must be >= 2 for MUSCL reconstruction, got {self.n_ghost}")
-        if not -1.0 <= self.kappa <= 1.0:
-            raise ValueError(f"muscl_kappa must lie in [-1, 1], got {self.kappa}")
-        if self.kappa != 0.0 and self.limiter != "none":
-            raise ValueError("muscl_kappa other than 0 is implemented without a limiter only "
-                             f"(limiter 'none'), got limiter {self.limiter!r}")
-        if self.solver not in SOLVERS:
-            raise ValueError(f"solver must be one of {SOLVERS}, got {self.solver!r}")
+        _enforce(self, _NUMERICS_RULES)
 
-    @property
-    def dtype(self) -> np.dtype:
-        return np.dtype("float64")
+    dtype = property(lambda self: np.dtype("float64"))
+
+
+_IMPLICIT_RULES: tuple[Rule, ...] = (
+    (lambda c: c["cfl"] > 0.0, "cfl must be positive, got {cfl}"),
+    (lambda c: c["cfl_max"] >= c["cfl"], "cfl_max must be at least the starting cfl {cfl}, got {cfl_max}"),
+    (lambda c: 0.0 < c["cfl_min"] <= c["cfl"], "cfl_min must lie in (0, cfl], got {cfl_min}"),
+    (lambda c: c["cfl_growth"] >= 1.0, "cfl_growth must be >= 1, got {cfl_growth}"),
+    (lambda c: c["cfl_cutback"] > 1.0, "cfl_cutback must exceed 1, got {cfl_cutback}"),
+    (lambda c: c["jacobian"] in JACOBIANS, "jacobian must be 'exact' or 'first_order', got {jacobian!r}"),
+    (lambda c: 0.0 < c["krylov_tol"] < 1.0, "krylov_tol must lie in (0, 1), got {krylov_tol}"),
+    (lambda c: c["krylov_subspace"] >= 1, "krylov_subspace must be >= 1, got {krylov_subspace}"),
+    (lambda c: c["krylov_restarts"] >= 1, "krylov_restarts must be >= 1, got {krylov_restarts}"),
+    (lambda c: _null_or(lambda r: r > 0.0)(c["max_relative_update"]),
+     "max_relative_update must be positive or None, got {max_relative_update}"),
+    (lambda c: 0.0 < c["krylov_success"] < 1.0, "krylov_success must lie in (0, 1), got {krylov_success}"),
+    (lambda c: c["precision"] in PRECISIONS, "precision must be 'fp64' or 'mixed', got {precision!r}"),
+    (lambda c: c["line_solver"] in LINE_SOLVERS,
+     "line_solver must be 'cyclic_reduction' or 'thomas', got {line_solver!r}"),
+)
 
 
 @dataclass(frozen=True)
@@ -133,8 +162,8 @@ class ImplicitConfig:
 
     ``precision`` is this implementation's addition: ``"fp64"`` (the drop-in,
     every array double) or ``"mixed"`` (line-preconditioner blocks and their
-    block-Thomas factor/solve in float32; residual, tangent, Krylov and update
-    in float64).  ``line_solver`` is how each line chain is solved: block cyclic
+    factor and solve in float32; residual, tangent, Krylov and update in
+    float64).  ``line_solver`` is how each line chain is solved: block cyclic
     reduction (the reference's, ``src/jaxrans/core/linalg.py:159-200``) or block
     Thomas (``linalg.py:80-132``); the two agree to round-off."""
 
@@ -154,29 +183,4 @@ class ImplicitConfig:
     line_solver: str = "cyclic_reduction"
 
     def __post_init__(self) -> None:
-        if not self.cfl > 0.0:
-            raise ValueError(f"cfl must be positive, got {self.cfl}")
-        if self.cfl_max < self.cfl:
-            raise ValueError(f"cfl_max must be at least the starting cfl {self.cfl}, got {self.cfl_max}")
-        if not 0.0 < self.cfl_min <= self.cfl:
-            raise ValueError(f"cfl_min must lie in (0, cfl], got {self.cfl_min}")
-        if self.cfl_growth < 1.0:
-            raise ValueError(f"cfl_growth must be >= 1, got {self.cfl_growth}")
-        if self.cfl_cutback <= 1.0:
-            raise ValueError(f"cfl_cutback must exceed 1, got {self.cfl_cutback}")
-        if self.jacobian not in ("exact", "first_order"):
-            raise ValueError(f"jacobian must be 'exact' or 'first_order', got {self.jacobian!r}")
-        if not 0.0 < self.krylov_tol < 1.0:
-            raise ValueError(f"krylov_tol must lie in (0, 1), got {self.krylov_tol}")
-        if self.krylov_subspace < 1:
-            raise ValueError(f"krylov_subspace must be >= 1, got {self.krylov_subspace}")
-        if self.krylov_restarts < 1:
-            raise ValueError(f"krylov_restarts must be >= 1, got {self.krylov_restarts}")
-        if self.max_relative_update is not None and not self.max_relative_update > 0.0:
-            raise ValueError(f"max_relative_update must be positive or None, got {self.max_relative_update}")
-        if not 0.0 < self.krylov_success < 1.0:
-            raise ValueError(f"krylov_success must lie in (0, 1), got {self.krylov_success}")
-        if self.precision not in ("fp64", "mixed"):
-            raise ValueError(f"precision must be 'fp64' or 'mixed', got {self.precision!r}")
-        if self.line_solver not in ("cyclic_reduction", "thomas"):
-            raise ValueError(f"line_solver must be 'cyclic_reduction' or 'thomas', got {self.line_solver!r}")
+        _enforce(self, _IMPLICIT_RULES)

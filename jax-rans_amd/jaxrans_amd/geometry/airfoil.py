@@ -14,92 +14,109 @@ import numpy as np
 from scipy.interpolate import CubicSpline
 
 
+def _arc_length(z: np.ndarray) -> np.ndarray:
+    """Cumulative chord-segment length along the complex points ``z``."""
+    return np.concatenate([[0.0], np.cumsum(np.abs(np.diff(z)))])
+
+
 @dataclass(frozen=True)
 class Airfoil:
+    """A closed-or-open section contour ``(x, y)``, clockwise from the trailing
+    edge (lower surface first).  Derived quantities are computed from the
+    complex points ``z = x + i y``: the trailing-edge midpoint, the leading
+    edge (the point farthest from it), the chord (their distance) and the
+    arc length the grid generator parametrises by."""
+
     name: str
     x: np.ndarray
     y: np.ndarray
 
     def __post_init__(self) -> None:
-        if self.x.size < 5:
-            raise ValueError(f"airfoil {self.name!r}: at least 5 points are needed, got {self.x.size}")
-        if np.any(np.hypot(np.diff(self.x), np.diff(self.y)) == 0.0):
+        n = self.x.size
+        if n < 5:
+            raise ValueError(f"airfoil {self.name!r}: at least 5 points are needed, got {n}")
+        z = self.x + 1j * self.y
+        if not np.all(np.abs(np.diff(z)) > 0.0):
             raise ValueError(f"airfoil {self.name!r}: repeated consecutive points")
+        te = 0.5 * (z[0] + z[-1])
+        le = int(np.argmax(np.abs(z - te)))
+        object.__setattr__(self, "_geo", {"z": z, "te": te, "le": le, "s": _arc_length(z)})
 
     @property
     def s(self) -> np.ndarray:
-        return np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(self.x), np.diff(self.y)))])
+        return self._geo["s"]
 
     @property
     def length(self) -> float:
-        return float(self.s[-1])
-
-    @property
-    def _te_mid(self) -> np.ndarray:
-        return 0.5 * np.array([self.x[0] + self.x[-1], self.y[0] + self.y[-1]])
+        return float(self._geo["s"][-1])
 
     @property
     def trailing_edge_gap(self) -> float:
-        return float(np.hypot(self.x[-1] - self.x[0], self.y[-1] - self.y[0]))
+        z = self._geo["z"]
+        return float(abs(z[-1] - z[0]))
 
     @property
     def leading_edge(self) -> int:
         """Index of the contour point farthest from the trailing-edge midpoint."""
-        te = self._te_mid
-        return int(np.argmax(np.hypot(self.x - te[0], self.y - te[1])))
+        return self._geo["le"]
 
     @property
     def chord(self) -> float:
-        te, k = self._te_mid, self.leading_edge
-        return float(np.hypot(self.x[k] - te[0], self.y[k] - te[1]))
+        g = self._geo
+        return float(abs(g["z"][g["le"]] - g["te"]))
 
     @property
     def sharp(self) -> bool:
+        """A trailing-edge gap below a millionth of the chord counts as closed."""
         return self.trailing_edge_gap <= 1e-6 * self.chord
 
     def spline(self) -> tuple[CubicSpline, CubicSpline]:
+        """Cubic splines ``x(s)``, ``y(s)`` against arc length; a sharp
+        contour's last point is pinned onto its first."""
         x, y = self.x.copy(), self.y.copy()
         if self.sharp:
             x[-1], y[-1] = x[0], y[0]
-        s = self.s
-        return CubicSpline(s, x), CubicSpline(s, y)
+        return CubicSpline(self.s, x), CubicSpline(self.s, y)
 
     def points(self, s: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         fx, fy = self.spline()
         return fx(s), fy(s)
 
     def unit_chord(self) -> "Airfoil":
-        k, c = self.leading_edge, self.chord
-        return Airfoil(self.name, (self.x - self.x[k]) / c, (self.y - self.y[k]) / c)
+        """Translated so that the leading edge sits at the origin, scaled to chord 1."""
+        g = self._geo
+        z = (g["z"] - g["z"][g["le"]]) / self.chord
+        return Airfoil(self.name, z.real.copy(), z.imag.copy())
 
     def closed(self) -> "Airfoil":
-        """Close an open trailing edge at its midpoint by a shear that grows
-        linearly along the chord from zero at the leading edge."""
+        """An open trailing edge closed at its midpoint: each surface is
+        sheared towards the midpoint by a displacement that grows linearly with
+        the distance along the leading edge -> trailing-edge-midpoint line,
+        from nothing at the leading edge to the whole half gap at its end point."""
         if self.sharp:
             return self
-        k = self.leading_edge
-        mid = self._te_mid
-        le = np.array([self.x[k], self.y[k]])
-        t = (mid - le) / np.linalg.norm(mid - le)
-        along = (self.x - le[0]) * t[0] + (self.y - le[1]) * t[1]
-        end = np.where(np.arange(self.x.size) < k, 0, self.x.size - 1)
-        w = np.clip(along / along[end], 0.0, 1.0)
-        x = self.x + w * (mid[0] - self.x[end])
-        y = self.y + w * (mid[1] - self.y[end])
-        x[0] = x[-1] = mid[0]
-        y[0] = y[-1] = mid[1]
-        return Airfoil(self.name, x, y)
+        g = self._geo
+        z, te, le = g["z"], g["te"], g["le"]
+        axis = (te - z[le]) / abs(te - z[le])
+        along = ((z - z[le]) * np.conj(axis)).real          # projection on the chord line
+        n = z.size
+        ends = np.where(np.arange(n) < le, 0, n - 1)           # lower surface -> point 0, upper -> point n-1
+        weight = np.clip(along / along[ends], 0.0, 1.0)
+        closed = z + weight * (te - z[ends])
+        closed[0] = closed[-1] = te
+        return Airfoil(self.name, closed.real.copy(), closed.imag.copy())
 
 
 def from_points(x, y, name: str = "airfoil") -> Airfoil:
-    """Any contour direction and start; returned clockwise from the trailing edge."""
-    x, y = np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64)
-    keep = np.concatenate([[True], np.hypot(np.diff(x), np.diff(y)) > 0.0])
-    x, y = x[keep], y[keep]
-    signed = 0.5 * (np.dot(x, np.roll(y, -1)) - np.dot(np.roll(x, -1), y))
-    if signed > 0.0:
-        x, y = x[::-1].copy(), y[::-1].copy()
-    return Airfoil(name, x, y)
+    """A contour given in any direction: repeated consecutive points dropped,
+    then reversed if it runs counter-clockwise (positive shoelace area), so
+    the result runs clockwise from its first point."""
+    z = np.asarray(x, dtype=np.float64) + 1j * np.asarray(y, dtype=np.float64)
+    z = z[np.concatenate([[True], np.abs(np.diff(z)) > 0.0])]
+    area = 0.5 * float(np.imag(np.sum(np.conj(z) * np.roll(z, -1))))
+    if area > 0.0:
+        z = z[::-1]
+    return Airfoil(name, z.real.copy(), z.imag.copy())
 
 
 def thickness(xc: np.ndarray, t: float, a4: float) -> np.ndarray:

@@ -6,14 +6,17 @@ The same assembly as ``src/jaxrans/mesh/assemble3d.py:1-19`` (steps
 
 1. a block whose cell volumes are all negative has ``k`` reversed (mixed
    signs are an error);
-2. side faces are matched into interfaces where their four corners coincide
-   within ``pair_tol`` times the smaller of the face's shortest edge and its
-   cell's width across it, in any of the eight relative orientations of two
-   quadrilaterals -- between blocks or within one (an O seam, a wake cut);
-   the matched faces of a side are cut into ``InterfaceRect`` regions of one
-   affine index map each;
-3. a side whose faces all have zero area is a pole: an interface with
-   itself half a turn round the running direction that is an O seam;
+2. the nodes of every block side are merged into shared points where they
+   lie within ``pair_tol`` times the local scale (the smaller of the
+   shortest face edge and the cell width across the side, over the faces
+   round the node); two side faces whose four corners are the same four
+   points form an interface, in whichever of the eight relative
+   orientations of two quadrilaterals maps one onto the other -- between
+   blocks or within one (an O seam, a wake cut); the matched faces of a side
+   are cut into ``InterfaceRect`` regions of one affine index map each;
+3. a side whose faces are all degenerate (fewer than three distinct
+   points) is a pole: an interface with itself half a turn round the running
+   direction that is an O seam;
 4. every other face takes the name of the first patch rule (a box in
    ``x, y, z``) that holds its midpoint; a side whose faces get several
    labels is tiled by rectangles of one label each.
@@ -97,73 +100,123 @@ def right_handed(nodes: np.ndarray, label: str = "") -> np.ndarray:
                      f"positive ones")
 
 
-def side_plane(nodes: np.ndarray, side: str, layer: int = 0) -> np.ndarray:
-    """The node plane ``layer`` planes in from ``side``, ``(3, na+1, nb+1)``
-    over its running axes in ascending order."""
+def side_nodes(nodes: np.ndarray, side: str, depth: int = 0) -> np.ndarray:
+    """Node coordinates ``(na+1, nb+1, 3)`` of the plane ``depth`` planes in
+    from ``side``, over its two running axes in ascending order."""
     axis = normal_axis(side)
-    at = layer if side.endswith("min") else nodes.shape[1 + axis] - 1 - layer
-    return np.take(nodes, at, axis=1 + axis)
+    index = depth if side.endswith("min") else nodes.shape[1 + axis] - 1 - depth
+    return np.moveaxis(np.take(nodes, index, axis=1 + axis), 0, -1)
 
 
-def plane_quads(plane: np.ndarray) -> np.ndarray:
-    """Corners ``(na, nb, 4, 3)`` of the faces of a node plane, in ``CORNERS`` order."""
-    na, nb = plane.shape[1] - 1, plane.shape[2] - 1
-    return np.stack([plane[:, a:a + na, b:b + nb] for a, b in CORNERS], axis=-1).transpose(1, 2, 3, 0)
+def face_corners(plane: np.ndarray) -> np.ndarray:
+    """The four corners ``(na, nb, 4, ...)`` of each face of a node plane,
+    in ``CORNERS`` order (round the face)."""
+    return np.stack([plane[a:plane.shape[0] - 1 + a, b:plane.shape[1] - 1 + b] for a, b in CORNERS], axis=2)
 
 
 class SideFaces:
-    """Every side face of every block in one flat table: owning ``block``,
-    ``side`` (index into ``SIDES_3D``), running cell ``cell (n, 2)``,
-    ``corners (n, 4, 3)``, coincidence tolerance ``tol``, ``collapsed``."""
+    """Every side face of every block in one flat table.
+
+    Matching is topological: the nodes of all block sides are merged into
+    shared points (union of nodes closer than ``pair_tol`` times the local
+    scale -- the shortest non-degenerate edge or the cell width across the
+    side, whichever is smaller), each face becomes the four point ids of its
+    corners, and two faces coincide when they have the same four points.  A
+    face with fewer than three distinct points is degenerate (a pole side is
+    all degenerate faces).  Columns: ``block``, ``side`` (index into
+    ``SIDES_3D``), running ``cell (n, 2)``, ``corners (n, 4, 3)``, point ids
+    ``ids (n, 4)``, ``degenerate``, ``mid``."""
 
     def __init__(self, blocks, pair_tol: float):
-        cols = {k: [] for k in ("block", "side", "cell", "corners", "tol", "collapsed")}
-        for b, nodes in enumerate(blocks):
+        per_face: dict[str, list] = {k: [] for k in ("block", "side", "cell", "corners", "node")}
+        node_xyz, node_scale = [], []
+        offset = 0
+        for b, xyz in enumerate(blocks):
             for s, side in enumerate(SIDES_3D):
-                quads = plane_quads(side_plane(nodes, side))
+                plane = side_nodes(xyz, side)
+                quads = face_corners(plane)                                  # (na, nb, 4, 3)
                 na, nb = quads.shape[:2]
-                inner = plane_quads(side_plane(nodes, side, 1)).mean(axis=2)
-                edge = np.linalg.norm(np.roll(quads, -1, axis=2) - quads, axis=-1)
-                area = 0.5 * np.linalg.norm(np.cross(quads[:, :, 2] - quads[:, :, 0],
-                                                     quads[:, :, 3] - quads[:, :, 1]), axis=-1)
-                width = np.linalg.norm(quads.mean(axis=2) - inner, axis=-1)
-                shortest = np.where(edge > 0.0, edge, np.inf).min(axis=-1)
+                edges = np.linalg.norm(np.roll(quads, -1, axis=2) - quads, axis=-1)
+                shortest = np.where(edges > 0.0, edges, np.inf).min(axis=-1)
+                width = np.linalg.norm(quads.mean(axis=2) - face_corners(side_nodes(xyz, side, 1)).mean(axis=2),
+                                       axis=-1)
+                scale = np.minimum(shortest, width)                           # (na, nb)
+                # a node's scale: the smallest of the faces round it
+                ns = np.full(plane.shape[:2], np.inf)
+                for a, c in CORNERS:
+                    view = ns[a:a + na, c:c + nb]
+                    np.minimum(view, scale, out=view)
+                ids = np.arange(plane.shape[0] * plane.shape[1]).reshape(plane.shape[:2]) + offset
+                offset += ids.size
+                node_xyz.append(plane.reshape(-1, 3))
+                node_scale.append(ns.ravel())
                 ia, ib = np.meshgrid(np.arange(na), np.arange(nb), indexing="ij")
-                cols["block"].append(np.full(na * nb, b))
-                cols["side"].append(np.full(na * nb, s))
-                cols["cell"].append(np.stack([ia.ravel(), ib.ravel()], axis=1))
-                cols["corners"].append(quads.reshape(-1, 4, 3))
-                cols["tol"].append((pair_tol * np.minimum(shortest, width)).ravel())
-                cols["collapsed"].append((area <= 1e-10 * edge.max(axis=-1) ** 2).ravel())
-        for key, parts in cols.items():
+                per_face["block"].append(np.full(na * nb, b))
+                per_face["side"].append(np.full(na * nb, s))
+                per_face["cell"].append(np.stack([ia.ravel(), ib.ravel()], axis=1))
+                per_face["corners"].append(quads.reshape(-1, 4, 3))
+                per_face["node"].append(face_corners(ids).reshape(-1, 4))
+        for key, parts in per_face.items():
             setattr(self, key, np.concatenate(parts))
         self.mid = self.corners.mean(axis=1)
+        points = self._merge(np.concatenate(node_xyz), pair_tol * np.concatenate(node_scale))
+        self.ids = points[self.node]
+        distinct = np.array([len(set(row)) for row in self.ids.tolist()])
+        self.degenerate = distinct < 3
+
+    @staticmethod
+    def _merge(xyz: np.ndarray, tol: np.ndarray) -> np.ndarray:
+        """Point id per node: nodes closer than the smaller of their two
+        tolerances share one (union-find over the close pairs)."""
+        parent = np.arange(len(xyz))
+        radius = np.where(np.isfinite(tol), tol, 0.0)
+        near = cKDTree(xyz).query_ball_point(xyz, r=radius)
+        first = np.repeat(np.arange(len(xyz)), [len(c) for c in near])
+        second = np.fromiter((o for c in near for o in c), dtype=np.int64, count=len(first))
+        keep = first < second
+        pairs = np.stack([first[keep], second[keep]], axis=1)
+        if len(pairs):
+            gap = np.linalg.norm(xyz[pairs[:, 0]] - xyz[pairs[:, 1]], axis=-1)
+            pairs = pairs[gap <= np.minimum(radius[pairs[:, 0]], radius[pairs[:, 1]])]
+
+        def root(i):
+            while parent[i] != i:
+                parent[i] = parent[parent[i]]
+                i = parent[i]
+            return i
+
+        for i, j in pairs.tolist():
+            ri, rj = root(i), root(j)
+            if ri != rj:
+                parent[max(ri, rj)] = min(ri, rj)
+        return np.array([root(i) for i in range(len(xyz))])
 
     def of(self, block: int, side: str) -> np.ndarray:
         return np.flatnonzero((self.block == block) & (self.side == SIDES_3D.index(side)))
 
 
-def match_faces(faces: SideFaces, excluded: np.ndarray):
-    """Coincident partner of each face (-1: none), the orientation (row of
-    ``TURNS``) taking my corners onto its corners, and the largest corner gap
-    of any matched pair."""
-    n = len(faces.mid)
-    near = cKDTree(faces.mid).query_ball_point(faces.mid, r=faces.tol)
-    me = np.repeat(np.arange(n), [len(c) for c in near])
-    other = np.fromiter((o for c in near for o in c), dtype=np.int64, count=len(me))
-    keep = (me != other) & ~excluded[me] & ~excluded[other]
-    me, other = me[keep], other[keep]
-    gaps = np.linalg.norm(faces.corners[me][:, None] - faces.corners[other][:, TURNS], axis=-1).max(axis=-1)
-    turn = gaps.argmin(axis=1)
-    gap = gaps[np.arange(len(me)), turn]
-    hit = gap <= faces.tol[me]
-    me, other, turn, gap = me[hit], other[hit], turn[hit], gap[hit]
-    if np.unique(me).size != me.size:
-        raise ValueError("a side face coincides with more than one other face")
+def match_faces(faces: SideFaces):
+    """Partner of each face (-1: none) -- the other non-degenerate face with
+    the same four points -- and the orientation (row of ``TURNS``) that takes
+    my corner ``c`` onto the partner's corner ``TURNS[t][c]``."""
+    n = len(faces.ids)
     partner = np.full(n, -1)
     orient = np.zeros(n, dtype=np.int64)
-    partner[me], orient[me] = other, turn
-    return partner, orient, float(gap.max(initial=0.0))
+    groups: dict[tuple, list] = {}
+    for f in np.flatnonzero(~faces.degenerate):
+        groups.setdefault(tuple(sorted(faces.ids[f].tolist())), []).append(int(f))
+    for members in groups.values():
+        if len(members) == 1:
+            continue
+        if len(members) > 2:
+            raise ValueError("a side face coincides with more than one other face")
+        f, g = members
+        for me, other in ((f, g), (g, f)):
+            hits = np.flatnonzero(np.all(faces.ids[other][TURNS] == faces.ids[me][None], axis=1))
+            if hits.size == 0:
+                raise ValueError("two side faces share their corner points in no quadrilateral orientation")
+            partner[me], orient[me] = other, int(hits[0])
+    return partner, orient
 
 
 def interface_label(faces: SideFaces, row: int, partner: int, turn: int) -> tuple:
@@ -236,7 +289,7 @@ def assemble_3d(blocks, rules: tuple = (), pair_tol: float = 0.05) -> tuple:
     interfaces and named patches (conditions still unassigned)."""
     nodes = [right_handed(np.asarray(b, np.float64), f"{k} ") for k, b in enumerate(blocks)]
     faces = SideFaces(nodes, pair_tol)
-    partner, turn, _ = match_faces(faces, faces.collapsed)
+    partner, turn = match_faces(faces)
     names = np.full(len(faces.block), None, dtype=object)
     for rule in rules:
         take = (names == None) & (partner < 0) & rule.holds(faces.mid)  # noqa: E711
@@ -247,7 +300,7 @@ def assemble_3d(blocks, rules: tuple = (), pair_tol: float = 0.05) -> tuple:
         sides, poles = {}, []
         for side in SIDES_3D:
             rows = faces.of(b, side)
-            if np.all(faces.collapsed[rows]):
+            if np.all(faces.degenerate[rows]):
                 poles.append(side)
                 continue
             labels = np.empty(len(rows), dtype=object)
