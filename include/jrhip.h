@@ -180,6 +180,22 @@ int jr_newton_init_case(jr_ctx* ctx, int case_index, double cfl0);
  * hist_host[n_steps][n_cases] receives |R(U)| before each update. */
 int jr_newton_block(jr_ctx* ctx, int n_steps, double* hist_host);
 
+/* jr_newton_block that allocates nothing: no hipMalloc, hipHostMalloc or
+ * hipFree, no graph, no host read inside the block.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory once their
+ * contexts exist (a serving loop's steady state, code under HIP graph
+ * capture).  The residual history is staged in a static device array of
+ * JR_NOALLOC_HISTORY_MAX doubles (512 KiB, one per device, held until the
+ * process ends), so n_steps * n_cases must not exceed JR_NOALLOC_HISTORY_MAX
+ * (larger blocks return an error).  Every GMRES cycle is launched, cycles no
+ * case needs exit on the device: results equal jr_newton_block's.  All calls
+ * on one device share the staging array: calls that overlap on one device
+ * (two contexts on two host threads) race and must be serialised by the
+ * caller. */
+#define JR_NOALLOC_HISTORY_MAX 65536
+#define JR_NOALLOC_MAX_DEVICES 64
+int jr_newton_block_noalloc(jr_ctx* ctx, int n_steps, double* hist_host);
+
 /* One solve of the Newton step's linear system, (V/dt + dR~/dU) x = b, at
  * the device state and the current CFL (set it with jr_set_cfl): GMRES from
  * zero with the context's subspace, restarts and tolerance, right-

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,6 +91,7 @@ struct jr_ctx {
   double* solve_buf = nullptr;
   bool per_case_geo = false;   // geometry arrays per case (jr_set_geometry)
   uint64_t topo_hash = 0;      // hash of the index plans: grids of one topology share them
+  int device = 0;              // the HIP device current at creation (its stream's device)
 };
 
 // FNV-1a over the index plans of a grid (stages, chains, walls, neighbours,
@@ -725,6 +727,7 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("jr_ctx_create: no HIP device");
   jr_ctx* c = new jr_ctx();
+  (void)hipGetDevice(&c->device);
   auto bail = [&](int) {
     jr_ctx_destroy(c);
     return -1;
@@ -1068,6 +1071,7 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("jr_ctx_create3: no HIP device");
   jr_ctx* c = new jr_ctx();
   c->dim = 3;
+  (void)hipGetDevice(&c->device);
   auto bail = [&](int) {
     jr_ctx_destroy(c);
     return -1;
@@ -1717,6 +1721,76 @@ int jr_newton_block(jr_ctx* c, int n_steps, double* hist_host) {
     HIPCHK(hipMemcpyAsync(hist_host, c->hist, (size_t)n_steps * c->n_cases * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// --- A block of Newton steps that allocates nothing --------------------------
+// The residual history of jr_newton_block_noalloc is staged in this array
+// instead of the context's lazily grown history buffer.  Every device the
+// library runs on holds its own copy (512 KiB) until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// that device, never through the array's name.
+static __device__ __align__(256) double scratch[JR_NOALLOC_HISTORY_MAX];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[JR_NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller's current device is restored).
+static int scratch_on(int device, double** out) {
+  if (device < 0 || device >= JR_NOALLOC_MAX_DEVICES)
+    return fail("jr_newton_block_noalloc: device " + std::to_string(device) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int current = 0;
+    HIPCHK(hipGetDevice(&current));
+    if (current != device) HIPCHK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (current != device) HIPCHK(hipSetDevice(current));
+    if (e != hipSuccess || !p)
+      return fail(std::string("jr_newton_block_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_zero(size_t n, double* __restrict__ p) {
+  const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e < n) p[e] = 0.0;
+}
+
+int jr_newton_block_noalloc(jr_ctx* c, int n_steps, double* hist_host) {
+  if (!c) return fail("jr_newton_block_noalloc: null context");
+  if (c->pending_steps) return fail("jr_newton_block_noalloc: an asynchronous block is pending: wait for it first");
+  if (n_steps < 1) return fail("jr_newton_block_noalloc: n_steps must be >= 1");
+  if (!c->has_q) return fail("jr_newton_block_noalloc: set the freestream first");
+  const size_t rows = (size_t)n_steps * c->n_cases;
+  if (rows > (size_t)JR_NOALLOC_HISTORY_MAX)
+    return fail("jr_newton_block_noalloc: n_steps x n_cases = " + std::to_string(rows) + " exceeds the " +
+                std::to_string(JR_NOALLOC_HISTORY_MAX) + " history entries of the static staging array");
+  double* hist = nullptr;
+  TRY(scratch_on(c->device, &hist));
+  // The staging array keeps the previous call's history: clear the rows this
+  // block writes (a fill kernel, not a memset, so a capture could hold it).
+  hipLaunchKernelGGL(k_fill_zero, grid1(rows, BLOCK, 1), dim3(BLOCK), 0, c->stream, rows, hist);
+  TRY(check_launch());
+  // Every GMRES cycle is launched (a cycle no case needs exits on the device,
+  // as on the asynchronous path): no host read of the control blocks, and the
+  // same results as jr_newton_block's kernel-by-kernel launches.
+  c->async_mode = true;
+  for (int s = 0; s < n_steps; ++s) {
+    step_dispatch(c, hist + (size_t)s * c->n_cases);
+    if (check_launch() != 0) {
+      c->async_mode = false;
+      return -1;
+    }
+  }
+  c->async_mode = false;
+  if (hist_host)
+    HIPCHK(hipMemcpyAsync(hist_host, hist, rows * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

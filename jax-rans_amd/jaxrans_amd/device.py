@@ -15,6 +15,7 @@ import numpy as np
 from .mesh.atlas import Atlas
 
 LIB_PATH = Path(__file__).with_name("libjrhip.so")
+NOALLOC_HISTORY_MAX = 65536      # JR_NOALLOC_HISTORY_MAX of include/jrhip.h
 
 FLUX_CODES = {"hllc": 0, "roe": 1}
 LIMITER_CODES = {"minmod": 0, "none": 1, "van_albada": 2, "van_albada_smooth": 3}
@@ -27,6 +28,7 @@ EXPORTS = (
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
     "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
+    "jr_newton_block_noalloc",
 )
 
 
@@ -354,6 +356,15 @@ class Context:
     def newton_block(self, n_steps: int) -> np.ndarray:
         hist = np.empty((n_steps, self.n_cases))
         _check(self._lib.jr_newton_block(self._h, int(n_steps), _ptr(hist, C.c_double)))
+        return hist
+
+    def newton_block_noalloc(self, n_steps: int) -> np.ndarray:
+        """``newton_block`` through the entry point that allocates nothing:
+        the history is staged in the library's static per-device array
+        (``n_steps * n_cases`` at most ``NOALLOC_HISTORY_MAX``); the same
+        results as ``newton_block``.  Calls that overlap on one device race."""
+        hist = np.empty((n_steps, self.n_cases))
+        _check(self._lib.jr_newton_block_noalloc(self._h, int(n_steps), _ptr(hist, C.c_double)))
         return hist
 
     def newton_block_async(self, n_steps: int, forces_ref=None) -> None:
