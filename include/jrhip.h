@@ -54,6 +54,12 @@ typedef struct {
   const int64_t* wall_face;
   const double* wall_into;
   const double* wall_mid;      /* [2][n_walls] */
+  /* Point-vortex far field only (jr_numerics.far_field = 1; NULL otherwise):
+   * face midpoints of the ghost-fill tasks, [2][n_task[s]] per stage, and the
+   * vortex centre, the quarter chord of the longest body
+   * (Atlas.quarter_chord, src/jaxrans/mesh/atlas.py:492-507). */
+  const double* task_mid[2];
+  double vortex_x, vortex_y;
 } jr_grid_desc;
 
 /* The 3D atlas (src/jaxrans/mesh/atlas3d.py:119-287), host arrays.  Canvas
@@ -113,6 +119,9 @@ typedef struct {
   double krylov_success;
   double cfl_growth, cfl_cutback, cfl_min, cfl_max;
   double max_relative_update;  /* <= 0: none */
+  int32_t far_field;           /* FAR_FIELDS (src/jaxrans/numerics/boundary.py:243-246): 0 characteristic,
+                                  1 characteristic_point_vortex (2D: the far-field reference is the
+                                  vortex of the current wall-pressure lift) */
 } jr_numerics;
 
 /* Context (replaces make_residual + make_newton_step set-up,
@@ -264,6 +273,14 @@ int jr_forces(jr_ctx* ctx, double length, double centre_x, double centre_y, doub
  * friction, per q_inf * area * length.  out[n_cases][8] = cl, cd, cm,
  * cd_friction, fx, fy, fz, cy. */
 int jr_forces3(jr_ctx* ctx, double area, double length, double centre_x, double centre_z, double* out_host);
+
+/* Per wall face of the device state, in the atlas's wall order (wall_loads,
+ * src/jaxrans/post/forces.py:48-68): out[n_cases][1 + dim][n_walls] = the
+ * gauge pressure p - 1/gamma of the wall cell, then the viscous force on the
+ * body (into-fluid sign times the viscous flux of the velocity rows at the
+ * face; zero when inviscid), per component.  The wall distributions of
+ * Result.surface / _surface.dat are formed from these on the host. */
+int jr_wall_loads(jr_ctx* ctx, double* out_host);
 
 /* Average device time of one launch of a hot kernel on the device state,
  * timed with HIP events on the context stream: 0 residual (all launches),

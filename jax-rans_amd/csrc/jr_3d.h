@@ -710,6 +710,52 @@ __global__ void __launch_bounds__(BLOCK) k_cell_speeds3(Plan3 P, Physics ph, con
   }
 }
 
+// --- explicit multistage Runge-Kutta (solvers/explicit.py:48-71) ------------
+// Per canvas cell dt / V as k_local_dt with three face families: the faces'
+// spectral radii |v . S| + a |S| at the cell's own state (lower then upper
+// face of each family), plus, when viscous, the diffusive limit
+// 4 (gamma / Pr) (mu / rho) (S_i^2 + S_j^2 + S_k^2) / V, S_a the mean area
+// of the two faces normal to a (numerics/residual.py:142-166).  Gap and
+// ghost rows get a zero factor (their R = U - U0 stays at its start).
+template <int NV, bool VISC>
+__global__ void __launch_bounds__(BLOCK) k_local_dt3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                     const double* __restrict__ U, double* __restrict__ fac) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const int cs = blockIdx.y;
+  if (c >= P.nc) return;
+  const int NJ = P.NJ, NK = P.NK;
+  const int i = c / (NJ * NK), j = (c / NK) % NJ, k = c % NK;
+  const int lo0 = (i * NJ + j) * NK + k, hi0 = lo0 + NJ * NK;
+  const int lo1 = P.nf[0] + (i * (NJ + 1) + j) * NK + k, hi1 = lo1 + NK;
+  const int lo2 = P.nf[0] + P.nf[1] + (i * NJ + j) * (NK + 1) + k, hi2 = lo2 + 1;
+  const int fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
+  const double gamma = ph.gamma;
+  double q[NV];
+  prim_at3<NV>(U, (size_t)cs * NV * P.nc, P.nc, c, gamma, q);
+  const double a = sqrt(gamma * q[4] / q[0]);
+  double total = 0.0, mean[3];
+  for (int fam = 0; fam < 3; ++fam) {
+    double sl[3], sh[3];
+    face_vec3(P, fl[2 * fam], sl);
+    face_vec3(P, fl[2 * fam + 1], sh);
+    const double ml = sqrt(dot3(sl, sl)), mh = sqrt(dot3(sh, sh));
+    const double ll = fabs(dot3(q + 1, sl)) + a * ml, lh = fabs(dot3(q + 1, sh)) + a * mh;
+    total = fam == 0 ? ll + lh : (total + ll) + lh;
+    mean[fam] = 0.5 * (ml + mh);
+  }
+  const double V = P.vol[c];
+  if constexpr (VISC) {
+    const double squares = (mean[0] * mean[0] + mean[1] * mean[1]) + mean[2] * mean[2];
+    const double mu_inf = ctl[cs].mu_inf;
+    double mu = viscosity<double>(gamma * q[4] / q[0], mu_inf, ph.sutherland);
+    if constexpr (NV > 5) mu = mu + eddy_viscosity<double>(q[0], q[5] * mu_inf, mu / q[0]);
+    const double factor = gamma / PR * mu / q[0];
+    total = total + 4.0 * factor * squares / V;
+  }
+  const double dt = ctl[cs].cfl * V / fmax(total, 1e-300);
+  fac[(size_t)cs * P.nc + c] = P.interior[c] ? dt / V : 0.0;
+}
+
 // --- line preconditioner: 5x5 block Thomas, one thread per chain -----------
 
 // d(F.S)/dU of the analytic flux by forward differentiation (euler.py:55-71).
@@ -1411,7 +1457,8 @@ __device__ void wall_shear3(const Plan3& P, const Physics& ph, const double* Q, 
 template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_forces3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
                                                    const double* __restrict__ Q, double area, double length,
-                                                   double cx, double cz, double* __restrict__ out) {
+                                                   double cx, double cz, double* __restrict__ out,
+                                                   double* __restrict__ loads) {
   __shared__ double sh[6][4];
   const int cs = blockIdx.x;
   const size_t base = (size_t)cs * NV * P.nc;
@@ -1436,6 +1483,11 @@ __global__ void __launch_bounds__(BLOCK) k_forces3(Plan3 P, Physics ph, const Ct
     if (ph.viscous) {
       wall_shear3<NV>(P, ph, Q, base, ctl[cs].mu_inf, fce, t);
       for (int c = 0; c < 3; ++c) t[c] *= into;
+    }
+    if (loads) {   // per wall face: gauge pressure, viscous force on the body (x, y, z)
+      double* lw = loads + (size_t)cs * 4 * P.n_walls + w;
+      lw[0] = p;
+      for (int c = 0; c < 3; ++c) lw[(size_t)(1 + c) * P.n_walls] = t[c];
     }
     double ld[3] = {pf[0] + t[0], pf[1] + t[1], pf[2] + t[2]};
     const double rx = P.w_mid[w] - cx, rz = P.w_mid[(size_t)2 * P.n_walls + w] - cz;

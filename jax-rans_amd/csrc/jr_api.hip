@@ -92,6 +92,8 @@ struct jr_ctx {
   bool per_case_geo = false;   // geometry arrays per case (jr_set_geometry)
   uint64_t topo_hash = 0;      // hash of the index plans: grids of one topology share them
   int device = 0;              // the HIP device current at creation (its stream's device)
+  double* loads = nullptr;     // jr_wall_loads: per wall face [case][1 + dim][n_walls]
+  double* circ = nullptr;      // point-vortex far field: [case][2] circulation and its tangent
 };
 
 // FNV-1a over the index plans of a grid (stages, chains, walls, neighbours,
@@ -187,6 +189,9 @@ static int probe_slot(jr_ctx* c, int id) {
 template <int NV, bool VISC>
 static void launch_residual(jr_ctx* c, const double* U, double* out, int first_order) {
   const Plan& P = c->P;
+  if (P.vortex)   // the far-field reference moves with the current lift
+    hipLaunchKernelGGL((k_circulation<double, NV>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
+                       c->ctl, U, nullptr, c->circ, 0);
   hipLaunchKernelGGL((k_fill<double, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
                      dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
   if (VISC)
@@ -208,6 +213,9 @@ template <int NV, bool VISC>
 static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double* out, int first_order,
                            const double* vdt, int check_active, bool dq_ready = false) {
   const Plan& P = c->P;
+  if (P.vortex && !dq_ready)   // the circulation's tangent for the far-field ghosts
+    hipLaunchKernelGGL((k_circulation<Dual, NV>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
+                       (const Ctl*)c->ctl, U, dU, c->circ, check_active);
   if (!dq_ready)
     JR_LAUNCH(PR_FILL, (k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
               dim3(BLOCK), 0, P, c->ph.gamma, c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
@@ -412,11 +420,6 @@ static void rk_step(jr_ctx* c, double* hist_row, const double* alphas, int n) {
   }
 }
 
-static void rk_dispatch(jr_ctx* c, double* hist_row, const double* alphas, int n) {
-  if (c->nv == 5) rk_step<5, true>(c, hist_row, alphas, n);
-  else if (c->ph.viscous) rk_step<4, true>(c, hist_row, alphas, n);
-  else rk_step<4, false>(c, hist_row, alphas, n);
-}
 
 // --- the 3D path ---------------------------------------------------------------
 
@@ -570,6 +573,37 @@ static void step3_dispatch(jr_ctx* c, double* hist_row) {
   else step3_m<5, false>(c, hist_row);
 }
 
+// The same step on a 3D canvas.
+template <int NV, bool VISC>
+static void rk_step3(jr_ctx* c, double* hist_row, const double* alphas, int n) {
+  const Plan& P = c->P;
+  const dim3 vg(c->nblk, c->n_cases);
+  const dim3 eg((unsigned)(((size_t)NV * P.nc + BLOCK - 1) / BLOCK), (unsigned)c->n_cases);
+  hipLaunchKernelGGL((k_local_dt3<NV, VISC>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, c->P3,
+                     c->ph, c->ctl, c->U, c->vdt);
+  launch_residual3<NV, VISC>(c, c->U, c->R, 0);
+  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
+                     hist_row, c->n_cases);
+  for (int k = 0; k < n; ++k) {
+    if (k > 0) launch_residual3<NV, VISC>(c, c->W, c->R, 0);
+    double* out = k + 1 < n ? c->W : c->U;
+    hipLaunchKernelGGL((k_rk_stage<NV>), eg, dim3(BLOCK), 0, c->stream, P.nc, (const double*)c->U,
+                       (const double*)c->vdt, (const double*)c->R, alphas[k], out);
+  }
+}
+
+static void rk_dispatch(jr_ctx* c, double* hist_row, const double* alphas, int n) {
+  if (c->dim == 3) {
+    if (c->nv == 6) rk_step3<6, true>(c, hist_row, alphas, n);
+    else if (c->ph.viscous) rk_step3<5, true>(c, hist_row, alphas, n);
+    else rk_step3<5, false>(c, hist_row, alphas, n);
+    return;
+  }
+  if (c->nv == 5) rk_step<5, true>(c, hist_row, alphas, n);
+  else if (c->ph.viscous) rk_step<4, true>(c, hist_row, alphas, n);
+  else rk_step<4, false>(c, hist_row, alphas, n);
+}
+
 // --- dispatch over the compile-time variants ------------------------------------
 
 template <int NV, bool VISC, int MM>
@@ -645,32 +679,33 @@ static void apply_dispatch(jr_ctx* c, const double* r, double* z) {
   else { if (c->mixed) launch_apply<float, 4>(c, r, z, 0, 0); else launch_apply<double, 4>(c, r, z, 0, 0); }
 }
 
-static int forces3_dispatch(jr_ctx* c, double area, double length, double cx, double cz) {
+static int forces3_dispatch(jr_ctx* c, double area, double length, double cx, double cz,
+                            double* loads = nullptr) {
   const Plan3& P = c->P3;
   if (c->nv == 6) {
     launch_residual3<6, true>(c, c->U, c->tmpA, 0);
     hipLaunchKernelGGL((k_forces3<6>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, area,
-                       length, cx, cz, c->coef);
+                       length, cx, cz, c->coef, loads);
   } else {
     if (c->ph.viscous) launch_residual3<5, true>(c, c->U, c->tmpA, 0);
     else launch_residual3<5, false>(c, c->U, c->tmpA, 0);
     hipLaunchKernelGGL((k_forces3<5>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, area,
-                       length, cx, cz, c->coef);
+                       length, cx, cz, c->coef, loads);
   }
   return 0;
 }
 
-static int forces_dispatch(jr_ctx* c, double length, double cx, double cy) {
+static int forces_dispatch(jr_ctx* c, double length, double cx, double cy, double* loads = nullptr) {
   const Plan& P = c->P;
   if (c->nv == 5) {
     launch_residual<5, true>(c, c->U, c->tmpA, 0);
     hipLaunchKernelGGL((k_forces<5>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, length, cx,
-                       cy, c->coef);
+                       cy, c->coef, loads);
   } else {
     if (c->ph.viscous) launch_residual<4, true>(c, c->U, c->tmpA, 0);
     else launch_residual<4, false>(c, c->U, c->tmpA, 0);
     hipLaunchKernelGGL((k_forces<4>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q, length, cx,
-                       cy, c->coef);
+                       cy, c->coef, loads);
   }
   return 0;
 }
@@ -766,6 +801,25 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
         upload(c, &P.t_kind[s], g->task_kind[s], g->n_task[s]) ||
         upload(c, &P.t_norm[s], g->task_normal[s], 2 * (size_t)g->n_task[s]))
       return bail(0);
+  }
+  if (num->far_field != 0 && num->far_field != 1) {
+    fail("jr_ctx_create: unknown far-field model");
+    return bail(0);
+  }
+  P.vortex = num->far_field == 1;
+  if (P.vortex) {
+    if (!g->task_mid[0] || !g->task_mid[1]) {
+      fail("jr_ctx_create: the point-vortex far field needs the task midpoints");
+      return bail(0);
+    }
+    P.vx = g->vortex_x;
+    P.vy = g->vortex_y;
+    for (int s = 0; s < 2; ++s)
+      if (upload(c, &P.t_mid[s], g->task_mid[s], 2 * (size_t)g->n_task[s])) return bail(0);
+    double* circ = nullptr;
+    if (dalloc(c, &circ, (size_t)n_cases * 2)) return bail(0);
+    c->circ = circ;
+    P.circ = circ;
   }
   std::vector<int> fq_task, fq_dest, fq_root;   // final fill tasks (below)
   bool roots_ok = true;
@@ -950,7 +1004,9 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   {
     const size_t need = c->nv == 5 ? cr_apply_lds<double, 5>(P.max_len, P.max_dep, true)
                                    : cr_apply_lds<double, 4>(P.max_len, P.max_dep, true);
-    c->dq_fused = !c->thomas && P.n_final > 0 && need <= 60 * 1024;
+    // the point-vortex ghosts also move with the circulation, a sum over all
+    // walls that no chain's apply can form: the tangent then runs the fill
+    c->dq_fused = !c->thomas && P.n_final > 0 && need <= 60 * 1024 && !P.vortex;
   }
   c->ktol = num->krylov_tol;
   c->max_rel = num->max_relative_update;
@@ -1067,6 +1123,7 @@ int jr_ctx_create3(const jr_grid_desc3* g, const jr_numerics* num, int n_cases, 
   if (num->flux < 0 || num->flux > 1) return fail("jr_ctx_create3: unknown flux");
   if (num->limiter < 0 || num->limiter > 3) return fail("jr_ctx_create3: unknown limiter");
   if (num->sa && !num->viscous) return fail("jr_ctx_create3: SA needs a viscous flow");
+  if (num->far_field != 0) return fail("jr_ctx_create3: the point-vortex far field is defined in 2D only");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("jr_ctx_create3: no HIP device");
   jr_ctx* c = new jr_ctx();
@@ -1399,6 +1456,7 @@ int jr_set_geometry(jr_ctx* c, int k, const jr_grid_desc* g) {
   if (g->NI != c->P.NI || g->NJ != c->P.NJ || topology_hash(g) != c->topo_hash)
     return fail("jr_set_geometry: the grid's index plans differ from the context's (another topology)");
   if (c->pending_steps) return fail("jr_set_geometry: an asynchronous block is pending");
+  if (c->P.vortex) return fail("jr_set_geometry: per-case geometry with the point-vortex far field is not built");
   Plan& P = c->P;
   const size_t nc = P.nc;
   const size_t n_area = nc, n_fi = 2 * (size_t)P.nfi, n_fj = 2 * (size_t)P.nfj,
@@ -1843,7 +1901,6 @@ int jr_linear_solve(jr_ctx* c, const double* b, double* x) {
 // |R(U)| before each step.
 int jr_rk_block(jr_ctx* c, int n_steps, const double* alphas, int n_alphas, double* hist_host) {
   if (!c || !alphas) return fail("jr_rk_block: null argument");
-  if (c->dim == 3) return fail("jr_rk_block: the explicit scheme is built for 2D contexts only");
   if (c->pending_steps) return fail("jr_rk_block: an asynchronous block is pending: wait for it first");
   if (n_steps < 1) return fail("jr_rk_block: n_steps must be >= 1");
   if (n_alphas < 1 || n_alphas > 16) return fail("jr_rk_block: between 1 and 16 stage coefficients");
@@ -1874,6 +1931,24 @@ int jr_forces(jr_ctx* c, double length, double cx, double cy, double* out) {
   forces_dispatch(c, length, cx, cy);
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_wall_loads(jr_ctx* c, double* out) {
+  if (!c || !out) return fail("jr_wall_loads: null argument");
+  if (!c->has_q) return fail("jr_wall_loads: set the freestream first");
+  const int nw = c->dim == 3 ? c->P3.n_walls : c->P.n_walls;
+  const size_t n = (size_t)c->n_cases * (c->dim + 1) * std::max(1, nw);
+  if (!c->loads) {
+    TRY(dalloc(c, &c->loads, n));
+    HIPCHK(hipDeviceSynchronize());   // dalloc clears on the null stream
+  }
+  if (c->dim == 3) forces3_dispatch(c, 1.0, 1.0, 0.0, 0.0, c->loads);
+  else forces_dispatch(c, 1.0, 0.0, 0.0, c->loads);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->loads, (size_t)c->n_cases * (c->dim + 1) * nw * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -757,7 +757,7 @@ __device__ void wall_shear(const Plan& P, const Physics& ph, const double* Q, si
 template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_forces(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                   const double* __restrict__ Q, double length, double cx, double cy,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ out, double* __restrict__ loads) {
   bind_case(P, blockIdx.x);
   __shared__ double sh[6][4];
   const int cs = blockIdx.x;
@@ -782,6 +782,12 @@ __global__ void __launch_bounds__(BLOCK) k_forces(Plan P, Physics ph, const Ctl*
       wall_shear(P, ph, Q, base, ctl[cs].mu_inf, fam, ff, &tx, &ty);
       tx *= into;
       ty *= into;
+    }
+    if (loads) {   // per wall face: gauge pressure, viscous force on the body (x, y)
+      double* lw = loads + (size_t)cs * 3 * P.n_walls + w;
+      lw[0] = p;
+      lw[P.n_walls] = tx;
+      lw[2 * (size_t)P.n_walls] = ty;
     }
     double ldx = px + tx, ldy = py + ty;
     double rx = P.w_mid[w] - cx, ry = P.w_mid[P.n_walls + w] - cy;

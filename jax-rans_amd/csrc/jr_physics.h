@@ -172,6 +172,66 @@ __device__ void roe(const T* qL, const T* qR, double sx, double sy, double gamma
   }
 }
 
+// A far-field reference value as the cell's scalar type: a constant
+// (freestream) is lifted, a value of the cell's own type kept.
+template <class T> __device__ __forceinline__ T to_t(double x) { return lift<T>(x); }
+template <class T> __device__ __forceinline__ T to_t(Dual x) { return x; }
+
+// characteristic_state (src/jaxrans/numerics/boundary.py:91-122) against the
+// reference primitives qf: one freestream (TR = double) or a per-face state
+// that may carry a tangent (TR = T, the point-vortex far field).
+template <class T, class TR, int NV>
+__device__ void far_field_state(const T* q, double nx, double ny, const TR* qf, double gamma, T* g) {
+  T a_i = jsqrt(gamma * q[3] / q[0]);
+  TR a_f = jsqrt(gamma * qf[3] / qf[0]);
+  double two_over = 2.0 / (gamma - 1.0);
+  T vn_i = q[1] * nx + q[2] * ny;
+  TR vn_f = qf[1] * nx + qf[2] * ny;
+  T rp = vn_i + two_over * a_i;
+  TR rm = vn_f - two_over * a_f;
+  T vn_b = 0.5 * (rp + rm);
+  T a_b = 0.25 * (gamma - 1.0) * (rp - rm);
+  bool out = val(vn_b) > 0.0;
+  T entropy = out ? q[3] / jpow(q[0], gamma) : to_t<T>(qf[3] / jpow(qf[0], gamma));
+  T tu = out ? q[1] - vn_i * nx : to_t<T>(qf[1] - vn_f * nx);
+  T tv = out ? q[2] - vn_i * ny : to_t<T>(qf[2] - vn_f * ny);
+  T rho_b = jpow(a_b * a_b / (gamma * entropy), 1.0 / (gamma - 1.0));
+  g[0] = rho_b;
+  g[1] = vn_b * nx + tu;
+  g[2] = vn_b * ny + tv;
+  g[3] = rho_b * a_b * a_b / gamma;
+  if constexpr (NV > 4) g[4] = out ? q[4] : to_t<T>(qf[4]);
+}
+
+// The point-vortex far-field reference (point_vortex_state,
+// src/jaxrans/numerics/boundary.py:181-214; Thomas and Salas, AIAA J. 24
+// (1986) 1074-1080): the freestream qi plus the Prandtl-Glauert-scaled
+// compressible vortex of circulation circ (clockwise positive) at offset
+// (dx, dy) from its centre, isentropic at the freestream total enthalpy.
+// Evaluated without contraction into FMAs, so that no circulation gives the
+// freestream bit for bit, as the reference's does.
+template <class TR, int NV>
+__device__ void vortex_state(double dx, double dy, const double* qi, double gamma, TR circ, TR* qr) {
+#pragma clang fp contract(off)
+  const double rho_f = qi[0], u_f = qi[1], v_f = qi[2], p_f = qi[3];
+  const double a2_f = gamma * p_f / rho_f;
+  const double speed2_f = u_f * u_f + v_f * v_f;
+  const double mach2 = speed2_f / a2_f;
+  const double alpha = atan2(v_f, u_f);
+  const double r = hypot_j(dx, dy);
+  const double theta = atan2(dy, dx);
+  const double s = sin(theta - alpha);
+  TR strength = circ * sqrt(1.0 - mach2) / (2.0 * M_PI * r * (1.0 - mach2 * (s * s)));
+  TR u = u_f + strength * sin(theta);
+  TR v = v_f - strength * cos(theta);
+  TR ratio = 1.0 + 0.5 * (gamma - 1.0) * (speed2_f - u * u - v * v) / a2_f;
+  qr[0] = rho_f * jpow(ratio, 1.0 / (gamma - 1.0));
+  qr[1] = u;
+  qr[2] = v;
+  qr[3] = p_f * jpow(ratio, gamma / (gamma - 1.0));
+  if constexpr (NV > 4) qr[4] = to_t<TR>(qi[4]);
+}
+
 // Ghost primitives from the source cell (src/jaxrans/numerics/boundary.py:54-157).
 // qf: the far-field reference (freestream) primitives.
 template <class T, int NV>
@@ -187,26 +247,7 @@ __device__ void bc_state(int kind, const T* q, double nx, double ny, const doubl
     g[0] = q[0]; g[1] = -q[1]; g[2] = -q[2]; g[3] = q[3];
     if constexpr (NV > 4) g[4] = -q[4];
   } else if (kind == K_FARFIELD) {
-    // characteristic_state (boundary.py:91-122)
-    T a_i = jsqrt(gamma * q[3] / q[0]);
-    double a_f = sqrt(gamma * qf[3] / qf[0]);
-    double two_over = 2.0 / (gamma - 1.0);
-    T vn_i = q[1] * nx + q[2] * ny;
-    double vn_f = qf[1] * nx + qf[2] * ny;
-    T rp = vn_i + two_over * a_i;
-    double rm = vn_f - two_over * a_f;
-    T vn_b = 0.5 * (rp + rm);
-    T a_b = 0.25 * (gamma - 1.0) * (rp - rm);
-    bool out = val(vn_b) > 0.0;
-    T entropy = out ? q[3] / jpow(q[0], gamma) : lift<T>(qf[3] / pow(qf[0], gamma));
-    T tu = out ? q[1] - vn_i * nx : lift<T>(qf[1] - vn_f * nx);
-    T tv = out ? q[2] - vn_i * ny : lift<T>(qf[2] - vn_f * ny);
-    T rho_b = jpow(a_b * a_b / (gamma * entropy), 1.0 / (gamma - 1.0));
-    g[0] = rho_b;
-    g[1] = vn_b * nx + tu;
-    g[2] = vn_b * ny + tv;
-    g[3] = rho_b * a_b * a_b / gamma;
-    if constexpr (NV > 4) g[4] = out ? q[4] : lift<T>(qf[4]);
+    far_field_state<T, double, NV>(q, nx, ny, qf, gamma, g);
   } else if (kind == K_INFLOW) {
     // total_condition_inflow (boundary.py:130-147)
     double g1 = gamma - 1.0;

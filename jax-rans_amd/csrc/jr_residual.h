@@ -39,15 +39,33 @@ __device__ __forceinline__ void prim_of(const double* __restrict__ U, const doub
   primitive<T, NV>(u, gamma, q);
 }
 
+// The circulation of case cs for the point-vortex far field (k_circulation):
+// its value, and for the tangent its derivative along the tangent's dU.
+template <class T> __device__ __forceinline__ T circ_of(const double* c, int cs);
+template <> __device__ __forceinline__ double circ_of<double>(const double* c, int cs) { return c[2 * cs]; }
+template <> __device__ __forceinline__ Dual circ_of<Dual>(const double* c, int cs) {
+  return mkd(c[2 * cs], c[2 * cs + 1]);
+}
+
 template <class T, int NV>
 __device__ __forceinline__ void ghost_of(const Plan& P, int stage, int t, const T* q, const double* q_inf,
-                                         double gamma, T* g) {
+                                         double gamma, T* g, int cs) {
   const int kind = P.t_kind[stage][t];
   if (kind == K_COPY) {
     for (int k = 0; k < NV; ++k) g[k] = q[k];
   } else {
     const int n = P.n_task[stage];
-    bc_state<T, NV>(kind, q, P.t_norm[stage][t], P.t_norm[stage][n + t], q_inf, gamma, g);
+    const double nx = P.t_norm[stage][t], ny = P.t_norm[stage][n + t];
+    if (kind == K_FARFIELD && P.vortex) {
+      // the reference state of this face: the vortex of the current lift
+      // (point_vortex_far_field, boundary.py:222-240)
+      T qr[NV];
+      vortex_state<T, NV>(P.t_mid[stage][t] - P.vx, P.t_mid[stage][n + t] - P.vy, q_inf, gamma,
+                          circ_of<T>(P.circ, cs), qr);
+      far_field_state<T, T, NV>(q, nx, ny, qr, gamma, g);
+    } else {
+      bc_state<T, NV>(kind, q, nx, ny, q_inf, gamma, g);
+    }
   }
 }
 
@@ -74,7 +92,7 @@ __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl*
   if (t < P.n_task[0]) {
     if (P.t_skip0[t]) return;
     prim_of<T, NV>(U, dU, base, nc, P.t_src[0][t], gamma, q);
-    ghost_of<T, NV>(P, 0, t, q, q_inf, gamma, g);
+    ghost_of<T, NV>(P, 0, t, q, q_inf, gamma, g, cs);
     dst = P.t_dest[0][t];
   } else {
     t -= P.n_task[0];
@@ -83,11 +101,11 @@ __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl*
     if (p >= 0) {
       T q0[NV];
       prim_of<T, NV>(U, dU, base, nc, P.t_src[0][p], gamma, q0);
-      ghost_of<T, NV>(P, 0, p, q0, q_inf, gamma, q);
+      ghost_of<T, NV>(P, 0, p, q0, q_inf, gamma, q, cs);
     } else {
       prim_of<T, NV>(U, dU, base, nc, P.t_src[1][t], gamma, q);
     }
-    ghost_of<T, NV>(P, 1, t, q, q_inf, gamma, g);
+    ghost_of<T, NV>(P, 1, t, q, q_inf, gamma, g, cs);
     dst = P.t_dest[1][t];
   }
   for (int k = 0; k < NV; ++k) Io<T>::st(Q, dQ, base + (size_t)k * nc + dst, g[k]);
@@ -117,15 +135,15 @@ __global__ void __launch_bounds__(BLOCK) k_ghost_jac(Plan P, double gamma, const
     Dual s[NV], g[NV];
     for (int k = 0; k < NV; ++k) s[k] = mkd(qr[k], k == col ? 1.0 : 0.0);
     if (task < n0) {
-      ghost_of<Dual, NV>(P, 0, task, s, q_inf, gamma, g);
+      ghost_of<Dual, NV>(P, 0, task, s, q_inf, gamma, g, cs);
     } else {
       const int t = task - n0, p = P.t_prev1[t];
       if (p >= 0) {
         Dual s1[NV];
-        ghost_of<Dual, NV>(P, 0, p, s, q_inf, gamma, s1);
-        ghost_of<Dual, NV>(P, 1, t, s1, q_inf, gamma, g);
+        ghost_of<Dual, NV>(P, 0, p, s, q_inf, gamma, s1, cs);
+        ghost_of<Dual, NV>(P, 1, t, s1, q_inf, gamma, g, cs);
       } else {
-        ghost_of<Dual, NV>(P, 1, t, s, q_inf, gamma, g);
+        ghost_of<Dual, NV>(P, 1, t, s, q_inf, gamma, g, cs);
       }
     }
     for (int r = 0; r < NV; ++r) J[r * NV + col] = g[r].d;

@@ -45,6 +45,45 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[4]) {
   __syncthreads();
 }
 
+// Kutta-Joukowski circulation of each case's wall pressure force for the
+// point-vortex far field (point_vortex_far_field,
+// src/jaxrans/numerics/boundary.py:222-240): F = -sum (p - p_inf) S over the
+// wall faces (S into the fluid), circ = (F_y u - F_x v) / |V| / (rho |V|) of
+// the freestream; with dU given, also its derivative along dU.  One workgroup
+// per case; circ[case][2] = (value, derivative).
+template <class T, int NV>
+__global__ void __launch_bounds__(BLOCK) k_circulation(Plan P, double gamma, const Ctl* __restrict__ ctl,
+                                                       const double* __restrict__ U, const double* __restrict__ dU,
+                                                       double* __restrict__ circ, int check_active) {
+  bind_case(P, blockIdx.x);
+  __shared__ double sh[4][4];
+  const int cs = blockIdx.x;
+  if (check_active && !ctl[cs].active) return;
+  const size_t base = (size_t)cs * NV * P.nc;
+  const double* qi = ctl[cs].q_inf;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};   // F_x, F_y and their derivatives
+  for (int w = threadIdx.x; w < P.n_walls; w += BLOCK) {
+    const int fam = P.w_fam[w], ff = P.w_face[w];
+    const double into = P.w_into[w];
+    const double sx = into * (fam == 0 ? P.fi[ff] : P.fj[ff]);
+    const double sy = into * (fam == 0 ? P.fi[P.nfi + ff] : P.fj[P.nfj + ff]);
+    T q[NV];
+    prim_of<T, NV>(U, dU, base, P.nc, P.w_cell[w], gamma, q);
+    const T p = q[3] - qi[3];
+    const T fx = -(p * sx), fy = -(p * sy);
+    v[0] += val(fx);
+    v[1] += val(fy);
+    v[2] += tan_of(fx);
+    v[3] += tan_of(fy);
+  }
+  block_sum<4>(v, sh);
+  if (threadIdx.x == 0) {
+    const double speed = hypot_j(qi[1], qi[2]);
+    circ[2 * cs] = (v[1] * qi[1] - v[0] * qi[2]) / speed / (qi[0] * speed);
+    circ[2 * cs + 1] = (v[3] * qi[1] - v[2] * qi[2]) / speed / (qi[0] * speed);
+  }
+}
+
 // Partial sums -> the last workgroup to finish adds them in block order and
 // leaves the NV totals in the shared array `out`.  Returns true in every
 // thread of that workgroup.

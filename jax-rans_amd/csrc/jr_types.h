@@ -62,6 +62,13 @@ struct Plan {
   // when the cases share the canvas geometry).  Per-case geometry lets one
   // batch hold different grids of one topology (the same index plans).
   size_t gs_area, gs_fi, gs_fj, gs_cen, gs_dist, gs_tn0, gs_tn1, gs_wmid;
+  // Point-vortex far field: far-field tasks take the vortex of each case's
+  // current circulation circ[case][2] (value, tangent), centred at (vx, vy),
+  // at their face midpoints t_mid[stage] ([2][n]).
+  int vortex;
+  double vx, vy;
+  const double* t_mid[2];
+  const double* circ;
 };
 
 // The geometry of case cs (identity when the cases share it).

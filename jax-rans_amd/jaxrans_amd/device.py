@@ -19,6 +19,7 @@ NOALLOC_HISTORY_MAX = 65536      # JR_NOALLOC_HISTORY_MAX of include/jrhip.h
 
 FLUX_CODES = {"hllc": 0, "roe": 1}
 LIMITER_CODES = {"minmod": 0, "none": 1, "van_albada": 2, "van_albada_smooth": 3}
+FAR_FIELD_CODES = {"characteristic": 0, "characteristic_point_vortex": 1}
 
 EXPORTS = (
     "jr_ctx_create", "jr_ctx_destroy", "jr_set_freestream", "jr_set_state", "jr_get_state", "jr_residual",
@@ -28,7 +29,7 @@ EXPORTS = (
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
     "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
-    "jr_newton_block_noalloc",
+    "jr_newton_block_noalloc", "jr_wall_loads",
 )
 
 
@@ -56,6 +57,7 @@ class GridDesc(C.Structure):
         ("line_down_face", _i64), ("line_down_sign", _i64),
         ("n_walls", C.c_int32),
         ("wall_cell", _i64), ("wall_family", _i64), ("wall_face", _i64), ("wall_into", _d), ("wall_mid", _d),
+        ("task_mid", _d * 2), ("vortex_x", C.c_double), ("vortex_y", C.c_double),
     ]
 
 
@@ -82,7 +84,7 @@ class Numerics(C.Structure):
         ("krylov_subspace", C.c_int32),
         ("krylov_restarts", C.c_int32), ("krylov_tol", C.c_double), ("krylov_success", C.c_double),
         ("cfl_growth", C.c_double), ("cfl_cutback", C.c_double), ("cfl_min", C.c_double), ("cfl_max", C.c_double),
-        ("max_relative_update", C.c_double),
+        ("max_relative_update", C.c_double), ("far_field", C.c_int32),
     ]
 
 
@@ -137,7 +139,7 @@ class Context:
 
     def __init__(self, atlas: Atlas, *, gamma: float, flux: str, limiter: str, kappa: float = 0.0,
                  viscous: bool = False, sa: bool = False, sutherland: float | None = None, c_cr1: float = 0.0,
-                 implicit=None, n_cases: int = 1):
+                 implicit=None, n_cases: int = 1, far_field: str = "characteristic"):
         lib = library()
         if device_count() < 1:
             raise DeviceUnavailable("no HIP device is visible")
@@ -153,7 +155,12 @@ class Context:
             self._keep.append(a)
             return a
 
-        g = self._desc3(atlas, arr) if self.dim == 3 else self._desc2(atlas, arr)
+        if far_field not in FAR_FIELD_CODES:
+            raise DeviceError(f"unknown far-field model {far_field!r}; known: {sorted(FAR_FIELD_CODES)}")
+        if far_field != "characteristic" and self.dim == 3:
+            raise DeviceError(f"far field model {far_field!r} is defined in 2D only")
+        self.far_field = far_field
+        g = self._desc3(atlas, arr) if self.dim == 3 else self._desc2(atlas, arr, far_field != "characteristic")
         from .core.config import ImplicitConfig
         cfg = implicit if implicit is not None else ImplicitConfig()
         n = Numerics()
@@ -169,6 +176,7 @@ class Context:
         n.krylov_tol, n.krylov_success = cfg.krylov_tol, cfg.krylov_success
         n.cfl_growth, n.cfl_cutback, n.cfl_min, n.cfl_max = cfg.cfl_growth, cfg.cfl_cutback, cfg.cfl_min, cfg.cfl_max
         n.max_relative_update = -1.0 if cfg.max_relative_update is None else cfg.max_relative_update
+        n.far_field = FAR_FIELD_CODES[far_field]
         if cfg.preconditioner != "line_j":
             raise DeviceError("only the line_j preconditioner is implemented on the device")
         self.config = cfg
@@ -179,7 +187,7 @@ class Context:
         self._lib = lib
 
     @staticmethod
-    def _desc2(atlas, arr) -> GridDesc:
+    def _desc2(atlas, arr, vortex: bool = False) -> GridDesc:
         g = GridDesc()
         g.NI, g.NJ = atlas.shape
         g.area = _ptr(arr(atlas.area, np.float64), C.c_double)
@@ -195,6 +203,11 @@ class Context:
             g.task_src[s] = _ptr(arr(st.src, np.int64), C.c_int64)
             g.task_kind[s] = _ptr(arr(st.kind, np.int32), C.c_int32)
             g.task_normal[s] = _ptr(arr(st.normal, np.float64), C.c_double)
+            if vortex:
+                g.task_mid[s] = _ptr(arr(st.midpoint, np.float64), C.c_double)
+        if vortex:
+            from .mesh.atlas import quarter_chord
+            g.vortex_x, g.vortex_y = quarter_chord(atlas.walls)
         Context._lines_walls(g, atlas, arr)
         return g
 
@@ -259,6 +272,7 @@ class Context:
         mu = None if mu_inf is None else np.ascontiguousarray(np.broadcast_to(mu_inf, (self.n_cases,)), np.float64)
         _check(self._lib.jr_set_freestream(self._h, _ptr(q, C.c_double),
                                            None if mu is None else _ptr(mu, C.c_double)))
+        self.freestream = q
 
     def set_geometry(self, k: int, atlas) -> None:
         """Case ``k`` steps on ``atlas``: another grid of this context's
@@ -437,6 +451,14 @@ class Context:
                                     C.c_double(centre[2]), _ptr(out, C.c_double)))
         return out
 
+    def wall_loads(self) -> np.ndarray:
+        """Per wall face of the device state (the atlas's wall order):
+        ``(n_cases, 1 + dim, n_walls)`` = gauge pressure, then the viscous
+        force on the body per component (zero when inviscid)."""
+        out = np.empty((self.n_cases, self.dim + 1, max(1, self.atlas.walls.n)))
+        _check(self._lib.jr_wall_loads(self._h, _ptr(out, C.c_double)))
+        return out[:, :, :self.atlas.walls.n]
+
     def forces(self, length: float = 1.0, centre=(0.25, 0.0)) -> np.ndarray:
         out = np.empty((self.n_cases, 8))
         _check(self._lib.jr_forces(self._h, C.c_double(length), C.c_double(centre[0]), C.c_double(centre[1]),
@@ -467,8 +489,7 @@ class Context:
 
 def context_for(atlas: Atlas, flow, numerics, implicit=None, n_cases: int = 1) -> Context:
     """A context for a case's flow and numerics (``FlowConditions``, ``NumericsConfig``)."""
-    if numerics.far_field != "characteristic":
-        raise DeviceError(f"far field model {numerics.far_field!r} is not implemented on the device")
     return Context(atlas, gamma=flow.gamma, flux=numerics.flux, limiter=numerics.limiter, kappa=numerics.kappa,
                    viscous=not flow.inviscid, sa=flow.turbulence is not None, sutherland=flow.sutherland_ratio,
-                   c_cr1=0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0, implicit=implicit, n_cases=n_cases)
+                   c_cr1=0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0, implicit=implicit, n_cases=n_cases,
+                   far_field=numerics.far_field)

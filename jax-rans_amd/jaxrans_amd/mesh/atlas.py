@@ -442,6 +442,24 @@ def _bodies(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return out
 
 
+def quarter_chord(walls: Walls) -> tuple[float, float]:
+    """The point-vortex centre (``Atlas.quarter_chord``,
+    ``src/jaxrans/mesh/atlas.py:492-507``): on the body of longest chord, a
+    quarter of the way from its leading edge (the wall node farthest from
+    the trailing edge) to its trailing edge (its wall node of largest ``x``,
+    the first in wall order on a tie); ``(0.25, 0)`` without walls."""
+    if walls.n == 0:
+        return 0.25, 0.0
+    chords = []
+    for body in range(walls.bodies):
+        pts = walls.a[walls.body == body]
+        te = pts[int(np.argmax(pts[:, 0]))]
+        le = pts[int(np.argmax(np.hypot(pts[:, 0] - te[0], pts[:, 1] - te[1])))]
+        chords.append((float(np.hypot(te[0] - le[0], te[1] - le[1])), te, le))
+    _, te, le = max(chords, key=lambda c: c[0])
+    return float(le[0] + 0.25 * (te[0] - le[0])), float(le[1] + 0.25 * (te[1] - le[1]))
+
+
 def wall_distance(mesh, walls: Walls, chunk: int = 4096) -> list[np.ndarray]:
     """Exact distance from each block's cell centres to the nearest wall face
     (clamped projection onto every segment ``a -> b``)."""

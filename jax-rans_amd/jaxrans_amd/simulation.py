@@ -176,6 +176,21 @@ class Result:
     def cells(self) -> int:
         return self.grid.cells
 
+    def surface(self) -> dict:
+        """Wall distributions per wall face (``post.surface``): 2D ``x``,
+        ``y``, ``body``, ``cp`` [, ``cf``]; 3D ``block``, ``i``, ``k``, ``x``,
+        ``y``, ``z``, ``cp`` [, ``cf_x``, ``cf_y``, ``cf_z``].  The loads are
+        evaluated on the device at ``U``."""
+        from .post.surface import surface
+        ctx = make_context(self.case, self.grid)
+        try:
+            q, mu = freestream_of(self.case.flow, self.grid.dim)
+            ctx.set_freestream(q, mu)
+            ctx.set_state(np.asarray(self.U)[None])
+            return surface(ctx, 0)
+        finally:
+            ctx.close()
+
     def write(self, directory=None, extra: dict | None = None):
         """Metadata, residual and lift histories and fields under the
         reference's file names (``simulation.py:135-160``); returns the directory."""
@@ -191,9 +206,6 @@ def _check_case(case: Case, dim: int | None = None) -> None:
     if case.numerics.solver == "explicit":
         if case.rk_scheme not in RK_SCHEMES:
             raise CaseError(f"unknown RK scheme {case.rk_scheme!r}; available: {sorted(RK_SCHEMES)}")
-        if dim == 3:
-            raise CaseError("the explicit Runge-Kutta solver is built for 2D cases only; set "
-                            "numerical.time_integration.solver = \"implicit\"")
     if case.flow.swept:
         raise CaseError("swept (2.5D) flow is not built by this implementation")
 

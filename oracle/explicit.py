@@ -8,6 +8,8 @@ a |S|`` at the cell's own state on each of its faces (lower i, upper i, lower
 j, upper j, summed in that order), plus the diffusive limit ``4 (gamma / Pr)
 (mu / rho) (S_i^2 + S_j^2) / V`` when viscous, ``S_a`` the mean area of the
 two faces normal to ``a`` and ``mu`` laminar plus the SA eddy viscosity.
+``local_time_step3`` is the same on a 3D canvas (three face families, lower
+and upper per family, ``|S| = sqrt(S . S)``, ``S_i^2 + S_j^2 + S_k^2``).
 ``rk_step`` follows ``src/jaxrans/solvers/explicit.py:48-71``:
 ``U^(k) = U^n - alpha_k (dt / V) R(U^(k-1))`` with ``dt`` frozen at ``U^n``;
 the reported norm is the first stage's.  Pinned by
@@ -46,12 +48,45 @@ def local_time_step(U, setup: Setup, cfl: float):
     return cfl * at.area / np.maximum(total, 1e-300)
 
 
-def rk_step(U, setup: Setup, alphas, cfl: float):
-    """``(U^{n+1}, |R(U^n)|)``."""
-    factor = local_time_step(U, setup, cfl) / setup.atlas.area
-    R = residual(U, setup)
-    nrm = norm(R, setup)
+def _lower_upper(s, a):
+    n = s.shape[1 + a]
+    return np.take(s, range(n - 1), axis=1 + a), np.take(s, range(1, n), axis=1 + a)
+
+
+def local_time_step3(U, setup, cfl: float):
+    """3D: the faces of family a (``(3, ...)`` with one more along a)."""
+    from . import physics3d as P3
+    at, gamma = setup.atlas, setup.gamma
+    q = np.stack(P3.primitive([U[k] for k in range(len(U))], gamma, 3))
+    a = np.sqrt(gamma * q[4] / q[0])
+    total, mean = None, []
+    for fam, s in enumerate(at.faces):
+        lo, hi = _lower_upper(s, fam)
+        mags = [np.sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]) for f in (lo, hi)]
+        lam = [np.abs((q[1] * f[0] + q[2] * f[1]) + q[3] * f[2]) + a * m for f, m in zip((lo, hi), mags)]
+        total = lam[0] + lam[1] if total is None else total + lam[0] + lam[1]
+        mean.append(0.5 * (mags[0] + mags[1]))
+    if setup.viscous:
+        squares = (mean[0] ** 2 + mean[1] ** 2) + mean[2] ** 2
+        mu = P.viscosity(gamma * q[4] / q[0], setup.mu_inf, setup.sutherland)
+        if setup.sa:
+            mu = mu + P.eddy_viscosity(q[0], q[5] * setup.mu_inf, mu / q[0])
+        total = total + 4.0 * (gamma / P.PR * mu / q[0]) * squares / at.volume
+    return cfl * at.volume / np.maximum(total, 1e-300)
+
+
+def rk_step(U, setup, alphas, cfl: float):
+    """``(U^{n+1}, |R(U^n)|)``, 2D (``Setup``) or 3D (``Setup3D``)."""
+    if getattr(setup.atlas, "dim", 2) == 3:
+        from . import residual3d as R3
+        factor = local_time_step3(U, setup, cfl) / setup.atlas.volume
+        res, nrm_of = R3.residual, R3.norm
+    else:
+        factor = local_time_step(U, setup, cfl) / setup.atlas.area
+        res, nrm_of = residual, norm
+    R = res(U, setup)
+    nrm = nrm_of(R, setup)
     new = U - alphas[0] * factor[None] * R
     for alpha in alphas[1:]:
-        new = U - alpha * factor[None] * residual(new, setup)
+        new = U - alpha * factor[None] * res(new, setup)
     return new, nrm

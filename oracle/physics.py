@@ -210,11 +210,12 @@ def bc_state(kind: int, q, n, q_inf, gamma):
 
 
 def characteristic(q, n, q_inf, gamma):
-    """(src/jaxrans/numerics/boundary.py:91-122)"""
+    """(src/jaxrans/numerics/boundary.py:91-122); ``q_inf`` one reference
+    state (numbers) or one per face (rows, possibly dual)."""
     rho_i, u_i, v_i, p_i = q[:4]
     rho_f, u_f, v_f, p_f = q_inf[:4]
     a_i = M.sqrt(gamma * p_i / rho_i)
-    a_f = np.sqrt(gamma * p_f / rho_f)
+    a_f = M.sqrt(gamma * p_f / rho_f)
     two_over = 2.0 / (gamma - 1.0)
     vn_i = u_i * n[0] + v_i * n[1]
     vn_f = u_f * n[0] + v_f * n[1]
@@ -232,6 +233,28 @@ def characteristic(q, n, q_inf, gamma):
     for k in range(4, len(q)):
         res.append(M.where(out, q[k], q_inf[k] + 0.0 * n[0]))
     return res
+
+
+def point_vortex_state(dx, dy, q_inf, gamma, circulation):
+    """Freestream plus the compressible point vortex of ``circulation``
+    (clockwise positive) at offsets ``(dx, dy)`` from its centre, isentropic
+    at the freestream total enthalpy; rows ``(rho, u, v, p[, scalars])``
+    (src/jaxrans/numerics/boundary.py:181-214: Thomas and Salas 1986,
+    the Prandtl-Glauert-scaled vortex).  ``circulation`` may be dual."""
+    rho_f, u_f, v_f, p_f = (np.float64(x) for x in q_inf[:4])
+    a2_f = gamma * p_f / rho_f
+    speed2_f = u_f * u_f + v_f * v_f
+    mach2 = speed2_f / a2_f
+    alpha = np.arctan2(v_f, u_f)
+    r = hypot_jnp(dx, dy)
+    theta = np.arctan2(dy, dx)
+    strength = circulation * np.sqrt(1.0 - mach2) / (2.0 * np.pi * r * (1.0 - mach2 * np.sin(theta - alpha) ** 2))
+    u = u_f + strength * np.sin(theta)
+    v = v_f - strength * np.cos(theta)
+    ratio = 1.0 + 0.5 * (gamma - 1.0) * (speed2_f - u * u - v * v) / a2_f
+    rho = rho_f * ratio ** (1.0 / (gamma - 1.0))
+    p = p_f * ratio ** (gamma / (gamma - 1.0))
+    return [rho, u, v, p] + [np.full(np.shape(dx), np.float64(x)) for x in q_inf[4:]]
 
 
 def inflow_total(q, q_inf, gamma):

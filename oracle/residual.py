@@ -36,6 +36,7 @@ class Setup:
     sutherland: float | None = None
     sa: bool = False
     c_cr1: float = 0.0
+    far_field: str = "characteristic"
 
     @property
     def nvar(self) -> int:
@@ -55,7 +56,7 @@ def setup_for(atlas, flow, numerics, freestream=None, mu_inf=None) -> Setup:
         q_inf = np.concatenate([q_inf, [flow.nu_tilde_ratio]])
     return Setup(atlas, flow.gamma, q_inf, numerics.flux, numerics.limiter, numerics.kappa, mu,
                  flow.sutherland_ratio, flow.turbulence is not None,
-                 0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0)
+                 0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0, numerics.far_field)
 
 
 def rows(U):
@@ -79,8 +80,49 @@ def _scatter(row, dest, vals):
     return out
 
 
+def quarter_chord(walls) -> tuple[float, float]:
+    """A quarter of the way from the leading to the trailing edge of the body
+    of longest chord: the trailing edge its wall node of largest ``x``, the
+    leading edge its node farthest from that (src/jaxrans/mesh/atlas.py:492-507)."""
+    if walls.n == 0:
+        return 0.25, 0.0
+    best = None
+    for k in range(walls.bodies):
+        nodes = walls.a[walls.body == k]
+        te = nodes[np.argmax(nodes[:, 0])]
+        le = nodes[np.argmax(np.hypot(nodes[:, 0] - te[0], nodes[:, 1] - te[1]))]
+        chord = np.hypot(te[0] - le[0], te[1] - le[1])
+        if best is None or chord > best[0]:
+            best = (chord, le[0] + 0.25 * (te[0] - le[0]), le[1] + 0.25 * (te[1] - le[1]))
+    return float(best[1]), float(best[2])
+
+
+def _dsum(x):
+    return M.D(np.sum(x.v), np.sum(x.d)) if isinstance(x, M.D) else np.sum(x)
+
+
+def circulation(q, setup: Setup):
+    """Kutta-Joukowski circulation of the walls' pressure force, ``(fy u - fx
+    v) / |V| / (rho |V|)`` with ``F = -sum (p - p_inf) S`` over the wall faces,
+    ``S`` into the fluid (src/jaxrans/numerics/boundary.py:222-240); ``q``
+    flat primitive rows (possibly dual)."""
+    at = setup.atlas
+    w = at.walls
+    fi, fj = at.face_i.reshape(2, -1), at.face_j.reshape(2, -1)
+    S = np.where(w.family[None] == 0, fi[:, np.minimum(w.face, fi.shape[1] - 1)],
+                 fj[:, np.minimum(w.face, fj.shape[1] - 1)]) * w.into_fluid
+    p = M.take(q[3], w.cell) - setup.q_inf[3]
+    fx, fy = -_dsum(p * S[0]), -_dsum(p * S[1])
+    rho_f, u_f, v_f = (np.float64(x) for x in setup.q_inf[:3])
+    speed = float(P.hypot_jnp(u_f, v_f))
+    return (fy * u_f - fx * v_f) / speed / (rho_f * speed)
+
+
 def fill_ghosts(q, setup: Setup):
     """Staged ghost fill on flat primitive rows ``(n_canvas,)`` each."""
+    vortex = None
+    if setup.far_field == "characteristic_point_vortex":
+        vortex = (circulation(q, setup), quarter_chord(setup.atlas.walls))
     for st in setup.atlas.stages:
         src_rows = [M.take(r, st.src) for r in q]
         new = [None] * len(q)
@@ -91,6 +133,11 @@ def fill_ghosts(q, setup: Setup):
             part = [M.take(r, sel) for r in src_rows]
             if kind == 0:
                 out = part
+            elif kind == 4 and vortex is not None:
+                n = (st.normal[0, sel], st.normal[1, sel])
+                (xc, yc), mid = vortex[1], st.midpoint[:, sel]
+                ref = P.point_vortex_state(mid[0] - xc, mid[1] - yc, setup.q_inf, setup.gamma, vortex[0])
+                out = P.characteristic(part, n, ref, setup.gamma)
             else:
                 n = (st.normal[0, sel], st.normal[1, sel])
                 out = P.bc_state(int(kind), part, n, setup.q_inf, setup.gamma)

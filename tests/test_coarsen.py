@@ -52,3 +52,37 @@ def test_odd_cell_counts_are_refused():
     case = load_case(CASE, ("case.domain.cells_normal=25",))
     with pytest.raises(CaseError, match="odd"):
         case.grid(coarsen=1)
+
+
+M6 = Path(__file__).resolve().parent / "golden" / "cases" / "onera_m6_sa"
+
+
+def test_3d_coarse_grid_of_the_m6_file_case():
+    """The reference's M6 grid (an O seam and a pole): every other node of
+    each block, every rectangle halved, the same condition kinds, and the
+    coarse atlas assembles (its interfaces and pole still match)."""
+    from jaxrans_amd.mesh.block3d import SIDES_3D
+    from jaxrans_amd.mesh.coarsen import coarsen
+    case = load_case(M6)
+    fine, coarse = case.grid(), case.grid(coarsen=1)
+    assert coarse.cells * 8 == fine.cells
+    assert sorted(set(coarse.chains.kind.tolist())) == sorted(set(fine.chains.kind.tolist()))
+    mesh = case.grid_spec.build(case.directory)
+    for b, h in zip(mesh, coarsen(mesh)):
+        assert np.array_equal(h.nodes, b.nodes[:, ::2, ::2, ::2])
+        for name in SIDES_3D:
+            for rf, rc in zip(b.sides.side(name), h.sides.side(name)):
+                assert rc.start == tuple(x // 2 for x in rf.start) and rc.stop == tuple(x // 2 for x in rf.stop)
+
+
+def test_3d_injection_gives_each_child_its_parents_value():
+    case = load_case(M6)
+    fine, coarse = case.grid(), case.grid(coarsen=1)
+    rng = np.random.default_rng(1)
+    U = rng.standard_normal((6,) + coarse.shape)
+    out = prolong(U, coarse, fine, np.zeros((6,) + fine.shape))
+    for wf, wc in zip(fine.windows, coarse.windows):
+        parent = U[(Ellipsis,) + tuple(wc)]
+        child = out[(Ellipsis,) + tuple(wf)]
+        for d in np.ndindex(2, 2, 2):
+            assert np.array_equal(child[:, d[0]::2, d[1]::2, d[2]::2], parent)

@@ -108,3 +108,35 @@ def test_rejected_steps_follow_the_oracle():
     assert ctx.scalars()[0, 7] == 3
     assert rel(ctx.get_state()[0] - U, s.U - U) < 1e-8
     ctx.close()
+
+
+@pytest.mark.parametrize("which", ["o_euler", "c_sa"])
+def test_point_vortex_far_field_matches_oracle(which):
+    """The point-vortex far field (numerics/boundary.py:181-246): residual
+    and tangent (through the circulation's dependence on the wall pressure)
+    and four Newton steps against the oracle on a lifting state."""
+    print(f"[gpu] paths point vortex {which}", flush=True)
+    if which == "o_euler":
+        atlas, flow = o_atlas(), FlowConditions(mach=0.6, alpha_deg=3.0)
+        num = NumericsConfig(flux="hllc", limiter="van_albada", far_field="characteristic_point_vortex")
+    else:
+        atlas = c_atlas()
+        flow = FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2")
+        num = NumericsConfig(flux="roe", limiter="van_albada_smooth", far_field="characteristic_point_vortex")
+    _check(atlas, flow, num)
+    cfg = ImplicitConfig(krylov_subspace=12, krylov_restarts=2)
+    st = OR.setup_for(atlas, flow, num)
+    U = perturbed(OR.U0_of(st), 1e-2)
+    s = OS.NewtonState(U, cfg.cfl)
+    norms = []
+    for _ in range(4):
+        s, n, _info = OS.newton_step(s, st, cfg)
+        norms.append(n)
+    ctx = context_for(atlas, flow, num, implicit=cfg)
+    ctx.set_freestream(st.q_inf, None if flow.inviscid else flow.mu_inf)
+    ctx.set_state(U)
+    ctx.newton_init(cfg.cfl)
+    got = ctx.newton_block(4)[:, 0]
+    assert np.allclose(got, norms, rtol=1e-9, atol=0.0), (got, norms)
+    assert rel(ctx.get_state()[0] - U, s.U - U) < 1e-8
+    ctx.close()

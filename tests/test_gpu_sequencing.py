@@ -49,3 +49,21 @@ def test_result_write_files(tmp_path):
         assert int(f["n_blocks"]) == 1
         assert np.array_equal(f["U_0"], result.grid.unpack(result.U)[0])
         assert f["x_node_0"].shape == (f["U_0"].shape[1] + 1, f["U_0"].shape[2] + 1)
+
+
+def test_sequenced_3d_run_converges_to_the_unsequenced_forces():
+    """The M6 file case (SA, a pole and an O seam) with one coarse level:
+    the coarse 3D grid runs first and its injected solution starts the fine
+    run, which converges to the forces of the march from the freestream."""
+    from jaxrans_amd import load_case, run
+    m6 = CASE.parent / "onera_m6_sa"
+    over = ("numerical.convergence.forces_tol=null", "numerical.convergence.tol=1e-9",
+            "numerical.convergence.max_iters=3000")
+    plain = run(load_case(m6, over))
+    seq = run(load_case(m6, over + ("numerical.convergence.grid_sequencing=1",)))
+    print(f"[gpu] 3D sequencing: levels {seq.levels}; fine {seq.history.summary()} vs plain "
+          f"{plain.history.summary()}", flush=True)
+    assert len(seq.levels) == 1 and seq.levels[0]["cells"] * 8 == seq.cells
+    assert seq.history.reason == "converged" and plain.history.reason == "converged"
+    for out in ("cl", "cd", "cm"):
+        assert abs(getattr(seq.forces, out) - getattr(plain.forces, out)) < 1e-7, out

@@ -67,3 +67,31 @@ def test_freestream_is_a_fixed_point_off_the_wall():
     wi, wj = st.atlas.windows[0]
     assert np.abs((U1 - U0)[:, wi, wj][:, :, 6:]).max() < 1e-12
     assert np.abs(U1 - U0).max() > 1e-8   # the wall moves it
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_local_time_step_3d_closed_form(viscous):
+    """A box of dx x dy x dz cells at one state: the six faces' spectral
+    radii and the diffusive limit in closed form."""
+    from oracle import physics3d as P3
+    nx, ny, nz, dx, dy, dz = 3, 4, 2, 0.3, 0.2, 0.5
+    faces = []
+    for a, d in enumerate((dy * dz, dx * dz, dx * dy)):
+        shape = [3, nx, ny, nz]
+        shape[1 + a] += 1
+        f = np.zeros(shape)
+        f[a] = d
+        faces.append(f)
+    flow = FlowConditions(mach=0.4, alpha_deg=0.0, reynolds=100.0 if viscous else None)
+    st = SimpleNamespace(atlas=SimpleNamespace(faces=tuple(faces), volume=np.full((nx, ny, nz), dx * dy * dz), dim=3),
+                         gamma=1.4, viscous=viscous, mu_inf=flow.mu_inf if viscous else None, sutherland=None, sa=False)
+    rho, u, v, w, p = 1.1, 0.3, -0.2, 0.15, 0.8
+    g = st.gamma
+    U = np.stack([np.full((nx, ny, nz), x) for x in P3.conservative([rho, u, v, w, p], g, 3)])
+    a = np.sqrt(g * p / rho)
+    total = 2 * (abs(u) * dy * dz + a * dy * dz) + 2 * (abs(v) * dx * dz + a * dx * dz) + 2 * (abs(w) * dx * dy + a * dx * dy)
+    if viscous:
+        mu = P.viscosity(g * p / rho, st.mu_inf, None)
+        total += 4 * g / P.PR * mu / rho * ((dy * dz) ** 2 + (dx * dz) ** 2 + (dx * dy) ** 2) / (dx * dy * dz)
+    dt = OX.local_time_step3(U, st, 1.5)
+    assert np.allclose(dt, 1.5 * dx * dy * dz / total, rtol=1e-14, atol=0)
