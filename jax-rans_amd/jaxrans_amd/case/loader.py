@@ -12,9 +12,11 @@ from __future__ import annotations
 
 import copy
 import json
+import math
 from dataclasses import dataclass, field
 from pathlib import Path
 
+import numpy as np
 
 from ..core import constants
 from ..core.config import FlowConditions, ImplicitConfig, NumericsConfig
@@ -23,7 +25,7 @@ from ..mesh.assemble import assemble
 from ..mesh.assemble3d import PatchRule, is_3d, load_grid_3d
 from ..mesh.atlas import Atlas, assign_conditions, build_atlas, patches
 from ..mesh.atlas3d import build_atlas_3d
-from ..mesh.block import folded
+from ..mesh.block import Grid, folded
 from ..mesh.coarsen import coarsen as coarsen_mesh
 from ..mesh.generate import c_grid, o_grid
 from . import schema
@@ -141,9 +143,15 @@ class Case:
     def grid(self, coarsen: int = 0) -> Atlas:
         """The case's grid, ``coarsen`` times coarsened by every other node
         (grid sequencing, ``src/jaxrans/mesh/coarsen.py:58-72``)."""
-        if self.flow.swept:
-            raise CaseError("'freestream.sweep_deg' (2.5D) is not built by this implementation")
         mesh = self.grid_spec.build(self.directory)
+        if self.flow.swept:
+            # the 2D grid of an infinite swept wing: a streamwise section is
+            # meshed normal to the sweep line (x scaled by cos(sweep),
+            # src/jaxrans/case/loader.py:60-64); run() extrudes it for the device
+            if mesh[0].dim != 2:
+                raise CaseError("'freestream.sweep_deg' needs a 2D grid (an infinite swept wing)")
+            if self.grid_spec.section == "streamwise":
+                mesh = scale_x(mesh, math.cos(self.flow.sweep_rad))
         kinds = resolve_conditions(self.conditions, patches(mesh), self.flow)
         mesh = assign_conditions(mesh, kinds)
         if coarsen:
@@ -159,6 +167,12 @@ class Case:
         if any(folded(b) for b in mesh):
             raise CaseError("the grid has folded cells")
         return build_atlas(mesh, self.numerics.n_ghost)
+
+
+def scale_x(mesh, factor: float) -> tuple:
+    """Every 2D block with ``x`` scaled by ``factor`` about ``x = 0`` and
+    ``y`` kept (the same thickness on a shorter chord)."""
+    return tuple(Grid(np.asarray(b.x_node) * factor, np.asarray(b.y_node), b.sides) for b in mesh)
 
 
 class _Document:

@@ -120,3 +120,35 @@ def surface(ctx, case_index: int = 0) -> dict:
         blocks, running = wall_running_indices(ctx.atlas)
         return surface_3d(loads, ctx.atlas, q_inf, ctx.viscous, blocks, running)
     return surface_2d(loads, ctx.atlas, q_inf, ctx.viscous)
+
+
+def swept_surface(ctx, atlas2, sweep_deg: float) -> dict:
+    """Wall distributions of a swept case (the reference's swept columns,
+    ``src/jaxrans/post/forces.py:203-226``) from a context on its extruded
+    canvas: per wall face of the 2D section (the faces of plane ``k = 0``),
+    ``x``, ``x_streamwise`` (``x / cos(sweep)``), ``y``, ``body``, ``cp``
+    and, viscous, ``cf`` (in the section's plane, along the downstream
+    tangent) and ``cf_z`` (along the sweep line), all over the whole
+    freestream's dynamic pressure."""
+    loads = ctx.wall_loads()[0]
+    at3 = ctx.atlas
+    w3, w2 = at3.walls, atlas2.walls
+    first = w3.mid[:, 2] < 0.5 * (at3.mesh[0].nodes[2, 0, 0, 1] + at3.mesh[0].nodes[2, 0, 0, 0]) + 1e-12
+    order = loop_order(w2)
+    mid2 = w2.mid[order]
+    # the plane-0 face over each 2D wall face: same (x, y) midpoint
+    from scipy.spatial import cKDTree
+    idx3 = np.flatnonzero(first)
+    _, near = cKDTree(w3.mid[idx3, :2]).query(mid2)
+    pick = idx3[near]
+    q = dynamic_pressure(ctx.freestream[0], 3)
+    out = {"x": mid2[:, 0], "x_streamwise": mid2[:, 0] / np.cos(np.deg2rad(sweep_deg)), "y": mid2[:, 1],
+           "body": w2.body[order].astype(float), "cp": loads[0, pick] / q}
+    if ctx.viscous:
+        S = w_face_vectors(at3)[:, pick] * w3.into_fluid[pick]
+        length = np.hypot(S[0], S[1])
+        tx, ty = S[1] / length, -S[0] / length
+        sign = np.where(tx >= 0.0, 1.0, -1.0)
+        out["cf"] = sign * (loads[1, pick] * tx + loads[2, pick] * ty) / length / q
+        out["cf_z"] = loads[3, pick] / length / q
+    return out

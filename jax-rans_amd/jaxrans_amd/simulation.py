@@ -181,15 +181,15 @@ class Result:
         ``y``, ``body``, ``cp`` [, ``cf``]; 3D ``block``, ``i``, ``k``, ``x``,
         ``y``, ``z``, ``cp`` [, ``cf_x``, ``cf_y``, ``cf_z``].  The loads are
         evaluated on the device at ``U``."""
-        from .post.surface import surface
-        ctx = make_context(self.case, self.grid)
+        from .post.surface import surface, swept_surface
+        dev = open_device(self.case, self.grid)
         try:
-            q, mu = freestream_of(self.case.flow, self.grid.dim)
-            ctx.set_freestream(q, mu)
-            ctx.set_state(np.asarray(self.U)[None])
-            return surface(ctx, 0)
+            dev.ctx.set_state(dev.to_device(np.asarray(self.U), self.grid)[None])
+            if dev.swept:
+                return swept_surface(dev.ctx, self.grid, self.case.flow.sweep_deg)
+            return surface(dev.ctx, 0)
         finally:
-            ctx.close()
+            dev.ctx.close()
 
     def write(self, directory=None, extra: dict | None = None):
         """Metadata, residual and lift histories and fields under the
@@ -206,8 +206,8 @@ def _check_case(case: Case, dim: int | None = None) -> None:
     if case.numerics.solver == "explicit":
         if case.rk_scheme not in RK_SCHEMES:
             raise CaseError(f"unknown RK scheme {case.rk_scheme!r}; available: {sorted(RK_SCHEMES)}")
-    if case.flow.swept:
-        raise CaseError("swept (2.5D) flow is not built by this implementation")
+    if case.flow.swept and case.numerics.far_field != "characteristic":
+        raise CaseError("the point-vortex far field with a swept flow is not built by this implementation")
 
 
 def make_context(case: Case, grid: Atlas, n_cases: int = 1) -> Context:
@@ -216,6 +216,74 @@ def make_context(case: Case, grid: Atlas, n_cases: int = 1) -> Context:
 
 def freestream_of(flow: FlowConditions, dim: int = 2) -> tuple[np.ndarray, float | None]:
     return q_infinity(flow, dim), (None if flow.inviscid else flow.mu_inf)
+
+
+@dataclass
+class _Device:
+    """What a march runs on the device: the context, the canvas it holds and
+    the maps between that canvas and the case's own.  A swept case's canvas
+    is its 2D section extruded two periodic cells deep (``mesh.swept``); its
+    state goes back to the reference's 2D layout with three velocity
+    components."""
+
+    ctx: Context
+    atlas: object
+    q: np.ndarray
+    mu: float | None
+    swept: bool
+    gamma: float
+
+    def to_device(self, U: np.ndarray, grid) -> np.ndarray:
+        if not self.swept:
+            return U
+        from .mesh.swept import section_to_planes
+        return section_to_planes(U, grid, self.atlas, uniform_canvas(self.q, self.atlas.shape, self.gamma))
+
+    def from_device(self, U: np.ndarray, grid) -> np.ndarray:
+        if not self.swept:
+            return U
+        from .mesh.swept import planes_to_section
+        return planes_to_section(U, self.atlas, grid)
+
+    def forces(self, case: Case) -> Forces:
+        if not self.swept:
+            return _forces(force_rows(self.ctx, case)[0], self.atlas.dim)
+        from .mesh.swept import swept_coefficients
+        spec = case.grid_spec
+        c = swept_coefficients(self.ctx.wall_loads()[0], self.atlas, self.q, case.flow.sweep_deg,
+                               spec.reference_length, tuple(spec.moment_centre), self.ctx.viscous)
+        f = c["force"]
+        return Forces(c["cl"], c["cd"], c["cm"], float(f[0]), float(f[1]), c["cd_friction"], fz=float(f[2]),
+                      cy=c["cy"])
+
+
+def swept_q(flow: FlowConditions) -> np.ndarray:
+    """A swept case's freestream primitives ``(rho, u, v, w, p[, nu ratio])``."""
+    from .mesh.swept import swept_freestream
+    q = swept_freestream(flow.mach, flow.alpha_deg, flow.sweep_deg, flow.gamma)
+    return np.concatenate([q, [flow.nu_tilde_ratio]]) if flow.turbulence is not None else q
+
+
+def section_state(q: np.ndarray, grid, gamma: float) -> np.ndarray:
+    """A swept case's uniform state on its 2D canvas (three velocity components)."""
+    from .core.state import conservative
+    u = conservative(np.asarray(q, np.float64), gamma, 3)
+    return np.broadcast_to(u.reshape((-1, 1, 1)), (u.size,) + tuple(grid.shape)).copy()
+
+
+def open_device(case: Case, grid: Atlas) -> _Device:
+    """The device context of a march of ``case`` on ``grid``, freestream set."""
+    flow = case.flow
+    mu = None if flow.inviscid else flow.mu_inf
+    if flow.swept:
+        from .mesh.swept import lift
+        atlas = lift(grid.mesh, case.numerics.n_ghost)
+        q = swept_q(flow)
+    else:
+        atlas, (q, mu) = grid, freestream_of(flow, grid.dim)
+    ctx = make_context(case, atlas)
+    ctx.set_freestream(q, mu)
+    return _Device(ctx, atlas, q, mu, flow.swept, flow.gamma)
 
 
 def run_to_steady(ctx: Context, *, max_iters: int, tol: float, block_iters: int,
@@ -293,10 +361,9 @@ def run(case: Case, grid: Atlas | None = None, progress: Progress | None = None,
 def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, start=None) -> Result:
     flow, numerics = case.flow, case.numerics
     _check_case(case, grid.dim)   # before the device context exists: a refusal leaves nothing open
-    ctx = make_context(case, grid)
-    q, mu = freestream_of(flow, grid.dim)
-    ctx.set_freestream(q, mu)
-    U0 = uniform_canvas(q, grid.shape, flow.gamma)
+    dev = open_device(case, grid)
+    ctx = dev.ctx
+    U0 = section_state(dev.q, grid, flow.gamma) if flow.swept else uniform_canvas(dev.q, grid.shape, flow.gamma)
     if initial is not None:
         initial = np.asarray(initial, np.float64)
         if initial.shape != U0.shape:
@@ -304,7 +371,7 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, 
         U0 = np.where(grid.interior[None], initial, U0)
     elif start is not None:
         U0 = prolong(start[0], start[1], grid, U0)
-    ctx.set_state(U0)
+    ctx.set_state(dev.to_device(U0, grid))
     block = None
     if numerics.solver == "explicit":   # solvers/explicit.py:48-71, CFL numerics.cfl
         ctx.newton_init(numerics.cfl)
@@ -318,7 +385,7 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, 
         finite = residuals[np.isfinite(residuals)]
         if finite.size:
             peak[0] = max(peak[0], float(finite.max()))
-        f = _forces(force_rows(ctx, case)[0], grid.dim)
+        f = dev.forces(case)
         cl_history.append(f.cl)
         coefficients.append((f.cl, f.cd, f.cm))
         if progress is not None:
@@ -330,8 +397,8 @@ def _march(case: Case, grid: Atlas, progress: Progress | None, *, initial=None, 
 
     history = run_to_steady(ctx, max_iters=numerics.max_iters, tol=numerics.tol, block_iters=case.block_iters,
                             on_block=on_block, block=block)
-    U = ctx.get_state()[0]
-    forces = _forces(force_rows(ctx, case)[0], grid.dim)
+    U = dev.from_device(ctx.get_state()[0], grid)
+    forces = dev.forces(case)
     timing = Timing(history.seconds, history.first_block_seconds, history.n_iters, case.block_iters, grid.cells)
     result = Result(case=case, grid=grid, U=U, forces=forces, history=history, cl_history=np.asarray(cl_history),
                     timing=timing, extra={"scalars": ctx.scalars()[0]})
@@ -394,6 +461,8 @@ def run_polar(case: Case, alphas, grid: Atlas | None = None, devices=None, progr
     case's own.  One GPU: ``devices`` above 1 is refused."""
     if devices is not None and int(devices) > 1:
         raise CaseError("this implementation runs a polar on one GPU; splitting it over devices is not built")
+    if case.flow.swept:
+        raise CaseError("a swept (2.5D) polar is not built by this implementation; run() each angle")
     grid = case.grid() if grid is None else grid
     _check_case(case, grid.dim)
     alphas = np.atleast_1d(np.asarray(alphas, np.float64))
