@@ -40,7 +40,7 @@ import numpy as np  # noqa: E402
 
 CASE = ROOT / "cases" / "kt10_c_sa"
 HBM_PEAK_GBS = 8000.0
-CPU_STEPS = 4          # oracle Newton steps timed for the CPU baseline (about 12 s)
+CPU_STEPS = 2          # Newton steps of the C++ oracle timed for the CPU baseline (about 10-20 s)
 
 
 def cr_factor_values(lengths) -> tuple[int, int]:
@@ -210,36 +210,56 @@ def measured_traffic(kernel: str, workload: str | None):
 
 
 def cpu_baseline() -> dict:
-    """CPU_STEPS oracle Newton steps of the same case, timed in a one-thread child process."""
+    """The oracle's C++/OpenMP restatement (oracle/cpu) on the same 2D SA case,
+    timed on the host's cores in a child process per component, as the
+    reference's scripts/benchmark.py:102-113 does (residual, tangent, line
+    preconditioner set-up and apply, Newton step; ns per cell), on the
+    benchmark input (the freestream times 1 + 1e-3 sin(k)), and
+    CPU_STEPS Newton steps from that state for the throughput.  The
+    polar's cases are independent, so the CPU's batch throughput is this
+    per-case rate."""
     code = r"""
 import os, sys, time, json
 sys.path[:0] = [sys.argv[1] + '/jax-rans_amd', sys.argv[1]]
 import numpy as np
 from jaxrans_amd import load_case
-from oracle import residual as OR, solver as OS
+from oracle import residual as OR, cpu as OC
 case = load_case(sys.argv[1] + '/cases/kt10_c_sa')
 at = case.grid()
 st = OR.setup_for(at, case.flow, case.numerics)
-s = OS.NewtonState(OR.U0_of(st), case.implicit.cfl)
-np.seterr(all='ignore')
+c = OC.Case(st, case.implicit)
+U0 = OR.U0_of(st)
+U = U0 * (1.0 + 1e-3 * np.sin(np.arange(U0.size)).reshape(U0.shape))
+dU = 1e-3 * np.cos(np.arange(U0.size)).reshape(U0.shape) * np.abs(U0)
+def tm(f, n=3):
+    f()
+    t = time.perf_counter()
+    for _ in range(n):
+        f()
+    return (time.perf_counter() - t) / n
+per = {"residual": tm(lambda: c.residual(U)), "jvp": tm(lambda: c.jvp(U, dU)),
+       "precond_setup": tm(lambda: c.precond_setup(U, 10.0)), "precond_apply": tm(lambda: c.precond_apply(dU))}
 steps = int(sys.argv[2])
+Uc, state = U.copy(), np.array([case.implicit.cfl, np.inf])
 t = time.perf_counter()
 for _ in range(steps):
-    s, n, info = OS.newton_step(s, st, case.implicit)
+    c.newton_step(Uc, state)
 dt = time.perf_counter() - t
-print(json.dumps({"seconds": dt, "cells": at.cells, "steps": steps}))
+print(json.dumps({"seconds": dt, "cells": at.cells, "steps": steps, "threads": OC.threads(),
+                  "ns_per_cell": {k: v / at.cells * 1e9 for k, v in per.items()} | {"newton_step": dt / steps / at.cells * 1e9}}))
 """
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-    out = subprocess.run([sys.executable, "-c", code, str(ROOT), str(CPU_STEPS)], env=env, capture_output=True,
-                         text=True, timeout=900)
+    out = subprocess.run([sys.executable, "-c", code, str(ROOT), str(CPU_STEPS)], capture_output=True, text=True,
+                         timeout=900)
     if out.returncode != 0:
-        return {"value": None, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-                "sample": "oracle Newton step failed: " + out.stderr[-300:]}
+        return {"value": None, "unit": "Mcell-steps/s", "cores": os.cpu_count(), "kind": "port",
+                "sample": "oracle C++ Newton step failed: " + out.stderr[-300:]}
     r = json.loads(out.stdout.strip().splitlines()[-1])
-    return {"value": r["cells"] * r["steps"] / r["seconds"] / 1e6, "unit": "Mcell-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{r['steps']} FP64 Newton steps (GMRES(30) x 2, line preconditioner) of the NumPy oracle on "
-                      f"the same {r['cells']}-cell case from the freestream, {r['seconds']:.1f} s, one thread"}
+    return {"value": r["cells"] * r["steps"] / r["seconds"] / 1e6, "unit": "Mcell-steps/s", "cores": r["threads"],
+            "kind": "port", "ns_per_cell": {k: round(v, 1) for k, v in r["ns_per_cell"].items()},
+            "sample": f"{r['steps']} FP64 Newton steps (GMRES(30) x 2, block-Thomas line preconditioner) of the "
+                      f"oracle's C++/OpenMP restatement on one {r['cells']}-cell case of the workload (the polar's "
+                      f"cases are independent), {r['seconds']:.1f} s on {r['threads']} threads; per-component "
+                      f"ns per cell as scripts/benchmark.py:102-113"}
 
 
 def polar_flows(flow, n: int):
@@ -438,6 +458,9 @@ def main(argv=None) -> int:
     ap.add_argument("--polar", type=int, metavar="B", default=0,
                     help="B cases of the batched polar instead of 32 (alpha uniform on [-4, 12] deg, Mach uniform "
                          "on [0.3, 0.78], Re 6e6, numpy default_rng(2)), every launch shared by the batch")
+    ap.add_argument("--steps-only", action="store_true",
+                    help="only the warmup and timed Newton steps (no probe, path timings, mixed run or CPU "
+                         "baseline): the process to run under a profiler's counters")
     ap.add_argument("--single", action="store_true",
                     help="instead of the batched polar: the 512 x 128 case alone (M 0.73, alpha 2.5, Re 6.5e6)")
     ap.add_argument("--converge", metavar="CASE_DIR", nargs="?", const=str(CASE), default=None,
@@ -517,6 +540,11 @@ def main(argv=None) -> int:
           f"Newton steps, FP64", file=sys.stderr, flush=True)
     ctx, dt, hist = run_steps(case, atlas, "fp64", args.steps, args.warmup, B)
     ms_per_step = 1e3 * dt / args.steps
+    if args.steps_only:
+        print(json.dumps({"workload": "polar" + str(B) if B else "single", "steps": args.steps,
+                          "ms_per_step": round(ms_per_step, 3)}), flush=True)
+        ctx.close()
+        return 0
     value = cells * args.steps / dt / 1e6
     m, restarts = case.implicit.krylov_subspace, case.implicit.krylov_restarts
     F64 = ctx.forces()                   # after warmup + steps, as the mixed run below

@@ -20,6 +20,10 @@ reference repository):
 * ``solver``   -- line preconditioner (Thomas and cyclic reduction), GMRES,
   Newton step, forces (``core/linalg.py``, ``solvers/preconditioners.py``,
   ``solvers/krylov.py``, ``solvers/implicit.py``, ``post/forces.py``).
+* ``cpu``      -- ctypes binding of ``cpu/newton2d.cpp``, a C++/OpenMP
+  restatement of the 2D residual, tangent, wave speeds, block-Thomas line
+  preconditioner, GMRES and Newton step above (the benchmark's CPU
+  baseline; pinned against this package by ``tests/test_oracle_cpu.py``).
 
 Pinning (each checked in ``tests/test_oracle_*.py``):
 
