@@ -1,7 +1,12 @@
-"""Writes tests/golden/euler_o48_converged.json: the oracle's converged forces
-on the small Euler case tests/golden/cases/euler_o48 (FP64, to a 1e-10 drop of
-the residual from its peak), the reference point of the device run() parity
-test.  Run from the repository root:  python tests/golden/make_converged.py
+"""Writes the oracle's converged forces of a case (FP64, to the case's own
+residual drop ``convergence.tol`` from its peak), the reference point of the
+device run() parity tests:
+
+    python tests/golden/make_converged.py                        # the small Euler case
+    python tests/golden/make_converged.py cases/naca0012_o_euler  # the 128 x 64 Euler O-grid
+
+Output: ``tests/golden/<case name>_converged.json``.  Run from the
+repository root.
 """
 
 from __future__ import annotations
@@ -18,19 +23,19 @@ from oracle import residual as OR  # noqa: E402
 from oracle import solver as OS  # noqa: E402
 
 
-def main() -> None:
-    case = load_case(ROOT / "tests/golden/cases/euler_o48")
+def main(case_dir: str = "tests/golden/cases/euler_o48") -> None:
+    case = load_case(ROOT / case_dir)
     atlas = case.grid()
     setup = OR.setup_for(atlas, case.flow, case.numerics)
     s, hist, f = OS.run_oracle(setup, case.implicit, tol=case.numerics.tol, max_iters=case.numerics.max_iters,
                                block_iters=case.block_iters)
-    out = {"case": "tests/golden/cases/euler_o48", "cells": atlas.cells, "iterations": int(hist.size),
+    out = {"case": case_dir, "cells": atlas.cells, "iterations": int(hist.size),
            "residual_first": float(hist[0]), "residual_peak": float(hist.max()), "residual_last": float(hist[-1]),
            "cl": float(f["cl"]), "cd": float(f["cd"]), "cm": float(f["cm"])}
-    path = ROOT / "tests/golden/euler_o48_converged.json"
+    path = ROOT / "tests/golden" / f"{case.name}_converged.json"
     path.write_text(json.dumps(out, indent=2) + "\n")
     print(json.dumps(out, indent=2))
 
 
 if __name__ == "__main__":
-    main()
+    main(*sys.argv[1:])
