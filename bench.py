@@ -78,10 +78,11 @@ def algorithmic_bytes(name: str, ctx, m: int, restarts: int) -> float:
         # per cycle (launches per step = cycles x (m + 1)):
         # m - 1 applies read w, write V_j and z; one reads V_0, writes z;
         # each of these m also reads U and writes the tangent's dQ (ghost cells
-        # through their nv x nv fill Jacobians); the cycle end reads T and X, writes X
-        cr = ctx.config.line_solver != "thomas" and lengths.max() <= 512
+        # through their nv x nv fill Jacobians; not with the point-vortex far
+        # field, whose tangent runs the fill); the cycle end reads T and X, writes X
+        fused = ctx.far_field == "characteristic"
         ghosts = len(at.stages[0].dest) + len(at.stages[1].dest)
-        tangent_in = (2 * N + ghosts * nv * nv * 8) if cr else 0
+        tangent_in = (2 * N + ghosts * nv * nv * 8) if fused else 0
         per_cycle = (m - 1) * 3 * N + 2 * N + m * tangent_in + 3 * N
         return ctx.n_cases * (factor + per_cycle / (m + 1))
     if name == "tangent_faces":
@@ -102,6 +103,8 @@ def algorithmic_bytes(name: str, ctx, m: int, restarts: int) -> float:
         return ctx.n_cases * N * (jmean + 1 + 2)          # V_0..V_j, w read and written
     if name == "precond_factor":
         flow, sav = cr_factor_values(lengths)
+        if ctx.config.line_solver == "thomas" or lengths.max() > 512:
+            flow, sav = 48 * lengths.sum(), 3 * lengths.sum()
         slots = lengths.sum()
         return ctx.n_cases * (slots * 36 * 8 + (flow + (sav if sa else 0)) * fb)
     raise KeyError(name)
@@ -467,7 +470,8 @@ def main(argv=None) -> int:
                     help="instead of the step benchmark: run CASE_DIR (default: the benchmark case) to "
                          "convergence in FP64 and mixed precision and print iterations and force differences")
     ap.add_argument("--set", action="append", default=[], metavar="FILE.SECTION.KEY=VALUE",
-                    help="--converge: a case override, as load_case takes it (repeatable)")
+                    help="a case override, as load_case takes it (repeatable), for --converge and the step "
+                         "benchmark, e.g. 'numerical.implicit.line_solver=\"thomas\"'")
     ap.add_argument("--wing", action="store_true",
                     help="instead of the 2D case: Newton steps of the 3D SA wing (four blocks, 1.09M cells)")
     ap.add_argument("--serve", type=int, metavar="N", default=0,
@@ -531,7 +535,7 @@ def main(argv=None) -> int:
         print(json.dumps(line), flush=True)
         return 0
     from jaxrans_amd import load_case
-    case = load_case(CASE)
+    case = load_case(CASE, tuple(args.set))
     atlas = case.grid()
     B = 0 if args.single else (args.polar or 32)
     ncases = max(1, B)

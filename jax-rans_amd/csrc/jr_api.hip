@@ -5,6 +5,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,6 +16,7 @@
 #include "../../include/jrhip.h"
 #include "jr_krylov.h"
 #include "jr_cr.h"
+#include "jr_lines.h"
 #include "jr_3d.h"
 
 using namespace jr;
@@ -77,7 +79,7 @@ struct jr_ctx {
   Red red{};
   Cg cg{};           // CGS2 partial sums
   double* gjac = nullptr;   // [n_cases][n_final][nv][nv] linearised ghost fill
-  bool dq_fused = false;    // the line apply writes the tangent input (cyclic reduction, fill linearised)
+  bool dq_fused = false;    // the line apply writes the tangent input (fill linearised)
   // Event probe: when armed, launches of the selected kernels are bracketed
   // by event pairs from a preallocated pool (measurement only).
   unsigned probe_mask = 0;
@@ -94,6 +96,13 @@ struct jr_ctx {
   int device = 0;              // the HIP device current at creation (its stream's device)
   double* loads = nullptr;     // jr_wall_loads: per wall face [case][1 + dim][n_walls]
   double* circ = nullptr;      // point-vortex far field: [case][2] circulation and its tangent
+  // 2D block Thomas apply (k_line_apply): the chains by length, longest first,
+  // in classes launched with one LDS size each
+  struct LineClass {
+    int first, count, lmax, gper;
+  };
+  const int* lorder = nullptr;
+  std::vector<LineClass> lclass;
 };
 
 // FNV-1a over the index plans of a grid (stages, chains, walls, neighbours,
@@ -246,8 +255,8 @@ static void launch_factor(jr_ctx* c, const double* U) {
   hipLaunchKernelGGL((k_blocks<NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl,
                      U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
   if (c->thomas)
-    JR_LAUNCH(PR_FACTOR, (k_thomas<R, NV>), dim3(P.n_lines, c->n_cases), dim3(64), 0, P, (const double*)c->blk,
-              (R*)c->fac, (R*)c->fac_sa);
+    JR_LAUNCH(PR_FACTOR, (k_thomas<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
+              (const int*)c->lorder, (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa);
   else
     JR_LAUNCH(PR_FACTOR, (k_cr_factor<R, NV>), dim3(P.n_lines, c->n_cases), dim3(c->cr_threads), 0, P,
               (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
@@ -257,12 +266,24 @@ template <class R, int NV>
 static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, int check_active,
                          double* vout = nullptr, double* dq_out = nullptr) {
   const Plan& P = c->P;
-  const size_t lds = (size_t)std::max(1, P.max_len) * NV * sizeof(R);
   if (c->thomas) {
-    const unsigned groups = (unsigned)(P.n_lines + (P.n_free + 63) / 64);
-    JR_LAUNCH(PR_APPLY, (k_apply<R, NV>), dim3(groups, c->n_cases), dim3(64), lds, P, (const Ctl*)c->ctl,
-              (const R*)c->fac, (const R*)c->fac_sa, r, z, accumulate, check_active, vout, (const double*)c->cg.p3,
-              c->cg.nblk);
+    // one launch per length class, the cells on no chain with the first; the
+    // probe brackets the whole sequence
+    const int pk = probe_slot(c, PR_APPLY);
+    if (pk >= 0) (void)hipEventRecord(c->probe_ev[2 * pk], c->stream);
+    const unsigned free_groups = (unsigned)((P.n_free + 63) / 64);
+    const size_t ncls = std::max<size_t>(1, c->lclass.size());
+    for (size_t k = 0; k < ncls; ++k) {
+      const jr_ctx::LineClass lc = k < c->lclass.size() ? c->lclass[k] : jr_ctx::LineClass{0, 0, LN_CH, 1};
+      const unsigned groups = (unsigned)((lc.count + lc.gper - 1) / lc.gper) + (k == 0 ? free_groups : 0u);
+      if (groups == 0) continue;
+      auto kern = accumulate ? k_line_apply<R, NV, true> : k_line_apply<R, NV, false>;
+      hipLaunchKernelGGL(kern, dim3(groups, c->n_cases), dim3(64), line_apply_lds<NV>(lc.lmax, lc.gper), c->stream, P,
+                         (const Ctl*)c->ctl, (const int*)c->lorder, lc.first, lc.count, lc.lmax, lc.gper,
+                         (const R*)c->fac, (const R*)c->fac_sa, r, z, check_active, vout, (const double*)c->cg.p3,
+                         c->cg.nblk, dq_out, (const double*)c->U, c->ph.gamma, (const double*)c->gjac);
+    }
+    if (pk >= 0) (void)hipEventRecord(c->probe_ev[2 * pk + 1], c->stream);
   } else {
     const int T = c->cr_threads;
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
@@ -955,8 +976,10 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
         upload(c, &P.gdep_q, dep_q.data(), dep_q.size()))
       return bail(0);
   }
-  if ((size_t)max_len * 5 * sizeof(double) > 60 * 1024) {
-    fail("jr_ctx_create: a line chain longer than " + std::to_string(60 * 1024 / 40) + " cells is not supported");
+  // the longest chain k_line_apply holds in 64 KiB of LDS, one per wavefront
+  if (line_apply_lds<5>((max_len + LN_CH - 1) / LN_CH * LN_CH, 1) > 64 * 1024) {
+    fail("jr_ctx_create: a line chain longer than " +
+         std::to_string(64 * 1024 / line_apply_lds<5>(LN_CH, 1) * LN_CH) + " cells is not supported");
     return bail(0);
   }
   std::vector<int> freec;
@@ -1001,12 +1024,32 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   // row); longer chains are solved by block Thomas, the same exact solve
   c->thomas = num->line_solver == 1 || P.max_len > CR_MAX_LEN;
   c->cr_threads = std::max(64, (P.max_len + 63) / 64 * 64);
+  if (c->thomas) {
+    // k_line_apply classes: chains longest first; a class takes the lengths
+    // in (lmax / 2, lmax], lmax a multiple of 16, and as many chains per
+    // wavefront (4, 2 or 1) as keep its LDS within 64 KiB
+    std::vector<int> ord(g->n_lines);
+    for (int l = 0; l < g->n_lines; ++l) ord[l] = l;
+    auto len = [&](int l) { return (int)(g->line_start[l + 1] - g->line_start[l]); };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return len(a) > len(b); });
+    for (int i = 0; i < g->n_lines;) {
+      const int lmax = std::max(LN_CH, (len(ord[i]) + LN_CH - 1) / LN_CH * LN_CH);
+      int j = i;
+      while (j < g->n_lines && (lmax <= LN_CH || 2 * len(ord[j]) > lmax)) ++j;
+      int gper = 4;
+      while (gper > 1 && line_apply_lds<5>(lmax, gper) > 64 * 1024) gper /= 2;
+      c->lclass.push_back({i, j - i, lmax, gper});
+      i = j;
+    }
+    if (ord.empty()) ord.push_back(0);
+    if (upload(c, &c->lorder, ord.data(), ord.size())) return bail(0);
+  }
   {
     const size_t need = c->nv == 5 ? cr_apply_lds<double, 5>(P.max_len, P.max_dep, true)
                                    : cr_apply_lds<double, 4>(P.max_len, P.max_dep, true);
     // the point-vortex ghosts also move with the circulation, a sum over all
     // walls that no chain's apply can form: the tangent then runs the fill
-    c->dq_fused = !c->thomas && P.n_final > 0 && need <= 60 * 1024 && !P.vortex;
+    c->dq_fused = P.n_final > 0 && !P.vortex && (c->thomas || need <= 60 * 1024);
   }
   c->ktol = num->krylov_tol;
   c->max_rel = num->max_relative_update;
@@ -1034,17 +1077,17 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
   {
     void* p;
-    // per chain slot: factor CR_FAC (+CR_FAC_SA) values (block Thomas needs 48 + 3),
-    // cyclic-reduction workspace 96 (+6 SA)
+    // per chain slot: factor CR_FAC (+CR_FAC_SA) values and the cyclic-reduction
+    // workspace 96 (+6 SA); block Thomas keeps 48 (+3) and needs no workspace
     const size_t slots = (size_t)n_cases * P.n_slots;
-    static_assert(CR_FAC >= 48 && CR_FAC_SA >= 3, "factor storage covers both line solvers");
-    if (dalloc(c, (char**)&p, slots * CR_FAC * rsz)) return bail(0);
+    const bool cr = !c->thomas;
+    if (dalloc(c, (char**)&p, slots * (cr ? CR_FAC : 48) * rsz)) return bail(0);
     c->fac = p;
-    if (dalloc(c, (char**)&p, slots * CR_FAC_SA * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, slots * (cr ? CR_FAC_SA : 3) * rsz)) return bail(0);
     c->fac_sa = p;
-    if (dalloc(c, (char**)&p, slots * 96 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, (cr ? slots * 96 : 1) * rsz)) return bail(0);
     c->ws = p;
-    if (dalloc(c, (char**)&p, slots * 6 * rsz)) return bail(0);
+    if (dalloc(c, (char**)&p, (cr ? slots * 6 : 1) * rsz)) return bail(0);
     c->ws_sa = p;
     if (dalloc(c, &c->blk, (size_t)n_cases * P.n_slots * BLKW)) return bail(0);
   }
@@ -1970,8 +2013,6 @@ int jr_forces3(jr_ctx* c, double area, double length, double cx, double cz, doub
 template <int NV, bool VISC, int MM, class R>
 static void profile_launch(jr_ctx* c, int which) {
   const Plan& P = c->P;
-  const dim3 vg(c->nblk, c->n_cases);
-  const size_t N = c->N, vstride = (size_t)c->n_cases * N;
   switch (which) {
     case 0: launch_residual<NV, VISC>(c, c->U, c->R, 0); break;
     case 1: launch_tangent<NV, VISC>(c, c->U, c->Z, c->W, c->exact ? 0 : 1, c->vdt, 0); break;

@@ -425,107 +425,69 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
   }
 }
 
-// Block Thomas factor (linalg.py:80-100): one wavefront per chain.  The
-// wavefront streams the chain's blocks through LDS in chunks of CH slots
-// (the next chunk's loads in flight while lane 0 runs the recurrence on the
-// current one) and writes the factor back coalesced:
+// Block Thomas factor (linalg.py:80-100), one lane per chain: a wavefront
+// takes 64 chains, order[t * 64 .. t * 64 + 63] (chains by length, longest
+// first, so a wavefront's chains end together), and runs the recurrence of
+// each in its own lane, the next slot's blocks loaded while the current one is
+// factored.  Per slot it writes
 //   fac[slot][32] = L (16), P^-1 (16);  fac[n_slots*32 + slot*16] = C (16);
 //   fac_sa[slot][3] = l, 1/p, c  (the scalar SA chain).
-constexpr int CH = 32;
-
-template <class R>
-__device__ __forceinline__ void invert4_gj(R A[4][4], R X[4][4]) { invert4<R>(A, X); }
-
 template <class R, int NV>
-__global__ void __launch_bounds__(64) k_thomas(Plan P, const double* __restrict__ blk, R* __restrict__ fac,
-                                               R* __restrict__ fac_sa) {
-  __shared__ double sb[CH * BLKW];
-  __shared__ R so[CH * 51];
-  const int lane = threadIdx.x, chain = blockIdx.x, cs = blockIdx.y;
-  if (chain >= P.n_lines) return;
-  const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
-  const int n = s1 - s0;
-  const double* B = blk + (size_t)cs * P.n_slots * BLKW;
+__global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ order, const double* __restrict__ blk,
+                                               R* __restrict__ fac, R* __restrict__ fac_sa) {
+  const int t = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
+  if (t >= P.n_lines) return;
+  const int chain = order[t];
+  const int s0 = P.l_start[chain], n = P.l_start[chain + 1] - s0;
+  const double* B = blk + ((size_t)cs * P.n_slots + s0) * BLKW;
   R* F = fac + (size_t)cs * P.n_slots * 48;
   R* Fc = F + (size_t)P.n_slots * 32;
   R* Fs = fac_sa + (size_t)cs * P.n_slots * 3;
-  constexpr int PER = (CH * BLKW + 63) / 64;
-  double pf[PER];
-  auto fetch = [&](int c) {
-    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
-    const double* src = B + (size_t)sc * BLKW;
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      int idx = t * 64 + lane;
-      pf[t] = idx < m * BLKW ? src[idx] : 0.0;
-    }
-  };
+  constexpr int NB = NV > 4 ? BLKW : 33;   // block values read per slot
+  double nb[NB];
+  if (n > 0)
+    for (int e = 0; e < NB; ++e) nb[e] = B[e];
   R Cp[4][4];
   for (int r = 0; r < 4; ++r)
     for (int q = 0; q < 4; ++q) Cp[r][q] = R(0);
   R cps = R(0);
-  const int nch = (n + CH - 1) / CH;
-  if (nch > 0) fetch(0);
-  for (int c = 0; c < nch; ++c) {
-    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      int idx = t * 64 + lane;
-      if (idx < CH * BLKW) sb[idx] = pf[t];
-    }
-    __syncthreads();
-    if (c + 1 < nch) fetch(c + 1);
-    if (lane == 0) {
-      for (int k = 0; k < m; ++k) {
-        const double* b = sb + k * BLKW;
-        const R d = (R)b[0];
-        R Pm[4][4], Pi[4][4], Cn[4][4];
-        for (int r = 0; r < 4; ++r)
-          for (int q = 0; q < 4; ++q) {
-            R acc = R(0);
-            for (int t = 0; t < 4; ++t) acc += (R)b[1 + r * 4 + t] * Cp[t][q];
-            Pm[r][q] = (r == q ? d : R(0)) - acc;
-          }
-        invert4<R>(Pm, Pi);
-        for (int r = 0; r < 4; ++r)
-          for (int q = 0; q < 4; ++q) {
-            R acc = R(0);
-            for (int t = 0; t < 4; ++t) acc += Pi[r][t] * (R)b[17 + t * 4 + q];
-            Cn[r][q] = acc;
-          }
-        R* o = so + k * 51;
-        for (int r = 0; r < 4; ++r)
-          for (int q = 0; q < 4; ++q) {
-            o[r * 4 + q] = (R)b[1 + r * 4 + q];
-            o[16 + r * 4 + q] = Pi[r][q];
-            o[32 + r * 4 + q] = Cn[r][q];
-            Cp[r][q] = Cn[r][q];
-          }
-        if constexpr (NV > 4) {
-          R lo = (R)b[33], ds = (R)b[34], up = (R)b[35];
-          R pinv = R(1) / (ds - lo * cps);
-          R cn = pinv * up;
-          o[48] = lo;
-          o[49] = pinv;
-          o[50] = cn;
-          cps = cn;
-        }
+  for (int k = 0; k < n; ++k) {
+    double b[NB];
+    for (int e = 0; e < NB; ++e) b[e] = nb[e];
+    if (k + 1 < n)
+      for (int e = 0; e < NB; ++e) nb[e] = B[(size_t)(k + 1) * BLKW + e];
+    const R d = (R)b[0];
+    R Pm[4][4], Pi[4][4];
+    for (int r = 0; r < 4; ++r)
+      for (int q = 0; q < 4; ++q) {
+        R acc = R(0);
+        for (int u = 0; u < 4; ++u) acc += (R)b[1 + r * 4 + u] * Cp[u][q];
+        Pm[r][q] = (r == q ? d : R(0)) - acc;
       }
+    invert4<R>(Pm, Pi);
+    const size_t s = (size_t)s0 + k;
+    for (int r = 0; r < 4; ++r)
+      for (int q = 0; q < 4; ++q) {
+        R acc = R(0);
+        for (int u = 0; u < 4; ++u) acc += Pi[r][u] * (R)b[17 + u * 4 + q];
+        Cp[r][q] = acc;
+      }
+    for (int e = 0; e < 16; ++e) {
+      F[s * 32 + e] = (R)b[1 + e];
+      F[s * 32 + 16 + e] = Pi[e / 4][e % 4];
+      Fc[s * 16 + e] = Cp[e / 4][e % 4];
     }
-    __syncthreads();
-    for (int idx = lane; idx < m * 32; idx += 64) F[(size_t)sc * 32 + idx] = so[(idx >> 5) * 51 + (idx & 31)];
-    for (int idx = lane; idx < m * 16; idx += 64) Fc[(size_t)sc * 16 + idx] = so[(idx >> 4) * 51 + 32 + (idx & 15)];
-    if constexpr (NV > 4)
-      for (int idx = lane; idx < m * 3; idx += 64) Fs[(size_t)sc * 3 + idx] = so[(idx / 3) * 51 + 48 + idx % 3];
-    __syncthreads();
+    if constexpr (NV > 4) {
+      const R lo = (R)b[33], ds = (R)b[34], up = (R)b[35];
+      const R pinv = R(1) / (ds - lo * cps);
+      cps = pinv * up;
+      Fs[s * 3] = lo;
+      Fs[s * 3 + 1] = pinv;
+      Fs[s * 3 + 2] = cps;
+    }
   }
 }
 
-// z <- M^-1 r (accumulate: z += M^-1 r): one wavefront per chain, factor and
-// right-hand side streamed through LDS, lane 0 runs the forward and backward
-// substitutions (linalg.py:103-132).  Workgroups past the chains copy the cells
-// on no chain (identity there, preconditioners.py:110-115), 64 per workgroup.
-// Dynamic LDS: max_len * NV values of R (the forward sweep's d).
 // Sum of n partials in an order independent of the launch shape: lane l adds
 // p[l], p[l + 64], ... in turn, then the xor butterfly leaves the same total in
 // every lane.  Called by one whole wave.
@@ -597,166 +559,6 @@ __device__ __forceinline__ void write_tangent_q(const Plan& P, const double* u, 
 // nblk values) when the apply normalises; called by every thread.
 __device__ __forceinline__ double apply_den(const double* __restrict__ p3, int nblk, const double* vout) {
   return vout ? fmax(column_norm(p3, nblk), 2.2250738585072014e-308) : 1.0;
-}
-
-template <class R, int NV>
-__global__ void __launch_bounds__(64) k_apply(Plan P, const Ctl* __restrict__ ctl, const R* __restrict__ fac,
-                                              const R* __restrict__ fac_sa, const double* __restrict__ r,
-                                              double* __restrict__ z, int accumulate, int check_active,
-                                              double* __restrict__ vout, const double* __restrict__ p3, int nblk) {
-  extern __shared__ unsigned char dyn[];
-  R* dbuf = reinterpret_cast<R*>(dyn);
-  __shared__ R fb[CH * 34];       // forward: L, P^-1, l, 1/p per slot; backward: C, c
-  __shared__ R rb[CH * NV];       // right-hand side, then the solution of the chunk
-  __shared__ int cb[CH];
-  const int lane = threadIdx.x, chain = blockIdx.x, cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
-  const int nc = P.nc;
-  const size_t base = (size_t)cs * NV * nc;
-  const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
-  if (chain >= P.n_lines) {
-    const int k = (chain - P.n_lines) * 64 + lane;
-    if (k >= P.n_free) return;
-    const int cell = P.free_cell[k];
-    for (int v = 0; v < NV; ++v) {
-      size_t o = base + (size_t)v * nc + cell;
-      const double x = apply_input(r, vout, o, den);
-      z[o] = accumulate ? z[o] + x : x;
-    }
-    return;
-  }
-  const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
-  const int n = s1 - s0;
-  const R* F = fac + (size_t)cs * P.n_slots * 48;
-  const R* Fc = F + (size_t)P.n_slots * 32;
-  const R* Fs = NV > 4 ? fac_sa + (size_t)cs * P.n_slots * 3 : nullptr;
-  const int nch = (n + CH - 1) / CH;
-  constexpr int PF = (CH * 34 + 63) / 64, PR = (CH * NV + 63) / 64;
-  R pf[PF];
-  R pr[PR];
-  // ---- forward sweep, chunks in order
-  auto fetch_fwd = [&](int c) {
-    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
-#pragma unroll
-    for (int t = 0; t < PF; ++t) {
-      int idx = t * 64 + lane;
-      R v = R(0);
-      if (idx < m * 32) v = F[(size_t)sc * 32 + idx];
-      else if (NV > 4 && idx >= CH * 32 && idx - CH * 32 < m * 2) {
-        int j = idx - CH * 32;
-        v = Fs[(size_t)(sc + (j >> 1)) * 3 + (j & 1)];
-      }
-      pf[t] = v;
-    }
-#pragma unroll
-    for (int t = 0; t < PR; ++t) {
-      int idx = t * 64 + lane;
-      R v = R(0);
-      if (idx < m * NV) {
-        int k = idx / NV, var = idx - k * NV;
-        v = (R)apply_input(r, vout, base + (size_t)var * nc + P.l_cell[sc + k], den);
-      }
-      pr[t] = v;
-    }
-  };
-  R dp[4] = {R(0), R(0), R(0), R(0)};
-  R dps = R(0);
-  if (nch > 0) fetch_fwd(0);
-  for (int c = 0; c < nch; ++c) {
-    const int sc = s0 + c * CH, m = min(CH, s1 - sc);
-#pragma unroll
-    for (int t = 0; t < PF; ++t) {
-      int idx = t * 64 + lane;
-      if (idx < CH * 34) fb[idx] = pf[t];
-    }
-#pragma unroll
-    for (int t = 0; t < PR; ++t) {
-      int idx = t * 64 + lane;
-      if (idx < CH * NV) rb[idx] = pr[t];
-    }
-    __syncthreads();
-    if (c + 1 < nch) fetch_fwd(c + 1);
-    if (lane == 0) {
-      for (int k = 0; k < m; ++k) {
-        const R* o = fb + k * 32;
-        R tmp[4];
-        for (int i = 0; i < 4; ++i) {
-          R acc = rb[k * NV + i];
-          for (int q = 0; q < 4; ++q) acc -= o[i * 4 + q] * dp[q];
-          tmp[i] = acc;
-        }
-        for (int i = 0; i < 4; ++i) {
-          R acc = R(0);
-          for (int q = 0; q < 4; ++q) acc += o[16 + i * 4 + q] * tmp[q];
-          dp[i] = acc;
-        }
-        R* dv = dbuf + (size_t)(c * CH + k) * NV;
-        for (int i = 0; i < 4; ++i) dv[i] = dp[i];
-        if constexpr (NV > 4) {
-          const R* os = fb + CH * 32 + k * 2;
-          dps = os[1] * (rb[k * NV + 4] - os[0] * dps);
-          dv[4] = dps;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // ---- backward sweep, chunks in reverse
-  constexpr int PB = (CH * 17 + 63) / 64;
-  R pb[PB];
-  auto fetch_bwd = [&](int c) {   // chunk c counted from the end
-    const int hi = s1 - c * CH, m = min(CH, hi - s0), sc = hi - m;
-#pragma unroll
-    for (int t = 0; t < PB; ++t) {
-      int idx = t * 64 + lane;
-      R v = R(0);
-      if (idx < m * 16) v = Fc[(size_t)sc * 16 + idx];
-      else if (NV > 4 && idx >= CH * 16 && idx - CH * 16 < m) v = Fs[(size_t)(sc + idx - CH * 16) * 3 + 2];
-      pb[t] = v;
-    }
-  };
-  R xn[4] = {R(0), R(0), R(0), R(0)};
-  R xns = R(0);
-  if (nch > 0) fetch_bwd(0);
-  for (int c = 0; c < nch; ++c) {
-    const int hi = s1 - c * CH, m = min(CH, hi - s0), sc = hi - m;
-#pragma unroll
-    for (int t = 0; t < PB; ++t) {
-      int idx = t * 64 + lane;
-      if (idx < CH * 17) fb[idx] = pb[t];
-    }
-    if (lane < m) cb[lane] = P.l_cell[sc + lane];
-    __syncthreads();
-    if (c + 1 < nch) fetch_bwd(c + 1);
-    if (lane == 0) {
-      for (int k = m - 1; k >= 0; --k) {
-        const R* o = fb + k * 16;
-        const R* dv = dbuf + (size_t)(sc - s0 + k) * NV;
-        R x[4];
-        for (int i = 0; i < 4; ++i) {
-          R acc = dv[i];
-          for (int q = 0; q < 4; ++q) acc -= o[i * 4 + q] * xn[q];
-          x[i] = acc;
-        }
-        for (int i = 0; i < 4; ++i) {
-          xn[i] = x[i];
-          rb[k * NV + i] = x[i];
-        }
-        if constexpr (NV > 4) {
-          xns = dv[4] - fb[CH * 16 + k] * xns;
-          rb[k * NV + 4] = xns;
-        }
-      }
-    }
-    __syncthreads();
-    for (int idx = lane; idx < m * NV; idx += 64) {
-      int k = idx / NV, var = idx - k * NV;
-      size_t o = base + (size_t)var * nc + cb[k];
-      double x = (double)rb[idx];
-      z[o] = accumulate ? z[o] + x : x;
-    }
-    __syncthreads();
-  }
 }
 
 }  // namespace jr

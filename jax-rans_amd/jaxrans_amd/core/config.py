@@ -163,9 +163,11 @@ class ImplicitConfig:
     ``precision`` is this implementation's addition: ``"fp64"`` (the drop-in,
     every array double) or ``"mixed"`` (line-preconditioner blocks and their
     factor and solve in float32; residual, tangent, Krylov and update in
-    float64).  ``line_solver`` is how each line chain is solved: block cyclic
-    reduction (the reference's, ``src/jaxrans/core/linalg.py:159-200``) or block
-    Thomas (``linalg.py:80-132``); the two agree to round-off."""
+    float64).  ``line_solver`` is how each line chain is solved: block Thomas
+    (``linalg.py:80-132``, the default: a lane of the wavefront per block entry,
+    the fastest exact solve on the device) or block cyclic reduction (the
+    reference's, ``src/jaxrans/core/linalg.py:159-200``); the two agree to
+    round-off."""
 
     cfl: float = 10.0
     cfl_max: float = 1.0e3
@@ -180,7 +182,7 @@ class ImplicitConfig:
     krylov_success: float = 0.5
     max_relative_update: float | None = None
     precision: str = "fp64"
-    line_solver: str = "cyclic_reduction"
+    line_solver: str = "thomas"
 
     def __post_init__(self) -> None:
         _enforce(self, _IMPLICIT_RULES)

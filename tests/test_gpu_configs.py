@@ -6,8 +6,9 @@
   tests/golden/make_converged.py).
 * 2D SA RANS, Karman-Trefftz 10 deg, C-grid 512 x 128 (cases/kt10_c_sa):
   residual (1e-12), first-order tangent (1e-11), the FP64 line
-  preconditioner (1e-10) and two Newton steps (norms 1e-9, state 1e-8)
+  preconditioner by block cyclic reduction (1e-10) and two Newton steps (norms 1e-9, state 1e-8)
   against the oracle on the same seeded state.
+* The same with the block Thomas line solve (lanes over block entries).
 * The batched polar on that grid, 32 cases (alpha U[-4, 12], M U[0.3,
   0.78], Re 6e6): every case of the batch equals the case alone bit for bit
   after three Newton steps (a sample of four cases); mixed precision run
@@ -66,7 +67,7 @@ def _kt10():
 def test_sa_c_grid_512x128_kernels_and_newton_steps_match_the_oracle():
     print("[gpu] configs C-grid 512x128 kernels", flush=True)
     case, atlas = _kt10()
-    cfg = replace(case.implicit, krylov_subspace=10, krylov_restarts=2)
+    cfg = replace(case.implicit, krylov_subspace=10, krylov_restarts=2, line_solver="cyclic_reduction")
     st = OR.setup_for(atlas, case.flow, case.numerics)
     ctx = context_for(atlas, case.flow, case.numerics, cfg)
     ctx.set_freestream(st.q_inf, case.flow.mu_inf)
@@ -79,6 +80,33 @@ def test_sa_c_grid_512x128_kernels_and_newton_steps_match_the_oracle():
     ctx.precond_factor(U)
     r = rng.standard_normal(U.shape)
     ref = OS.LinePreconditioner(U, OR.wave_speeds(U, st), 25.0, st, "cyclic_reduction")(r)
+    assert rel(ctx.precond_apply(r)[0], ref) < 1e-10
+    ctx.set_state(U)
+    ctx.newton_init(cfg.cfl)
+    s, norms = OS.NewtonState(U, cfg.cfl), []
+    for _ in range(2):
+        s, n, _info = OS.newton_step(s, st, cfg)
+        norms.append(n)
+    hist = ctx.newton_block(2)[:, 0]
+    assert np.allclose(hist, norms, rtol=1e-9, atol=0.0), (hist, norms)
+    assert rel(ctx.get_state()[0] - U, s.U - U) < 1e-8
+    ctx.close()
+
+
+def test_sa_c_grid_512x128_block_thomas_lines_and_newton_steps_match_the_oracle():
+    """The lane-parallel block Thomas apply on the real grid (chains of 128
+    cells and of 256 across the wake: two launch classes)."""
+    print("[gpu] configs C-grid 512x128 block Thomas", flush=True)
+    case, atlas = _kt10()
+    cfg = replace(case.implicit, krylov_subspace=10, krylov_restarts=2, line_solver="thomas")
+    st = OR.setup_for(atlas, case.flow, case.numerics)
+    ctx = context_for(atlas, case.flow, case.numerics, cfg)
+    ctx.set_freestream(st.q_inf, case.flow.mu_inf)
+    U = perturbed(OR.U0_of(st), 1e-3)
+    ctx.set_cfl(25.0)
+    ctx.precond_factor(U)
+    r = np.random.default_rng(12).standard_normal(U.shape)
+    ref = OS.LinePreconditioner(U, OR.wave_speeds(U, st), 25.0, st, "thomas")(r)
     assert rel(ctx.precond_apply(r)[0], ref) < 1e-10
     ctx.set_state(U)
     ctx.newton_init(cfg.cfl)

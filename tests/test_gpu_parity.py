@@ -112,14 +112,16 @@ def test_line_preconditioner_matches_oracle(name, kind, flowf, flux, lim, precis
     assert rel(got, ref) < tol, rel(got, ref)
 
 
+@pytest.mark.parametrize("solver", ["cyclic_reduction", "thomas"])
 @pytest.mark.parametrize("name, kind, flowf, flux, lim", [CASES[1], CASES[4], CASES[5]],
                          ids=[CASES[1][0], CASES[4][0], CASES[5][0]])
-def test_newton_steps_match_oracle(name, kind, flowf, flux, lim):
+def test_newton_steps_match_oracle(name, kind, flowf, flux, lim, solver):
     """Four Newton steps from a perturbed state (no exact ties in the
     tangent): reported norms, the state and the CFL ramp follow the oracle;
-    GMRES internals differ only in round-off."""
-    print(f"[gpu] newton {name}", flush=True)
-    cfg = ImplicitConfig(krylov_subspace=12, krylov_restarts=2)
+    GMRES internals differ only in round-off.  Both line solvers: the apply
+    also normalises the Arnoldi vector and writes the tangent's input."""
+    print(f"[gpu] newton {name} {solver}", flush=True)
+    cfg = ImplicitConfig(krylov_subspace=12, krylov_restarts=2, line_solver=solver)
     atlas, flow, num, st, ctx = _setup(kind, flowf, flux, lim, implicit=cfg)
     U = perturbed(OR.U0_of(st), 1e-2)
     ctx.set_state(U)
