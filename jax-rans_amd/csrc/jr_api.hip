@@ -96,13 +96,10 @@ struct jr_ctx {
   int device = 0;              // the HIP device current at creation (its stream's device)
   double* loads = nullptr;     // jr_wall_loads: per wall face [case][1 + dim][n_walls]
   double* circ = nullptr;      // point-vortex far field: [case][2] circulation and its tangent
-  // 2D block Thomas apply (k_line_apply): the chains by length, longest first,
-  // in classes launched with one LDS size each
-  struct LineClass {
-    int first, count, lmax, gper;
-  };
+  // 2D block Thomas (k_thomas, k_line_apply): the chains by length, longest
+  // first, and their length classes
   const int* lorder = nullptr;
-  std::vector<LineClass> lclass;
+  LineLaunch lines{};
 };
 
 // FNV-1a over the index plans of a grid (stages, chains, walls, neighbours,
@@ -267,23 +264,14 @@ static void launch_apply(jr_ctx* c, const double* r, double* z, int accumulate, 
                          double* vout = nullptr, double* dq_out = nullptr) {
   const Plan& P = c->P;
   if (c->thomas) {
-    // one launch per length class, the cells on no chain with the first; the
-    // probe brackets the whole sequence
-    const int pk = probe_slot(c, PR_APPLY);
-    if (pk >= 0) (void)hipEventRecord(c->probe_ev[2 * pk], c->stream);
-    const unsigned free_groups = (unsigned)((P.n_free + 63) / 64);
-    const size_t ncls = std::max<size_t>(1, c->lclass.size());
-    for (size_t k = 0; k < ncls; ++k) {
-      const jr_ctx::LineClass lc = k < c->lclass.size() ? c->lclass[k] : jr_ctx::LineClass{0, 0, LN_CH, 1};
-      const unsigned groups = (unsigned)((lc.count + lc.gper - 1) / lc.gper) + (k == 0 ? free_groups : 0u);
-      if (groups == 0) continue;
+    const unsigned groups = (unsigned)(c->lines.wave0[c->lines.n] + (P.n_free + 63) / 64);
+    if (groups > 0) {
       auto kern = accumulate ? k_line_apply<R, NV, true> : k_line_apply<R, NV, false>;
-      hipLaunchKernelGGL(kern, dim3(groups, c->n_cases), dim3(64), line_apply_lds<NV>(lc.lmax, lc.gper), c->stream, P,
-                         (const Ctl*)c->ctl, (const int*)c->lorder, lc.first, lc.count, lc.lmax, lc.gper,
-                         (const R*)c->fac, (const R*)c->fac_sa, r, z, check_active, vout, (const double*)c->cg.p3,
-                         c->cg.nblk, dq_out, (const double*)c->U, c->ph.gamma, (const double*)c->gjac);
+      JR_LAUNCH(PR_APPLY, kern, dim3(groups, c->n_cases), dim3(64), line_apply_lds<NV>(c->lines.rows), P,
+                (const Ctl*)c->ctl, (const int*)c->lorder, c->lines, (const R*)c->fac, (const R*)c->fac_sa, r, z,
+                check_active, vout, (const double*)c->cg.p3, c->cg.nblk, dq_out, (const double*)c->U, c->ph.gamma,
+                (const double*)c->gjac);
     }
-    if (pk >= 0) (void)hipEventRecord(c->probe_ev[2 * pk + 1], c->stream);
   } else {
     const int T = c->cr_threads;
     const unsigned groups = (unsigned)(P.n_lines + (P.n_free + T - 1) / T);
@@ -977,9 +965,10 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
       return bail(0);
   }
   // the longest chain k_line_apply holds in 64 KiB of LDS, one per wavefront
-  if (line_apply_lds<5>((max_len + LN_CH - 1) / LN_CH * LN_CH, 1) > 64 * 1024) {
-    fail("jr_ctx_create: a line chain longer than " +
-         std::to_string(64 * 1024 / line_apply_lds<5>(LN_CH, 1) * LN_CH) + " cells is not supported");
+  // rows of one k_line_apply wavefront: whole multiples of 16 within 64 KiB
+  const int line_rows_cap = (int)((64 * 1024 - line_apply_lds<5>(0)) / line_rows_lds<5>(16)) * 16;
+  if ((max_len + 15) / 16 * 16 > line_rows_cap) {
+    fail("jr_ctx_create: a line chain longer than " + std::to_string(line_rows_cap) + " cells is not supported");
     return bail(0);
   }
   std::vector<int> freec;
@@ -1025,21 +1014,39 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
   c->thomas = num->line_solver == 1 || P.max_len > CR_MAX_LEN;
   c->cr_threads = std::max(64, (P.max_len + 63) / 64 * 64);
   if (c->thomas) {
-    // k_line_apply classes: chains longest first; a class takes the lengths
-    // in (lmax / 2, lmax], lmax a multiple of 16, and as many chains per
-    // wavefront (4, 2 or 1) as keep its LDS within 64 KiB
+    // length classes: lengths in (lmax / 2, lmax], lmax a multiple of 16; one
+    // wavefront's LDS holds `rows` rows, four chains of the most populous
+    // class or one longest chain, whichever is more, within 64 KiB; each class
+    // takes rows / lmax chains per wavefront, at most four
     std::vector<int> ord(g->n_lines);
     for (int l = 0; l < g->n_lines; ++l) ord[l] = l;
     auto len = [&](int l) { return (int)(g->line_start[l + 1] - g->line_start[l]); };
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return len(a) > len(b); });
+    LineLaunch& LL = c->lines;
+    LL.n = 0;
     for (int i = 0; i < g->n_lines;) {
       const int lmax = std::max(LN_CH, (len(ord[i]) + LN_CH - 1) / LN_CH * LN_CH);
       int j = i;
       while (j < g->n_lines && (lmax <= LN_CH || 2 * len(ord[j]) > lmax)) ++j;
-      int gper = 4;
-      while (gper > 1 && line_apply_lds<5>(lmax, gper) > 64 * 1024) gper /= 2;
-      c->lclass.push_back({i, j - i, lmax, gper});
+      if (LL.n == LN_MAX_CLASSES) {
+        fail("jr_ctx_create: too many chain length classes");
+        return bail(0);
+      }
+      LL.first[LL.n] = i;
+      LL.count[LL.n] = j - i;
+      LL.lmax[LL.n] = lmax;
+      ++LL.n;
       i = j;
+    }
+    const int cap = line_rows_cap;
+    int most = 0;
+    for (int k = 1; k < LL.n; ++k)
+      if (LL.count[k] > LL.count[most]) most = k;
+    LL.rows = LL.n ? std::min(cap, std::max(LL.lmax[0], 4 * LL.lmax[most])) : LN_CH;
+    LL.wave0[0] = 0;
+    for (int k = 0; k < LL.n; ++k) {
+      LL.gper[k] = std::max(1, std::min(4, LL.rows / LL.lmax[k]));
+      LL.wave0[k + 1] = LL.wave0[k] + (LL.count[k] + LL.gper[k] - 1) / LL.gper[k];
     }
     if (ord.empty()) ord.push_back(0);
     if (upload(c, &c->lorder, ord.data(), ord.size())) return bail(0);

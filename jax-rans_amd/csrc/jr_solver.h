@@ -429,8 +429,11 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
 // takes 64 chains, order[t * 64 .. t * 64 + 63] (chains by length, longest
 // first, so a wavefront's chains end together), and runs the recurrence of
 // each in its own lane, the next slot's blocks loaded while the current one is
-// factored.  Per slot it writes
-//   fac[slot][32] = L (16), P^-1 (16);  fac[n_slots*32 + slot*16] = C (16);
+// factored.  Per slot, in the order the lanes of the line apply read them
+// (entry e = 4 r + q):
+//   fac[slot][e][2] = (L[r][q], P^-1[q][r]);
+//   fac[n_slots*32 + slot*16 + e] = C[q][r] on the even steps counted from
+//     the chain's top (slot L - 1, L - 3, ...), C[r][q] on the odd ones;
 //   fac_sa[slot][3] = l, 1/p, c  (the scalar SA chain).
 template <class R, int NV>
 __global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ order, const double* __restrict__ blk,
@@ -472,10 +475,12 @@ __global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ o
         for (int u = 0; u < 4; ++u) acc += Pi[r][u] * (R)b[17 + u * 4 + q];
         Cp[r][q] = acc;
       }
+    const bool even = (n - 1 - k) % 2 == 0;
     for (int e = 0; e < 16; ++e) {
-      F[s * 32 + e] = (R)b[1 + e];
-      F[s * 32 + 16 + e] = Pi[e / 4][e % 4];
-      Fc[s * 16 + e] = Cp[e / 4][e % 4];
+      const int r = e / 4, q = e % 4;
+      F[s * 32 + 2 * e] = (R)b[1 + e];
+      F[s * 32 + 2 * e + 1] = Pi[q][r];
+      Fc[s * 16 + e] = even ? Cp[q][r] : Cp[r][q];
     }
     if constexpr (NV > 4) {
       const R lo = (R)b[33], ds = (R)b[34], up = (R)b[35];
