@@ -993,6 +993,20 @@ int jr_ctx_create(const jr_grid_desc* g, const jr_numerics* num, int n_cases, jr
       upload(c, &P.w_mid, g->wall_mid, 2 * (size_t)g->n_walls))
     return bail(0);
 
+  {
+    // the viscous faces' diamond geometry, once per grid
+    double* dg = nullptr;
+    if (dalloc(c, &dg, num->viscous ? 4 * ((size_t)P.nfi + P.nfj) : 1)) return bail(0);
+    P.dg = dg;
+    if (num->viscous) {
+      hipLaunchKernelGGL(k_diamond, dim3((unsigned)((P.nfi + P.nfj + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, 0, P, 0,
+                         dg);
+      if (hipDeviceSynchronize() != hipSuccess) {
+        fail("jr_ctx_create: the diamond geometry kernel failed");
+        return bail(0);
+      }
+    }
+  }
   Physics& ph = c->ph;
   ph.gamma = num->gamma;
   ph.kappa = num->kappa;
@@ -1526,6 +1540,7 @@ int jr_set_geometry(jr_ctx* c, int k, const jr_grid_desc* g) {
     };
     if (widen(&P.area, n_area, &P.gs_area) || widen(&P.fi, n_fi, &P.gs_fi) || widen(&P.fj, n_fj, &P.gs_fj) ||
         widen(&P.cen, n_cen, &P.gs_cen) || widen(&P.dist, n_area, &P.gs_dist) ||
+        (c->ph.viscous && widen(&P.dg, 4 * ((size_t)P.nfi + P.nfj), &P.gs_dg)) ||
         widen(&P.t_norm[0], n_tn0, &P.gs_tn0) || widen(&P.t_norm[1], n_tn1, &P.gs_tn1) ||
         widen(&P.w_mid, n_wm, &P.gs_wmid))
       return -1;
@@ -1547,6 +1562,9 @@ int jr_set_geometry(jr_ctx* c, int k, const jr_grid_desc* g) {
   if (P.n_task[0]) TRY(put(P.t_norm[0], P.gs_tn0, g->task_normal[0], 2 * (size_t)P.n_task[0]));
   if (P.n_task[1]) TRY(put(P.t_norm[1], P.gs_tn1, g->task_normal[1], 2 * (size_t)P.n_task[1]));
   if (P.n_walls) TRY(put(P.w_mid, P.gs_wmid, g->wall_mid, 2 * (size_t)P.n_walls));
+  if (c->ph.viscous)   // this case's diamonds from its centres
+    hipLaunchKernelGGL(k_diamond, dim3((unsigned)((P.nfi + P.nfj + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, 0, P, k,
+                       (double*)P.dg + (size_t)k * P.gs_dg);
   // the fills and copies above ran on the null stream; the context's stream
   // does not wait for it
   HIPCHK(hipDeviceSynchronize());

@@ -194,6 +194,57 @@ __global__ void __launch_bounds__(BLOCK) k_nodes(Plan P, double gamma, const Ctl
     Io<T>::st(Nd, dNd, o + (size_t)k * nn, 0.25 * (((p00[k] + p10[k]) + p01[k]) + p11[k]));
 }
 
+// The diamond of every face (gradients.py:31-95): d1 = the centre difference
+// of the face's two cells, d2 = the difference of its two nodes, each node
+// the mean of the four edge-padded centres around it, ((c00 + c10) + c01) +
+// c11.  Pure geometry: computed once per grid (per case with per-case
+// geometry) into out[4][nfi + nfj], (d1x, d1y, d2x, d2y); zero on the faces
+// the face kernel does not evaluate.  The geometry of case cs.
+__global__ void __launch_bounds__(BLOCK) k_diamond(Plan P, int cs, double* __restrict__ out) {
+  bind_case(P, cs);
+  const int f = blockIdx.x * BLOCK + threadIdx.x;
+  const int nft = P.nfi + P.nfj;
+  if (f >= nft) return;
+  const int NJ = P.NJ;
+  const bool ifam = f < P.nfi;
+  int a, b, ff;
+  if (ifam) {
+    ff = f;
+    a = f / NJ; b = f - a * NJ;
+  } else {
+    ff = f - P.nfi;
+    a = ff / (NJ + 1); b = ff - a * (NJ + 1);
+  }
+  int cL, cR;
+  bool inside;
+  if (ifam) {
+    inside = a >= 1 && a <= P.NI - 1;
+    cR = a * NJ + b; cL = cR - NJ;
+  } else {
+    inside = b >= 1 && b <= NJ - 1;
+    cR = a * NJ + b; cL = cR - 1;
+  }
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  if (inside && (P.interior[cL] || P.interior[cR])) {
+    const int NJ2 = NJ + 2;
+    const int iL = cL / NJ, jL = cL - iL * NJ, iR = cR / NJ, jR = cR - iR * NJ;
+    const size_t ycen = (size_t)(P.NI + 2) * NJ2;
+    d[0] = P.cen[(iR + 1) * NJ2 + jR + 1] - P.cen[(iL + 1) * NJ2 + jL + 1];
+    d[1] = P.cen[ycen + (iR + 1) * NJ2 + jR + 1] - P.cen[ycen + (iL + 1) * NJ2 + jL + 1];
+    // nodes n0, n1 of the face: the lower-left corners (node index = canvas
+    // cell index) of cell R and of cell R shifted one step along the face
+    const int n0i = iR, n0j = jR;
+    const int n1i = ifam ? iR : iR + 1, n1j = ifam ? jR + 1 : jR;
+    auto node_c = [&](int ni, int nj, int row) -> double {
+      const double* c = P.cen + (size_t)row * ycen;
+      return 0.25 * (((c[ni * NJ2 + nj] + c[(ni + 1) * NJ2 + nj]) + c[ni * NJ2 + nj + 1]) + c[(ni + 1) * NJ2 + nj + 1]);
+    };
+    d[2] = node_c(n1i, n1j, 0) - node_c(n0i, n0j, 0);
+    d[3] = node_c(n1i, n1j, 1) - node_c(n0i, n0j, 1);
+  }
+  for (int k = 0; k < 4; ++k) out[(size_t)k * nft + f] = d[k];
+}
+
 template <class T, int NV, bool VISC>
 __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
@@ -268,23 +319,14 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
 
   if (VISC) {
     const double mu_inf = ctl[cs].mu_inf;
-    // Diamond gradient: the cells' centre difference and the face's node difference.
-    const int NJ2 = NJ + 2;
-    int iL = cL / NJ, jL = cL - iL * NJ, iR = cR / NJ, jR = cR - iR * NJ;
-    const size_t ycen = (size_t)(P.NI + 2) * NJ2;
-    double d1x = P.cen[(iR + 1) * NJ2 + jR + 1] - P.cen[(iL + 1) * NJ2 + jL + 1];
-    double d1y = P.cen[ycen + (iR + 1) * NJ2 + jR + 1] - P.cen[ycen + (iL + 1) * NJ2 + jL + 1];
-    // Nodes n0, n1 of the face: the lower-left corners (p1 node index = canvas cell
-    // index) of cell R and of cell R shifted one step along the face.
-    int n0i = iR, n0j = jR;
-    int n1i = ifam ? iR : iR + 1, n1j = ifam ? jR + 1 : jR;
-    auto node_c = [&](int ni, int nj, int row) -> double {
-      // average of padded centres (ni, nj), (ni+1, nj), (ni, nj+1), (ni+1, nj+1)
-      const double* c = P.cen + (size_t)row * ycen;
-      return 0.25 * (((c[ni * NJ2 + nj] + c[(ni + 1) * NJ2 + nj]) + c[ni * NJ2 + nj + 1]) + c[(ni + 1) * NJ2 + nj + 1]);
-    };
-    double d2x = node_c(n1i, n1j, 0) - node_c(n0i, n0j, 0);
-    double d2y = node_c(n1i, n1j, 1) - node_c(n0i, n0j, 1);
+    // Diamond gradient: the cells' centre difference d1 and the face's node
+    // difference d2, precomputed per grid (k_diamond).
+    const int nft = P.nfi + P.nfj;
+    const double d1x = P.dg[f], d1y = P.dg[(size_t)nft + f], d2x = P.dg[2 * (size_t)nft + f],
+                 d2y = P.dg[3 * (size_t)nft + f];
+    const int iR = cR / NJ, jR = cR - iR * NJ;
+    const int n0i = iR, n0j = jR;
+    const int n1i = ifam ? iR : iR + 1, n1j = ifam ? jR + 1 : jR;
     double det = d1x * d2y - d2x * d1y;
     // phi = (u, v, T[, nu_hat]) at the face's two nodes (k_nodes): node
     // (ni, nj) averages canvas cells (ni-1, nj-1), (ni, nj-1), (ni-1, nj), (ni, nj).

@@ -20,6 +20,7 @@ struct Plan {
   const double* fj;                // [2][nfj] j-face area vectors
   const double* cen;               // [2][(NI+2)*(NJ+2)] centres, edge-padded by one
   const double* dist;              // [nc] wall distance
+  const double* dg;                // [4][nfi + nfj] viscous diamond of each face: d1x, d1y, d2x, d2y
   const uint8_t* interior;         // [nc]
   const int* nbr;                  // [nc] face-neighbour map
   int n_task[2];                   // ghost-fill stages
@@ -61,7 +62,7 @@ struct Plan {
   // Geometry per case: case cs reads area + cs * gs_area, and so on (all 0
   // when the cases share the canvas geometry).  Per-case geometry lets one
   // batch hold different grids of one topology (the same index plans).
-  size_t gs_area, gs_fi, gs_fj, gs_cen, gs_dist, gs_tn0, gs_tn1, gs_wmid;
+  size_t gs_area, gs_fi, gs_fj, gs_cen, gs_dist, gs_tn0, gs_tn1, gs_wmid, gs_dg;
   // Point-vortex far field: far-field tasks take the vortex of each case's
   // current circulation circ[case][2] (value, tangent), centred at (vx, vy),
   // at their face midpoints t_mid[stage] ([2][n]).
@@ -78,6 +79,7 @@ __device__ __forceinline__ void bind_case(Plan& P, int cs) {
   P.fj += cs * P.gs_fj;
   P.cen += cs * P.gs_cen;
   P.dist += cs * P.gs_dist;
+  P.dg += cs * P.gs_dg;
   P.t_norm[0] += cs * P.gs_tn0;
   P.t_norm[1] += cs * P.gs_tn1;
   P.w_mid += cs * P.gs_wmid;
