@@ -215,6 +215,30 @@ int jr_newton_block_noalloc(jr_ctx* ctx, int n_steps, double* hist_host);
  * src/jaxrans/derivatives.py:94-120. */
 int jr_linear_solve(jr_ctx* ctx, const double* b_host, double* x_host);
 
+/* Parameter tangents of the sensitivity solve, 2D contexts.  The flow
+ * parameters reach the residual and the force coefficients only through the
+ * freestream primitives and mu_inf (flow_inputs, src/jaxrans/derivatives.py:
+ * 65-70), so a parameter direction is the freestream's tangent:
+ * dq_inf_host[n_cases][5] (the layout of jr_set_freestream's q_inf) and
+ * dmu_inf_host[n_cases] (null: zero).  dU_host may be null (zero).
+ *
+ * jr_tangent_freestream: d/de R(U + e dU; q_inf + e dq_inf, mu_inf + e dmu_inf)
+ * at e = 0, the exact (second-order) residual -- jax.jvp of the residual
+ * over the parameters (src/jaxrans/solvers/sensitivity.py:60) and, with dU,
+ * over the state as well.  Ghost states, the U - U0 rows off the interior,
+ * the viscous and SA terms and the point-vortex reference all carry the
+ * freestream's tangent.  out_host: a field [n_cases][nv][cells].
+ *
+ * jr_forces_tangent: the tangent of jr_forces' coefficients at U along the
+ * same direction, the output JVP of src/jaxrans/solvers/sensitivity.py:63
+ * (force_coefficients over the state and the freestream,
+ * src/jaxrans/derivatives.py:82-84).  out_host[n_cases][8] =
+ * d(cl, cd, cm, cd_friction, fx, fy), 0, 0. */
+int jr_tangent_freestream(jr_ctx* ctx, const double* U_host, const double* dU_host, const double* dq_inf_host,
+                          const double* dmu_inf_host, double* out_host);
+int jr_forces_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, const double* dq_inf_host,
+                      const double* dmu_inf_host, double length, double centre_x, double centre_y, double* out_host);
+
 /* n_steps explicit multistage Runge-Kutta steps on the device state, 2D
  * contexts: U^(k) = U^n - alphas[k] dt/V R(U^(k-1)) with the local time step
  * dt frozen at U^n (CFL: each case's control value, set by jr_newton_init or

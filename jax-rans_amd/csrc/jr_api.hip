@@ -96,6 +96,8 @@ struct jr_ctx {
   int device = 0;              // the HIP device current at creation (its stream's device)
   double* loads = nullptr;     // jr_wall_loads: per wall face [case][1 + dim][n_walls]
   double* circ = nullptr;      // point-vortex far field: [case][2] circulation and its tangent
+  double* dfree = nullptr;     // parameter tangents: [case][8] the freestream's tangent (Plan::dfree)
+  double* zero_field = nullptr;   // parameter tangents at dU = 0: a zero state tangent
   // 2D block Thomas (k_thomas, k_line_apply): the chains by length, longest
   // first, and their length classes
   const int* lorder = nullptr;
@@ -728,6 +730,47 @@ static int check_launch() {
 }
 
 static size_t field_bytes(const jr_ctx* c) { return (size_t)c->n_cases * c->N * sizeof(double); }
+
+// --- parameter tangents (sensitivity) -----------------------------------------
+
+// The exact residual's tangent along (dU, dq_inf, dmu_inf) at U: the
+// parameter-tangent kernels (PT) over the primal primitives, node values and
+// circulation of a residual evaluation at U.
+template <int NV, bool VISC>
+static void launch_param_tangent(jr_ctx* c, const double* U, const double* dU, double* out) {
+  const Plan& P = c->P;
+  launch_residual<NV, VISC>(c, U, c->R, 0);
+  if (P.vortex)
+    hipLaunchKernelGGL((k_circulation<Dual, NV, true>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
+                       (const Ctl*)c->ctl, U, dU, c->circ, 0);
+  hipLaunchKernelGGL((k_fill<Dual, NV, true>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
+                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, 0);
+  if (VISC)
+    hipLaunchKernelGGL((k_nodes<Dual, NV>), grid1((size_t)(P.NI + 1) * (P.NJ + 1), BLOCK, c->n_cases), dim3(BLOCK), 0,
+                       c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ,
+                       (double*)nullptr, c->dNd, 0);
+  hipLaunchKernelGGL((k_faces<Dual, NV, VISC, true>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph, 0, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ,
+                     (const double*)c->Nd, (const double*)c->dNd, c->Fi, c->Fj, 0);
+  hipLaunchKernelGGL((k_cells<Dual, NV, true>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Fi,
+                     (const double*)c->Fj, (const double*)nullptr, (const double*)c->salin, out, 0);
+}
+
+// The force coefficients' tangent along (dU, dq_inf, dmu_inf) at U.
+template <int NV, bool VISC>
+static void launch_forces_tangent(jr_ctx* c, const double* U, const double* dU, double length, double cx, double cy,
+                                  double* out) {
+  const Plan& P = c->P;
+  launch_residual<NV, VISC>(c, U, c->R, 0);
+  if (P.vortex)
+    hipLaunchKernelGGL((k_circulation<Dual, NV, true>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
+                       (const Ctl*)c->ctl, U, dU, c->circ, 0);
+  hipLaunchKernelGGL((k_fill<Dual, NV, true>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
+                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, 0);
+  hipLaunchKernelGGL((k_forces_tangent<NV>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, length, cx, cy, out);
+}
 
 extern "C" {
 
@@ -1959,6 +2002,71 @@ int jr_linear_solve(jr_ctx* c, const double* b, double* x) {
   c->solve_b = nullptr;
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(x, c->X, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Stage U, dU (null: zero) and the freestream's tangent for a parameter
+// tangent: U in tmpA, dU in Z (or the zero field), dq_inf / dmu_inf into
+// dfree.  Returns the device dU.
+static int stage_param_tangent(jr_ctx* c, const char* who, const double* U, const double* dU, const double* dq_inf,
+                               const double* dmu_inf, const double** dU_dev) {
+  if (!c || !U || !dq_inf) return fail(std::string(who) + ": null argument");
+  if (c->dim == 3) return fail(std::string(who) + ": 2D contexts only");
+  if (c->pending_steps) return fail(std::string(who) + ": an asynchronous block is pending: wait for it first");
+  if (!c->has_q) return fail(std::string(who) + ": set the freestream first");
+  if (!c->dfree) {
+    TRY(dalloc(c, &c->dfree, (size_t)c->n_cases * 8));
+    HIPCHK(hipDeviceSynchronize());   // dalloc clears on the null stream
+    c->P.dfree = c->dfree;
+  }
+  std::vector<double> h((size_t)c->n_cases * 8, 0.0);
+  for (int k = 0; k < c->n_cases; ++k) {
+    for (int v = 0; v < 5; ++v) h[(size_t)k * 8 + v] = dq_inf[k * 5 + v];
+    h[(size_t)k * 8 + 6] = dmu_inf ? dmu_inf[k] : 0.0;
+    for (int v = 0; v < 7; ++v)
+      if (!std::isfinite(h[(size_t)k * 8 + v])) return fail(std::string(who) + ": non-finite freestream tangent");
+  }
+  HIPCHK(hipMemcpyAsync(c->dfree, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->tmpA, U, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+  if (dU) {
+    HIPCHK(hipMemcpyAsync(c->Z, dU, field_bytes(c), hipMemcpyHostToDevice, c->stream));
+    *dU_dev = c->Z;
+  } else {
+    if (!c->zero_field) {
+      TRY(dalloc(c, &c->zero_field, (size_t)c->n_cases * c->N));
+      HIPCHK(hipDeviceSynchronize());
+    }
+    *dU_dev = c->zero_field;
+  }
+  return 0;
+}
+
+int jr_tangent_freestream(jr_ctx* c, const double* U, const double* dU, const double* dq_inf, const double* dmu_inf,
+                          double* out) {
+  if (!out) return fail("jr_tangent_freestream: null argument");
+  const double* dUd = nullptr;
+  TRY(stage_param_tangent(c, "jr_tangent_freestream", U, dU, dq_inf, dmu_inf, &dUd));
+  if (c->nv == 5) launch_param_tangent<5, true>(c, c->tmpA, dUd, c->tmpB);
+  else if (c->ph.viscous) launch_param_tangent<4, true>(c, c->tmpA, dUd, c->tmpB);
+  else launch_param_tangent<4, false>(c, c->tmpA, dUd, c->tmpB);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->tmpB, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_forces_tangent(jr_ctx* c, const double* U, const double* dU, const double* dq_inf, const double* dmu_inf,
+                      double length, double cx, double cy, double* out) {
+  if (!out) return fail("jr_forces_tangent: null argument");
+  if (!(length > 0.0)) return fail("jr_forces_tangent: length must be positive");
+  const double* dUd = nullptr;
+  TRY(stage_param_tangent(c, "jr_forces_tangent", U, dU, dq_inf, dmu_inf, &dUd));
+  if (c->nv == 5) launch_forces_tangent<5, true>(c, c->tmpA, dUd, length, cx, cy, c->coef);
+  else if (c->ph.viscous) launch_forces_tangent<4, true>(c, c->tmpA, dUd, length, cx, cy, c->coef);
+  else launch_forces_tangent<4, false>(c, c->tmpA, dUd, length, cx, cy, c->coef);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

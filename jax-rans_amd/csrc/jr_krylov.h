@@ -694,8 +694,11 @@ __global__ void __launch_bounds__(BLOCK) k_update(int nc, const Ctl* __restrict_
 // the ghost-filled primitives Q; CL, CD, CM (pressure only, 2D), CD friction,
 // Fx, Fy per case.  One workgroup per case, a fixed-order block sum.
 
-__device__ void wall_shear(const Plan& P, const Physics& ph, const double* Q, size_t base, double mu_inf, int fam,
-                           int ff, double* fx, double* fy) {
+// T = Dual: the shear's tangent, from dQ (the fill's tangent) and a mu_inf
+// that may carry a parameter tangent (TM = Dual).
+template <class T, class TM>
+__device__ void wall_shear(const Plan& P, const Physics& ph, const double* Q, const double* dQ, size_t base,
+                           TM mu_inf, int fam, int ff, T* fx, T* fy) {
   const int NJ = P.NJ, nc = P.nc, NJ2 = NJ + 2;
   const double gamma = ph.gamma;
   int cL, cR;
@@ -721,35 +724,35 @@ __device__ void wall_shear(const Plan& P, const Physics& ph, const double* Q, si
   double d2x = node_c(n1i, n1j, 0) - node_c(n0i, n0j, 0);
   double d2y = node_c(n1i, n1j, 1) - node_c(n0i, n0j, 1);
   double det = d1x * d2y - d2x * d1y;
-  auto phi_at = [&](int i, int j, double* out) {
-    double q[4];
-    for (int k = 0; k < 4; ++k) q[k] = Q[base + (size_t)k * nc + i * NJ + j];
+  auto phi_at = [&](int i, int j, T* out) {
+    T q[4];
+    for (int k = 0; k < 4; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + i * NJ + j);
     out[0] = q[1]; out[1] = q[2]; out[2] = gamma * q[3] / q[0];
   };
-  auto node_phi = [&](int ni, int nj, double* out) {
-    double p00[3], p10[3], p01[3], p11[3];
+  auto node_phi = [&](int ni, int nj, T* out) {
+    T p00[3], p10[3], p01[3], p11[3];
     phi_at(ni - 1, nj - 1, p00); phi_at(ni, nj - 1, p10); phi_at(ni - 1, nj, p01); phi_at(ni, nj, p11);
     for (int k = 0; k < 3; ++k) out[k] = 0.25 * (((p00[k] + p10[k]) + p01[k]) + p11[k]);
   };
-  double pn0[3], pn1[3], pl[3], pr[3];
+  T pn0[3], pn1[3], pl[3], pr[3];
   node_phi(n0i, n0j, pn0);
   node_phi(n1i, n1j, pn1);
   phi_at(iL, jL, pl);
   phi_at(iR, jR, pr);
-  double gx[3], gy[3];
+  T gx[3], gy[3];
   for (int k = 0; k < 3; ++k) {
-    double dp1 = pr[k] - pl[k], dp2 = pn1[k] - pn0[k];
+    T dp1 = pr[k] - pl[k], dp2 = pn1[k] - pn0[k];
     gx[k] = (dp1 * d2y - dp2 * d1y) / det;
     gy[k] = (d1x * dp2 - d2x * dp1) / det;
   }
-  double gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
-  double gT[2] = {gx[2], gy[2]};
-  double vel[2] = {0.5 * (pl[0] + pr[0]), 0.5 * (pl[1] + pr[1])};
-  double Tf = 0.5 * (pl[2] + pr[2]);
-  double mu = viscosity<double>(Tf, mu_inf, ph.sutherland);
-  double kc = mu / (PR * (gamma - 1.0));
-  double f[4];
-  viscous_flux<double>(vel, gv, gT, sx, sy, mu, kc, nullptr, f);
+  T gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
+  T gT[2] = {gx[2], gy[2]};
+  T vel[2] = {0.5 * (pl[0] + pr[0]), 0.5 * (pl[1] + pr[1])};
+  T Tf = 0.5 * (pl[2] + pr[2]);
+  T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
+  T kc = mu / (PR * (gamma - 1.0));
+  T f[4];
+  viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, nullptr, f);
   *fx = f[1];
   *fy = f[2];
 }
@@ -779,7 +782,7 @@ __global__ void __launch_bounds__(BLOCK) k_forces(Plan P, Physics ph, const Ctl*
     double px = -p * sx, py = -p * sy;
     double tx = 0.0, ty = 0.0;
     if (ph.viscous) {
-      wall_shear(P, ph, Q, base, ctl[cs].mu_inf, fam, ff, &tx, &ty);
+      wall_shear<double>(P, ph, Q, nullptr, base, ctl[cs].mu_inf, fam, ff, &tx, &ty);
       tx *= into;
       ty *= into;
     }
@@ -812,6 +815,74 @@ __global__ void __launch_bounds__(BLOCK) k_forces(Plan P, Physics ph, const Ctl*
     o[3] = (dx * v[3] + dy * v[4]) / q_area;
     o[4] = f2[0];
     o[5] = f2[1];
+    o[6] = 0.0;
+    o[7] = 0.0;
+  }
+}
+
+// The tangent of k_forces' coefficients along (dU, dq_inf, dmu_inf): the
+// same integration on dual numbers, with the primitives' tangent dQ (the
+// parameter-tangent fill) and the freestream's tangent P.dfree, which turns
+// the lift and drag directions and scales the dynamic pressure (the
+// reference's output JVP, src/jaxrans/solvers/sensitivity.py:63 over
+// derivatives.py:82-84).  out[case][8] = d(CL, CD, CM, CD friction, Fx, Fy).
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_forces_tangent(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                          const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                          double length, double cx, double cy,
+                                                          double* __restrict__ out) {
+  bind_case(P, blockIdx.x);
+  __shared__ double sh[14][4];
+  const int cs = blockIdx.x;
+  const size_t base = (size_t)cs * NV * P.nc;
+  const double gamma = ph.gamma;
+  Dual qi[4];
+  for (int k = 0; k < 4; ++k) qi[k] = free_val<true>(ctl[cs].q_inf[k], P.dfree, cs, k);
+  const Dual mu_inf = free_val<true>(ctl[cs].mu_inf, P.dfree, cs, 6);
+  const Dual speed = hypot_j(qi[1], qi[2]);
+  const Dual dx = qi[1] / speed, dy = qi[2] / speed;   // drag direction
+  const Dual lx = -dy, ly = dx;                        // lift direction
+  const Dual q_area = 0.5 * qi[0] * speed * speed * length;
+  // lift, drag, moment, shear x, shear y, Fx, Fy: values then tangents
+  Dual acc[7];
+  for (int k = 0; k < 7; ++k) acc[k] = mkd(0.0, 0.0);
+  for (int w = threadIdx.x; w < P.n_walls; w += BLOCK) {
+    const int cell = P.w_cell[w], fam = P.w_fam[w], ff = P.w_face[w];
+    const double into = P.w_into[w];
+    const double sx = into * (fam == 0 ? P.fi[ff] : P.fj[ff]);
+    const double sy = into * (fam == 0 ? P.fi[P.nfi + ff] : P.fj[P.nfj + ff]);
+    const Dual p = Io<Dual>::ld(Q, dQ, base + (size_t)3 * P.nc + cell) - 1.0 / gamma;
+    const Dual px = -p * sx, py = -p * sy;
+    Dual tx = mkd(0.0, 0.0), ty = mkd(0.0, 0.0);
+    if (ph.viscous) {
+      wall_shear<Dual>(P, ph, Q, dQ, base, mu_inf, fam, ff, &tx, &ty);
+      tx = tx * into;
+      ty = ty * into;
+    }
+    const Dual ldx = px + tx, ldy = py + ty;
+    const double rx = P.w_mid[w] - cx, ry = P.w_mid[P.n_walls + w] - cy;
+    const Dual moment = ry * px - rx * py;
+    acc[0] = acc[0] + (lx * ldx + ly * ldy) / q_area;
+    acc[1] = acc[1] + (dx * ldx + dy * ldy) / q_area;
+    acc[2] = acc[2] + (moment / length) / q_area;
+    acc[3] = acc[3] + tx;
+    acc[4] = acc[4] + ty;
+    acc[5] = acc[5] + ldx;
+    acc[6] = acc[6] + ldy;
+  }
+  double v[14];
+  for (int k = 0; k < 7; ++k) { v[k] = acc[k].v; v[7 + k] = acc[k].d; }
+  block_sum<14>(v, sh);
+  if (threadIdx.x == 0) {
+    const Dual shear_x = mkd(v[3], v[10]), shear_y = mkd(v[4], v[11]);
+    const Dual friction = (dx * shear_x + dy * shear_y) / q_area;
+    double* o = out + (size_t)cs * 8;
+    o[0] = v[7];
+    o[1] = v[8];
+    o[2] = v[9];
+    o[3] = friction.d;
+    o[4] = v[12];
+    o[5] = v[13];
     o[6] = 0.0;
     o[7] = 0.0;
   }

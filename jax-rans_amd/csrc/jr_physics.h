@@ -24,6 +24,12 @@ __device__ __forceinline__ double hypot_j(double x, double y) {
   double r = lo / hi;
   return hi * sqrt(1.0 + r * r);
 }
+// ... and its derivative (x dx + y dy) / hypot for a freestream that carries
+// a parameter tangent.
+__device__ __forceinline__ Dual hypot_j(Dual x, Dual y) {
+  const double h = hypot_j(x.v, y.v);
+  return mkd(h, (x.v * x.d + y.v * y.d) / h);
+}
 
 // conservative -> primitive (src/jaxrans/core/state.py:141-146, numerics/residual.py:46-51)
 template <class T, int NV>
@@ -210,18 +216,20 @@ __device__ void far_field_state(const T* q, double nx, double ny, const TR* qf, 
 // (dx, dy) from its centre, isentropic at the freestream total enthalpy.
 // Evaluated without contraction into FMAs, so that no circulation gives the
 // freestream bit for bit, as the reference's does.
-template <class TR, int NV>
-__device__ void vortex_state(double dx, double dy, const double* qi, double gamma, TR circ, TR* qr) {
+// qi: the freestream, plain (TQ = double) or carrying a parameter tangent
+// (TQ = TR = Dual).
+template <class TR, int NV, class TQ>
+__device__ void vortex_state(double dx, double dy, const TQ* qi, double gamma, TR circ, TR* qr) {
 #pragma clang fp contract(off)
-  const double rho_f = qi[0], u_f = qi[1], v_f = qi[2], p_f = qi[3];
-  const double a2_f = gamma * p_f / rho_f;
-  const double speed2_f = u_f * u_f + v_f * v_f;
-  const double mach2 = speed2_f / a2_f;
-  const double alpha = atan2(v_f, u_f);
+  const TQ rho_f = qi[0], u_f = qi[1], v_f = qi[2], p_f = qi[3];
+  const TQ a2_f = gamma * p_f / rho_f;
+  const TQ speed2_f = u_f * u_f + v_f * v_f;
+  const TQ mach2 = speed2_f / a2_f;
+  const TQ alpha = jatan2(v_f, u_f);
   const double r = hypot_j(dx, dy);
   const double theta = atan2(dy, dx);
-  const double s = sin(theta - alpha);
-  TR strength = circ * sqrt(1.0 - mach2) / (2.0 * M_PI * r * (1.0 - mach2 * (s * s)));
+  const TQ s = jsin(theta - alpha);
+  TR strength = circ * jsqrt(1.0 - mach2) / (2.0 * M_PI * r * (1.0 - mach2 * (s * s)));
   TR u = u_f + strength * sin(theta);
   TR v = v_f - strength * cos(theta);
   TR ratio = 1.0 + 0.5 * (gamma - 1.0) * (speed2_f - u * u - v * v) / a2_f;
@@ -233,9 +241,10 @@ __device__ void vortex_state(double dx, double dy, const double* qi, double gamm
 }
 
 // Ghost primitives from the source cell (src/jaxrans/numerics/boundary.py:54-157).
-// qf: the far-field reference (freestream) primitives.
-template <class T, int NV>
-__device__ void bc_state(int kind, const T* q, double nx, double ny, const double* qf, double gamma, T* g) {
+// qf: the far-field reference (freestream) primitives, plain (TQ = double)
+// or carrying a parameter tangent (TQ = T = Dual).
+template <class T, int NV, class TQ>
+__device__ void bc_state(int kind, const T* q, double nx, double ny, const TQ* qf, double gamma, T* g) {
   if (kind == K_SLIP || kind == K_SYMMETRY) {
     T vn = q[1] * nx + q[2] * ny;
     g[0] = q[0];
@@ -247,14 +256,14 @@ __device__ void bc_state(int kind, const T* q, double nx, double ny, const doubl
     g[0] = q[0]; g[1] = -q[1]; g[2] = -q[2]; g[3] = q[3];
     if constexpr (NV > 4) g[4] = -q[4];
   } else if (kind == K_FARFIELD) {
-    far_field_state<T, double, NV>(q, nx, ny, qf, gamma, g);
+    far_field_state<T, TQ, NV>(q, nx, ny, qf, gamma, g);
   } else if (kind == K_INFLOW) {
     // total_condition_inflow (boundary.py:130-147)
     double g1 = gamma - 1.0;
-    double t_f = gamma * qf[3] / qf[0];
-    double speed_f = hypot_j(qf[1], qf[2]);
-    double t_tot = t_f + 0.5 * g1 * speed_f * speed_f;
-    double p_tot = qf[3] * pow(t_tot / t_f, gamma / g1);
+    TQ t_f = gamma * qf[3] / qf[0];
+    TQ speed_f = hypot_j(qf[1], qf[2]);
+    TQ t_tot = t_f + 0.5 * g1 * speed_f * speed_f;
+    TQ p_tot = qf[3] * jpow(t_tot / t_f, gamma / g1);
     T p_b = q[3];
     T mach2 = jmax(2.0 / g1 * (jpow(p_tot / p_b, g1 / gamma) - 1.0), lift<T>(0.0));
     T t_b = t_tot / (1.0 + 0.5 * g1 * mach2);
@@ -263,9 +272,9 @@ __device__ void bc_state(int kind, const T* q, double nx, double ny, const doubl
     g[1] = speed * qf[1] / speed_f;
     g[2] = speed * qf[2] / speed_f;
     g[3] = p_b;
-    if constexpr (NV > 4) g[4] = lift<T>(qf[4]);
+    if constexpr (NV > 4) g[4] = to_t<T>(qf[4]);
   } else if (kind == K_OUTFLOW) {
-    g[0] = q[0]; g[1] = q[1]; g[2] = q[2]; g[3] = lift<T>(qf[3]);
+    g[0] = q[0]; g[1] = q[1]; g[2] = q[2]; g[3] = to_t<T>(qf[3]);
     if constexpr (NV > 4) g[4] = q[4];
   } else {
     for (int k = 0; k < NV; ++k) g[k] = q[k];
@@ -273,9 +282,10 @@ __device__ void bc_state(int kind, const T* q, double nx, double ny, const doubl
 }
 
 // Viscosity law (src/jaxrans/physics/navier_stokes.py:154-175).
-template <class T>
-__device__ __forceinline__ T viscosity(T Tm, double mu_inf, double suth) {
-  if (suth <= 0.0) return lift<T>(mu_inf);
+// mu_inf plain (TM = double) or carrying a parameter tangent (TM = T = Dual).
+template <class T, class TM>
+__device__ __forceinline__ T viscosity(T Tm, TM mu_inf, double suth) {
+  if (suth <= 0.0) return to_t<T>(mu_inf);
   return mu_inf * Tm * jsqrt(Tm) * (1.0 + suth) / (Tm + suth);
 }
 

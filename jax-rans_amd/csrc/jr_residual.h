@@ -47,8 +47,8 @@ template <> __device__ __forceinline__ Dual circ_of<Dual>(const double* c, int c
   return mkd(c[2 * cs], c[2 * cs + 1]);
 }
 
-template <class T, int NV>
-__device__ __forceinline__ void ghost_of(const Plan& P, int stage, int t, const T* q, const double* q_inf,
+template <class T, int NV, class TQ>
+__device__ __forceinline__ void ghost_of(const Plan& P, int stage, int t, const T* q, const TQ* q_inf,
                                          double gamma, T* g, int cs) {
   const int kind = P.t_kind[stage][t];
   if (kind == K_COPY) {
@@ -69,7 +69,10 @@ __device__ __forceinline__ void ghost_of(const Plan& P, int stage, int t, const 
   }
 }
 
-template <class T, int NV>
+//
+// PT (with T = Dual): the freestream carries its tangent along a flow
+// parameter (P.dfree), so the tangent is that of the fill along (dU, dq_inf).
+template <class T, int NV, bool PT = false>
 __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl* __restrict__ ctl,
                                                 const double* __restrict__ U, const double* __restrict__ dU,
                                                 double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
@@ -79,7 +82,9 @@ __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl*
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
-  const double* q_inf = ctl[cs].q_inf;
+  typename Free<PT>::type qfree[6];
+  for (int k = 0; k < 6; ++k) qfree[k] = free_val<PT>(ctl[cs].q_inf[k], P.dfree, cs, k);
+  const typename Free<PT>::type* q_inf = qfree;
   T q[NV], g[NV];
   int dst;
   if (t < nc) {
@@ -245,7 +250,8 @@ __global__ void __launch_bounds__(BLOCK) k_diamond(Plan P, int cs, double* __res
   for (int k = 0; k < 4; ++k) out[(size_t)k * nft + f] = d[k];
 }
 
-template <class T, int NV, bool VISC>
+// PT (with T = Dual): mu_inf carries its tangent along a flow parameter.
+template <class T, int NV, bool VISC, bool PT = false>
 __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
                                                  const double* __restrict__ Nd, const double* __restrict__ dNd,
@@ -318,7 +324,7 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
   if constexpr (NV > 4) Fl[4] = lift<T>(0.0);
 
   if (VISC) {
-    const double mu_inf = ctl[cs].mu_inf;
+    const auto mu_inf = free_val<PT>(ctl[cs].mu_inf, P.dfree, cs, 6);
     // Diamond gradient: the cells' centre difference d1 and the face's node
     // difference d2, precomputed per grid (k_diamond).
     const int nft = P.nfi + P.nfj;
@@ -452,7 +458,10 @@ __global__ void __launch_bounds__(BLOCK) k_sa_lin(Plan P, Physics ph, const Ctl*
   L[(size_t)5 * nc] = rg * (2.0 * gny);
 }
 
-template <class T, int NV>
+// PT (with T = Dual): the freestream and mu_inf carry their tangent along a
+// flow parameter -- U0 moves with it, and the SA source is differentiated in
+// full (its partials at the state hold dU only).
+template <class T, int NV, bool PT = false>
 __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ U, const double* __restrict__ dU,
                                                  const double* __restrict__ Q, const double* __restrict__ dQ,
@@ -473,6 +482,11 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
       double u0[5];
       conservative<double, NV>(ctl[cs].q_inf, ph.gamma, u0);
       for (int k = 0; k < NV; ++k) out[base + (size_t)k * nc + c] = U[base + (size_t)k * nc + c] - u0[k];
+    } else if constexpr (PT) {
+      Dual qf[6], u0[5];
+      for (int k = 0; k < 6; ++k) qf[k] = free_val<true>(ctl[cs].q_inf[k], P.dfree, cs, k);
+      conservative<Dual, NV>(qf, ph.gamma, u0);
+      for (int k = 0; k < NV; ++k) out[base + (size_t)k * nc + c] = dU[base + (size_t)k * nc + c] - u0[k].d;
     } else {
       for (int k = 0; k < NV; ++k) {
         double d = dU[base + (size_t)k * nc + c];
@@ -490,7 +504,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
     r[k] = (Fi[bi + (size_t)k * P.nfi + fi1] - Fi[bi + (size_t)k * P.nfi + fi0]) +
            (Fj[bj + (size_t)k * P.nfj + fj1] - Fj[bj + (size_t)k * P.nfj + fj0]);
   }
-  if constexpr (NV > 4 && tangent) {
+  if constexpr (NV > 4 && tangent && !PT) {
     // the source's tangent from its partials at the state (k_sa_lin): the
     // face-average gradients are linear in dq, the rest is a dot product
     const double* dq = dQ + base;
@@ -517,7 +531,8 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
   } else if constexpr (NV > 4) {
     // SA source at the cell (turbulence.py:140-151): vorticity and grad nu_hat by
     // Green-Gauss over the face-averaged neighbours.
-    const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
+    const double gamma = ph.gamma;
+    const auto nu_inf = free_val<PT>(ctl[cs].mu_inf, P.dfree, cs, 6);
     T qc[NV], qw[NV], qe[NV], qs[NV], qn[NV];
     load_q<T, NV>(Q, dQ, base, nc, c, qc);
     load_q<T, NV>(Q, dQ, base, nc, c - NJ, qw);

@@ -86,6 +86,28 @@ __device__ __forceinline__ double jpow(double x, double c) { return pow(x, c); }
 __device__ __forceinline__ Dual jpow(Dual x, double c) {
   return mkd(pow(x.v, c), c * pow(x.v, c - 1.0) * x.d);
 }
+// Trigonometry for the freestream's parameter tangent (the point-vortex
+// reference turns with the flow angle): atan2's derivative
+// (x dy - y dx) / (x^2 + y^2), sin' = cos, cos' = -sin.
+__device__ __forceinline__ double jsin(double x) { return sin(x); }
+__device__ __forceinline__ Dual jsin(Dual x) { return mkd(sin(x.v), cos(x.v) * x.d); }
+__device__ __forceinline__ double jatan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ Dual jatan2(Dual y, Dual x) {
+  return mkd(atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / (x.v * x.v + y.v * y.v));
+}
+
+// The freestream as the kernels read it: plain values (double), or, in the
+// parameter-tangent kernels, values carrying their tangent along a flow
+// parameter (Dual).  dfree[case][8] = (dq_inf[0..5], dmu_inf, unused).
+template <bool PT> struct Free { using type = double; };
+template <> struct Free<true> { using type = Dual; };
+
+template <bool PT>
+__device__ __forceinline__ typename Free<PT>::type free_val(double v, const double* dfree, int cs, int k) {
+  if constexpr (PT) return mkd(v, dfree[(size_t)cs * 8 + k]);
+  else return v;
+}
+
 // x ** n for a small constant integer exponent (repeated products).
 template <class T> __device__ __forceinline__ T ipow2(T x) { return x * x; }
 template <class T> __device__ __forceinline__ T ipow3(T x) { return x * x * x; }

@@ -51,7 +51,10 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[4]) {
 // wall faces (S into the fluid), circ = (F_y u - F_x v) / |V| / (rho |V|) of
 // the freestream; with dU given, also its derivative along dU.  One workgroup
 // per case; circ[case][2] = (value, derivative).
-template <class T, int NV>
+//
+// PT (with T = Dual): the freestream carries its tangent along a flow
+// parameter (P.dfree), and the derivative is along (dU, dq_inf).
+template <class T, int NV, bool PT = false>
 __global__ void __launch_bounds__(BLOCK) k_circulation(Plan P, double gamma, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ U, const double* __restrict__ dU,
                                                        double* __restrict__ circ, int check_active) {
@@ -60,7 +63,8 @@ __global__ void __launch_bounds__(BLOCK) k_circulation(Plan P, double gamma, con
   const int cs = blockIdx.x;
   if (check_active && !ctl[cs].active) return;
   const size_t base = (size_t)cs * NV * P.nc;
-  const double* qi = ctl[cs].q_inf;
+  typename Free<PT>::type qi[4];
+  for (int k = 0; k < 4; ++k) qi[k] = free_val<PT>(ctl[cs].q_inf[k], P.dfree, cs, k);
   double v[4] = {0.0, 0.0, 0.0, 0.0};   // F_x, F_y and their derivatives
   for (int w = threadIdx.x; w < P.n_walls; w += BLOCK) {
     const int fam = P.w_fam[w], ff = P.w_face[w];
@@ -78,9 +82,17 @@ __global__ void __launch_bounds__(BLOCK) k_circulation(Plan P, double gamma, con
   }
   block_sum<4>(v, sh);
   if (threadIdx.x == 0) {
-    const double speed = hypot_j(qi[1], qi[2]);
-    circ[2 * cs] = (v[1] * qi[1] - v[0] * qi[2]) / speed / (qi[0] * speed);
-    circ[2 * cs + 1] = (v[3] * qi[1] - v[2] * qi[2]) / speed / (qi[0] * speed);
+    if constexpr (PT) {
+      const Dual fx = mkd(v[0], v[2]), fy = mkd(v[1], v[3]);
+      const Dual speed = hypot_j(qi[1], qi[2]);
+      const Dual cc = (fy * qi[1] - fx * qi[2]) / speed / (qi[0] * speed);
+      circ[2 * cs] = cc.v;
+      circ[2 * cs + 1] = cc.d;
+    } else {
+      const double speed = hypot_j(qi[1], qi[2]);
+      circ[2 * cs] = (v[1] * qi[1] - v[0] * qi[2]) / speed / (qi[0] * speed);
+      circ[2 * cs + 1] = (v[3] * qi[1] - v[2] * qi[2]) / speed / (qi[0] * speed);
+    }
   }
 }
 

@@ -70,6 +70,9 @@ struct Plan {
   double vx, vy;
   const double* t_mid[2];
   const double* circ;
+  // The freestream's tangent along a flow parameter, dfree[case][8] =
+  // (dq_inf[0..5], dmu_inf, unused); read only by the parameter-tangent kernels.
+  const double* dfree;
 };
 
 // The geometry of case cs (identity when the cases share it).

@@ -18,23 +18,27 @@ line-preconditioned).
 
 The parameters reach the residual and the force coefficients only through
 the freestream primitives and ``mu_inf = M / Re`` (``flow_inputs``, as
-``derivatives.py:65-70``).  The two parameter products that the reference
-takes by forward-mode differentiation of its traced functions -- ``dR/dp`` at
-fixed ``U`` and the output tangent ``dJ/dp|_U + dJ/dU dU`` -- are formed here
-from device evaluations of the residual and the force integration at
-perturbed freestreams and states, by Richardson's fourth-order central
-difference ``(4 D(h/2) - D(h)) / 3`` of ``D(h) = (f(h) - f(-h)) / 2h``, with
-``h`` = 1e-4 degree for ``alpha`` and 1e-5 for ``mach``: truncation
-``O(h^4)`` and round-off near ``1e-16 / h`` relative.  The steps are small so
-that no far-field face changes between inflow and outflow within them (the
-characteristic state switches on the sign of the boundary normal velocity).
+``derivatives.py:65-70``): ``q = (1, M cos a, M sin a, 1/gamma[, nu ratio])``
+with ``a`` in radians from degrees, so a parameter direction is the
+freestream's tangent ``(dq, dmu)`` in closed form (``freestream_tangent``).
+On 2D contexts the two parameter products that the reference takes by
+forward-mode differentiation -- ``dR/dp`` at fixed ``U``
+(``solvers/sensitivity.py:60``) and the output tangent ``dJ/dp|_U + dJ/dU dU``
+(``:63``) -- are exact: the device carries the freestream's tangent through
+the residual's and the force integration's dual-number kernels
+(``jr_tangent_freestream``, ``jr_forces_tangent``).  3D contexts form the
+two products from device evaluations at perturbed freestreams and states instead, by Richardson's
+fourth-order central difference ``(4 D(h/2) - D(h)) / 3`` of ``D(h) = (f(h) -
+f(-h)) / 2h`` with ``h`` = 1e-4 degree for ``alpha`` and 1e-5 for ``mach``.
 The derivative exists where the discretisation is differentiable
 (``minmod`` and ``van_albada`` have kinks: use ``van_albada_smooth`` or no
-limiter) and is as accurate as the result is converged.
+limiter) and is as accurate as the result is converged.  A swept (2.5D)
+case is refused.
 """
 
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, replace
 
@@ -76,6 +80,24 @@ def flow_inputs(flow: FlowConditions, dim: int, p: dict) -> tuple[np.ndarray, fl
     return freestream_of(replace(flow, alpha_deg=float(p["alpha"]), mach=float(p["mach"])), dim)
 
 
+def freestream_tangent(flow: FlowConditions, dim: int, name: str) -> tuple[np.ndarray, float | None]:
+    """``(dq_inf, dmu_inf)``: the derivative of the freestream primitives and
+    of ``mu_inf = M / Re`` along ``name`` (``alpha`` per degree, ``mach``) at
+    the flow's parameters (``freestream_vector``, ``core/state.py:176-192``;
+    ``None`` for ``dmu_inf`` when inviscid)."""
+    a = math.radians(flow.alpha_deg)
+    if name == "alpha":
+        per = math.pi / 180.0
+        du, dw, dm = -flow.mach * math.sin(a) * per, flow.mach * math.cos(a) * per, 0.0
+    elif name == "mach":
+        du, dw, dm = math.cos(a), math.sin(a), 1.0
+    else:
+        raise ValueError(f"unknown parameter {name!r}; available: {PARAMETERS}")
+    velocity = [du, dw] if dim == 2 else [du, 0.0, dw]
+    dq = np.array([0.0, *velocity, 0.0] + ([0.0] if flow.turbulence is not None else []))
+    return dq, (None if flow.inviscid else dm / flow.reynolds)
+
+
 def defect_correction(matvec, b: np.ndarray, approximate_inverse, tol: float, maxiter: int):
     """``x <- x + G (b - A x)`` from zero until ``|b - A x| <= tol |b|`` or
     ``maxiter`` steps (``solvers/sensitivity.py:34-50``); returns ``x``, its
@@ -110,6 +132,8 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
             raise ValueError(f"unknown {kind}(s) {unknown or given}; available: {known}")
     case, grid = result.case, result.grid
     flow, dim = case.flow, grid.dim
+    if flow.swept:
+        raise ValueError("sensitivity() of a swept (2.5D) case is not supported")
     p0 = {"alpha": float(flow.alpha_deg), "mach": float(flow.mach)}
     U = np.asarray(result.U, np.float64)
     started = time.perf_counter()
@@ -133,7 +157,28 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
             f = _forces(force_rows(ctx, case)[0], dim)
             return np.array([getattr(f, out) for out in outputs])
 
-        rhs = {name: _richardson(lambda t, n=name: residual_at(n, t), STEPS[name]) for name in parameters}
+        exact = ctx.dim == 2
+        reference = case.grid_spec
+        rows = {"cl": 0, "cd": 1, "cm": 2, "cd_friction": 3}
+
+        def parameter_rhs(name: str) -> np.ndarray:
+            if exact:
+                dq, dmu = freestream_tangent(flow, dim, name)
+                return ctx.tangent_freestream(U, dq, dmu)[0]
+            return _richardson(lambda t: residual_at(name, t), STEPS[name])
+
+        def output_tangent(name: str, dU: np.ndarray) -> np.ndarray:
+            if exact:
+                dq, dmu = freestream_tangent(flow, dim, name)
+                row = ctx.forces_tangent(U, dq, dmu, dU=dU, length=reference.reference_length,
+                                         centre=tuple(reference.moment_centre[:2]))[0]
+                return np.array([row[rows[out]] for out in outputs])
+            return _richardson(lambda t: coefficients_at(name, dU, t), STEPS[name])
+
+        rhs = {}
+        for name in parameters:
+            at(p0)
+            rhs[name] = parameter_rhs(name)
 
         def matvec(x: np.ndarray) -> np.ndarray:
             return ctx.tangent(U, x, first_order=False)[0]
@@ -146,7 +191,7 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
             at(p0, U)   # the output differences below move the freestream and the state
             ctx.newton_init(case.implicit.cfl_max)
             dU, errors[name], iterations[name] = defect_correction(matvec, -rhs[name], inverse, tol, maxiter)
-            columns[name] = _richardson(lambda t, n=name, d=dU: coefficients_at(n, d, t), STEPS[name])
+            columns[name] = output_tangent(name, dU)
     finally:
         ctx.close()
     derivatives = {out: {name: float(columns[name][i]) for name in parameters} for i, out in enumerate(outputs)}

@@ -199,3 +199,37 @@ def pad_edge(x, widths):
 
 def isfinite(x):
     return np.isfinite(_val(x))
+
+
+# --- the freestream as a function of the flow parameters (sensitivity) --------------
+
+def sin(x):
+    if isinstance(x, D):
+        return D(np.sin(x.v), np.cos(x.v) * x.d)
+    return np.sin(x)
+
+
+def cos(x):
+    if isinstance(x, D):
+        return D(np.cos(x.v), -np.sin(x.v) * x.d)
+    return np.cos(x)
+
+
+def arctan2(y, x):
+    """``atan2`` with the tangent ``(x dy - y dx) / (x^2 + y^2)``."""
+    if not is_dual(y, x):
+        return np.arctan2(y, x)
+    yv, xv = _val(y), _val(x)
+    return D(np.arctan2(yv, xv), (xv * _tan(y, xv) - yv * _tan(x, yv)) / (xv * xv + yv * yv))
+
+
+def scalar(x):
+    """A freestream component: a dual scalar kept, a number as ``np.float64``."""
+    return x if isinstance(x, D) else np.float64(x)
+
+
+def full(shape, x):
+    """``x`` (number or dual scalar) broadcast to ``shape``."""
+    if isinstance(x, D):
+        return D(np.full(shape, x.v), np.full(shape, x.d))
+    return np.full(shape, np.float64(x))

@@ -37,6 +37,17 @@ def hypot_jnp(x, y):
     return np.where(hi == 0, hi, hi * np.sqrt(1.0 + (lo / safe) ** 2))
 
 
+def speed_of(u, v):
+    """The freestream speed ``hypot(u, v)`` as a scalar: ``hypot_jnp`` on
+    numbers; on dual scalars (a freestream carrying a parameter tangent) its
+    value with the tangent ``(u du + v dv) / hypot``."""
+    if M.is_dual(u, v):
+        uv, vv = M.value(u), M.value(v)
+        h = float(hypot_jnp(np.float64(uv), np.float64(vv)))
+        return M.D(h, (uv * M.tangent(u) + vv * M.tangent(v)) / h)
+    return float(hypot_jnp(np.float64(u), np.float64(v)))
+
+
 def primitive(U, gamma):
     """``(rho, u, v, p[, nu_hat])`` rows from conservative rows
     (src/jaxrans/numerics/residual.py:46-51, core/state.py:141-146)."""
@@ -241,20 +252,20 @@ def point_vortex_state(dx, dy, q_inf, gamma, circulation):
     at the freestream total enthalpy; rows ``(rho, u, v, p[, scalars])``
     (src/jaxrans/numerics/boundary.py:181-214: Thomas and Salas 1986,
     the Prandtl-Glauert-scaled vortex).  ``circulation`` may be dual."""
-    rho_f, u_f, v_f, p_f = (np.float64(x) for x in q_inf[:4])
+    rho_f, u_f, v_f, p_f = (M.scalar(x) for x in q_inf[:4])
     a2_f = gamma * p_f / rho_f
     speed2_f = u_f * u_f + v_f * v_f
     mach2 = speed2_f / a2_f
-    alpha = np.arctan2(v_f, u_f)
+    alpha = M.arctan2(v_f, u_f)
     r = hypot_jnp(dx, dy)
     theta = np.arctan2(dy, dx)
-    strength = circulation * np.sqrt(1.0 - mach2) / (2.0 * np.pi * r * (1.0 - mach2 * np.sin(theta - alpha) ** 2))
+    strength = circulation * M.sqrt(1.0 - mach2) / (2.0 * np.pi * r * (1.0 - mach2 * M.sin(theta - alpha) ** 2))
     u = u_f + strength * np.sin(theta)
     v = v_f - strength * np.cos(theta)
     ratio = 1.0 + 0.5 * (gamma - 1.0) * (speed2_f - u * u - v * v) / a2_f
     rho = rho_f * ratio ** (1.0 / (gamma - 1.0))
     p = p_f * ratio ** (gamma / (gamma - 1.0))
-    return [rho, u, v, p] + [np.full(np.shape(dx), np.float64(x)) for x in q_inf[4:]]
+    return [rho, u, v, p] + [M.full(np.shape(dx), x) for x in q_inf[4:]]
 
 
 def inflow_total(q, q_inf, gamma):
@@ -262,7 +273,7 @@ def inflow_total(q, q_inf, gamma):
     g1 = gamma - 1.0
     rho_f, u_f, v_f, p_f = q_inf[:4]
     t_f = gamma * p_f / rho_f
-    speed_f = float(hypot_jnp(np.float64(u_f), np.float64(v_f)))
+    speed_f = speed_of(u_f, v_f)
     t_tot = t_f + 0.5 * g1 * speed_f * speed_f
     p_tot = p_f * (t_tot / t_f) ** (gamma / g1)
     p_b = q[3]

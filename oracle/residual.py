@@ -63,9 +63,14 @@ def rows(U):
     return [U[k] for k in range(len(U))]
 
 
-def U0_of(setup: Setup) -> np.ndarray:
+def U0_of(setup: Setup):
+    """The uniform freestream state on the canvas ``(nvar, NI, NJ)``; dual
+    when the freestream carries a parameter tangent."""
     NI, NJ = setup.atlas.shape
-    c = np.array([np.float64(x) for x in P.conservative(list(setup.q_inf), setup.gamma)])
+    rows_ = P.conservative([M.scalar(x) for x in setup.q_inf], setup.gamma)
+    if M.is_dual(*rows_):
+        return M.stack([M.full((NI, NJ), x) for x in rows_])
+    c = np.array([np.float64(x) for x in rows_])
     return np.broadcast_to(c[:, None, None], (c.size, NI, NJ)).copy()
 
 
@@ -113,8 +118,8 @@ def circulation(q, setup: Setup):
                  fj[:, np.minimum(w.face, fj.shape[1] - 1)]) * w.into_fluid
     p = M.take(q[3], w.cell) - setup.q_inf[3]
     fx, fy = -_dsum(p * S[0]), -_dsum(p * S[1])
-    rho_f, u_f, v_f = (np.float64(x) for x in setup.q_inf[:3])
-    speed = float(P.hypot_jnp(u_f, v_f))
+    rho_f, u_f, v_f = (M.scalar(x) for x in setup.q_inf[:3])
+    speed = P.speed_of(u_f, v_f)
     return (fy * u_f - fx * v_f) / speed / (rho_f * speed)
 
 

@@ -18,7 +18,7 @@ import numpy as np
 from . import dual as M
 from . import physics as P
 from .residual import (Setup, face_averages, green_gauss, jvp, norm, residual, rows, wave_speed_sum,
-                       wave_speeds, fill_ghosts, _lr, face_gradients)
+                       wave_speeds, fill_ghosts, _dsum, _lr, face_gradients)
 
 # --- small dense blocks ---------------------------------------------------------------
 
@@ -391,7 +391,10 @@ def newton_step(state: NewtonState, setup: Setup, cfg, method: str | None = None
 
 
 def forces(U, setup: Setup, length=1.0, centre=(0.25, 0.0)):
-    """CL, CD, CM (pressure-only moment), friction drag (forces.py:48-161)."""
+    """CL, CD, CM (pressure-only moment), friction drag (forces.py:48-161).
+    Dual when ``U`` or the freestream (``setup.q_inf`` components,
+    ``setup.mu_inf``) carries a tangent: the output JVP of the sensitivity
+    (``src/jaxrans/solvers/sensitivity.py:63``)."""
     at, gamma = setup.atlas, setup.gamma
     W = at.walls
     NI, NJ = at.shape
@@ -405,7 +408,7 @@ def forces(U, setup: Setup, length=1.0, centre=(0.25, 0.0)):
         lam_setup = Setup(at, gamma, setup.q_inf[:4], setup.flux, setup.limiter, setup.kappa, setup.mu_inf,
                           setup.sutherland, False, 0.0)
         flat = fill_ghosts([r.reshape(-1) for r in q], lam_setup)
-        qc = [np.pad(r.reshape(NI, NJ), 1, mode="edge") for r in flat]
+        qc = [M.pad_edge(r.reshape(NI, NJ), 1) for r in flat]
         T1 = P.temperature(qc, gamma)
         grads = face_gradients([qc[1], qc[2], T1], at.centres)
         vis = []
@@ -418,22 +421,23 @@ def forces(U, setup: Setup, length=1.0, centre=(0.25, 0.0)):
             k = mu / (P.PR * (gamma - 1.0))
             f = P.viscous_flux(vel, [[gx[0], gy[0]], [gx[1], gy[1]]], [gx[2], gy[2]],
                                (at.face_i, at.face_j)[a], mu, k)
-            vis.append(np.stack([f[1].reshape(-1), f[2].reshape(-1)]))
-        shear = W.into_fluid * np.where(W.family == 0, vis[0][:, np.where(W.family == 0, W.face, 0)],
-                                        vis[1][:, np.where(W.family == 1, W.face, 0)])
-    rho, u, v = setup.q_inf[0], setup.q_inf[1], setup.q_inf[2]
-    speed = float(P.hypot_jnp(np.float64(u), np.float64(v)))
+            vis.append(M.stack([f[1].reshape(-1), f[2].reshape(-1)]))
+        shear = W.into_fluid * M.where(W.family == 0, vis[0][:, np.where(W.family == 0, W.face, 0)],
+                                       vis[1][:, np.where(W.family == 1, W.face, 0)])
+    rho, u, v = (M.scalar(x) for x in setup.q_inf[:3])
+    speed = P.speed_of(u, v)
     pressure = -p * S
     load = pressure + shear
-    drag = np.array([u, v]) / speed
-    lift = np.array([-drag[1], drag[0]])
+    dx, dy = u / speed, v / speed           # drag direction
+    lx, ly = -dy, dx                        # lift direction
     q_area = 0.5 * rho * speed**2 * length
     r = W.mid - np.asarray(centre[:2])
     moment = r[:, 1] * pressure[0] - r[:, 0] * pressure[1]
-    per = np.stack([lift @ load, drag @ load, moment / length]) / q_area
-    tot = per.sum(axis=1)
-    return {"cl": tot[0], "cd": tot[1], "cm": tot[2], "cd_friction": drag @ shear.sum(axis=1) / q_area,
-            "fx": load.sum(axis=1)[0], "fy": load.sum(axis=1)[1]}
+    cl = _dsum((lx * load[0] + ly * load[1]) / q_area)
+    cd = _dsum((dx * load[0] + dy * load[1]) / q_area)
+    cm = _dsum(moment / length / q_area)
+    return {"cl": cl, "cd": cd, "cm": cm, "cd_friction": (dx * _dsum(shear[0]) + dy * _dsum(shear[1])) / q_area,
+            "fx": _dsum(load[0]), "fy": _dsum(load[1])}
 
 
 # --- driver ----------------------------------------------------------------------------

@@ -21,6 +21,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from jaxrans_amd.core.config import FlowConditions, NumericsConfig
+
 pytestmark = pytest.mark.gpu
 
 CASE = Path(__file__).resolve().parent / "golden" / "cases" / "euler_o48"
@@ -74,6 +76,93 @@ def test_derivatives_match_differences_of_converged_runs(base):
             got = s.derivatives[out][name]
             print(f"[gpu]   {flow} d{out}/d{name}: sensitivity {got:+.10e}  converged differences {fd:+.10e}", flush=True)
             assert got == pytest.approx(fd, rel=1e-4, abs=1e-8), (out, name)
+
+
+PT_CASES = {
+    "euler-hllc-vas": ("o", FlowConditions(mach=0.5, alpha_deg=1.25),
+                       NumericsConfig(flux="hllc", limiter="van_albada_smooth")),
+    "laminar-sutherland": ("ol", FlowConditions(mach=0.5, alpha_deg=1.0, reynolds=5e3, sutherland_ratio=198.6 / 460.0),
+                           NumericsConfig(flux="hllc", limiter="van_albada_smooth")),
+    "sa-roe": ("c", FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2"),
+               NumericsConfig(flux="roe", limiter="van_albada_smooth")),
+    "qcr-hllc": ("c", FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2_qcr"),
+                 NumericsConfig(flux="hllc", limiter="van_albada_smooth")),
+    "vortex": ("o", FlowConditions(mach=0.6, alpha_deg=3.0),
+               NumericsConfig(flux="hllc", limiter="van_albada_smooth", far_field="characteristic_point_vortex")),
+}
+
+
+@pytest.mark.parametrize("case", sorted(PT_CASES))
+def test_parameter_tangents_match_the_oracle(case):
+    """jr_tangent_freestream and jr_forces_tangent against the oracle's dual
+    freestream (oracle/sensitivity.py) on a perturbed state: the residual's
+    parameter JVP, with and without a state direction, to 1e-11 normwise and
+    the force coefficients' JVP to 1e-10 relative."""
+    from conftest import c_atlas, o_atlas, perturbed
+    from jaxrans_amd.derivatives import freestream_tangent
+    from jaxrans_amd.device import context_for
+    from oracle import residual as OR
+    from oracle import sensitivity as SE
+    print(f"[gpu] parameter tangent {case}", flush=True)
+    kind, flow, num = PT_CASES[case]
+    atlas = {"o": lambda: o_atlas(), "ol": lambda: o_atlas(wall="no_slip_wall"), "c": lambda: c_atlas()}[kind]()
+    st = OR.setup_for(atlas, flow, num)
+    U = perturbed(OR.U0_of(st), 1e-2)
+    rng = np.random.default_rng(13)
+    dU = rng.standard_normal(U.shape) * np.abs(U).mean(axis=(1, 2), keepdims=True) * 1e-2
+    ctx = context_for(atlas, flow, num)
+    try:
+        ctx.set_freestream(st.q_inf, None if flow.inviscid else flow.mu_inf)
+        for name in ("alpha", "mach"):
+            q, mu = SE.freestream_dual(flow, name, U.shape[0])
+            dq, dmu = freestream_tangent(flow, 2, name)
+            assert np.allclose(dq, SE.tangents(q, mu)[0], rtol=1e-14, atol=1e-17)
+            ref = SE.residual_parameter_jvp(U, st, q, mu)
+            got = ctx.tangent_freestream(U, dq, dmu)[0]
+            e0 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            ref_u = SE.residual_parameter_jvp(U, st, q, mu, dU)
+            got_u = ctx.tangent_freestream(U, dq, dmu, dU=dU)[0]
+            e1 = np.linalg.norm(got_u - ref_u) / np.linalg.norm(ref_u)
+            f_ref = SE.forces_jvp(U, dU, st, q, mu)
+            f_got = ctx.forces_tangent(U, dq, dmu, dU=dU)[0]
+            keys = ("cl", "cd", "cm", "cd_friction", "fx", "fy")
+            fr = np.array([f_ref[k] for k in keys])
+            e2 = np.abs(f_got[:6] - fr).max() / np.abs(fr).max()
+            print(f"[gpu]   d/d{name}: residual {e0:.2e}, with dU {e1:.2e}, forces {e2:.2e}", flush=True)
+            assert e0 < 1e-11 and e1 < 1e-11 and e2 < 1e-10, (name, e0, e1, e2)
+        # the state tangent alone through the same kernels is jr_tangent's
+        zero = np.zeros(5)
+        got = ctx.tangent_freestream(U, zero, 0.0 if not flow.inviscid else None, dU=dU)[0]
+        assert np.linalg.norm(got - OR.jvp(U, dU, st, first_order=False)) <= 1e-11 * np.linalg.norm(got)
+    finally:
+        ctx.close()
+
+
+def test_sensitivity_matches_the_oracle_linearisation():
+    """sensitivity() with the case's own smooth limiter against the oracle's
+    sensitivity solve at the same converged state: the exact parameter
+    tangents, the defect correction with the Newton step's GMRES, and the
+    output tangents, to 1e-8 relative."""
+    from jaxrans_amd import load_case, run, sensitivity
+    from oracle import residual as OR
+    from oracle import sensitivity as SE
+    case = load_case(CASE, ("numerical.convective_fluxes.limiter=van_albada_smooth", "numerical.convergence.tol=1e-11",
+                            "numerical.convergence.max_iters=3000"))
+    result = run(case)
+    print(f"[gpu] sensitivity vs oracle: {result.history.summary()}", flush=True)
+    assert result.history.reason == "converged"
+    s = sensitivity(result, ("cl", "cd", "cm"), ("alpha", "mach"), tol=1e-11, maxiter=3000)
+    st = OR.setup_for(result.grid, case.flow, case.numerics)
+    spec = case.grid_spec
+    d, errors, steps = SE.sensitivity(result.U, st, case.flow, case.implicit, ("cl", "cd", "cm"), ("alpha", "mach"),
+                                      tol=1e-11, maxiter=3000, length=spec.reference_length,
+                                      centre=tuple(spec.moment_centre[:2]))
+    print(f"[gpu]   device steps {s.iterations}, oracle steps {steps}", flush=True)
+    for out in ("cl", "cd", "cm"):
+        for name in ("alpha", "mach"):
+            got, ref = s.derivatives[out][name], d[out][name]
+            print(f"[gpu]   d{out}/d{name}: device {got:+.12e}  oracle {ref:+.12e}", flush=True)
+            assert got == pytest.approx(ref, rel=1e-8, abs=1e-12), (out, name)
 
 
 def test_state_and_result_are_left_unchanged(base):

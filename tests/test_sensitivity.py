@@ -8,8 +8,32 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from jaxrans_amd.derivatives import OUTPUTS, PARAMETERS, _richardson, defect_correction, flow_inputs, sensitivity
+from jaxrans_amd.derivatives import (OUTPUTS, PARAMETERS, _richardson, defect_correction, flow_inputs,
+                                     freestream_tangent, sensitivity)
 from jaxrans_amd.core.config import FlowConditions
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("name", PARAMETERS)
+@pytest.mark.parametrize("flow", [FlowConditions(mach=0.5, alpha_deg=1.25),
+                                  FlowConditions(mach=0.3, alpha_deg=-4.0, reynolds=5e3),
+                                  FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2")])
+def test_freestream_tangent_is_the_derivative_of_the_flow_inputs(flow, name, dim):
+    """The closed-form (dq_inf, dmu_inf) against Richardson differences of
+    flow_inputs (derivatives.py:65-70) in both parameters."""
+    dq, dmu = freestream_tangent(flow, dim, name)
+    p0 = {"alpha": flow.alpha_deg, "mach": flow.mach}
+
+    def q_at(t):
+        q, mu = flow_inputs(flow, dim, {k: v + (t if k == name else 0.0) for k, v in p0.items()})
+        return np.concatenate([q, [0.0 if mu is None else mu]])
+
+    fd = _richardson(q_at, 1e-2)   # trigonometric in alpha, linear in M: O(h^4) is round-off here
+    assert dq.shape == flow_inputs(flow, dim, p0)[0].shape
+    assert np.allclose(dq, fd[:-1], rtol=1e-9, atol=1e-13)
+    assert (dmu is None) == flow.inviscid
+    if dmu is not None:
+        assert dmu == pytest.approx(fd[-1], rel=1e-9)
 
 
 def test_defect_correction_solves_with_an_approximate_inverse():
