@@ -774,12 +774,13 @@ __device__ __forceinline__ void flux_jacobian3(const double* q, const double* s,
 // blk3[slot] = [d, Lo(25), Up(25), sa_lo, sa_d, sa_up]
 constexpr int BLKW3 = 54;
 
-template <int NV>
+// Stored in the factor's precision R (FP32 in the mixed path).
+template <class R, int NV>
 __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
                                                    const double* __restrict__ U, const double* __restrict__ lam,
                                                    const double* __restrict__ lsum, const double* __restrict__ sam,
                                                    const double* __restrict__ sad, const double* __restrict__ sa_diag,
-                                                   double* __restrict__ blk) {
+                                                   R* __restrict__ blk) {
   // thread t: step k = t / n_lines of chain ch = t mod n_lines, so that the
   // block components are written (and read by k_thomas3) chain-minor
   const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -797,9 +798,9 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
   const int cell = P.l_cell[s];
   // component e of this step at o[e * cstride]
   const size_t cstride = (size_t)P.max_len * nl;
-  double* o = blk + (size_t)cs * BLKW3 * cstride + (size_t)k * nl + ch;
+  R* o = blk + (size_t)cs * BLKW3 * cstride + (size_t)k * nl + ch;
   double blk_lo[25], blk_up[25];
-  o[0] = lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5);
+  o[0] = (R)(lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5));
   auto link = [&](int nb_cell, int fce, double sg, double* dst) {
     double sv[3];
     face_vec3(P, fce, sv);
@@ -818,8 +819,8 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
   if (has_up) link(P.l_cell[s + 1], P.l_up[s], (double)P.l_ups[s], blk_up);
   else for (int e = 0; e < 25; ++e) blk_up[e] = 0.0;
   for (int e = 0; e < 25; ++e) {
-    o[(1 + e) * cstride] = blk_lo[e];
-    o[(26 + e) * cstride] = blk_up[e];
+    o[(1 + e) * cstride] = (R)blk_lo[e];
+    o[(26 + e) * cstride] = (R)blk_up[e];
   }
   if constexpr (NV > 5) {
     const double* mj = sam + (size_t)cs * P.nft;
@@ -835,9 +836,9 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
       const double sg = P.l_ups[s];
       up = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s + 1]];
     }
-    o[51 * cstride] = lo;
-    o[52 * cstride] = sa_diag[(size_t)cs * nc + cell];
-    o[53 * cstride] = up;
+    o[51 * cstride] = (R)lo;
+    o[52 * cstride] = (R)sa_diag[(size_t)cs * nc + cell];
+    o[53 * cstride] = (R)up;
   }
 }
 
@@ -901,13 +902,13 @@ __device__ __forceinline__ void fac3_values(const R* __restrict__ step_base, R* 
 }
 
 template <class R, int NV>
-__global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restrict__ blk, R* __restrict__ fac,
+__global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const R* __restrict__ blk, R* __restrict__ fac,
                                                 R* __restrict__ fac_sa) {
   const int ch = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
   if (ch >= P.n_lines) return;
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0, nl = P.n_lines;
   const size_t cstride = (size_t)P.max_len * nl;
-  const double* B = blk + (size_t)cs * BLKW3 * cstride + ch;   // component e of step k at B[e cstride + k nl]
+  const R* B = blk + (size_t)cs * BLKW3 * cstride + ch;   // component e of step k at B[e cstride + k nl]
   // factor tiles of 64 chains, values packed W to a vector (FAC3_VALUES);
   // SA [case][chain / 64][step][3][64]
   constexpr int W = Vec16<R>::W;
@@ -921,8 +922,8 @@ __global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const double* __restric
   R cps = R(0);
   for (int k = 0; k < n; ++k) {
     // the step's 54 block values, all loaded before the first factor store
-    const double* b0 = B + (size_t)k * nl;
-    double bk[BLKW3];
+    const R* b0 = B + (size_t)k * nl;
+    R bk[BLKW3];
 #pragma unroll
     for (int e = 0; e < BLKW3; ++e) bk[e] = b0[(size_t)e * cstride];
     auto b = [&](int e) { return bk[e]; };

@@ -251,14 +251,16 @@ static void launch_speeds(jr_ctx* c, const double* U) {
 template <class R, int NV>
 static void launch_factor(jr_ctx* c, const double* U) {
   const Plan& P = c->P;
-  hipLaunchKernelGGL((k_blocks<NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl,
-                     U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
+  // the chain blocks in the factor's precision (the buffer holds FP64 blocks, so FP32 ones fit)
+  R* blk = reinterpret_cast<R*>(c->blk);
+  hipLaunchKernelGGL((k_blocks<R, NV>), grid1(P.n_slots, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, blk);
   if (c->thomas)
     JR_LAUNCH(PR_FACTOR, (k_thomas<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
-              (const int*)c->lorder, (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa);
+              (const int*)c->lorder, (const R*)blk, (R*)c->fac, (R*)c->fac_sa);
   else
     JR_LAUNCH(PR_FACTOR, (k_cr_factor<R, NV>), dim3(P.n_lines, c->n_cases), dim3(c->cr_threads), 0, P,
-              (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
+              (const R*)blk, (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
 }
 
 template <class R, int NV>
@@ -483,10 +485,11 @@ static void launch_speeds3(jr_ctx* c, const double* U) {
 template <class R, int NV>
 static void launch_factor3(jr_ctx* c, const double* U) {
   const Plan3& P = c->P3;
-  hipLaunchKernelGGL((k_blocks3<NV>), grid1((size_t)P.max_len * P.n_lines, BLOCK, c->n_cases), dim3(BLOCK), 0,
-                     c->stream, P, c->ph, c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, c->blk);
+  R* blk = reinterpret_cast<R*>(c->blk);   // blocks in the factor's precision
+  hipLaunchKernelGGL((k_blocks3<R, NV>), grid1((size_t)P.max_len * P.n_lines, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph, c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, blk);
   JR_LAUNCH(PR_FACTOR, (k_thomas3<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
-            (const double*)c->blk, (R*)c->fac, (R*)c->fac_sa);
+            (const R*)blk, (R*)c->fac, (R*)c->fac_sa);
 }
 
 template <class R, int NV>

@@ -103,7 +103,7 @@ __device__ __forceinline__ void inv4(const R* A, R* X) {
 // levels (diag, lower, upper) live in the workspace, two halves of [48][L]
 // (SA [3][L]) at 96 s0 (SA 6 s0), alternating with the level parity.
 template <class R, int NV>
-__global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* __restrict__ blk,
+__global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const R* __restrict__ blk,
                                                           R* __restrict__ fac, R* __restrict__ fac_sa,
                                                           R* __restrict__ ws, R* __restrict__ ws_sa) {
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x;
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* 
   if (threadIdx.x == 0) cr_levels(L, c);
   __syncthreads();
   const int O = uni(c.odd0[c.nl]), E = uni(c.even0[c.nl]), NL = uni(c.nl);
-  const double* B = blk + (size_t)cs * P.n_slots * BLKW + (size_t)s0 * BLKW;
+  const R* B = blk + (size_t)cs * P.n_slots * BLKW + (size_t)s0 * BLKW;
   R* Fo = fac + ((size_t)cs * P.n_slots + s0) * CR_FAC;
   R* Fe = Fo + (size_t)32 * O;
   R* Ff = Fe + (size_t)48 * E;
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* 
   // workspace half of the level's parity
   auto ld_blk = [&](int l, int j, int q, R* out) {
     if (l == 0) {
-      const double* b = B + (size_t)j * BLKW;
+      const R* b = B + (size_t)j * BLKW;
       if (q == 0)
         for (int k = 0; k < 16; ++k) out[k] = (k % 5 == 0) ? (R)b[0] : R(0);
       else
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const double* 
   auto ld_sa = [&](int l, int j, R& sd, R& sa_, R& su) {
     if constexpr (NV > 4) {
       if (l == 0) {
-        const double* b = B + (size_t)j * BLKW;
+        const R* b = B + (size_t)j * BLKW;
         sa_ = (R)b[33];
         sd = (R)b[34];
         su = (R)b[35];

@@ -381,14 +381,16 @@ __device__ __forceinline__ void link_block(const double* Ujs, size_t base, int n
 // blk[slot] = [d, Lo(16), Up(16), sa_lo, sa_d, sa_up, pad].  A slot couples to
 // its predecessor when the predecessor's link is live: the last slot of every
 // chain has no link, so the first slot of the next chain never reads across.
+// Stored in the factor's precision R: FP32 in the mixed path (each entry
+// evaluated in FP64 and rounded once), FP64 otherwise.
 constexpr int BLKW = 36;
 
-template <int NV>
+template <class R, int NV>
 __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl* __restrict__ ctl,
                                                   const double* __restrict__ U, const double* __restrict__ lam,
                                                   const double* __restrict__ lsum, const double* __restrict__ sam,
                                                   const double* __restrict__ sad, const double* __restrict__ sa_diag,
-                                                  double* __restrict__ blk) {
+                                                  R* __restrict__ blk) {
   bind_case(P, blockIdx.y);
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
@@ -398,8 +400,8 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
   const double gamma = ph.gamma, cfl = ctl[cs].cfl;
   const double* L = lam + (size_t)cs * nf + P.nfi;
   const int cell = P.l_cell[s];
-  double* o = blk + ((size_t)cs * P.n_slots + s) * BLKW;
-  o[0] = lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5);
+  R* o = blk + ((size_t)cs * P.n_slots + s) * BLKW;
+  o[0] = (R)(lsum[(size_t)cs * nc + cell] * (1.0 / cfl + 0.5));
   double B[4][4];
   const bool has_lo = s > 0 && P.l_dns[s - 1] != 0;
   if (has_lo) {
@@ -408,7 +410,7 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
     link_block<double>(U, base, nc, P.l_cell[s - 1], sg * P.fj[fce], sg * P.fj[P.nfj + fce], L[fce], gamma, B);
   }
   for (int r = 0; r < 4; ++r)
-    for (int c = 0; c < 4; ++c) o[1 + r * 4 + c] = has_lo ? B[r][c] : 0.0;
+    for (int c = 0; c < 4; ++c) o[1 + r * 4 + c] = (R)(has_lo ? B[r][c] : 0.0);
   const bool has_up = P.l_ups[s] != 0;
   if (has_up) {
     int fce = P.l_up[s];
@@ -416,7 +418,7 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
     link_block<double>(U, base, nc, P.l_cell[s + 1], sg * P.fj[fce], sg * P.fj[P.nfj + fce], L[fce], gamma, B);
   }
   for (int r = 0; r < 4; ++r)
-    for (int c = 0; c < 4; ++c) o[17 + r * 4 + c] = has_up ? B[r][c] : 0.0;
+    for (int c = 0; c < 4; ++c) o[17 + r * 4 + c] = (R)(has_up ? B[r][c] : 0.0);
   if constexpr (NV > 4) {
     const double* mj = sam + (size_t)cs * nf + P.nfi;
     const double* dj = sad + (size_t)cs * nf + P.nfi;
@@ -431,9 +433,9 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
       double sg = P.l_ups[s];
       up = -(fmax(-sg * mj[fce], 0.0) + dj[fce]) / U[base + P.l_cell[s + 1]];
     }
-    o[33] = lo;
-    o[34] = sa_diag[(size_t)cs * nc + cell];
-    o[35] = up;
+    o[33] = (R)lo;
+    o[34] = (R)sa_diag[(size_t)cs * nc + cell];
+    o[35] = (R)up;
   }
 }
 
@@ -448,18 +450,18 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
 //     the chain's top (slot L - 1, L - 3, ...), C[r][q] on the odd ones;
 //   fac_sa[slot][3] = l, 1/p, c  (the scalar SA chain).
 template <class R, int NV>
-__global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ order, const double* __restrict__ blk,
+__global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ order, const R* __restrict__ blk,
                                                R* __restrict__ fac, R* __restrict__ fac_sa) {
   const int t = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
   if (t >= P.n_lines) return;
   const int chain = order[t];
   const int s0 = P.l_start[chain], n = P.l_start[chain + 1] - s0;
-  const double* B = blk + ((size_t)cs * P.n_slots + s0) * BLKW;
+  const R* B = blk + ((size_t)cs * P.n_slots + s0) * BLKW;
   R* F = fac + (size_t)cs * P.n_slots * 48;
   R* Fc = F + (size_t)P.n_slots * 32;
   R* Fs = fac_sa + (size_t)cs * P.n_slots * 3;
   constexpr int NB = NV > 4 ? BLKW : 33;   // block values read per slot
-  double nb[NB];
+  R nb[NB];
   if (n > 0)
     for (int e = 0; e < NB; ++e) nb[e] = B[e];
   R Cp[4][4];
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ o
     for (int q = 0; q < 4; ++q) Cp[r][q] = R(0);
   R cps = R(0);
   for (int k = 0; k < n; ++k) {
-    double b[NB];
+    R b[NB];
     for (int e = 0; e < NB; ++e) b[e] = nb[e];
     if (k + 1 < n)
       for (int e = 0; e < NB; ++e) nb[e] = B[(size_t)(k + 1) * BLKW + e];
