@@ -397,8 +397,8 @@ SERVE_MODES = {"fp64_serial": ("fp64", False), "fp64_served": ("fp64", True), "m
                "mixed_serial": ("mixed", False)}
 
 
-def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iters: int, driver: str = "threads",
-                modes=("fp64_serial", "fp64_served", "mixed_served"), slots: int = 4) -> dict:
+def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iters: int, driver: str = "slots",
+                modes=("fp64_serial", "fp64_served", "mixed_served"), slots: int = 16, start: int = 0) -> dict:
     """The first ``n`` requests of the 256-request stream (Poisson arrivals at
     ``rate`` per second, numpy default_rng(3)) served three ways: the plain
     FP64 path one solve at a time; served FP64; served mixed, ``in_flight``
@@ -410,7 +410,10 @@ def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iter
     import multiprocessing as mp
 
     from jaxrans_amd import serve
-    reqs = serve.make_requests(256, rate)[:n]
+    reqs = serve.make_requests(256, rate)[start:start + n]
+    if start:   # a later part of the stream: its clock starts at its first arrival
+        from dataclasses import replace as _replace
+        reqs = [_replace(r, arrival=r.arrival - reqs[0].arrival) for r in reqs]
     cfg = serve.default_config("fp64", max_iters=max_iters, forces_tol=forces_tol, max_in_flight=in_flight,
                                driver=driver, slots=slots)
     t0 = time.perf_counter()
@@ -438,8 +441,13 @@ def serve_bench(n: int, rate: float, forces_tol: float, in_flight: int, max_iter
         dcl = max(abs(by_id[k].cl - base[k].cl) for k in base)
         dcd = max(abs(by_id[k].cd - base[k].cd) for k in base)
         dit = [by_id[k].iterations - base[k].iterations for k in base]
+        # the force bar against the first mode (CL within 1e-6, CD within 0.01 count), over the requests
+        # both runs counted (converged or settled)
+        both = [k for k in base if by_id[k].reason in serve.COUNTED and base[k].reason in serve.COUNTED]
+        within = [k for k in both if abs(by_id[k].cl - base[k].cl) <= 1e-6 and abs(by_id[k].cd - base[k].cd) <= 1e-6]
         out[label] = {**summ, f"max_abs_dcl_vs_{modes[0]}": dcl, f"max_abs_dcd_counts_vs_{modes[0]}": dcd * 1e4,
-                      f"mean_diterations_vs_{modes[0]}": float(np.mean(dit))}
+                      f"mean_diterations_vs_{modes[0]}": float(np.mean(dit)),
+                      f"counted_in_both_vs_{modes[0]}": len(both), f"within_force_bar_vs_{modes[0]}": len(within)}
     for label in modes[1:]:
         out[f"solves_per_min_{label}_over_{modes[0]}"] = round(
             out[label]["solves_per_min"] / out[modes[0]]["solves_per_min"], 3)
@@ -480,13 +488,17 @@ def main(argv=None) -> int:
     ap.add_argument("--serve", type=int, metavar="N", default=0,
                     help="instead of the step benchmark: the served stream, first N of its 256 requests")
     ap.add_argument("--rate", type=float, default=0.5, help="--serve: Poisson arrival rate, requests per second")
+    ap.add_argument("--serve-from", type=int, default=0, metavar="K",
+                    help="--serve: requests K .. K+N-1 of the stream, its clock started at request K's arrival "
+                         "(a long plain-path baseline measured in parts)")
     ap.add_argument("--forces-tol", type=float, default=1e-5, help="--serve: settle tolerance on CL, CD, CM")
     ap.add_argument("--in-flight", type=int, default=8, help="--serve: solves on the device together")
     ap.add_argument("--max-iters", type=int, default=4000, help="--serve: Newton budget per solve")
-    ap.add_argument("--driver", choices=("threads", "async", "slots"), default="threads",
+    ap.add_argument("--driver", choices=("threads", "async", "slots"), default="slots",
                     help="--serve: a host thread per solve, one thread polling asynchronous blocks, or one batched "
                          "context per grid size with the requests in its slots (shared launches)")
-    ap.add_argument("--slots", type=int, default=4, help="--serve --driver slots: slots per grid size")
+    ap.add_argument("--slots", type=int, default=16, help="--serve --driver slots: slots per grid size (at most; "
+                                                          "about 2^20 cells per size)")
     ap.add_argument("--serve-modes", default="fp64_serial,fp64_served,mixed_served",
                     help=f"--serve: comma-separated runs of {sorted(SERVE_MODES)}; the first is the reference "
                          f"for the force differences")
@@ -522,14 +534,15 @@ def main(argv=None) -> int:
         return 0
     if args.serve:
         line = {"metric": "solves/min + p50/p95/p99 latency, served stream", "n_gpus": 1,
-                "config": {"workload": f"served stream: first {args.serve} of 256 requests, Poisson {args.rate}/s, "
+                "config": {"workload": f"served stream: requests {args.serve_from}..{args.serve_from + args.serve - 1} "
+                                       f"of 256, Poisson {args.rate}/s, "
                                        "Karman-Trefftz (x0 U[-0.12,-0.04], y0 U[0,0.06], tau U[5,15]), C-grids "
                                        "{257x65, 513x129, 769x193, 1025x257} nodes, M U[0.3,0.8], alpha U[-2,6], "
                                        "Re log-U[1e6,2e7], SA, Roe + van_albada_smooth, GMRES(30)x2, line_j",
                            "forces_tol": args.forces_tol, "max_iters": args.max_iters, "in_flight": args.in_flight,
                            "driver": args.driver, "slots": args.slots},
                 **serve_bench(args.serve, args.rate, args.forces_tol, args.in_flight, args.max_iters, args.driver,
-                              tuple(args.serve_modes.split(",")), args.slots)}
+                              tuple(args.serve_modes.split(",")), args.slots, args.serve_from)}
         print(json.dumps(line), flush=True)
         return 0
     if args.converge:

@@ -184,6 +184,16 @@ int jr_newton_init(jr_ctx* ctx, const double* cfl0);
 /* The same for one case (a slot refilled between blocks, sweep.py:229-264). */
 int jr_newton_init_case(jr_ctx* ctx, int case_index, double cfl0);
 
+/* Park (parked = 1) or release (0) case k of a batched context: a parked
+ * case's GMRES never starts (its Newton steps leave its state as it is and
+ * skip every Krylov launch for it), so an empty serving slot costs only its
+ * share of the residual, wave-speed and factor launches.  A released case
+ * continues from its state; refill it first with jr_set_geometry,
+ * jr_set_state_case and jr_newton_init_case.  The slot mask of the
+ * reference's slot scheduler (masked, src/jaxrans/sweep.py:203-207), which
+ * keeps an idle slot's state; here its Krylov launches are skipped too. */
+int jr_park_case(jr_ctx* ctx, int case_index, int parked);
+
 /* n_steps pseudo-transient Newton-Krylov steps on the device state
  * (one block of run_to_steady, src/jaxrans/solvers/driver.py:87-104);
  * hist_host[n_steps][n_cases] receives |R(U)| before each update. */

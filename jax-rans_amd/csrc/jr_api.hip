@@ -1725,6 +1725,16 @@ int jr_newton_init_case(jr_ctx* c, int k, double cfl0) {
   return put_ctl(c, h);
 }
 
+int jr_park_case(jr_ctx* c, int k, int parked) {
+  if (!c) return fail("jr_park_case: null context");
+  if (k < 0 || k >= c->n_cases) return fail("jr_park_case: case out of range");
+  if (c->pending_steps) return fail("jr_park_case: an asynchronous block is pending: wait for it first");
+  std::vector<Ctl> h;
+  TRY(get_ctl(c, h));
+  h[k].parked = parked != 0;
+  return put_ctl(c, h);
+}
+
 int jr_set_cfl(jr_ctx* c, const double* cfl) {
   if (!c || !cfl) return fail("jr_set_cfl: null argument");
   std::vector<Ctl> h;

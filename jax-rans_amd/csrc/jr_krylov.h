@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(BLOCK) k_cycle_begin(int nc, Ctl* __restrict__
                                                        double tol, double* __restrict__ G) {
   const int cs = blockIdx.y;
   Ctl& c = ctl[cs];
-  const bool act = c.cycles < maxiter && c.rnorm > tol * c.bnorm;
+  const bool act = !c.parked && c.cycles < maxiter && c.rnorm > tol * c.bnorm;
   if (blockIdx.x == 0) {
     double* g = G + (size_t)cs * GIV + (size_t)(MMAX + 1) * MMAX + 2 * MMAX;
     for (int k = threadIdx.x; k <= MMAX; k += BLOCK) g[k] = k == 0 ? c.rnorm : 0.0;

@@ -118,6 +118,7 @@ struct Ctl {
   int steps;           // Newton steps taken
   int rejected;
   int cycles_total;    // GMRES cycles run since the context was made (measurement)
+  int parked;          // an empty serving slot: its GMRES never starts, so its state stays put
 };
 
 }  // namespace jr

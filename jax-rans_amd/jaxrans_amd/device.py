@@ -29,7 +29,7 @@ EXPORTS = (
     "jr_device_count", "jr_profile", "jr_set_graphs", "jr_probe_arm", "jr_probe_read", "jr_newton_block_async",
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
     "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
-    "jr_newton_block_noalloc", "jr_wall_loads", "jr_tangent_freestream", "jr_forces_tangent",
+    "jr_newton_block_noalloc", "jr_wall_loads", "jr_tangent_freestream", "jr_forces_tangent", "jr_park_case",
 )
 
 
@@ -450,6 +450,11 @@ class Context:
         out = C.c_size_t()
         _check(self._lib.jr_ctx_bytes(self._h, C.byref(out)))
         return int(out.value)
+
+    def park_case(self, k: int, parked: bool = True) -> None:
+        """Park (or release) case ``k``: a parked case's Newton steps skip its
+        GMRES and leave its state as it is (an empty serving slot)."""
+        _check(self._lib.jr_park_case(self._h, int(k), int(bool(parked))))
 
     def set_graphs(self, enable: bool) -> None:
         """Replay Newton blocks from a captured HIP graph, or launch every

@@ -1,29 +1,32 @@
 """Served stream of uneven solves on one GPU.
 
 Requests (a Karman-Trefftz section, a C-grid size, Mach, alpha, Reynolds)
-arrive over time; each admitted request gets its own device context (its own
-HIP stream, its state and Krylov basis resident in HBM) and is driven in
-blocks of Newton steps through the asynchronous block API, so the blocks of
-every admitted solve are in flight together and overlap on the device.
-Admission is by free device memory (a request is started when its estimated
-footprint fits under the free memory less a reserve) and by a cap on the
-number of solves in flight.  A solve retires when the driver's block test of
-``driver.py:161-201`` says converged, non-finite, stalled or out of
+arrive over time.  The default driver batches them continuously: every grid
+size (one topology) has one batched context whose slots hold requests of
+that size, each slot with its own geometry (``jr_set_geometry``), freestream
+and Newton control, so the requests of a size share every launch of a
+Newton step, their states and Krylov bases resident in HBM; the contexts of
+the four sizes run on their own streams (four, the box's hardware queues)
+and overlap on the device.  Between blocks of Newton steps a finished slot
+is refilled from the arrived requests of its size, or parked
+(``jr_park_case``: its GMRES skipped, its state kept) when none is waiting.
+Slots per size: about ``slot_cells`` cells per context, within the free
+device memory less a reserve.  A solve retires when the driver's block test
+of ``driver.py:161-201`` says converged, non-finite, stalled or out of
 iterations, or when its forces have settled (CL, CD and CM each moving less
 than ``forces_tol`` over the last quarter of the blocks,
 ``simulation.py:254-272``) -- the same tests, in the same order, as
-``simulation.run``; its context is freed and the next waiting request is
-admitted.  Each block carries its final force coefficients back with its
-residual history, so the settle test needs no extra synchronisation.
+``simulation.run``.
 
-Each context is deterministic and independent of the others, so a request's
-result is the same alone and in the mix (a GPU test checks it bit for bit).
+Every kernel of a batched step works case by case, so a request's result is
+the same alone and in the mix (a GPU test checks it bit for bit).  The
+other drivers give each request its own context and stream, marched by a
+host thread per solve or by one thread polling asynchronous blocks.
 
 The reference has no serving loop; its closest analogue is the slot
-scheduler of ``src/jaxrans/sweep.py:172-327`` (slots refilled between blocks,
-idle slots masked), which needs one grid for all slots.  Contexts per request
-lift that restriction: requests of different grids and geometries share the
-device through concurrent streams rather than one launch.
+scheduler of ``src/jaxrans/sweep.py:172-327`` (slots refilled between
+blocks, idle slots masked), which holds one grid and geometry for all slots;
+per-slot geometry and one context per grid size lift that restriction.
 """
 
 from __future__ import annotations
@@ -97,9 +100,11 @@ class ServeConfig:
     forces_tol: float | None = 1e-5
     max_in_flight: int = 8
     reserve_bytes: int = 8 << 30          # device memory kept free
-    driver: str = "threads"               # "threads": a host thread per solve; "async": one thread polls all;
-    #                                       "slots": one batched context per grid size, requests in its slots
-    slots: int = 4                        # "slots": cases per batched context (at most)
+    driver: str = "slots"                 # "slots": one batched context per grid size, requests in its slots;
+    #                                       "threads": a context and host thread per solve; "async": one
+    #                                       thread polls every solve's context
+    slots: int = 16                       # "slots": cases per batched context (at most) ...
+    slot_cells: int = 1 << 20             # ... and about this many cells per batched context
 
     def __post_init__(self) -> None:
         if self.driver not in ("threads", "async", "slots"):
@@ -338,11 +343,23 @@ def _serve_threads(requests, atlases, cfg: ServeConfig, clock, on_done) -> list[
     return done
 
 
+def slots_for(cells: int, cfg: ServeConfig) -> int:
+    """Slots of the batched context of a grid of ``cells`` cells: about
+    ``slot_cells`` cells per context, at most ``slots``."""
+    return int(max(1, min(cfg.slots, cfg.slot_cells // max(cells, 1))))
+
+
 def _serve_slots(requests, atlases, cfg: ServeConfig, clock, on_done) -> list[Outcome]:
-    """One batched context per grid size, ``cfg.slots`` slots each (fewer if
-    the free device memory less the reserve does not hold them), marched by
-    one host thread per size; a slot is refilled from the requests of its
-    size that have arrived, between blocks."""
+    """Continuous batching: one batched context per grid size (one
+    topology: the same index plans, every slot with its own geometry,
+    freestream and Newton control), each on its own stream and marched by
+    its own host thread, so the sizes overlap on the device while the
+    requests of one size share every launch.  ``slots_for`` slots per size
+    (fewer if the free device memory less the reserve does not hold them).
+    Between blocks a finished slot is refilled from the requests of its size
+    that have arrived, or parked when none has: a parked slot's Newton steps
+    skip its GMRES and keep its state (the reference's slot mask,
+    ``src/jaxrans/sweep.py:203-207``)."""
     t0 = clock()
     by_size: dict = {}
     for r in sorted(requests, key=lambda r: r.arrival):
@@ -354,21 +371,23 @@ def _serve_slots(requests, atlases, cfg: ServeConfig, clock, on_done) -> list[Ou
     def run_size(queue: list) -> None:
         first = queue[0]
         flow0 = request_flow(first, cfg)
-        probe = context_for(atlases[first.rid], flow0, cfg.numerics, cfg.implicit, n_cases=1)
+        atlas0 = atlases[first.rid]
+        probe = context_for(atlas0, flow0, cfg.numerics, cfg.implicit, n_cases=1)
         per_case = probe.device_bytes
         probe.close()
-        free, _ = mem_info()
         with lock:
-            n = int(max(1, min(cfg.slots, len(queue), (free - cfg.reserve_bytes) // max(per_case, 1))))
-        ctx = context_for(atlases[first.rid], flow0, cfg.numerics, cfg.implicit, n_cases=n)
-        nq = 5
-        qs, mus = np.zeros((n, nq)), np.full(n, flow0.mu_inf if not flow0.inviscid else 0.0)
+            free, _ = mem_info()
+            n = int(max(1, min(slots_for(atlas0.cells, cfg), len(queue),
+                               (free - cfg.reserve_bytes) // max(per_case, 1))))
+            ctx = context_for(atlas0, flow0, cfg.numerics, cfg.implicit, n_cases=n)
+        qs, mus = np.zeros((n, 5)), np.full(n, 0.0 if flow0.inviscid else flow0.mu_inf)
         q0 = q_infinity(flow0)
         qs[:, :q0.size] = q0
         slot: list = [None] * n
+        parked = [False] * n
         waiting = list(queue)
         ctx.set_freestream(qs, None if flow0.inviscid else mus)
-        shape = atlases[first.rid].shape
+        shape = atlas0.shape
         ctx.set_state(np.broadcast_to(uniform_canvas(q0, shape, flow0.gamma), (n,) + ctx.shape[1:]))
         ctx.newton_init(np.full(n, cfg.implicit.cfl))
         try:
@@ -386,8 +405,14 @@ def _serve_slots(requests, atlases, cfg: ServeConfig, clock, on_done) -> list[Ou
                         ctx.set_geometry(k, atlases[req.rid])
                         ctx.set_state_case(k, uniform_canvas(q, shape, flow.gamma))
                         ctx.newton_init_case(k, cfg.implicit.cfl)
+                        if parked[k]:
+                            ctx.park_case(k, False)
+                            parked[k] = False
                         slot[k] = Solve(req, ctx, now)
                         changed = True
+                    elif slot[k] is None and not parked[k]:
+                        ctx.park_case(k, True)
+                        parked[k] = True
                 if changed:
                     ctx.set_freestream(qs, None if flow0.inviscid else mus)
                 if not any(s is not None for s in slot):
@@ -451,10 +476,18 @@ def serve_serial(requests: list[Request], atlases: dict, cfg: ServeConfig, clock
     return done
 
 
+COUNTED = (ExitReason.CONVERGED, ExitReason.SETTLED)
+
+
 def summary(outcomes: list[Outcome], wall: float) -> dict:
-    lat = np.array([o.latency for o in outcomes])
-    return {"requests": len(outcomes), "wall_s": round(wall, 3),
-            "solves_per_min": round(60.0 * len(outcomes) / wall, 3) if wall > 0 else None,
+    """Throughput and latency of a served stream.  ``solves_per_min`` and
+    the latency percentiles count the solves that ended converged or settled
+    (``COUNTED``); ``finished_per_min`` counts every solve, whatever ended it."""
+    counted = [o for o in outcomes if o.reason in COUNTED]
+    lat = np.array([o.latency for o in counted]) if counted else np.array([np.nan])
+    return {"requests": len(outcomes), "counted": len(counted), "wall_s": round(wall, 3),
+            "solves_per_min": round(60.0 * len(counted) / wall, 3) if wall > 0 else None,
+            "finished_per_min": round(60.0 * len(outcomes) / wall, 3) if wall > 0 else None,
             "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
             "latency_p95_s": round(float(np.percentile(lat, 95)), 3),
             "latency_p99_s": round(float(np.percentile(lat, 99)), 3),

@@ -125,3 +125,50 @@ def test_batched_geometries_equal_each_grid_alone():
         assert np.array_equal(alone.get_state()[0], Ub[k]), k
         assert np.array_equal(alone.forces()[0], Fb[k]), k
         alone.close()
+
+
+def test_parked_slot_keeps_its_state_and_the_others_are_unchanged():
+    """A parked case of a batch (jr_park_case) keeps its state bit for bit
+    through a block of Newton steps, and the other cases step exactly as in
+    the same batch without parking; released and refilled, it steps again."""
+    print("[gpu] parked slot", flush=True)
+    from jaxrans_amd.core.state import q_infinity, uniform_canvas
+    reqs = serve.make_requests(3, 0.0, seed=21, sizes=((96, 24),))
+    cfg = serve.default_config("fp64", wall_spacing=1e-3, far_field=15.0)
+    atlases = [serve.request_grid(r, cfg) for r in reqs]
+    flows = [serve.request_flow(r, cfg) for r in reqs]
+    imp = ImplicitConfig(krylov_subspace=12, krylov_restarts=2, max_relative_update=0.2)
+    qs = np.zeros((3, 5))
+    for k, f in enumerate(flows):
+        q = q_infinity(f)
+        qs[k, :q.size] = q
+    U0 = [uniform_canvas(q_infinity(f), a.shape, f.gamma) for a, f in zip(atlases, flows)]
+
+    def batch():
+        b = context_for(atlases[0], flows[0], cfg.numerics, imp, n_cases=3)
+        for k in range(3):
+            b.set_geometry(k, atlases[k])
+        b.set_freestream(qs, np.array([f.mu_inf for f in flows]))
+        for k in range(3):
+            b.set_state_case(k, U0[k])
+            b.newton_init_case(k, imp.cfl)
+        return b
+
+    plain, parked = batch(), batch()
+    h_plain = plain.newton_block(8)
+    parked.newton_block(4)
+    before = parked.get_state()[1].copy()
+    parked.park_case(1, True)
+    h_parked = parked.newton_block(4)
+    after = parked.get_state()
+    assert np.array_equal(after[1], before)
+    for k in (0, 2):
+        assert np.array_equal(h_parked[:, k], h_plain[4:, k]), k
+        assert np.array_equal(after[k], plain.get_state()[k]), k
+    parked.set_state_case(1, U0[1])
+    parked.newton_init_case(1, imp.cfl)
+    parked.park_case(1, False)
+    h = parked.newton_block(2)
+    assert np.all(np.isfinite(h)) and not np.array_equal(parked.get_state()[1], U0[1])
+    plain.close()
+    parked.close()

@@ -102,3 +102,20 @@ def test_summary_percentiles():
     assert s["latency_p50_s"] == 50.5 and s["latency_p99_s"] == pytest.approx(99.01)
     assert s["reasons"] == {ExitReason.SETTLED: 100}
     assert s["newton_steps"] == 10000
+
+
+def test_summary_counts_only_converged_or_settled_solves():
+    reasons = [ExitReason.CONVERGED, ExitReason.SETTLED, ExitReason.MAX_ITERS, ExitReason.STALLED]
+    outs = [serve.Outcome(k, 0.0, 0.0, float(k + 1), 100, reasons[k % 4], 0.0, 0.0, 0.0, 1000, 3.0)
+            for k in range(100)]
+    s = serve.summary(outs, 60.0)
+    assert s["requests"] == 100 and s["counted"] == 50
+    assert s["solves_per_min"] == 50.0 and s["finished_per_min"] == 100.0
+    lat = [k + 1.0 for k in range(100) if k % 4 < 2]
+    assert s["latency_p50_s"] == pytest.approx(float(np.percentile(lat, 50)))
+
+
+def test_slots_per_grid_size():
+    cfg = serve.default_config("mixed")
+    assert [serve.slots_for(a * n, cfg) for a, n in serve.GRID_SIZES] == [16, 16, 7, 4]
+    assert serve.slots_for(10, cfg) == cfg.slots and serve.slots_for(10 ** 9, cfg) == 1
