@@ -257,6 +257,15 @@ int jr_forces_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, 
  * src/jaxrans/numerics/residual.py:142-166. */
 int jr_rk_block(jr_ctx* ctx, int n_steps, const double* alphas, int n_alphas, double* hist_host);
 
+/* The Krylov operator's first-order tangent by LDS tiles (k_tangent_tile:
+ * node values, face fluxes and cell divergence in one pass, enable = 1) or
+ * by the separate node, face and cell kernels (enable = 0, the default: the
+ * faster on MI355X), for 2D contexts whose line apply writes the tangent
+ * primitives (block Thomas, or cyclic reduction within its LDS budget).  The
+ * two agree to round-off.  A switch for measurement and tests; no reference
+ * counterpart. */
+int jr_set_tile_tangent(jr_ctx* ctx, int enable);
+
 /* Execution option: replay a block of Newton steps from a HIP graph captured
  * on the first block of that length (1), or launch every kernel on the
  * context stream (0, the default: faster on MI355X with ROCm 7).  Results are

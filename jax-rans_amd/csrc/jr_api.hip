@@ -98,6 +98,7 @@ struct jr_ctx {
   double* circ = nullptr;      // point-vortex far field: [case][2] circulation and its tangent
   double* dfree = nullptr;     // parameter tangents: [case][8] the freestream's tangent (Plan::dfree)
   double* zero_field = nullptr;   // parameter tangents at dU = 0: a zero state tangent
+  bool tile_tangent = false;   // the Krylov tangent by LDS tiles (k_tangent_tile) where it applies
   // 2D block Thomas (k_thomas, k_line_apply): the chains by length, longest
   // first, and their length classes
   const int* lorder = nullptr;
@@ -227,6 +228,14 @@ static void launch_tangent(jr_ctx* c, const double* U, const double* dU, double*
   if (!dq_ready)
     JR_LAUNCH(PR_FILL, (k_fill<Dual, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
               dim3(BLOCK), 0, P, c->ph.gamma, c->ctl, U, dU, (double*)nullptr, c->dQ, check_active);
+  if (dq_ready && first_order && c->tile_tangent) {
+    // nodes, faces and cells in one pass over LDS tiles
+    const unsigned tiles = (unsigned)(((P.NI + TT_I - 1) / TT_I) * ((P.NJ + TT_J - 1) / TT_J));
+    JR_LAUNCH(PR_FACES, (k_tangent_tile<NV, VISC>), dim3(tiles, c->n_cases), dim3(TT_THREADS), 0, P, c->ph,
+              (const Ctl*)c->ctl, dU, (const double*)c->Q, (const double*)c->dQ, vdt, (const double*)c->salin, out,
+              check_active);
+    return;
+  }
   if (VISC)
     hipLaunchKernelGGL((k_nodes<Dual, NV>), grid1((size_t)(P.NI + 1) * (P.NJ + 1), BLOCK, c->n_cases), dim3(BLOCK), 0,
                        c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ,
@@ -1741,6 +1750,15 @@ int jr_set_cfl(jr_ctx* c, const double* cfl) {
   TRY(get_ctl(c, h));
   for (int k = 0; k < c->n_cases; ++k) h[k].cfl = cfl[k];
   return put_ctl(c, h);
+}
+
+int jr_set_tile_tangent(jr_ctx* c, int enable) {
+  if (!c) return fail("jr_set_tile_tangent: null context");
+  if (c->pending_steps) return fail("jr_set_tile_tangent: an asynchronous block is pending: wait for it first");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graph(c);   // captured launches hold the old kernel path
+  c->tile_tangent = enable != 0;
+  return 0;
 }
 
 int jr_set_graphs(jr_ctx* c, int enable) {

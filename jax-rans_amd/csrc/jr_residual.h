@@ -160,6 +160,88 @@ __device__ __forceinline__ void load_q(const double* Q, const double* dQ, size_t
   for (int k = 0; k < NV; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + c);
 }
 
+// One face's flux, or its tangent (T = Dual): MUSCL states from the cells
+// either side (qm, qp) and, second order, their outer neighbours (qmm,
+// qpp), the Roe or HLLC flux through S = (sx, sy), and, viscous, minus the
+// viscous flux from the diamond gradient (the face's node values pn0, pn1,
+// and its diamond dg = d1x, d1y, d2x, d2y) with mu_inf (TM: Dual when it
+// carries a parameter tangent); row 4 (SA) the nu_hat face flux.
+// (residual.py:54-124, reconstruction.py:29-80, gradients.py:59-95,
+// turbulence.py:78-90.)
+// diamond(pn0, pn1, dg) fetches the node values and the diamond when the
+// viscous part needs them (after the inviscid flux, so that they are not
+// live across it).
+template <class T, int NV, bool VISC, class TM, class DiamondFn>
+__device__ __forceinline__ void face_flux(const Physics& ph, int first_order, const T* qmm, const T* qm, const T* qp,
+                                          const T* qpp, double sx, double sy, DiamondFn diamond, TM mu_inf, T* Fl) {
+  const double gamma = ph.gamma;
+  T qL[4], qR[4];
+  if (first_order) {
+    for (int k = 0; k < 4; ++k) { qL[k] = qm[k]; qR[k] = qp[k]; }
+  } else if (ph.kappa != 0.0) {
+    double lo = 0.25 * (1.0 - ph.kappa), hi = 0.25 * (1.0 + ph.kappa);
+    for (int k = 0; k < 4; ++k) {
+      qL[k] = qm[k] + lo * (qm[k] - qmm[k]) + hi * (qp[k] - qm[k]);
+      qR[k] = qp[k] - lo * (qpp[k] - qp[k]) - hi * (qp[k] - qm[k]);
+    }
+  } else {
+    for (int k = 0; k < 4; ++k) {
+      qL[k] = qm[k] + 0.5 * limit<T>(ph.limiter, qm[k] - qmm[k], qp[k] - qm[k]);
+      qR[k] = qp[k] - 0.5 * limit<T>(ph.limiter, qp[k] - qm[k], qpp[k] - qp[k]);
+    }
+  }
+  if (ph.flux == FLUX_ROE) roe<T>(qL, qR, sx, sy, gamma, Fl);
+  else hllc<T>(qL, qR, sx, sy, gamma, Fl);
+  if constexpr (NV > 4) Fl[4] = lift<T>(0.0);
+
+  if constexpr (VISC) {
+    T pn0[4], pn1[4];
+    double dg[4];
+    diamond(pn0, pn1, dg);
+    const double d1x = dg[0], d1y = dg[1], d2x = dg[2], d2y = dg[3];
+    const double det = d1x * d2y - d2x * d1y;
+    constexpr int NPHI = NV > 4 ? 4 : 3;
+    T pl[4], pr[4];
+    pl[0] = qm[1]; pl[1] = qm[2]; pl[2] = temperature(qm, gamma);
+    pr[0] = qp[1]; pr[1] = qp[2]; pr[2] = temperature(qp, gamma);
+    if (NPHI > 3) { pl[3] = qm[NV - 1]; pr[3] = qp[NV - 1]; }
+    T gx[4], gy[4];
+    const double idet = 1.0 / det;
+    for (int k = 0; k < NPHI; ++k) {
+      T dp1 = pr[k] - pl[k];
+      T dp2 = pn1[k] - pn0[k];
+      gx[k] = (dp1 * d2y - dp2 * d1y) * idet;
+      gy[k] = (d1x * dp2 - d2x * dp1) * idet;
+    }
+    T gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
+    T gT[2] = {gx[2], gy[2]};
+    T vel[2] = {0.5 * (qm[1] + qp[1]), 0.5 * (qm[2] + qp[2])};
+    T Tf = 0.5 * (temperature(qm, gamma) + temperature(qp, gamma));
+    T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
+    T kc = mu / (PR * (gamma - 1.0));
+    T extra[2][2];
+    bool has_extra = false;
+    if constexpr (NV > 4) {
+      T face[5];
+      for (int k = 0; k < 5; ++k) face[k] = 0.5 * (qm[k] + qp[k]);
+      T muf = viscosity<T>(temperature(face, gamma), mu_inf, ph.sutherland);
+      T mut = eddy_viscosity<T>(face[0], face[4] * mu_inf, muf / face[0]);
+      mu = mu + mut;
+      kc = kc + mut / (PR_T * (gamma - 1.0));
+      if (ph.c_cr1 != 0.0) { qcr_stress<T>(gv, mut, ph.c_cr1, extra); has_extra = true; }
+      // SA face flux: upwind convection on the mass flux, diffusion through the face gradient.
+      T up = val(Fl[0]) > 0.0 ? qm[4] : qp[4];
+      T coef = diffusion_coefficient<T>(face[0], face[4] * mu_inf, muf);
+      Fl[4] = Fl[0] * up - coef * (gx[3] * sx + gy[3] * sy);
+    }
+    T fv[4];
+    // two inlined calls keep extra in registers (a selected pointer sent it to scratch)
+    if (has_extra) viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, extra, fv);
+    else viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, nullptr, fv);
+    for (int k = 0; k < 4; ++k) Fl[k] = Fl[k] - fv[k];
+  }
+}
+
 // Face fluxes of both families.  Face index f < nfi: i-face (fi, j) between
 // cells (fi-1, j) and (fi, j); else j-face (i, fj) between (i, fj-1) and (i, fj).
 // Faces not adjacent to an interior cell are zero.  Rows 0..3 hold the
@@ -294,94 +376,36 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
     return;
   }
   const size_t base = (size_t)cs * NV * nc;
-  const double gamma = ph.gamma;
-  T qm[NV], qp[NV];
+  T qm[NV], qp[NV], qmm[NV], qpp[NV];
   load_q<T, NV>(Q, dQ, base, nc, cL, qm);
   load_q<T, NV>(Q, dQ, base, nc, cR, qp);
-  T qL[4], qR[4];
-  if (first_order) {
-    for (int k = 0; k < 4; ++k) { qL[k] = qm[k]; qR[k] = qp[k]; }
-  } else {
-    T qmm[NV], qpp[NV];
+  if (!first_order) {
     load_q<T, NV>(Q, dQ, base, nc, cL - dL, qmm);
     load_q<T, NV>(Q, dQ, base, nc, cR + dL, qpp);
-    if (ph.kappa != 0.0) {
-      double lo = 0.25 * (1.0 - ph.kappa), hi = 0.25 * (1.0 + ph.kappa);
-      for (int k = 0; k < 4; ++k) {
-        qL[k] = qm[k] + lo * (qm[k] - qmm[k]) + hi * (qp[k] - qm[k]);
-        qR[k] = qp[k] - lo * (qpp[k] - qp[k]) - hi * (qp[k] - qm[k]);
-      }
-    } else {
-      for (int k = 0; k < 4; ++k) {
-        qL[k] = qm[k] + 0.5 * limit<T>(ph.limiter, qm[k] - qmm[k], qp[k] - qm[k]);
-        qR[k] = qp[k] - 0.5 * limit<T>(ph.limiter, qp[k] - qm[k], qpp[k] - qp[k]);
-      }
-    }
   }
   T Fl[NV];
-  if (ph.flux == FLUX_ROE) roe<T>(qL, qR, sx, sy, gamma, Fl);
-  else hllc<T>(qL, qR, sx, sy, gamma, Fl);
-  if constexpr (NV > 4) Fl[4] = lift<T>(0.0);
-
   if (VISC) {
-    const auto mu_inf = free_val<PT>(ctl[cs].mu_inf, P.dfree, cs, 6);
-    // Diamond gradient: the cells' centre difference d1 and the face's node
-    // difference d2, precomputed per grid (k_diamond).
-    const int nft = P.nfi + P.nfj;
-    const double d1x = P.dg[f], d1y = P.dg[(size_t)nft + f], d2x = P.dg[2 * (size_t)nft + f],
-                 d2y = P.dg[3 * (size_t)nft + f];
-    const int iR = cR / NJ, jR = cR - iR * NJ;
-    const int n0i = iR, n0j = jR;
-    const int n1i = ifam ? iR : iR + 1, n1j = ifam ? jR + 1 : jR;
-    double det = d1x * d2y - d2x * d1y;
     // phi = (u, v, T[, nu_hat]) at the face's two nodes (k_nodes): node
     // (ni, nj) averages canvas cells (ni-1, nj-1), (ni, nj-1), (ni-1, nj), (ni, nj).
-    const int NPHI = NV > 4 ? 4 : 3;
+    constexpr int NPHI = NV > 4 ? 4 : 3;
+    const int iR = cR / NJ, jR = cR - iR * NJ;
+    const int n1i = ifam ? iR : iR + 1, n1j = ifam ? jR + 1 : jR;
     const int nn = (P.NI + 1) * (NJ + 1);
-    auto node_phi = [&](int ni, int nj, T* out) {
-      const size_t o = (size_t)cs * NPHI * nn + (size_t)ni * (NJ + 1) + nj;
-      for (int k = 0; k < NPHI; ++k) out[k] = Io<T>::ld(Nd, dNd, o + (size_t)k * nn);
+    // the diamond: the cells' centre difference d1 and the face's node
+    // difference d2, precomputed per grid (k_diamond)
+    const int nft = P.nfi + P.nfj;
+    auto diamond = [&](T* pn0, T* pn1, double* dg) {
+      for (int k = 0; k < 4; ++k) dg[k] = P.dg[(size_t)k * nft + f];
+      const size_t o0 = (size_t)cs * NPHI * nn + (size_t)iR * (NJ + 1) + jR;
+      const size_t o1 = (size_t)cs * NPHI * nn + (size_t)n1i * (NJ + 1) + n1j;
+      for (int k = 0; k < NPHI; ++k) pn0[k] = Io<T>::ld(Nd, dNd, o0 + (size_t)k * nn);
+      for (int k = 0; k < NPHI; ++k) pn1[k] = Io<T>::ld(Nd, dNd, o1 + (size_t)k * nn);
     };
-    T pn0[4], pn1[4], pl[4], pr[4];
-    node_phi(n0i, n0j, pn0);
-    node_phi(n1i, n1j, pn1);
-    pl[0] = qm[1]; pl[1] = qm[2]; pl[2] = temperature(qm, gamma);
-    pr[0] = qp[1]; pr[1] = qp[2]; pr[2] = temperature(qp, gamma);
-    if (NPHI > 3) { pl[3] = qm[NV - 1]; pr[3] = qp[NV - 1]; }
-    T gx[4], gy[4];
-    const double idet = 1.0 / det;
-    for (int k = 0; k < NPHI; ++k) {
-      T dp1 = pr[k] - pl[k];
-      T dp2 = pn1[k] - pn0[k];
-      gx[k] = (dp1 * d2y - dp2 * d1y) * idet;
-      gy[k] = (d1x * dp2 - d2x * dp1) * idet;
-    }
-    T gv[2][2] = {{gx[0], gy[0]}, {gx[1], gy[1]}};
-    T gT[2] = {gx[2], gy[2]};
-    T vel[2] = {0.5 * (qm[1] + qp[1]), 0.5 * (qm[2] + qp[2])};
-    T Tf = 0.5 * (temperature(qm, gamma) + temperature(qp, gamma));
-    T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
-    T kc = mu / (PR * (gamma - 1.0));
-    T extra[2][2];
-    bool has_extra = false;
-    if constexpr (NV > 4) {
-      T face[5];
-      for (int k = 0; k < 5; ++k) face[k] = 0.5 * (qm[k] + qp[k]);
-      T muf = viscosity<T>(temperature(face, gamma), mu_inf, ph.sutherland);
-      T mut = eddy_viscosity<T>(face[0], face[4] * mu_inf, muf / face[0]);
-      mu = mu + mut;
-      kc = kc + mut / (PR_T * (gamma - 1.0));
-      if (ph.c_cr1 != 0.0) { qcr_stress<T>(gv, mut, ph.c_cr1, extra); has_extra = true; }
-      // SA face flux: upwind convection on the mass flux, diffusion through the face gradient.
-      T up = val(Fl[0]) > 0.0 ? qm[4] : qp[4];
-      T coef = diffusion_coefficient<T>(face[0], face[4] * mu_inf, muf);
-      Fl[4] = Fl[0] * up - coef * (gx[3] * sx + gy[3] * sy);
-    }
-    T fv[4];
-    // two inlined calls keep extra in registers (a selected pointer sent it to scratch)
-    if (has_extra) viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, extra, fv);
-    else viscous_flux<T>(vel, gv, gT, sx, sy, mu, kc, nullptr, fv);
-    for (int k = 0; k < 4; ++k) Fl[k] = Fl[k] - fv[k];
+    face_flux<T, NV, true>(ph, first_order, qmm, qm, qp, qpp, sx, sy, diamond,
+                           free_val<PT>(ctl[cs].mu_inf, P.dfree, cs, 6), Fl);
+  } else {
+    auto none = [](T*, T*, double*) {};
+    face_flux<T, NV, false>(ph, first_order, qmm, qm, qp, qpp, sx, sy, none, 0.0, Fl);
   }
   for (int k = 0; k < NV; ++k) {
     if (sizeof(T) == sizeof(double)) F[fbase + (size_t)k * nfam] = val(Fl[k]);
@@ -564,6 +588,191 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
   for (int k = 0; k < NV; ++k) {
     if (tangent && vdt) r[k] = vd * dU[base + (size_t)k * nc + c] + r[k];
     out[base + (size_t)k * nc + c] = r[k];
+  }
+}
+
+
+// The first-order tangent of the Krylov operator in one pass over tiles of
+// TI x TJ cells (out = vdt dU + dR~/dU dU, the node, face and cell kernels
+// fused): a workgroup stages its tile's cells with a one-cell halo --
+// primitives Q and their tangent dQ, complete on ghost cells (the line
+// apply wrote them) -- in LDS, forms the tile's node values from them,
+// computes every face of the tile from LDS with face_flux (the arithmetic of
+// k_faces), keeps the face fluxes' tangent in LDS and finishes each cell's
+// divergence and SA source there (the arithmetic of k_cells).  Each cell's
+// primitives are read from HBM once per tile instead of once per face and
+// node, and no face flux goes through HBM.  Measured on the 32-case polar
+// it takes 460-470 us per launch against 400 us for the node, face and cell
+// kernels together (the face arithmetic, not HBM, bounds both), so it is
+// off by default (jr_set_tile_tangent).
+constexpr int TT_I = 8, TT_J = 32, TT_THREADS = 256;
+
+template <int NV, bool VISC>
+__global__ void __launch_bounds__(TT_THREADS, 2) k_tangent_tile(Plan P, Physics ph, const Ctl* __restrict__ ctl,
+                                                                const double* __restrict__ dU,
+                                                                const double* __restrict__ Q,
+                                                                const double* __restrict__ dQ,
+                                                                const double* __restrict__ vdt,
+                                                                const double* __restrict__ salin,
+                                                                double* __restrict__ out, int check_active) {
+  constexpr int HI = TT_I + 2, HJ = TT_J + 2;           // cells with the halo
+  constexpr int NI1 = TT_I + 1, NJ1 = TT_J + 1;         // nodes
+  constexpr int NPHI = NV > 4 ? 4 : 3;
+  __shared__ double sq[NV][HI][HJ], sd[NV][HI][HJ];
+  __shared__ double snv[VISC ? NPHI : 1][VISC ? NI1 : 1][VISC ? NJ1 : 1];
+  __shared__ double snd[VISC ? NPHI : 1][VISC ? NI1 : 1][VISC ? NJ1 : 1];
+  __shared__ double sfi[NV][NI1][TT_J], sfj[NV][TT_I][NJ1];
+  bind_case(P, blockIdx.y);
+  const int cs = blockIdx.y;
+  if (check_active && !ctl[cs].active) return;
+  const int NI = P.NI, NJ = P.NJ, nc = P.nc;
+  const int tiles_j = (NJ + TT_J - 1) / TT_J;
+  const int i0 = (int)(blockIdx.x / tiles_j) * TT_I, j0 = (int)(blockIdx.x % tiles_j) * TT_J;
+  const size_t base = (size_t)cs * NV * nc;
+  const int tid = threadIdx.x;
+  // 1. the tile's cells and halo
+  for (int e = tid; e < HI * HJ; e += TT_THREADS) {
+    const int li = e / HJ, lj = e - li * HJ;
+    const int gi = i0 + li - 1, gj = j0 + lj - 1;
+    const bool on = gi >= 0 && gi < NI && gj >= 0 && gj < NJ;
+    const int c = on ? gi * NJ + gj : 0;
+    for (int k = 0; k < NV; ++k) {
+      sq[k][li][lj] = on ? Q[base + (size_t)k * nc + c] : 0.0;
+      sd[k][li][lj] = on ? dQ[base + (size_t)k * nc + c] : 0.0;
+    }
+  }
+  __syncthreads();
+  auto cell = [&](int li, int lj, Dual* q) {
+    for (int k = 0; k < NV; ++k) q[k] = mkd(sq[k][li][lj], sd[k][li][lj]);
+  };
+  // 2. node values phi = (u, v, T[, nu_hat]) and their tangent (k_nodes):
+  // node (ni, nj) averages cells (ni-1..ni, nj-1..nj); nodes on the canvas
+  // edge are never read
+  if constexpr (VISC) {
+    const double gamma = ph.gamma;
+    for (int e = tid; e < NI1 * NJ1; e += TT_THREADS) {
+      const int ln = e / NJ1, mn = e - ln * NJ1;
+      const int ni = i0 + ln, nj = j0 + mn;
+      if (ni < 1 || nj < 1 || ni > NI - 1 || nj > NJ - 1) continue;
+      Dual p[4][4];
+      const int ci[4] = {ln, ln + 1, ln, ln + 1}, cj[4] = {mn, mn, mn + 1, mn + 1};
+      for (int t = 0; t < 4; ++t) {
+        Dual q[NV];
+        cell(ci[t], cj[t], q);
+        p[t][0] = q[1]; p[t][1] = q[2]; p[t][2] = temperature(q, gamma);
+        if constexpr (NPHI > 3) p[t][3] = q[NV - 1];
+      }
+      for (int k = 0; k < NPHI; ++k) {
+        const Dual v = 0.25 * (((p[0][k] + p[1][k]) + p[2][k]) + p[3][k]);
+        snv[k][ln][mn] = v.v;
+        snd[k][ln][mn] = v.d;
+      }
+    }
+    __syncthreads();
+  }
+  // 3. the tile's faces: i-faces (a, b) between cells (a-1, b), (a, b) for
+  // a in i0..i0+TI; j-faces (a, b) between (a, b-1), (a, b) for b in j0..j0+TJ
+  const int nft = P.nfi + P.nfj;
+  auto face = [&](bool ifam, int la, int lb) {
+    const int a = i0 + la, b = j0 + lb;
+    double* F = ifam ? &sfi[0][0][0] : &sfj[0][0][0];
+    const int fstride = ifam ? NI1 * TT_J : TT_I * NJ1;
+    const int fo = ifam ? la * TT_J + lb : la * NJ1 + lb;
+    int f, cR;
+    bool inside;
+    double sx, sy;
+    if (ifam) {
+      if (a > NI || b >= NJ) return;
+      f = a * NJ + b;
+      inside = a >= 1 && a <= NI - 1;
+      cR = f;
+      sx = P.fi[f]; sy = P.fi[P.nfi + f];
+    } else {
+      if (a >= NI || b > NJ) return;
+      const int ff = a * (NJ + 1) + b;
+      f = P.nfi + ff;
+      inside = b >= 1 && b <= NJ - 1;
+      cR = a * NJ + b;
+      sx = P.fj[ff]; sy = P.fj[P.nfj + ff];
+    }
+    const int cL = cR - (ifam ? NJ : 1);
+    if (!inside || !(P.interior[cL] || P.interior[cR])) {
+      for (int k = 0; k < NV; ++k) F[k * fstride + fo] = 0.0;
+      return;
+    }
+    // the cells either side in LDS: (a - 1, b) | (a, b - 1) and (a, b)
+    Dual qm[NV], qp[NV];
+    cell(ifam ? la : la + 1, ifam ? lb + 1 : lb, qm);
+    cell(la + 1, lb + 1, qp);
+    Dual Fl[NV];
+    if constexpr (VISC) {
+      // nodes (a, b) and (a, b + 1) | (a + 1, b)
+      const int m1 = ifam ? la : la + 1, n1 = ifam ? lb + 1 : lb;
+      auto diamond = [&](Dual* pn0, Dual* pn1, double* dg) {
+        for (int k = 0; k < 4; ++k) dg[k] = P.dg[(size_t)k * nft + f];
+        for (int k = 0; k < NPHI; ++k) {
+          pn0[k] = mkd(snv[k][la][lb], snd[k][la][lb]);
+          pn1[k] = mkd(snv[k][m1][n1], snd[k][m1][n1]);
+        }
+      };
+      face_flux<Dual, NV, true>(ph, 1, qm, qm, qp, qp, sx, sy, diamond, ctl[cs].mu_inf, Fl);
+    } else {
+      auto none = [](Dual*, Dual*, double*) {};
+      face_flux<Dual, NV, false>(ph, 1, qm, qm, qp, qp, sx, sy, none, 0.0, Fl);
+    }
+    for (int k = 0; k < NV; ++k) F[k * fstride + fo] = Fl[k].d;
+  };
+  constexpr int NFI = NI1 * TT_J, NFJ = TT_I * NJ1;
+  for (int e = tid; e < NFI + NFJ; e += TT_THREADS) {
+    if (e < NFI) face(true, e / TT_J, e % TT_J);
+    else face(false, (e - NFI) / NJ1, (e - NFI) % NJ1);
+  }
+  __syncthreads();
+  // 4. the tile's cells (k_cells): divergence of the face tangents, the SA
+  // source's tangent from its partials, the pseudo-time term
+  for (int e = tid; e < TT_I * TT_J; e += TT_THREADS) {
+    const int li = e / TT_J, lj = e - li * TT_J;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= NI || j >= NJ) continue;
+    const int c = i * NJ + j;
+    const double vd = vdt ? vdt[(size_t)cs * nc + c] : 0.0;
+    if (!P.interior[c]) {
+      for (int k = 0; k < NV; ++k) {
+        const double d = dU[base + (size_t)k * nc + c];
+        out[base + (size_t)k * nc + c] = vdt ? vd * d + d : d;
+      }
+      continue;
+    }
+    double r[NV];
+    for (int k = 0; k < NV; ++k) r[k] = (sfi[k][li + 1][lj] - sfi[k][li][lj]) + (sfj[k][li][lj + 1] - sfj[k][li][lj]);
+    if constexpr (NV > 4) {
+      const int fi0 = i * NJ + j, fi1 = fi0 + NJ;
+      const int fj0 = i * (NJ + 1) + j, fj1 = fj0 + 1;
+      const int ci = li + 1, cj = lj + 1;
+      const double sxi0 = P.fi[fi0], syi0 = P.fi[P.nfi + fi0], sxi1 = P.fi[fi1], syi1 = P.fi[P.nfi + fi1];
+      const double sxj0 = P.fj[fj0], syj0 = P.fj[P.nfj + fj0], sxj1 = P.fj[fj1], syj1 = P.fj[P.nfj + fj1];
+      const double V = P.area[c];
+      auto gg = [&](int k, double& gxo, double& gyo) {
+        const double qc = sd[k][ci][cj];
+        const double fw = 0.5 * (sd[k][ci - 1][cj] + qc), fe = 0.5 * (qc + sd[k][ci + 1][cj]);
+        const double fs = 0.5 * (sd[k][ci][cj - 1] + qc), fn = 0.5 * (qc + sd[k][ci][cj + 1]);
+        gxo = ((fe * sxi1 - fw * sxi0) + (fn * sxj1 - fs * sxj0)) / V;
+        gyo = ((fe * syi1 - fw * syi0) + (fn * syj1 - fs * syj0)) / V;
+      };
+      double gux, guy, gvx, gvy, gnx, gny;
+      gg(1, gux, guy);
+      gg(2, gvx, gvy);
+      gg(4, gnx, gny);
+      const double* L = salin + (size_t)cs * SALIN2 * nc + c;
+      double ds = (L[0] * sd[0][ci][cj] + L[(size_t)nc] * sd[3][ci][cj]) + L[(size_t)2 * nc] * sd[4][ci][cj];
+      ds += L[(size_t)3 * nc] * (gvx - guy);
+      ds += L[(size_t)4 * nc] * gnx + L[(size_t)5 * nc] * gny;
+      r[4] = r[4] - ds * V;
+    }
+    for (int k = 0; k < NV; ++k) {
+      if (vdt) r[k] = vd * dU[base + (size_t)k * nc + c] + r[k];
+      out[base + (size_t)k * nc + c] = r[k];
+    }
   }
 }
 

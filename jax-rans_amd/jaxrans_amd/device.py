@@ -30,6 +30,7 @@ EXPORTS = (
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
     "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
     "jr_newton_block_noalloc", "jr_wall_loads", "jr_tangent_freestream", "jr_forces_tangent", "jr_park_case",
+    "jr_set_tile_tangent",
 )
 
 
@@ -455,6 +456,11 @@ class Context:
         """Park (or release) case ``k``: a parked case's Newton steps skip its
         GMRES and leave its state as it is (an empty serving slot)."""
         _check(self._lib.jr_park_case(self._h, int(k), int(bool(parked))))
+
+    def set_tile_tangent(self, enable: bool) -> None:
+        """2D: the Krylov tangent by LDS tiles (default) or by the separate
+        node, face and cell kernels (they agree to round-off)."""
+        _check(self._lib.jr_set_tile_tangent(self._h, int(bool(enable))))
 
     def set_graphs(self, enable: bool) -> None:
         """Replay Newton blocks from a captured HIP graph, or launch every

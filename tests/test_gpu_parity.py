@@ -138,6 +138,30 @@ def test_newton_steps_match_oracle(name, kind, flowf, flux, lim, solver):
     assert ctx.scalars()[0, 0] == pytest.approx(s.cfl, rel=1e-12)
 
 
+@pytest.mark.parametrize("name, kind, flowf, flux, lim", [CASES[1], CASES[4], CASES[5]],
+                         ids=[CASES[1][0], CASES[4][0], CASES[5][0]])
+def test_tile_tangent_equals_the_node_face_and_cell_kernels(name, kind, flowf, flux, lim):
+    """The Krylov tangent by LDS tiles (k_tangent_tile) against the separate
+    node, face and cell kernels: four Newton steps from the same state give
+    the same residual history and state to round-off, Euler, laminar and SA
+    on the O- and C-grids (tiles cut by the canvas edges in both directions)."""
+    print(f"[gpu] tile tangent {name}", flush=True)
+    cfg = ImplicitConfig(krylov_subspace=12, krylov_restarts=2)
+    runs = []
+    for tile in (True, False):
+        atlas, flow, num, st, ctx = _setup(kind, flowf, flux, lim, implicit=cfg)
+        U = perturbed(OR.U0_of(st), 1e-2)
+        ctx.set_tile_tangent(tile)
+        ctx.set_state(U)
+        ctx.newton_init(cfg.cfl)
+        runs.append((ctx.newton_block(4)[:, 0], ctx.get_state()[0] - U))
+        ctx.close()
+    (h1, d1), (h0, d0) = runs
+    print(f"[gpu]   histories {h1} vs {h0}; state {rel(d1, d0):.2e}", flush=True)
+    assert np.allclose(h1, h0, rtol=1e-11, atol=0.0)
+    assert rel(d1, d0) < 1e-10
+
+
 def test_run_converges_to_the_oracle_forces():
     """The drop-in run() on the small Euler case converges (1e-10 drop) to
     the oracle's forces within 1e-8 (tests/golden/euler_o48_converged.json)."""
