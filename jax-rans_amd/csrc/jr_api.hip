@@ -432,8 +432,8 @@ static void rk_step(jr_ctx* c, double* hist_row, const double* alphas, int n) {
   hipLaunchKernelGGL((k_local_dt<NV, VISC>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
                      c->ctl, c->U, c->vdt);
   launch_residual<NV, VISC>(c, c->U, c->R, 0);
-  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
-                     hist_row, c->n_cases);
+  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, (double*)nullptr, (double*)nullptr,
+                     (double*)nullptr, c->red, hist_row, c->n_cases);   // norms only: no GMRES start
   for (int k = 0; k < n; ++k) {
     if (k > 0) launch_residual<NV, VISC>(c, c->W, c->R, 0);
     double* out = k + 1 < n ? c->W : c->U;
@@ -605,8 +605,8 @@ static void rk_step3(jr_ctx* c, double* hist_row, const double* alphas, int n) {
   hipLaunchKernelGGL((k_local_dt3<NV, VISC>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, c->P3,
                      c->ph, c->ctl, c->U, c->vdt);
   launch_residual3<NV, VISC>(c, c->U, c->R, 0);
-  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, c->X, c->Act, c->Rv, c->red,
-                     hist_row, c->n_cases);
+  hipLaunchKernelGGL((k_norms<NV>), vg, dim3(BLOCK), 0, c->stream, P, c->ctl, c->R, (double*)nullptr, (double*)nullptr,
+                     (double*)nullptr, c->red, hist_row, c->n_cases);   // norms only: no GMRES start
   for (int k = 0; k < n; ++k) {
     if (k > 0) launch_residual3<NV, VISC>(c, c->W, c->R, 0);
     double* out = k + 1 < n ? c->W : c->U;

@@ -36,11 +36,14 @@ __global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, 
   const int nc = P.nc;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   double v[3] = {0.0, 0.0, 0.0};
+  const bool gmres_start = X != nullptr;   // null for the explicit solver: norms only
   JR_VEC_LOOP(N) {
     double r = R[base + e];
-    X[base + e] = 0.0;
-    Act[base + e] = 0.0;
-    Rv[base + e] = -r;
+    if (gmres_start) {
+      X[base + e] = 0.0;
+      Act[base + e] = 0.0;
+      Rv[base + e] = -r;
+    }
     v[2] += r * r;
     if (e < (size_t)nc && P.interior[e]) {
       double w = P.area[e];
