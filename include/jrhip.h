@@ -225,11 +225,11 @@ int jr_newton_block_noalloc(jr_ctx* ctx, int n_steps, double* hist_host);
  * src/jaxrans/derivatives.py:94-120. */
 int jr_linear_solve(jr_ctx* ctx, const double* b_host, double* x_host);
 
-/* Parameter tangents of the sensitivity solve, 2D contexts.  The flow
- * parameters reach the residual and the force coefficients only through the
- * freestream primitives and mu_inf (flow_inputs, src/jaxrans/derivatives.py:
- * 65-70), so a parameter direction is the freestream's tangent:
- * dq_inf_host[n_cases][5] (the layout of jr_set_freestream's q_inf) and
+/* Parameter tangents of the sensitivity solve.  The flow parameters reach
+ * the residual and the force coefficients only through the freestream
+ * primitives and mu_inf (flow_inputs, src/jaxrans/derivatives.py:65-70), so
+ * a parameter direction is the freestream's tangent: dq_inf_host[n_cases][5]
+ * (3D: [6]; the layout of jr_set_freestream's q_inf) and
  * dmu_inf_host[n_cases] (null: zero).  dU_host may be null (zero).
  *
  * jr_tangent_freestream: d/de R(U + e dU; q_inf + e dq_inf, mu_inf + e dmu_inf)
@@ -248,6 +248,11 @@ int jr_tangent_freestream(jr_ctx* ctx, const double* U_host, const double* dU_ho
                           const double* dmu_inf_host, double* out_host);
 int jr_forces_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, const double* dq_inf_host,
                       const double* dmu_inf_host, double length, double centre_x, double centre_y, double* out_host);
+/* ... and of jr_forces3's coefficients (3D contexts): out_host[n_cases][8] =
+ * d(cl, cd, cm, cd_friction, fx, fy, fz, cy). */
+int jr_forces3_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, const double* dq_inf_host,
+                       const double* dmu_inf_host, double area, double length, double centre_x, double centre_z,
+                       double* out_host);
 
 /* n_steps explicit multistage Runge-Kutta steps on the device state, 2D
  * contexts: U^(k) = U^n - alphas[k] dt/V R(U^(k-1)) with the local time step

@@ -46,6 +46,9 @@ struct Plan3 {
   const double* w_into;
   const double* w_mid;             // [3][n_walls]
   const double* dg;                // [10][nft] each face's diamond: d2 x d3, d3 x d1, d1 x d2, det (viscous)
+  // The freestream's tangent along a flow parameter, dfree[case][8] =
+  // (dq_inf[0..5], dmu_inf, unused); read only by the parameter-tangent kernels.
+  const double* dfree;
 };
 
 __device__ __forceinline__ void face_vec3(const Plan3& P, int f, double* s) {
@@ -60,7 +63,9 @@ __device__ __forceinline__ void face_vec3(const Plan3& P, int f, double* s) {
 // Primitives with the ghost fill in one launch (residual.py:312-322): threads
 // [0, nc) convert the cells no stage writes; thread nc + g evaluates ghost g
 // from its root cell through its boundary conditions in order.
-template <class T, int NV>
+// PT (with T = Dual): the freestream carries its tangent along a flow
+// parameter (P.dfree), so the tangent is that of the fill along (dU, dq_inf).
+template <class T, int NV, bool PT = false>
 __global__ void __launch_bounds__(BLOCK) k_fill3(Plan3 P, double gamma, const Ctl* __restrict__ ctl,
                                                  const double* __restrict__ U, const double* __restrict__ dU,
                                                  double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
@@ -69,6 +74,8 @@ __global__ void __launch_bounds__(BLOCK) k_fill3(Plan3 P, double gamma, const Ct
   if (check_active && !ctl[cs].active) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
+  typename Free<PT>::type qfree[6];
+  for (int k = 0; k < 6; ++k) qfree[k] = free_val<PT>(ctl[cs].q_inf[k], P.dfree, cs, k);
   int src, dst, g = -1;
   if (t < nc) {
     if (P.ghost[t]) return;
@@ -88,7 +95,7 @@ __global__ void __launch_bounds__(BLOCK) k_fill3(Plan3 P, double gamma, const Ct
       const int task = P.g_ops[(size_t)g * MAX_OPS + o];
       const double n[3] = {P.t_norm[task], P.t_norm[(size_t)P.n_tasks + task], P.t_norm[(size_t)2 * P.n_tasks + task]};
       T gq[NV];
-      bc_state3<T, NV>(P.t_kind[task], q, n, ctl[cs].q_inf, gamma, gq);
+      bc_state3<T, NV>(P.t_kind[task], q, n, qfree, gamma, gq);
       for (int k = 0; k < NV; ++k) q[k] = gq[k];
     }
   }
@@ -234,7 +241,7 @@ __global__ void __launch_bounds__(BLOCK) k_diamond3(Plan3 P, double* __restrict_
 
 // The family a is a template parameter so that every index below is known
 // at compile time (a runtime family index sent the index arrays to scratch).
-template <class T, int NV, bool VISC, int a>
+template <class T, int NV, bool VISC, int a, bool PT = false>
 __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int first_order, const Ctl* __restrict__ ctl,
                                       const double* __restrict__ Q, const double* __restrict__ dQ,
                                       const double* __restrict__ Nd, const double* __restrict__ dNd,
@@ -289,7 +296,7 @@ __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int fir
   if constexpr (NV > 5) Fl[5] = lift<T>(0.0);
 
   if (VISC) {
-    const double mu_inf = ctl[cs].mu_inf;
+    const auto mu_inf = free_val<PT>(ctl[cs].mu_inf, P.dfree, cs, 6);
     // the diamond's geometry (state independent, k_diamond3): the cross
     // products of the cell line d1 and the two node diagonals d2, d3
     double c23[3], c31[3], c12[3];
@@ -376,7 +383,8 @@ __device__ __forceinline__ void face3(const Plan3& P, const Physics& ph, int fir
   }
 }
 
-template <class T, int NV, bool VISC>
+// PT (with T = Dual): mu_inf carries its tangent along a flow parameter.
+template <class T, int NV, bool VISC, bool PT = false>
 __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int first_order, const Ctl* __restrict__ ctl,
                                                     const double* __restrict__ Q, const double* __restrict__ dQ,
                                                     const double* __restrict__ Nd, const double* __restrict__ dNd,
@@ -385,10 +393,10 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int fi
   const int cs = blockIdx.y;
   if (check_active && !ctl[cs].active) return;
   if (f >= P.nft) return;
-  if (f < P.nf[0]) face3<T, NV, VISC, 0>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f, cs);
+  if (f < P.nf[0]) face3<T, NV, VISC, 0, PT>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f, cs);
   else if (f < P.nf[0] + P.nf[1])
-    face3<T, NV, VISC, 1>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f - P.nf[0], cs);
-  else face3<T, NV, VISC, 2>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f - P.nf[0] - P.nf[1], cs);
+    face3<T, NV, VISC, 1, PT>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f - P.nf[0], cs);
+  else face3<T, NV, VISC, 2, PT>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f - P.nf[0] - P.nf[1], cs);
 }
 
 // Per cell: R = div F (- V * SA source) at interior cells, U - U0 elsewhere;
@@ -470,7 +478,10 @@ __global__ void __launch_bounds__(BLOCK) k_sa_lin3(Plan3 P, Physics ph, const Ct
   for (int b = 0; b < 3; ++b) L[(size_t)(6 + b) * nc] = rg * (2.0 * gn[b]);
 }
 
-template <class T, int NV>
+// PT (with T = Dual): the freestream and mu_inf carry their tangent along a
+// flow parameter -- U0 moves with it, and the SA source is differentiated in
+// full (its partials at the state hold dU only).
+template <class T, int NV, bool PT = false>
 __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
                                                     const double* __restrict__ U, const double* __restrict__ dU,
                                                     const double* __restrict__ Q, const double* __restrict__ dQ,
@@ -490,6 +501,11 @@ __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const 
       double u0[6];
       conservative3<double, NV>(ctl[cs].q_inf, ph.gamma, u0);
       for (int k = 0; k < NV; ++k) out[base + (size_t)k * nc + c] = U[base + (size_t)k * nc + c] - u0[k];
+    } else if constexpr (PT) {
+      Dual qf[6], u0[6];
+      for (int k = 0; k < 6; ++k) qf[k] = free_val<true>(ctl[cs].q_inf[k], P.dfree, cs, k);
+      conservative3<Dual, NV>(qf, ph.gamma, u0);
+      for (int k = 0; k < NV; ++k) out[base + (size_t)k * nc + c] = dU[base + (size_t)k * nc + c] - u0[k].d;
     } else {
       for (int k = 0; k < NV; ++k) {
         double d = dU[base + (size_t)k * nc + c];
@@ -510,7 +526,7 @@ __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const 
     r[v] = ((Fv[hi0] - Fv[lo0]) + (Fv[hi1] - Fv[lo1])) + (Fv[hi2] - Fv[lo2]);
   }
   if constexpr (NV > 5) {
-   if constexpr (tangent) {
+   if constexpr (tangent && !PT) {
     // the source's tangent from its partials at the state (k_sa_lin3): the
     // face-average gradients are linear in dq, the rest is a dot product
     const int nb[6] = {c - NJ * NK, c + NJ * NK, c - NK, c + NK, c - 1, c + 1};
@@ -547,7 +563,8 @@ __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const 
    } else {
     // SA source (turbulence.py:140-151): vorticity and grad nu_hat by
     // Green-Gauss over the face averages with the six neighbours
-    const double gamma = ph.gamma, nu_inf = ctl[cs].mu_inf;
+    const double gamma = ph.gamma;
+    const auto nu_inf = free_val<PT>(ctl[cs].mu_inf, P.dfree, cs, 6);
     const int nb[6] = {c - NJ * NK, c + NJ * NK, c - NK, c + NK, c - 1, c + 1};
     const int fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
     T qc[NV];
@@ -1371,9 +1388,11 @@ __global__ void __launch_bounds__(BLOCK) k_check3(Plan3 P, double gamma, Ctl* __
 
 // Laminar viscous force of wall face `fce` (concatenated numbering) from the
 // ghost-filled primitives, the 3D diamond as in k_faces3.
-template <int NV>
-__device__ void wall_shear3(const Plan3& P, const Physics& ph, const double* Q, size_t base, double mu_inf, int fce,
-                            double* out) {
+// T = Dual: the shear's tangent, from dQ (the fill's tangent) and a mu_inf
+// that may carry a parameter tangent (TM = Dual).
+template <int NV, class T = double, class TM = double>
+__device__ void wall_shear3(const Plan3& P, const Physics& ph, const double* Q, const double* dQ, size_t base,
+                            TM mu_inf, int fce, T* out) {
   const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
   const double gamma = ph.gamma;
   int a = 0, ff = fce;
@@ -1412,43 +1431,43 @@ __device__ void wall_shear3(const Plan3& P, const Physics& ph, const double* Q, 
   double c31[3] = {d3[1] * d1[2] - d3[2] * d1[1], d3[2] * d1[0] - d3[0] * d1[2], d3[0] * d1[1] - d3[1] * d1[0]};
   double c12[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
   const double det = (d1[0] * c23[0] + d1[1] * c23[1]) + d1[2] * c23[2];
-  auto phi_at = [&](int cell, double* o) {
-    double q[5];
-    for (int k = 0; k < 5; ++k) q[k] = Q[base + (size_t)k * nc + cell];
+  auto phi_at = [&](int cell, T* o) {
+    T q[5];
+    for (int k = 0; k < 5; ++k) q[k] = Io<T>::ld(Q, dQ, base + (size_t)k * nc + cell);
     o[0] = q[1]; o[1] = q[2]; o[2] = q[3]; o[3] = gamma * q[4] / q[0];
   };
-  double pn[4][4];
+  T pn[4][4];
   for (int e = 0; e < 4; ++e) {
-    double v[8][4];
+    T v[8][4];
     for (int di = 0; di < 2; ++di)
       for (int dj2 = 0; dj2 < 2; ++dj2)
         for (int dk2 = 0; dk2 < 2; ++dk2)
           phi_at(((nds[e][0] - 1 + di) * NJ + nds[e][1] - 1 + dj2) * NK + nds[e][2] - 1 + dk2, v[(di * 2 + dj2) * 2 + dk2]);
     for (int k = 0; k < 4; ++k) {
-      double w[8];
+      T w[8];
       for (int m = 0; m < 8; ++m) w[m] = v[m][k];
-      pn[e][k] = node_mean<double>(w);
+      pn[e][k] = node_mean<T>(w);
     }
   }
-  double pl[4], pr[4];
+  T pl[4], pr[4];
   phi_at(cL, pl);
   phi_at(cR, pr);
-  double gr[3][4];
+  T gr[3][4];
   for (int k = 0; k < 4; ++k) {
-    const double dp1 = pr[k] - pl[k], dp2 = pn[1][k] - pn[0][k], dp3 = pn[3][k] - pn[2][k];
+    const T dp1 = pr[k] - pl[k], dp2 = pn[1][k] - pn[0][k], dp3 = pn[3][k] - pn[2][k];
     for (int b = 0; b < 3; ++b) gr[b][k] = (dp1 * c23[b] + dp2 * c31[b] + dp3 * c12[b]) / det;
   }
-  double gv[3][3], gT[3];
+  T gv[3][3], gT[3];
   for (int c = 0; c < 3; ++c) {
     for (int b = 0; b < 3; ++b) gv[c][b] = gr[b][c];
     gT[c] = gr[c][3];
   }
-  double vel[3] = {0.5 * (pl[0] + pr[0]), 0.5 * (pl[1] + pr[1]), 0.5 * (pl[2] + pr[2])};
-  const double Tf = 0.5 * (pl[3] + pr[3]);
-  const double mu = viscosity<double>(Tf, mu_inf, ph.sutherland);
-  const double kc = mu / (PR * (gamma - 1.0));
-  double f[5];
-  viscous_flux3<double>(vel, gv, gT, s, mu, kc, nullptr, f);
+  T vel[3] = {0.5 * (pl[0] + pr[0]), 0.5 * (pl[1] + pr[1]), 0.5 * (pl[2] + pr[2])};
+  const T Tf = 0.5 * (pl[3] + pr[3]);
+  const T mu = viscosity<T>(Tf, mu_inf, ph.sutherland);
+  const T kc = mu / (PR * (gamma - 1.0));
+  T f[5];
+  viscous_flux3<T>(vel, gv, gT, s, mu, kc, nullptr, f);
   out[0] = f[1]; out[1] = f[2]; out[2] = f[3];
 }
 
@@ -1482,7 +1501,7 @@ __global__ void __launch_bounds__(BLOCK) k_forces3(Plan3 P, Physics ph, const Ct
     double pf[3] = {-p * s[0], -p * s[1], -p * s[2]};
     double t[3] = {0.0, 0.0, 0.0};
     if (ph.viscous) {
-      wall_shear3<NV>(P, ph, Q, base, ctl[cs].mu_inf, fce, t);
+      wall_shear3<NV>(P, ph, Q, nullptr, base, ctl[cs].mu_inf, fce, t);
       for (int c = 0; c < 3; ++c) t[c] *= into;
     }
     if (loads) {   // per wall face: gauge pressure, viscous force on the body (x, y, z)
@@ -1511,6 +1530,74 @@ __global__ void __launch_bounds__(BLOCK) k_forces3(Plan3 P, Physics ph, const Ct
     o[5] = fs[1];
     o[6] = fs[2];
     o[7] = fs[1] / q_area;
+  }
+}
+
+// The tangent of k_forces3's coefficients along (dU, dq_inf, dmu_inf): the
+// same integration on dual numbers over the parameter-tangent fill's dQ and
+// the freestream's tangent P.dfree (the output JVP of the sensitivity,
+// src/jaxrans/solvers/sensitivity.py:63).  out[case][8] = d(CL, CD, CM,
+// CD friction, Fx, Fy, Fz, CY).
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_forces3_tangent(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
+                                                           const double* __restrict__ Q, const double* __restrict__ dQ,
+                                                           double area, double length, double cx, double cz,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[18][4];
+  const int cs = blockIdx.x;
+  const size_t base = (size_t)cs * NV * P.nc;
+  const double gamma = ph.gamma;
+  Dual qi[5];
+  for (int k = 0; k < 5; ++k) qi[k] = free_val<true>(ctl[cs].q_inf[k], P.dfree, cs, k);
+  const Dual mu_inf = free_val<true>(ctl[cs].mu_inf, P.dfree, cs, 6);
+  const Dual speed = jsqrt(dot3(qi + 1, qi + 1));
+  const Dual dr[3] = {qi[1] / speed, qi[2] / speed, qi[3] / speed};
+  const Dual li[3] = {-dr[2], mkd(0.0, 0.0), dr[0]};
+  const Dual q_area = 0.5 * qi[0] * speed * speed * area;
+  Dual acc[9];   // lift, drag, moment, shear x, y, z, force x, y, z
+  for (int k = 0; k < 9; ++k) acc[k] = mkd(0.0, 0.0);
+  for (int w = threadIdx.x; w < P.n_walls; w += BLOCK) {
+    const int cell = P.w_cell[w], fam = P.w_fam[w];
+    const int fce = (fam > 0 ? P.nf[0] : 0) + (fam > 1 ? P.nf[1] : 0) + P.w_face[w];
+    const double into = P.w_into[w];
+    double s[3];
+    face_vec3(P, fce, s);
+    for (int c = 0; c < 3; ++c) s[c] *= into;
+    const Dual p = Io<Dual>::ld(Q, dQ, base + (size_t)4 * P.nc + cell) - 1.0 / gamma;
+    const Dual pf[3] = {-p * s[0], -p * s[1], -p * s[2]};
+    Dual t[3] = {mkd(0.0, 0.0), mkd(0.0, 0.0), mkd(0.0, 0.0)};
+    if (ph.viscous) {
+      wall_shear3<NV, Dual, Dual>(P, ph, Q, dQ, base, mu_inf, fce, t);
+      for (int c = 0; c < 3; ++c) t[c] = t[c] * into;
+    }
+    const Dual ld[3] = {pf[0] + t[0], pf[1] + t[1], pf[2] + t[2]};
+    const double rx = P.w_mid[w] - cx, rz = P.w_mid[(size_t)2 * P.n_walls + w] - cz;
+    const Dual moment = rz * ld[0] - rx * ld[2];
+    acc[0] = acc[0] + dot3(li, ld) / q_area;
+    acc[1] = acc[1] + dot3(dr, ld) / q_area;
+    acc[2] = acc[2] + (moment / length) / q_area;
+    for (int c = 0; c < 3; ++c) {
+      acc[3 + c] = acc[3 + c] + t[c];
+      acc[6 + c] = acc[6 + c] + ld[c];
+    }
+  }
+  double v[18];
+  for (int k = 0; k < 9; ++k) { v[k] = acc[k].v; v[9 + k] = acc[k].d; }
+  block_sum<18>(v, sh);
+  if (threadIdx.x == 0) {
+    Dual sv[3];
+    for (int c = 0; c < 3; ++c) sv[c] = mkd(v[3 + c], v[12 + c]);
+    const Dual friction = dot3(dr, sv) / q_area;
+    const Dual cy = mkd(v[7], v[16]) / q_area;
+    double* o = out + (size_t)cs * 8;
+    o[0] = v[9];
+    o[1] = v[10];
+    o[2] = v[11];
+    o[3] = friction.d;
+    o[4] = v[15];
+    o[5] = v[16];
+    o[6] = v[17];
+    o[7] = cy.d;
   }
 }
 

@@ -784,6 +784,35 @@ static void launch_forces_tangent(jr_ctx* c, const double* U, const double* dU, 
                      (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, length, cx, cy, out);
 }
 
+template <int NV, bool VISC>
+static void launch_param_tangent3(jr_ctx* c, const double* U, const double* dU, double* out) {
+  const Plan3& P = c->P3;
+  launch_residual3<NV, VISC>(c, U, c->R, 0);
+  hipLaunchKernelGGL((k_fill3<Dual, NV, true>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, 0);
+  const size_t nn = (size_t)(P.NI + 1) * (P.NJ + 1) * (P.NK + 1);
+  if (VISC)
+    hipLaunchKernelGGL((k_nodes3<Dual, NV>), grid1(nn, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
+                       (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, (double*)nullptr, c->dNd, 0);
+  hipLaunchKernelGGL((k_faces3<Dual, NV, VISC, true>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
+                     c->ph, 0, (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, (const double*)c->Nd,
+                     (const double*)c->dNd, c->F3, 0);
+  hipLaunchKernelGGL((k_cells3<Dual, NV, true>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     (const Ctl*)c->ctl, U, dU, (const double*)c->Q, (const double*)c->dQ, (const double*)c->F3,
+                     (const double*)nullptr, (const double*)c->salin, out, 0);
+}
+
+template <int NV, bool VISC>
+static void launch_forces3_tangent(jr_ctx* c, const double* U, const double* dU, double area, double length,
+                                   double cx, double cz, double* out) {
+  const Plan3& P = c->P3;
+  launch_residual3<NV, VISC>(c, U, c->R, 0);
+  hipLaunchKernelGGL((k_fill3<Dual, NV, true>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0,
+                     c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, 0);
+  hipLaunchKernelGGL((k_forces3_tangent<NV>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
+                     (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, area, length, cx, cz, out);
+}
+
 extern "C" {
 
 const char* jr_last_error(void) { return g_error.c_str(); }
@@ -2043,17 +2072,18 @@ int jr_linear_solve(jr_ctx* c, const double* b, double* x) {
 static int stage_param_tangent(jr_ctx* c, const char* who, const double* U, const double* dU, const double* dq_inf,
                                const double* dmu_inf, const double** dU_dev) {
   if (!c || !U || !dq_inf) return fail(std::string(who) + ": null argument");
-  if (c->dim == 3) return fail(std::string(who) + ": 2D contexts only");
   if (c->pending_steps) return fail(std::string(who) + ": an asynchronous block is pending: wait for it first");
   if (!c->has_q) return fail(std::string(who) + ": set the freestream first");
   if (!c->dfree) {
     TRY(dalloc(c, &c->dfree, (size_t)c->n_cases * 8));
     HIPCHK(hipDeviceSynchronize());   // dalloc clears on the null stream
     c->P.dfree = c->dfree;
+    c->P3.dfree = c->dfree;
   }
+  const int nq = c->dim == 3 ? 6 : 5;   // the layout of jr_set_freestream's q_inf
   std::vector<double> h((size_t)c->n_cases * 8, 0.0);
   for (int k = 0; k < c->n_cases; ++k) {
-    for (int v = 0; v < 5; ++v) h[(size_t)k * 8 + v] = dq_inf[k * 5 + v];
+    for (int v = 0; v < nq; ++v) h[(size_t)k * 8 + v] = dq_inf[k * nq + v];
     h[(size_t)k * 8 + 6] = dmu_inf ? dmu_inf[k] : 0.0;
     for (int v = 0; v < 7; ++v)
       if (!std::isfinite(h[(size_t)k * 8 + v])) return fail(std::string(who) + ": non-finite freestream tangent");
@@ -2078,9 +2108,17 @@ int jr_tangent_freestream(jr_ctx* c, const double* U, const double* dU, const do
   if (!out) return fail("jr_tangent_freestream: null argument");
   const double* dUd = nullptr;
   TRY(stage_param_tangent(c, "jr_tangent_freestream", U, dU, dq_inf, dmu_inf, &dUd));
-  if (c->nv == 5) launch_param_tangent<5, true>(c, c->tmpA, dUd, c->tmpB);
-  else if (c->ph.viscous) launch_param_tangent<4, true>(c, c->tmpA, dUd, c->tmpB);
-  else launch_param_tangent<4, false>(c, c->tmpA, dUd, c->tmpB);
+  if (c->dim == 3) {
+    if (c->nv == 6) launch_param_tangent3<6, true>(c, c->tmpA, dUd, c->tmpB);
+    else if (c->ph.viscous) launch_param_tangent3<5, true>(c, c->tmpA, dUd, c->tmpB);
+    else launch_param_tangent3<5, false>(c, c->tmpA, dUd, c->tmpB);
+  } else if (c->nv == 5) {
+    launch_param_tangent<5, true>(c, c->tmpA, dUd, c->tmpB);
+  } else if (c->ph.viscous) {
+    launch_param_tangent<4, true>(c, c->tmpA, dUd, c->tmpB);
+  } else {
+    launch_param_tangent<4, false>(c, c->tmpA, dUd, c->tmpB);
+  }
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(out, c->tmpB, field_bytes(c), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2091,11 +2129,28 @@ int jr_forces_tangent(jr_ctx* c, const double* U, const double* dU, const double
                       double length, double cx, double cy, double* out) {
   if (!out) return fail("jr_forces_tangent: null argument");
   if (!(length > 0.0)) return fail("jr_forces_tangent: length must be positive");
+  if (c && c->dim == 3) return fail("jr_forces_tangent: a 3D context needs jr_forces3_tangent (a reference area)");
   const double* dUd = nullptr;
   TRY(stage_param_tangent(c, "jr_forces_tangent", U, dU, dq_inf, dmu_inf, &dUd));
   if (c->nv == 5) launch_forces_tangent<5, true>(c, c->tmpA, dUd, length, cx, cy, c->coef);
   else if (c->ph.viscous) launch_forces_tangent<4, true>(c, c->tmpA, dUd, length, cx, cy, c->coef);
   else launch_forces_tangent<4, false>(c, c->tmpA, dUd, length, cx, cy, c->coef);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_forces3_tangent(jr_ctx* c, const double* U, const double* dU, const double* dq_inf, const double* dmu_inf,
+                       double area, double length, double cx, double cz, double* out) {
+  if (!out) return fail("jr_forces3_tangent: null argument");
+  if (c && c->dim != 3) return fail("jr_forces3_tangent: not a 3D context");
+  if (!(area > 0.0) || !(length > 0.0)) return fail("jr_forces3_tangent: area and length must be positive");
+  const double* dUd = nullptr;
+  TRY(stage_param_tangent(c, "jr_forces3_tangent", U, dU, dq_inf, dmu_inf, &dUd));
+  if (c->nv == 6) launch_forces3_tangent<6, true>(c, c->tmpA, dUd, area, length, cx, cz, c->coef);
+  else if (c->ph.viscous) launch_forces3_tangent<5, true>(c, c->tmpA, dUd, area, length, cx, cz, c->coef);
+  else launch_forces3_tangent<5, false>(c, c->tmpA, dUd, area, length, cx, cz, c->coef);
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

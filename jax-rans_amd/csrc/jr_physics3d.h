@@ -138,8 +138,9 @@ __device__ void roe3(const T* qL, const T* qR, const double* s, double gamma, T*
 
 // Ghost primitives from the source cell (src/jaxrans/numerics/boundary.py:54-157);
 // n the outward unit normal, qf the far-field reference primitives.
-template <class T, int NV>
-__device__ void bc_state3(int kind, const T* q, const double* n, const double* qf, double gamma, T* g) {
+// qf plain (TQ = double) or carrying a parameter tangent (TQ = T = Dual).
+template <class T, int NV, class TQ>
+__device__ void bc_state3(int kind, const T* q, const double* n, const TQ* qf, double gamma, T* g) {
   if (kind == K_SLIP || kind == K_SYMMETRY) {
     T vn = dot3(q + 1, n);
     g[0] = q[0];
@@ -151,30 +152,30 @@ __device__ void bc_state3(int kind, const T* q, const double* n, const double* q
     if constexpr (NV > 5) g[5] = -q[5];
   } else if (kind == K_FARFIELD) {
     T a_i = jsqrt(gamma * q[4] / q[0]);
-    double a_f = sqrt(gamma * qf[4] / qf[0]);
+    TQ a_f = jsqrt(gamma * qf[4] / qf[0]);
     double two_over = 2.0 / (gamma - 1.0);
     T vn_i = dot3(q + 1, n);
-    double vn_f = dot3(qf + 1, n);
+    TQ vn_f = dot3(qf + 1, n);
     T rp = vn_i + two_over * a_i;
-    double rm = vn_f - two_over * a_f;
+    TQ rm = vn_f - two_over * a_f;
     T vn_b = 0.5 * (rp + rm);
     T a_b = 0.25 * (gamma - 1.0) * (rp - rm);
     bool out = val(vn_b) > 0.0;
-    T entropy = out ? q[4] / jpow(q[0], gamma) : lift<T>(qf[4] / pow(qf[0], gamma));
+    T entropy = out ? q[4] / jpow(q[0], gamma) : to_t<T>(qf[4] / jpow(qf[0], gamma));
     T rho_b = jpow(a_b * a_b / (gamma * entropy), 1.0 / (gamma - 1.0));
     g[0] = rho_b;
     for (int c = 0; c < 3; ++c) {
-      T t = out ? q[1 + c] - vn_i * n[c] : lift<T>(qf[1 + c] - vn_f * n[c]);
+      T t = out ? q[1 + c] - vn_i * n[c] : to_t<T>(qf[1 + c] - vn_f * n[c]);
       g[1 + c] = vn_b * n[c] + t;
     }
     g[4] = rho_b * a_b * a_b / gamma;
-    if constexpr (NV > 5) g[5] = out ? q[5] : lift<T>(qf[5]);
+    if constexpr (NV > 5) g[5] = out ? q[5] : to_t<T>(qf[5]);
   } else if (kind == K_INFLOW) {
     double g1 = gamma - 1.0;
-    double t_f = gamma * qf[4] / qf[0];
-    double speed_f = sqrt(dot3(qf + 1, qf + 1));
-    double t_tot = t_f + 0.5 * g1 * speed_f * speed_f;
-    double p_tot = qf[4] * pow(t_tot / t_f, gamma / g1);
+    TQ t_f = gamma * qf[4] / qf[0];
+    TQ speed_f = jsqrt(dot3(qf + 1, qf + 1));
+    TQ t_tot = t_f + 0.5 * g1 * speed_f * speed_f;
+    TQ p_tot = qf[4] * jpow(t_tot / t_f, gamma / g1);
     T p_b = q[4];
     T mach2 = jmax(2.0 / g1 * (jpow(p_tot / p_b, g1 / gamma) - 1.0), lift<T>(0.0));
     T t_b = t_tot / (1.0 + 0.5 * g1 * mach2);
@@ -182,10 +183,10 @@ __device__ void bc_state3(int kind, const T* q, const double* n, const double* q
     g[0] = gamma * p_b / t_b;
     for (int c = 0; c < 3; ++c) g[1 + c] = speed * qf[1 + c] / speed_f;
     g[4] = p_b;
-    if constexpr (NV > 5) g[5] = lift<T>(qf[5]);
+    if constexpr (NV > 5) g[5] = to_t<T>(qf[5]);
   } else if (kind == K_OUTFLOW) {
     for (int k = 0; k < NV; ++k) g[k] = q[k];
-    g[4] = lift<T>(qf[4]);
+    g[4] = to_t<T>(qf[4]);
   } else {
     for (int k = 0; k < NV; ++k) g[k] = q[k];
   }

@@ -21,19 +21,15 @@ the freestream primitives and ``mu_inf = M / Re`` (``flow_inputs``, as
 ``derivatives.py:65-70``): ``q = (1, M cos a, M sin a, 1/gamma[, nu ratio])``
 with ``a`` in radians from degrees, so a parameter direction is the
 freestream's tangent ``(dq, dmu)`` in closed form (``freestream_tangent``).
-On 2D contexts the two parameter products that the reference takes by
-forward-mode differentiation -- ``dR/dp`` at fixed ``U``
-(``solvers/sensitivity.py:60``) and the output tangent ``dJ/dp|_U + dJ/dU dU``
-(``:63``) -- are exact: the device carries the freestream's tangent through
-the residual's and the force integration's dual-number kernels
-(``jr_tangent_freestream``, ``jr_forces_tangent``).  3D contexts form the
-two products from device evaluations at perturbed freestreams and states instead, by Richardson's
-fourth-order central difference ``(4 D(h/2) - D(h)) / 3`` of ``D(h) = (f(h) -
-f(-h)) / 2h`` with ``h`` = 1e-4 degree for ``alpha`` and 1e-5 for ``mach``.
-The derivative exists where the discretisation is differentiable
-(``minmod`` and ``van_albada`` have kinks: use ``van_albada_smooth`` or no
-limiter) and is as accurate as the result is converged.  A swept (2.5D)
-case is refused.
+The two parameter products that the reference takes by forward-mode
+differentiation -- ``dR/dp`` at fixed ``U`` (``solvers/sensitivity.py:60``)
+and the output tangent ``dJ/dp|_U + dJ/dU dU`` (``:63``) -- are exact here
+too: the device carries the freestream's tangent through the residual's and
+the force integration's dual-number kernels (``jr_tangent_freestream``,
+``jr_forces_tangent``, ``jr_forces3_tangent``), in 2D and 3D.  The derivative
+exists where the discretisation is differentiable (``minmod`` and
+``van_albada`` have kinks: use ``van_albada_smooth`` or no limiter) and is as
+accurate as the result is converged.  A swept (2.5D) case is refused.
 """
 
 from __future__ import annotations
@@ -45,11 +41,10 @@ from dataclasses import dataclass, replace
 import numpy as np
 
 from .core.config import FlowConditions
-from .simulation import Result, _forces, force_rows, freestream_of, make_context
+from .simulation import Result, freestream_of, make_context
 
 PARAMETERS: tuple[str, ...] = ("alpha", "mach")
 OUTPUTS: tuple[str, ...] = ("cl", "cd", "cm", "cd_friction")
-STEPS = {"alpha": 1e-4, "mach": 1e-5}   # central-difference steps h of the parameter products
 
 
 @dataclass(frozen=True)
@@ -113,14 +108,6 @@ def defect_correction(matvec, b: np.ndarray, approximate_inverse, tol: float, ma
     return x, float(np.linalg.norm(r)) / b_norm, k
 
 
-def _richardson(f, h: float):
-    """``(4 D(h/2) - D(h)) / 3``, ``D(s) = (f(s) - f(-s)) / 2s``: the
-    derivative of ``f`` at 0 to ``O(h^4)``."""
-    d_h = (f(h) - f(-h)) / (2.0 * h)
-    d_half = (f(0.5 * h) - f(-0.5 * h)) / h
-    return (4.0 * d_half - d_h) / 3.0
-
-
 def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), parameters: tuple[str, ...] = ("alpha",),
                 tol: float = 1e-10, maxiter: int = 10_000) -> Sensitivity:
     """Derivatives of ``outputs`` (``OUTPUTS``) with respect to ``parameters``
@@ -139,46 +126,28 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
     started = time.perf_counter()
     ctx = make_context(case, grid)
     try:
-        def at(p: dict, state: np.ndarray | None = None) -> None:
-            q, mu = flow_inputs(flow, dim, p)
-            ctx.set_freestream(q, mu)
-            if state is not None:
-                ctx.set_state(state)
-
-        def shifted(name: str, t: float) -> dict:
-            return {k: v + (t if k == name else 0.0) for k, v in p0.items()}
-
-        def residual_at(name: str, t: float) -> np.ndarray:
-            at(shifted(name, t))
-            return ctx.residual(U)[0]
-
-        def coefficients_at(name: str, dU: np.ndarray, t: float) -> np.ndarray:
-            at(shifted(name, t), U + t * dU)
-            f = _forces(force_rows(ctx, case)[0], dim)
-            return np.array([getattr(f, out) for out in outputs])
-
-        exact = ctx.dim == 2
-        reference = case.grid_spec
+        q, mu = flow_inputs(flow, dim, p0)
+        ctx.set_freestream(q, mu)
+        ctx.set_state(U)
+        ctx.newton_init(case.implicit.cfl_max)
+        spec = case.grid_spec
         rows = {"cl": 0, "cd": 1, "cm": 2, "cd_friction": 3}
 
         def parameter_rhs(name: str) -> np.ndarray:
-            if exact:
-                dq, dmu = freestream_tangent(flow, dim, name)
-                return ctx.tangent_freestream(U, dq, dmu)[0]
-            return _richardson(lambda t: residual_at(name, t), STEPS[name])
+            dq, dmu = freestream_tangent(flow, dim, name)
+            return ctx.tangent_freestream(U, dq, dmu)[0]
 
         def output_tangent(name: str, dU: np.ndarray) -> np.ndarray:
-            if exact:
-                dq, dmu = freestream_tangent(flow, dim, name)
-                row = ctx.forces_tangent(U, dq, dmu, dU=dU, length=reference.reference_length,
-                                         centre=tuple(reference.moment_centre[:2]))[0]
-                return np.array([row[rows[out]] for out in outputs])
-            return _richardson(lambda t: coefficients_at(name, dU, t), STEPS[name])
+            dq, dmu = freestream_tangent(flow, dim, name)
+            if dim == 3:
+                row = ctx.forces3_tangent(U, dq, dmu, dU=dU, area=spec.reference_area,
+                                          length=spec.reference_length, centre=tuple(spec.moment_centre))[0]
+            else:
+                row = ctx.forces_tangent(U, dq, dmu, dU=dU, length=spec.reference_length,
+                                         centre=tuple(spec.moment_centre[:2]))[0]
+            return np.array([row[rows[out]] for out in outputs])
 
-        rhs = {}
-        for name in parameters:
-            at(p0)
-            rhs[name] = parameter_rhs(name)
+        rhs = {name: parameter_rhs(name) for name in parameters}
 
         def matvec(x: np.ndarray) -> np.ndarray:
             return ctx.tangent(U, x, first_order=False)[0]
@@ -188,8 +157,6 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
 
         columns, errors, iterations = {}, {}, {}
         for name in parameters:
-            at(p0, U)   # the output differences below move the freestream and the state
-            ctx.newton_init(case.implicit.cfl_max)
             dU, errors[name], iterations[name] = defect_correction(matvec, -rhs[name], inverse, tol, maxiter)
             columns[name] = output_tangent(name, dU)
     finally:

@@ -30,7 +30,7 @@ EXPORTS = (
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
     "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
     "jr_newton_block_noalloc", "jr_wall_loads", "jr_tangent_freestream", "jr_forces_tangent", "jr_park_case",
-    "jr_set_tile_tangent",
+    "jr_set_tile_tangent", "jr_forces3_tangent",
 )
 
 
@@ -320,14 +320,14 @@ class Context:
         return out
 
     def _freestream_tangent(self, dq_inf, dmu_inf):
-        dq = np.zeros((self.n_cases, 5))
+        dq = np.zeros((self.n_cases, 6 if self.dim == 3 else 5))
         d = np.atleast_2d(np.asarray(dq_inf, np.float64))
         dq[:, :d.shape[1]] = d
         dmu = None if dmu_inf is None else np.ascontiguousarray(np.broadcast_to(dmu_inf, (self.n_cases,)), np.float64)
         return dq, dmu
 
     def tangent_freestream(self, U, dq_inf, dmu_inf=None, dU=None) -> np.ndarray:
-        """2D: the exact residual's tangent at ``U`` along the freestream's
+        """The exact residual's tangent at ``U`` along the freestream's
         tangent ``dq_inf`` (per case, the layout of ``set_freestream``),
         ``dmu_inf`` and the state's ``dU`` (None: zero)."""
         U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
@@ -350,6 +350,20 @@ class Context:
                                            _ptr(dq, C.c_double), None if dmu is None else _ptr(dmu, C.c_double),
                                            C.c_double(length), C.c_double(centre[0]), C.c_double(centre[1]),
                                            _ptr(out, C.c_double)))
+        return out
+
+    def forces3_tangent(self, U, dq_inf, dmu_inf=None, dU=None, area: float = 1.0, length: float = 1.0,
+                        centre=(0.25, 0.0, 0.0)) -> np.ndarray:
+        """3D: the tangent of ``forces3`` at ``U`` along ``(dU, dq_inf,
+        dmu_inf)``: ``(n_cases, 8)`` = d(cl, cd, cm, cd_friction, fx, fy, fz, cy)."""
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        dU = None if dU is None else np.ascontiguousarray(np.broadcast_to(dU, self.shape), np.float64)
+        dq, dmu = self._freestream_tangent(dq_inf, dmu_inf)
+        out = np.empty((self.n_cases, 8))
+        _check(self._lib.jr_forces3_tangent(self._h, _ptr(U, C.c_double), None if dU is None else _ptr(dU, C.c_double),
+                                            _ptr(dq, C.c_double), None if dmu is None else _ptr(dmu, C.c_double),
+                                            C.c_double(area), C.c_double(length), C.c_double(centre[0]),
+                                            C.c_double(centre[2]), _ptr(out, C.c_double)))
         return out
 
     def wave_speeds(self, U):

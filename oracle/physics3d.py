@@ -177,7 +177,7 @@ def characteristic(q, n, q_inf, gamma, d):
     rho_i, v_i, p_i = q[0], q[1:d + 1], q[d + 1]
     rho_f, v_f, p_f = q_inf[0], list(q_inf[1:d + 1]), q_inf[d + 1]
     a_i = M.sqrt(gamma * p_i / rho_i)
-    a_f = np.sqrt(gamma * p_f / rho_f)
+    a_f = M.sqrt(gamma * p_f / rho_f)
     two_over = 2.0 / (gamma - 1.0)
     vn_i, vn_f = dot(v_i, n), dot(v_f, n)
     r_plus = vn_i + two_over * a_i
@@ -198,9 +198,12 @@ def characteristic(q, n, q_inf, gamma, d):
 def inflow_total(q, q_inf, gamma, d):
     """(src/jaxrans/numerics/boundary.py:130-147)"""
     g1 = gamma - 1.0
-    rho_f, v_f, p_f = q_inf[0], [np.float64(v) for v in q_inf[1:d + 1]], q_inf[d + 1]
+    rho_f, v_f, p_f = q_inf[0], [M.scalar(v) for v in q_inf[1:d + 1]], q_inf[d + 1]
     t_f = gamma * p_f / rho_f
-    speed_f = float(magnitude_const(v_f))
+    if M.is_dual(*v_f):   # a freestream carrying a parameter tangent
+        speed_f = P.speed_of(v_f[0], v_f[1]) if d == 2 else M.sqrt(dot(v_f, v_f))
+    else:
+        speed_f = float(magnitude_const(v_f))
     t_tot = t_f + 0.5 * g1 * speed_f * speed_f
     p_tot = p_f * (t_tot / t_f) ** (gamma / g1)
     p_b = q[d + 1]

@@ -58,8 +58,13 @@ def setup_for(atlas, flow, numerics) -> Setup3D:
                    0.3 if flow.turbulence == "sa_noft2_qcr" else 0.0)
 
 
-def U0_of(setup: Setup3D) -> np.ndarray:
-    c = np.array([np.float64(x) for x in P3.conservative(list(setup.q_inf), setup.gamma, D3)])
+def U0_of(setup: Setup3D):
+    """The uniform freestream state on the canvas; dual when the freestream
+    carries a parameter tangent."""
+    rows_ = P3.conservative([M.scalar(x) for x in setup.q_inf], setup.gamma, D3)
+    if M.is_dual(*rows_):
+        return M.stack([M.full(setup.atlas.shape, x) for x in rows_])
+    c = np.array([np.float64(x) for x in rows_])
     return np.broadcast_to(c[:, None, None, None], (c.size,) + setup.atlas.shape).copy()
 
 

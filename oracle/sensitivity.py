@@ -1,4 +1,4 @@
-"""Parameter tangent of a converged 2D state: the sensitivity solve
+"""Parameter tangent of a converged state: the sensitivity solve
 (``src/jaxrans/solvers/sensitivity.py:34-64``) over the oracle's residual,
 linear solve and forces (``src/jaxrans/derivatives.py:65-147``).
 
@@ -11,13 +11,14 @@ through the freestream primitives and ``mu_inf = M / Re`` (``flow_inputs``,
 ``derivatives.py:65-70``, over ``freestream_vector``,
 ``core/state.py:176-192``), so one parameter direction is the freestream
 carried as dual scalars: ``q = (1, M cos a, M sin a, 1/gamma[, nu ratio])``
-with ``a`` in radians from degrees.  The residual's parameter JVP is the
+(3D: velocity ``M (cos a, 0, sin a)``) with ``a`` in radians from degrees.  The residual's parameter JVP is the
 oracle residual evaluated on that freestream (ghost states, the ``U - U0``
 rows, the viscous and SA terms and the point-vortex reference all see it),
 and the output JVP is the oracle's force integration on the state's and the
 freestream's tangents together.  The linear solve is the reference's defect
 correction with ``G`` = the Newton step's GMRES at ``cfl_max``
-(``newton_inverse``, ``derivatives.py:94-120``).
+(``newton_inverse``, ``derivatives.py:94-120``; the solve is restated for
+2D setups, the parameter and output JVPs for 2D and 3D).
 """
 
 from __future__ import annotations
@@ -33,19 +34,26 @@ from .solver import LinePreconditioner, forces, gmres_right
 PARAMETERS = ("alpha", "mach")
 
 
-def freestream_dual(flow, name: str, nvar: int):
+def freestream_dual(flow, name: str, nvar: int, dim: int = 2):
     """The freestream primitives (``nvar`` dual scalars) and ``mu_inf`` (dual,
     or None if inviscid) at the flow's parameters, with tangent one along
-    ``name`` (``alpha`` per degree, ``mach``), the Reynolds number held."""
+    ``name`` (``alpha`` per degree, ``mach``), the Reynolds number held; in
+    3D the velocity is ``M (cos a, 0, sin a)``."""
     if name not in PARAMETERS:
         raise ValueError(name)
     a = M.D(np.deg2rad(flow.alpha_deg), np.deg2rad(1.0) if name == "alpha" else 0.0)
     mach = M.D(flow.mach, 1.0 if name == "mach" else 0.0)
-    q = [M.D(1.0, 0.0), mach * M.cos(a), mach * M.sin(a), M.D(1.0 / flow.gamma, 0.0)]
-    if nvar > 4:
+    u, w = mach * M.cos(a), mach * M.sin(a)
+    vel = [u, w] if dim == 2 else [u, M.D(0.0, 0.0), w]
+    q = [M.D(1.0, 0.0), *vel, M.D(1.0 / flow.gamma, 0.0)]
+    if nvar > dim + 2:
         q.append(M.D(flow.nu_tilde_ratio, 0.0))
     mu = None if flow.inviscid else mach / flow.reynolds
     return q, mu
+
+
+def _is_3d(setup) -> bool:
+    return len(setup.atlas.shape) == 3
 
 
 def tangents(q, mu):
@@ -58,15 +66,25 @@ def with_freestream(setup: Setup, q, mu) -> Setup:
 
 
 def residual_parameter_jvp(U, setup: Setup, q, mu, dU=None) -> np.ndarray:
-    """``d/de R(U + e dU; q + e dq, mu + e dmu)``: the exact residual's tangent."""
+    """``d/de R(U + e dU; q + e dq, mu + e dmu)``: the exact residual's tangent
+    (2D or 3D setup)."""
     dU = np.zeros_like(U) if dU is None else dU
+    if _is_3d(setup):
+        from .residual3d import residual as residual3
+        return residual3(M.D(U, dU), with_freestream(setup, q, mu)).d
     return residual(M.D(U, dU), with_freestream(setup, q, mu)).d
 
 
-def forces_jvp(U, dU, setup: Setup, q, mu, length=1.0, centre=(0.25, 0.0)) -> dict:
-    """Tangent of ``forces`` along the state's ``dU`` and the freestream's tangent."""
+def forces_jvp(U, dU, setup: Setup, q, mu, length=1.0, centre=(0.25, 0.0), area=None) -> dict:
+    """Tangent of ``forces`` along the state's ``dU`` and the freestream's
+    tangent (3D: ``forces3`` per ``area``, the moment about y through
+    ``centre = (x, y, z)``)."""
     dU = np.zeros_like(U) if dU is None else dU
-    f = forces(M.D(U, dU), with_freestream(setup, q, mu), length, centre)
+    if _is_3d(setup):
+        from .solver3d import forces as forces3
+        f = forces3(M.D(U, dU), with_freestream(setup, q, mu), area, length, centre)
+    else:
+        f = forces(M.D(U, dU), with_freestream(setup, q, mu), length, centre)
     return {k: float(M.tangent(v)) for k, v in f.items()}
 
 

@@ -18,6 +18,7 @@ import numpy as np
 from . import dual as M
 from . import physics as P
 from . import physics3d as P3
+from .residual import _dsum
 from .residual3d import (D3, Setup3D, _lr, face_averages, face_gradients, fill_ghosts, green_gauss, jvp, norm,
                          residual, wave_speed_sum, wave_speeds)
 from .solver import (NewtonState, _padded_chains, cr_factor, cr_solve, gmres_right, thomas_factor,
@@ -217,7 +218,9 @@ def newton_step(state: NewtonState, setup: Setup3D, cfg, method: str = "cr", exa
 
 
 def forces(U, setup: Setup3D, area: float, length: float = 1.0, centre=(0.25, 0.0, 0.0)):
-    """CL, CD, CM (pressure and friction about ``y``), CD friction, CY (forces.py:48-161)."""
+    """CL, CD, CM (pressure and friction about ``y``), CD friction, CY (forces.py:48-161).
+    Dual when ``U`` or the freestream carries a tangent (the sensitivity's
+    output JVP)."""
     at, gamma = setup.atlas, setup.gamma
     W = at.walls
     q = P3.primitive(_rows(U[:5]), gamma, D3)
@@ -225,6 +228,9 @@ def forces(U, setup: Setup3D, area: float, length: float = 1.0, centre=(0.25, 0.
     fam = [f.reshape(3, -1) for f in at.faces]
 
     def pick(fields):
+        if M.is_dual(*fields):
+            v = pick([M.value(f) for f in fields])
+            return M.D(v, pick([M.tangent(f) for f in fields]))
         out = np.zeros((3, W.n))
         for a in range(3):
             sel = W.family == a
@@ -237,7 +243,7 @@ def forces(U, setup: Setup3D, area: float, length: float = 1.0, centre=(0.25, 0.
         lam_setup = Setup3D(at, gamma, setup.q_inf[:5], setup.flux, setup.limiter, setup.kappa, setup.mu_inf,
                             setup.sutherland, False, 0.0)
         flat = fill_ghosts([r.reshape(-1) for r in q], lam_setup)
-        qc = [np.pad(r.reshape(at.shape), 1, mode="edge") for r in flat]
+        qc = [M.pad_edge(r.reshape(at.shape), 1) for r in flat]
         T1 = P3.temperature(qc, gamma, D3)
         grads = face_gradients([qc[1], qc[2], qc[3], T1], at.centres)
         vis = []
@@ -250,19 +256,23 @@ def forces(U, setup: Setup3D, area: float, length: float = 1.0, centre=(0.25, 0.
             k = mu / (P.PR * (gamma - 1.0))
             gv = [[gr[b][c] for b in range(3)] for c in range(3)]
             f = P3.viscous_flux(vel, gv, [gr[b][3] for b in range(3)], [at.faces[a][c] for c in range(3)], mu, k)
-            vis.append(np.stack([f[1 + c].reshape(-1) for c in range(3)]))
+            vis.append(M.stack([f[1 + c].reshape(-1) for c in range(3)]))
         shear = W.into_fluid * pick(vis)
-    rho, vel = setup.q_inf[0], np.asarray(setup.q_inf[1:4], np.float64)
-    speed = float(np.sqrt(P3.dot(list(vel), list(vel))))
+    rho, vel = M.scalar(setup.q_inf[0]), [M.scalar(v) for v in setup.q_inf[1:4]]
+    speed = M.sqrt(P3.dot(vel, vel)) if M.is_dual(*vel) else float(np.sqrt(P3.dot(list(vel), list(vel))))
     pressure = -p * S
     load = pressure + shear
-    drag = vel / speed
-    lift = np.array([-drag[2], 0.0, drag[0]])
+    drag = [v / speed for v in vel]
+    lift = [-drag[2], 0.0 * drag[0], drag[0]]
     q_area = 0.5 * rho * speed**2 * area
     r = W.mid - np.asarray(centre[:3])
     moment = r[:, 2] * load[0] - r[:, 0] * load[2]
-    per = np.stack([lift @ load, drag @ load, moment / length]) / q_area
-    tot = per.sum(axis=1)
-    force = load.sum(axis=1)
-    return {"cl": tot[0], "cd": tot[1], "cm": tot[2], "cd_friction": drag @ shear.sum(axis=1) / q_area,
+
+    def along(d, v):
+        return d[0] * v[0] + d[1] * v[1] + d[2] * v[2]
+
+    tot = [_dsum(along(lift, load) / q_area), _dsum(along(drag, load) / q_area), _dsum(moment / length / q_area)]
+    force = [_dsum(load[c]) for c in range(3)]
+    shear_sum = [_dsum(shear[c]) for c in range(3)]
+    return {"cl": tot[0], "cd": tot[1], "cm": tot[2], "cd_friction": along(drag, shear_sum) / q_area,
             "cy": force[1] / q_area, "fx": force[0], "fy": force[1], "fz": force[2]}

@@ -138,6 +138,49 @@ def test_parameter_tangents_match_the_oracle(case):
         ctx.close()
 
 
+@pytest.mark.parametrize("case", ["wing-euler", "wing-sa"])
+def test_3d_parameter_tangents_match_the_oracle(case):
+    """3D: jr_tangent_freestream and jr_forces3_tangent against the 3D
+    oracle's dual freestream on a perturbed state of a small four-block
+    wing (slip wall, Euler; no-slip, SA with Sutherland)."""
+    from grids3d import small_wing
+    from jaxrans_amd.derivatives import freestream_tangent
+    from jaxrans_amd.device import context_for
+    from oracle import residual3d as OR3
+    from oracle import sensitivity as SE
+    print(f"[gpu] 3D parameter tangent {case}", flush=True)
+    flow, wall = {
+        "wing-euler": (FlowConditions(mach=0.6, alpha_deg=2.0), "slip_wall"),
+        "wing-sa": (FlowConditions(mach=0.6, alpha_deg=2.0, reynolds=1e6, sutherland_ratio=198.6 / 460.0,
+                                   turbulence="sa_noft2"), "no_slip_wall")}[case]
+    atlas = small_wing(wall=wall)
+    num = NumericsConfig(flux="hllc", limiter="van_albada_smooth")
+    st = OR3.setup_for(atlas, flow, num)
+    from conftest import perturbed
+    U = perturbed(OR3.U0_of(st), 1e-2)
+    rng = np.random.default_rng(17)
+    dU = rng.standard_normal(U.shape) * np.abs(U).mean(axis=tuple(range(1, U.ndim)), keepdims=True) * 1e-2
+    ctx = context_for(atlas, flow, num)
+    try:
+        ctx.set_freestream(st.q_inf, None if flow.inviscid else flow.mu_inf)
+        for name in ("alpha", "mach"):
+            q, mu = SE.freestream_dual(flow, name, U.shape[0], dim=3)
+            dq, dmu = freestream_tangent(flow, 3, name)
+            assert np.allclose(dq, SE.tangents(q, mu)[0], rtol=1e-14, atol=1e-17)
+            ref = SE.residual_parameter_jvp(U, st, q, mu, dU)
+            got = ctx.tangent_freestream(U, dq, dmu, dU=dU)[0]
+            e0 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            f_ref = SE.forces_jvp(U, dU, st, q, mu, 1.0, (0.25, 0.0, 0.0), area=2.0)
+            f_got = ctx.forces3_tangent(U, dq, dmu, dU=dU, area=2.0, length=1.0, centre=(0.25, 0.0, 0.0))[0]
+            keys = ("cl", "cd", "cm", "cd_friction", "fx", "fy", "fz", "cy")
+            fr = np.array([f_ref[k] for k in keys])
+            e1 = np.abs(f_got - fr).max() / np.abs(fr).max()
+            print(f"[gpu]   d/d{name}: residual {e0:.2e}, forces {e1:.2e}", flush=True)
+            assert e0 < 1e-11 and e1 < 1e-10, (name, e0, e1)
+    finally:
+        ctx.close()
+
+
 def test_sensitivity_matches_the_oracle_linearisation():
     """sensitivity() with the case's own smooth limiter against the oracle's
     sensitivity solve at the same converged state: the exact parameter

@@ -18,12 +18,15 @@ import numpy as np
 import pytest
 
 from conftest import c_atlas, o_atlas, perturbed
+from grids3d import small_wing
 from jaxrans_amd.core.config import FlowConditions, ImplicitConfig, NumericsConfig
 from jaxrans_amd.core.state import q_infinity
 from jaxrans_amd.mesh.atlas import assign_conditions, build_atlas
 from jaxrans_amd.mesh.block import Grid, rectangle_sides
 from oracle import residual as OR
+from oracle import residual3d as OR3
 from oracle import sensitivity as SE
+from oracle import solver3d as OS3
 from oracle import solver as OS
 
 PARAMETER_NAMES = ("alpha", "mach")
@@ -166,3 +169,43 @@ def test_newton_inverse_solves_the_pseudo_time_system():
     err = rel(vdt[None] / cfg.cfl_max * x + OR.jvp(U, x, st, first_order=True), r)
     print(f"newton_inverse: |r - A G r| / |r| = {err:.2e}")
     assert err <= 1e-6
+
+
+WING_FLOWS = {
+    "wing-euler": (FlowConditions(mach=0.6, alpha_deg=2.0), "slip_wall"),
+    "wing-sa": (FlowConditions(mach=0.6, alpha_deg=2.0, reynolds=1e6, sutherland_ratio=SUTH, turbulence="sa_noft2"),
+                "no_slip_wall"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(WING_FLOWS))
+def test_3d_parameter_jvps_match_differences(case):
+    """3D: the residual's parameter JVP and the force coefficients' JVP (per
+    reference area, moment about y) on a small four-block wing against
+    Richardson differences at shifted freestreams (velocity M (cos a, 0,
+    sin a))."""
+    flow, wall = WING_FLOWS[case]
+    atlas = small_wing(wall=wall)
+    num = NumericsConfig(flux="hllc", limiter="van_albada_smooth")
+    st = OR3.setup_for(atlas, flow, num)
+    U = perturbed(OR3.U0_of(st), 1e-2)
+    rng = np.random.default_rng(5)
+    dU = rng.standard_normal(U.shape) * np.abs(U).mean(axis=tuple(range(1, U.ndim)), keepdims=True) * 1e-2
+    for name in PARAMETER_NAMES:
+        q, mu = SE.freestream_dual(flow, name, U.shape[0], dim=3)
+        assert np.allclose([float(x.v) for x in q], st.q_inf, rtol=1e-15, atol=0.0)
+        got = SE.residual_parameter_jvp(U, st, q, mu)
+        fd = richardson(lambda t: OR3.residual(U, OR3.setup_for(atlas, shifted(flow, name, t), num)), STEP[name])
+        err = rel(got, fd)
+        print(f"{case} dR/d{name}: rel to differences {err:.2e}")
+        assert err < 1e-7, (name, err)
+        f = SE.forces_jvp(U, dU, st, q, mu, 1.0, (0.25, 0.0, 0.0), area=2.0)
+
+        def coefficients(t):
+            c = OS3.forces(U + t * dU, OR3.setup_for(atlas, shifted(flow, name, t), num), 2.0, 1.0, (0.25, 0.0, 0.0))
+            return np.array([c[k] for k in ("cl", "cd", "cm", "cd_friction", "cy")])
+
+        ref = richardson(coefficients, STEP[name])
+        vec = np.array([f[k] for k in ("cl", "cd", "cm", "cd_friction", "cy")])
+        print(f"{case} dJ/d{name}: {vec}  differences {ref}")
+        assert np.allclose(vec, ref, rtol=1e-7, atol=1e-10 * np.abs(ref).max()), (name, vec, ref)
