@@ -253,6 +253,12 @@ int jr_forces_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, 
 int jr_forces3_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, const double* dq_inf_host,
                        const double* dmu_inf_host, double area, double length, double centre_x, double centre_z,
                        double* out_host);
+/* ... and of jr_wall_loads' per-face values (3D contexts), out_host
+ * [n_cases][4][n_walls] = d(gauge pressure, viscous force x, y, z): the
+ * swept (2.5D) section forms its coefficients from them on the host
+ * (force_coefficients of a swept flow, src/jaxrans/post/forces.py:136-160). */
+int jr_wall_loads_tangent(jr_ctx* ctx, const double* U_host, const double* dU_host, const double* dq_inf_host,
+                          const double* dmu_inf_host, double* out_host);
 
 /* n_steps explicit multistage Runge-Kutta steps on the device state, 2D
  * contexts: U^(k) = U^n - alphas[k] dt/V R(U^(k-1)) with the local time step

@@ -1538,11 +1538,13 @@ __global__ void __launch_bounds__(BLOCK) k_forces3(Plan3 P, Physics ph, const Ct
 // the freestream's tangent P.dfree (the output JVP of the sensitivity,
 // src/jaxrans/solvers/sensitivity.py:63).  out[case][8] = d(CL, CD, CM,
 // CD friction, Fx, Fy, Fz, CY).
+// dloads (may be null): the tangent of jr_wall_loads' per-face values,
+// [case][4][n_walls] = d(gauge pressure, viscous force x, y, z).
 template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_forces3_tangent(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
                                                            const double* __restrict__ Q, const double* __restrict__ dQ,
                                                            double area, double length, double cx, double cz,
-                                                           double* __restrict__ out) {
+                                                           double* __restrict__ out, double* __restrict__ dloads) {
   __shared__ double sh[18][4];
   const int cs = blockIdx.x;
   const size_t base = (size_t)cs * NV * P.nc;
@@ -1569,6 +1571,11 @@ __global__ void __launch_bounds__(BLOCK) k_forces3_tangent(Plan3 P, Physics ph, 
     if (ph.viscous) {
       wall_shear3<NV, Dual, Dual>(P, ph, Q, dQ, base, mu_inf, fce, t);
       for (int c = 0; c < 3; ++c) t[c] = t[c] * into;
+    }
+    if (dloads) {
+      double* lw = dloads + (size_t)cs * 4 * P.n_walls + w;
+      lw[0] = p.d;
+      for (int c = 0; c < 3; ++c) lw[(size_t)(1 + c) * P.n_walls] = t[c].d;
     }
     const Dual ld[3] = {pf[0] + t[0], pf[1] + t[1], pf[2] + t[2]};
     const double rx = P.w_mid[w] - cx, rz = P.w_mid[(size_t)2 * P.n_walls + w] - cz;

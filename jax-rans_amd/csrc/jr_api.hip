@@ -98,6 +98,7 @@ struct jr_ctx {
   double* circ = nullptr;      // point-vortex far field: [case][2] circulation and its tangent
   double* dfree = nullptr;     // parameter tangents: [case][8] the freestream's tangent (Plan::dfree)
   double* zero_field = nullptr;   // parameter tangents at dU = 0: a zero state tangent
+  double* dloads = nullptr;    // jr_wall_loads_tangent: [case][4][n_walls]
   bool tile_tangent = false;   // the Krylov tangent by LDS tiles (k_tangent_tile) where it applies
   // 2D block Thomas (k_thomas, k_line_apply): the chains by length, longest
   // first, and their length classes
@@ -804,13 +805,13 @@ static void launch_param_tangent3(jr_ctx* c, const double* U, const double* dU, 
 
 template <int NV, bool VISC>
 static void launch_forces3_tangent(jr_ctx* c, const double* U, const double* dU, double area, double length,
-                                   double cx, double cz, double* out) {
+                                   double cx, double cz, double* out, double* dloads = nullptr) {
   const Plan3& P = c->P3;
   launch_residual3<NV, VISC>(c, U, c->R, 0);
   hipLaunchKernelGGL((k_fill3<Dual, NV, true>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0,
                      c->stream, P, c->ph.gamma, (const Ctl*)c->ctl, U, dU, (double*)nullptr, c->dQ, 0);
   hipLaunchKernelGGL((k_forces3_tangent<NV>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, area, length, cx, cz, out);
+                     (const Ctl*)c->ctl, (const double*)c->Q, (const double*)c->dQ, area, length, cx, cz, out, dloads);
 }
 
 extern "C" {
@@ -2153,6 +2154,28 @@ int jr_forces3_tangent(jr_ctx* c, const double* U, const double* dU, const doubl
   else launch_forces3_tangent<5, false>(c, c->tmpA, dUd, area, length, cx, cz, c->coef);
   TRY(check_launch());
   HIPCHK(hipMemcpyAsync(out, c->coef, (size_t)c->n_cases * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int jr_wall_loads_tangent(jr_ctx* c, const double* U, const double* dU, const double* dq_inf, const double* dmu_inf,
+                          double* out) {
+  if (!out) return fail("jr_wall_loads_tangent: null argument");
+  if (c && c->dim != 3) return fail("jr_wall_loads_tangent: 3D contexts only");
+  const double* dUd = nullptr;
+  TRY(stage_param_tangent(c, "jr_wall_loads_tangent", U, dU, dq_inf, dmu_inf, &dUd));
+  const int nw = c->P3.n_walls;
+  const size_t n = (size_t)c->n_cases * 4 * std::max(1, nw);
+  if (!c->dloads) {
+    TRY(dalloc(c, &c->dloads, n));
+    HIPCHK(hipDeviceSynchronize());   // dalloc clears on the null stream
+  }
+  if (c->nv == 6) launch_forces3_tangent<6, true>(c, c->tmpA, dUd, 1.0, 1.0, 0.0, 0.0, c->coef, c->dloads);
+  else if (c->ph.viscous) launch_forces3_tangent<5, true>(c, c->tmpA, dUd, 1.0, 1.0, 0.0, 0.0, c->coef, c->dloads);
+  else launch_forces3_tangent<5, false>(c, c->tmpA, dUd, 1.0, 1.0, 0.0, 0.0, c->coef, c->dloads);
+  TRY(check_launch());
+  HIPCHK(hipMemcpyAsync(out, c->dloads, (size_t)c->n_cases * 4 * nw * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -29,7 +29,10 @@ the force integration's dual-number kernels (``jr_tangent_freestream``,
 ``jr_forces_tangent``, ``jr_forces3_tangent``), in 2D and 3D.  The derivative
 exists where the discretisation is differentiable (``minmod`` and
 ``van_albada`` have kinks: use ``van_albada_smooth`` or no limiter) and is as
-accurate as the result is converged.  A swept (2.5D) case is refused.
+accurate as the result is converged.  A swept (2.5D) case runs on its
+extruded section's 3D canvas, its parameters turning the freestream of the
+section-normal frame, and its coefficients come from the wall loads and
+their device tangent (``jr_wall_loads_tangent``) by the complex step.
 """
 
 from __future__ import annotations
@@ -41,7 +44,7 @@ from dataclasses import dataclass, replace
 import numpy as np
 
 from .core.config import FlowConditions
-from .simulation import Result, freestream_of, make_context
+from .simulation import Result, freestream_of, open_device
 
 PARAMETERS: tuple[str, ...] = ("alpha", "mach")
 OUTPUTS: tuple[str, ...] = ("cl", "cd", "cm", "cd_friction")
@@ -71,8 +74,13 @@ class Sensitivity:
 def flow_inputs(flow: FlowConditions, dim: int, p: dict) -> tuple[np.ndarray, float | None]:
     """Freestream primitives and ``mu_inf`` (``None`` if inviscid) at
     parameters ``p`` (``alpha`` in degrees, ``mach``), the Reynolds number
-    held (``derivatives.py:65-70``)."""
-    return freestream_of(replace(flow, alpha_deg=float(p["alpha"]), mach=float(p["mach"])), dim)
+    held (``derivatives.py:65-70``); a swept flow keeps its sweep (the
+    section-normal frame's primitives)."""
+    at = replace(flow, alpha_deg=float(p["alpha"]), mach=float(p["mach"]))
+    if flow.swept:
+        from .simulation import swept_q
+        return swept_q(at), (None if at.inviscid else at.mu_inf)
+    return freestream_of(at, dim)
 
 
 def freestream_tangent(flow: FlowConditions, dim: int, name: str) -> tuple[np.ndarray, float | None]:
@@ -88,24 +96,30 @@ def freestream_tangent(flow: FlowConditions, dim: int, name: str) -> tuple[np.nd
         du, dw, dm = math.cos(a), math.sin(a), 1.0
     else:
         raise ValueError(f"unknown parameter {name!r}; available: {PARAMETERS}")
-    velocity = [du, dw] if dim == 2 else [du, 0.0, dw]
+    if flow.swept:   # the section-normal frame: M (cos a cos L, sin a, cos a sin L)
+        c, s = math.cos(math.radians(flow.sweep_deg)), math.sin(math.radians(flow.sweep_deg))
+        velocity = [du * c, dw, du * s]
+    else:
+        velocity = [du, dw] if dim == 2 else [du, 0.0, dw]
     dq = np.array([0.0, *velocity, 0.0] + ([0.0] if flow.turbulence is not None else []))
     return dq, (None if flow.inviscid else dm / flow.reynolds)
 
 
-def defect_correction(matvec, b: np.ndarray, approximate_inverse, tol: float, maxiter: int):
+def defect_correction(matvec, b: np.ndarray, approximate_inverse, tol: float, maxiter: int, norm=None):
     """``x <- x + G (b - A x)`` from zero until ``|b - A x| <= tol |b|`` or
     ``maxiter`` steps (``solvers/sensitivity.py:34-50``); returns ``x``, its
-    relative residual and the step count."""
-    b_norm = max(float(np.linalg.norm(b)), np.finfo(np.float64).tiny)
+    relative residual and the step count.  ``norm``: the vector norm of the
+    test (default the 2-norm of the array)."""
+    norm = norm or (lambda v: float(np.linalg.norm(v)))
+    b_norm = max(norm(b), np.finfo(np.float64).tiny)
     x = np.zeros_like(b)
     r = b
     k = 0
-    while k < maxiter and float(np.linalg.norm(r)) > tol * b_norm:
+    while k < maxiter and norm(r) > tol * b_norm:
         x = x + approximate_inverse(r)
         r = b - matvec(x)
         k += 1
-    return x, float(np.linalg.norm(r)) / b_norm, k
+    return x, norm(r) / b_norm, k
 
 
 def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), parameters: tuple[str, ...] = ("alpha",),
@@ -118,16 +132,12 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
         if unknown or not given:
             raise ValueError(f"unknown {kind}(s) {unknown or given}; available: {known}")
     case, grid = result.case, result.grid
-    flow, dim = case.flow, grid.dim
-    if flow.swept:
-        raise ValueError("sensitivity() of a swept (2.5D) case is not supported")
-    p0 = {"alpha": float(flow.alpha_deg), "mach": float(flow.mach)}
-    U = np.asarray(result.U, np.float64)
+    flow = case.flow
     started = time.perf_counter()
-    ctx = make_context(case, grid)
+    dev = open_device(case, grid)   # a swept section runs on its extruded 3D canvas
+    ctx, dim = dev.ctx, dev.atlas.dim
+    U = dev.to_device(np.asarray(result.U, np.float64), grid)
     try:
-        q, mu = flow_inputs(flow, dim, p0)
-        ctx.set_freestream(q, mu)
         ctx.set_state(U)
         ctx.newton_init(case.implicit.cfl_max)
         spec = case.grid_spec
@@ -139,6 +149,12 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
 
         def output_tangent(name: str, dU: np.ndarray) -> np.ndarray:
             dq, dmu = freestream_tangent(flow, dim, name)
+            if flow.swept:   # the swept coefficients from the wall loads and their tangent
+                from .mesh.swept import swept_coefficients_tangent
+                d = swept_coefficients_tangent(ctx.wall_loads()[0], ctx.wall_loads_tangent(U, dq, dmu, dU=dU)[0],
+                                               dev.atlas, dev.q, dq, flow.sweep_deg, spec.reference_length,
+                                               tuple(spec.moment_centre), ctx.viscous)
+                return np.array([d[out] for out in outputs])
             if dim == 3:
                 row = ctx.forces3_tangent(U, dq, dmu, dU=dU, area=spec.reference_area,
                                           length=spec.reference_length, centre=tuple(spec.moment_centre))[0]
@@ -155,9 +171,16 @@ def sensitivity(result: Result, outputs: tuple[str, ...] = ("cl", "cd"), paramet
         def inverse(r: np.ndarray) -> np.ndarray:
             return ctx.linear_solve(r)[0]
 
+        # the test's norm over the case's own state layout: a swept section's
+        # 2D canvas, not its extruded planes and their ghost layers
+        norm = None
+        if flow.swept:
+            def norm(v: np.ndarray) -> float:
+                return float(np.linalg.norm(dev.from_device(v, grid)))
+
         columns, errors, iterations = {}, {}, {}
         for name in parameters:
-            dU, errors[name], iterations[name] = defect_correction(matvec, -rhs[name], inverse, tol, maxiter)
+            dU, errors[name], iterations[name] = defect_correction(matvec, -rhs[name], inverse, tol, maxiter, norm)
             columns[name] = output_tangent(name, dU)
     finally:
         ctx.close()

@@ -30,7 +30,7 @@ EXPORTS = (
     "jr_newton_block_poll", "jr_newton_block_wait", "jr_ctx_bytes", "jr_mem_info", "jr_ctx_create3", "jr_forces3",
     "jr_set_geometry", "jr_set_state_case", "jr_newton_init_case", "jr_linear_solve", "jr_rk_block", "jr_last_error",
     "jr_newton_block_noalloc", "jr_wall_loads", "jr_tangent_freestream", "jr_forces_tangent", "jr_park_case",
-    "jr_set_tile_tangent", "jr_forces3_tangent",
+    "jr_set_tile_tangent", "jr_forces3_tangent", "jr_wall_loads_tangent",
 )
 
 
@@ -364,6 +364,19 @@ class Context:
                                             _ptr(dq, C.c_double), None if dmu is None else _ptr(dmu, C.c_double),
                                             C.c_double(area), C.c_double(length), C.c_double(centre[0]),
                                             C.c_double(centre[2]), _ptr(out, C.c_double)))
+        return out
+
+    def wall_loads_tangent(self, U, dq_inf, dmu_inf=None, dU=None) -> np.ndarray:
+        """3D: the tangent of ``wall_loads`` at ``U`` along ``(dU, dq_inf,
+        dmu_inf)``: ``(n_cases, 4, n_walls)``."""
+        U = np.ascontiguousarray(np.broadcast_to(U, self.shape), np.float64)
+        dU = None if dU is None else np.ascontiguousarray(np.broadcast_to(dU, self.shape), np.float64)
+        dq, dmu = self._freestream_tangent(dq_inf, dmu_inf)
+        out = np.empty((self.n_cases, 4, max(1, self.atlas.walls.n)))
+        _check(self._lib.jr_wall_loads_tangent(self._h, _ptr(U, C.c_double),
+                                               None if dU is None else _ptr(dU, C.c_double), _ptr(dq, C.c_double),
+                                               None if dmu is None else _ptr(dmu, C.c_double),
+                                               _ptr(out, C.c_double)))
         return out
 
     def wave_speeds(self, U):

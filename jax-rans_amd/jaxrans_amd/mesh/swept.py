@@ -110,17 +110,10 @@ def section_to_planes(U2: np.ndarray, atlas2, atlas3: Atlas3D, fill: np.ndarray)
     return canvas
 
 
-def swept_coefficients(loads: np.ndarray, atlas3: Atlas3D, q_inf: np.ndarray, sweep_deg: float, length: float,
-                       centre, viscous: bool) -> dict:
-    """Force coefficients of a swept case from the device's per-wall-face
-    loads of the extruded canvas (``(4, n_walls)``: gauge pressure, viscous
-    force), as ``force_coefficients`` forms them for a swept flow
-    (``src/jaxrans/post/forces.py:11-21``, ``136-160``): the loads per unit
-    length along the sweep line taken to the wing frame per unit span,
-    ``F_X = F_x + (s / c) F_z``, ``F_Y = F_y / c``, ``F_S = (c F_z - s F_x) /
-    c``; lift and drag in the ``X-Y`` plane per ``q_inf`` (the whole
-    freestream's) times ``length``; the pressure moment about ``centre`` at
-    streamwise positions ``x / c``; ``cy`` the spanwise force."""
+def _coefficients(loads, atlas3: Atlas3D, q_inf, sweep_deg: float, length: float, centre, viscous: bool) -> dict:
+    """``swept_coefficients`` on arrays of any numeric type: real, or complex
+    for the complex-step derivative (every operation is analytic in the loads
+    and the freestream)."""
     w = atlas3.walls
     span = float(atlas3.mesh[0].nodes[2, 0, 0, -1] - atlas3.mesh[0].nodes[2, 0, 0, 0])
     S = np.empty((3, w.n))
@@ -139,13 +132,41 @@ def swept_coefficients(loads: np.ndarray, atlas3: Atlas3D, q_inf: np.ndarray, sw
     side = (c * load[2] - s * load[0]) / c
     load_w, pressure_w, shear_w = wing(load), wing(pressure), wing(shear)
     vel = np.array([c * q_inf[1] + s * q_inf[3], q_inf[2]])
-    speed = float(np.sqrt(q_inf[1] ** 2 + q_inf[2] ** 2 + q_inf[3] ** 2))
+    speed = np.sqrt(q_inf[1] ** 2 + q_inf[2] ** 2 + q_inf[3] ** 2)
     drag = vel / speed
     lift_dir = np.array([-drag[1], drag[0]])
     q_area = 0.5 * q_inf[0] * speed ** 2 * length
     mid = w.mid[:, :2] / np.array([c, 1.0])
     r = mid - np.asarray(centre[:2], np.float64)
     moment = r[:, 1] * pressure_w[0] - r[:, 0] * pressure_w[1]
-    return {"cl": float(lift_dir @ load_w.sum(axis=1) / q_area), "cd": float(drag @ load_w.sum(axis=1) / q_area),
-            "cm": float(np.sum(moment / length) / q_area), "cd_friction": float(drag @ shear_w.sum(axis=1) / q_area),
-            "cy": float(side.sum() / q_area), "force": np.concatenate([load_w.sum(axis=1), [side.sum()]])}
+    return {"cl": lift_dir @ load_w.sum(axis=1) / q_area, "cd": drag @ load_w.sum(axis=1) / q_area,
+            "cm": np.sum(moment / length) / q_area, "cd_friction": drag @ shear_w.sum(axis=1) / q_area,
+            "cy": side.sum() / q_area, "force": np.concatenate([load_w.sum(axis=1), [side.sum()]])}
+
+
+def swept_coefficients(loads: np.ndarray, atlas3: Atlas3D, q_inf: np.ndarray, sweep_deg: float, length: float,
+                       centre, viscous: bool) -> dict:
+    """Force coefficients of a swept case from the device's per-wall-face
+    loads of the extruded canvas (``(4, n_walls)``: gauge pressure, viscous
+    force), as ``force_coefficients`` forms them for a swept flow
+    (``src/jaxrans/post/forces.py:11-21``, ``136-160``): the loads per unit
+    length along the sweep line taken to the wing frame per unit span,
+    ``F_X = F_x + (s / c) F_z``, ``F_Y = F_y / c``, ``F_S = (c F_z - s F_x) /
+    c``; lift and drag in the ``X-Y`` plane per ``q_inf`` (the whole
+    freestream's) times ``length``; the pressure moment about ``centre`` at
+    streamwise positions ``x / c``; ``cy`` the spanwise force."""
+    out = _coefficients(np.asarray(loads, np.float64), atlas3, np.asarray(q_inf, np.float64), sweep_deg, length,
+                        centre, viscous)
+    return {k: (v if k == "force" else float(v)) for k, v in out.items()}
+
+
+def swept_coefficients_tangent(loads, dloads, atlas3: Atlas3D, q_inf, dq_inf, sweep_deg: float, length: float,
+                               centre, viscous: bool) -> dict:
+    """The derivative of ``swept_coefficients`` along ``(dloads, dq_inf)``,
+    exact to round-off by the complex step (Squire and Trapp, SIAM Review 40
+    (1998) 110-112): Im f(x + i h dx) / h with h = 1e-30, no subtraction."""
+    h = 1e-30
+    z = _coefficients(np.asarray(loads, np.float64) + 1j * h * np.asarray(dloads, np.float64), atlas3,
+                      np.asarray(q_inf, np.float64) + 1j * h * np.asarray(dq_inf, np.float64), sweep_deg, length,
+                      centre, viscous)
+    return {k: np.imag(v) / h if k == "force" else float(np.imag(v) / h) for k, v in z.items()}

@@ -99,3 +99,38 @@ def test_independence_principle_of_the_inviscid_swept_wing():
         assert rel(U25[r], U2[r]) < 1e-8, r
     assert rel(U25[3], U25[0] * w) < 1e-8
     assert rel(U25[4] - 0.5 * U25[0] * w * w, U2[3]) < 1e-8
+
+
+def test_unswept_sensitivity_equals_the_2d_one():
+    """sensitivity() through the swept path at zero sweep -- the section on
+    its extruded 3D canvas, the parameter tangents of the 3D kernels, the
+    coefficients' tangent from the wall loads' by the complex step -- equals
+    the 2D path's at the same converged state (the 2D solution with a zero
+    spanwise momentum row)."""
+    from dataclasses import replace as _replace
+
+    from jaxrans_amd import load_case, run, sensitivity
+    print("[gpu] unswept sensitivity vs 2D", flush=True)
+    base = ("numerical.convective_fluxes.limiter=none", "numerical.convergence.tol=1e-11",
+            "numerical.convergence.max_iters=3000")
+    flat = run(load_case(CASE, base))
+    swept = run(load_case(CASE, base + ("case.freestream.sweep_deg=0.0",)))
+    assert flat.history.reason == swept.history.reason == "converged"
+    U25 = np.concatenate([flat.U[:3], np.zeros_like(flat.U[:1]), flat.U[3:]])
+    same = _replace(swept, U=U25)
+    # tight solves: the test |b - A x| <= tol |b| is dominated by the rows off
+    # the interior (the U - U0 rows' dU0/dM), so tol 1e-11 leaves dCL/dM good
+    # to about 1e-7 only, on either path
+    s2 = sensitivity(flat, ("cl", "cd", "cm"), ("alpha", "mach"), tol=1e-14, maxiter=5000)
+    s3 = sensitivity(same, ("cl", "cd", "cm"), ("alpha", "mach"), tol=1e-14, maxiter=5000)
+    print(f"[gpu]   linear residuals 2D {s2.linear_residuals}, swept path {s3.linear_residuals}", flush=True)
+    own = sensitivity(swept, ("cl",), ("alpha", "mach"), tol=1e-14, maxiter=5000)
+    a = np.array([s2.derivatives[o][n] for o in ("cl", "cd", "cm") for n in ("alpha", "mach")])
+    b = np.array([s3.derivatives[o][n] for o in ("cl", "cd", "cm") for n in ("alpha", "mach")])
+    for k, (o, n) in enumerate((o, n) for o in ("cl", "cd", "cm") for n in ("alpha", "mach")):
+        print(f"[gpu]   d{o}/d{n}: 2D {a[k]:+.10e}  swept path {b[k]:+.10e}", flush=True)
+    assert np.linalg.norm(b - a) <= 1e-8 * np.linalg.norm(a)
+    assert np.abs(b - a).max() <= 1e-10
+    for name in ("alpha", "mach"):   # at the swept path's own converged state: the states' difference
+        print(f"[gpu]   dcl/d{name} at the swept run's own state {own.derivatives['cl'][name]:+.10e}", flush=True)
+        assert own.derivatives["cl"][name] == pytest.approx(s2.derivatives["cl"][name], rel=1e-6)

@@ -24,10 +24,14 @@ def _richardson(f, h: float):
 @pytest.mark.parametrize("name", PARAMETERS)
 @pytest.mark.parametrize("flow", [FlowConditions(mach=0.5, alpha_deg=1.25),
                                   FlowConditions(mach=0.3, alpha_deg=-4.0, reynolds=5e3),
-                                  FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2")])
+                                  FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2"),
+                                  FlowConditions(mach=0.73, alpha_deg=2.5, reynolds=6.5e6, turbulence="sa_noft2",
+                                                 sweep_deg=25.0)])
 def test_freestream_tangent_is_the_derivative_of_the_flow_inputs(flow, name, dim):
     """The closed-form (dq_inf, dmu_inf) against Richardson differences of
     flow_inputs (derivatives.py:65-70) in both parameters."""
+    if flow.swept and dim == 2:
+        pytest.skip("a swept section's freestream has three velocity components")
     dq, dmu = freestream_tangent(flow, dim, name)
     p0 = {"alpha": flow.alpha_deg, "mach": flow.mach}
 

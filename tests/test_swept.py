@@ -15,7 +15,8 @@ import pytest
 
 from conftest import c_atlas, o_atlas, perturbed
 from jaxrans_amd.core.config import FlowConditions, NumericsConfig
-from jaxrans_amd.mesh.swept import lift, planes_to_section, section_to_planes, swept_coefficients, swept_freestream
+from jaxrans_amd.mesh.swept import (lift, planes_to_section, section_to_planes, swept_coefficients,
+                                    swept_coefficients_tangent, swept_freestream)
 from jaxrans_amd.simulation import swept_q
 from oracle import residual as OR
 from oracle import residual3d as O3
@@ -103,3 +104,27 @@ def test_unswept_swept_coefficients_are_the_2d_ones():
     for k in ("cl", "cd", "cm"):
         assert abs(c[k] - ref[k]) < 1e-13, (k, c[k], ref[k])
     assert abs(c["cy"]) < 1e-15
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_swept_coefficient_tangent_is_the_derivative(viscous):
+    """The complex-step tangent of the swept coefficients (loads and
+    freestream moving together) against a fourth-order central difference."""
+    a3 = lift(o_atlas().mesh)
+    n = a3.walls.n
+    rng = np.random.default_rng(2)
+    loads = np.stack([0.1 * rng.standard_normal(n), *(1e-3 * rng.standard_normal((3, n)))])
+    dloads = rng.standard_normal((4, n)) * np.array([[0.1], [1e-3], [1e-3], [1e-3]])
+    q = swept_freestream(0.6, 2.5, 30.0, 1.4)
+    dq = np.array([0.0, 0.3, -0.2, 0.1, 0.0])
+    args = (a3, 30.0, 1.0, (0.25, 0.0), viscous)
+    got = swept_coefficients_tangent(loads, dloads, a3, q, dq, *args[1:])
+
+    def f(t):
+        c = swept_coefficients(loads + t * dloads, a3, q + t * dq, *args[1:])
+        return np.array([c[k] for k in ("cl", "cd", "cm", "cd_friction", "cy")])
+
+    h = 1e-4
+    fd = (4.0 * (f(h / 2) - f(-h / 2)) / h - (f(h) - f(-h)) / (2 * h)) / 3.0
+    vec = np.array([got[k] for k in ("cl", "cd", "cm", "cd_friction", "cy")])
+    assert np.allclose(vec, fd, rtol=1e-8, atol=1e-12), (vec, fd)
