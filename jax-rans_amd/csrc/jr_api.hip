@@ -328,11 +328,13 @@ static void cgs2_passes_pw(jr_ctx* c, int j, int passes) {
 }
 
 // Two entries per thread and loop step (16-byte loads) when the field length
-// is even (every case offset then keeps the alignment) and the batch of basis
-// values fits the registers twice (j < 32).
+// is even (every case offset then keeps the alignment) and j < 16: for the
+// later columns the doubled batch of basis values leaves one wave per SIMD
+// (256 VGPRs), and one entry per step at two or three waves per SIMD is
+// faster (the 32-case polar's CGS2 passes: 26.7 -> 25.3 ms per Newton step).
 template <int NV, int NCH>
 static void cgs2_passes(jr_ctx* c, int j, int passes) {
-  if constexpr (NCH <= 4) {
+  if constexpr (NCH <= 2) {
     if (c->N % 2 == 0) {
       cgs2_passes_pw<NV, NCH, 2>(c, j, passes);
       return;
