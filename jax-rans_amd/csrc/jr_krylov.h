@@ -355,9 +355,10 @@ __global__ void __launch_bounds__(BLOCK) k_dots(int nc, const Ctl* __restrict__ 
 // CGS2 pass 2: h = the pass-1 totals; w <- w - sum_{k<=j} h_k V_k, then
 // h2_k = V_k . w.  Workgroup 0 records h.  (Loading the first grid-stride
 // step ahead of the totals, as the other two passes do, needs a second set
-// of NK basis registers here and spills at NK = 32.)
+// of NK basis registers here and spills at NK = 32.)  Three waves per SIMD
+// where 168 VGPRs hold the batch without scratch (NCH 3, one entry per step).
 template <int NV, int NCH, int PW>
-__global__ void __launch_bounds__(BLOCK) k_update_dots(int nc, const Ctl* __restrict__ ctl,
+__global__ void __launch_bounds__(BLOCK, (NCH == 3 && PW == 1) ? 3 : 1) k_update_dots(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
                                                        double* __restrict__ W, int j, double* __restrict__ hv,
                                                        Cg cg) {
