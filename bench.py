@@ -387,6 +387,42 @@ def converge(case_dir: Path, grid=None, overrides=()) -> dict:
     return out
 
 
+def polar_forces(max_iters: int) -> dict:
+    """The 32-case batched polar through run_polar in FP64 and in mixed
+    precision for up to ``max_iters`` Newton steps (every case stops when all
+    have converged, tol 1e-10): per case the residual drop, the CL spread over
+    the last quarter of its blocks and mixed - FP64 in CL and CD."""
+    from dataclasses import replace
+
+    from jaxrans_amd import load_case, run_polar
+    flows = polar_flows(load_case(CASE).flow, 32)
+    alphas = [f.alpha_deg for f in flows]
+    machs = [f.mach for f in flows]
+    out = {}
+    for precision in ("fp64", "mixed"):
+        case = load_case(CASE, (f"numerical.implicit.precision={json.dumps(precision)}",
+                                f"numerical.convergence.max_iters={max_iters}"))
+        t0 = time.perf_counter()
+
+        def progress(it, cl, precision=precision):
+            if it % 500 == 0:
+                print(f"[polar {precision}] {it} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+        polar = run_polar(case, alphas, machs=machs, reynolds=6e6, progress=progress)
+        out[precision] = (polar, time.perf_counter() - t0)
+        print(f"[polar {precision}] {polar.timing.iterations} steps in {out[precision][1]:.1f} s", file=sys.stderr,
+              flush=True)
+    f64, mixed = out["fp64"][0], out["mixed"][0]
+    rows = []
+    for k, (a, m) in enumerate(zip(alphas, machs)):
+        rf, rm = f64.results[k], mixed.results[k]
+        rows.append({"alpha": round(a, 3), "mach": round(m, 4), "orders_fp64": round(float(rf.history.orders_dropped), 2),
+                     "orders_mixed": round(float(rm.history.orders_dropped), 2),
+                     "cl_spread_fp64": float(rf.cl_spread), "cl": rf.forces.cl, "cd": rf.forces.cd,
+                     "dcl": rm.forces.cl - rf.forces.cl, "dcd_counts": (rm.forces.cd - rf.forces.cd) * 1e4})
+    return {"max_iters": max_iters, "seconds": {k: round(v[1], 1) for k, v in out.items()}, "cases": rows}
+
+
 def _serve_grid(args):
     from jaxrans_amd import serve
     req, cfg = args
@@ -485,6 +521,9 @@ def main(argv=None) -> int:
                          "benchmark, e.g. 'numerical.implicit.line_solver=\"thomas\"'")
     ap.add_argument("--wing", action="store_true",
                     help="instead of the 2D case: Newton steps of the 3D SA wing (four blocks, 1.09M cells)")
+    ap.add_argument("--polar-forces", type=int, metavar="MAX_ITERS", default=0,
+                    help="instead of the step benchmark: the 32-case polar through run_polar in FP64 and mixed "
+                         "precision, per-case residual drop and mixed - FP64 forces")
     ap.add_argument("--serve", type=int, metavar="N", default=0,
                     help="instead of the step benchmark: the served stream, first N of its 256 requests")
     ap.add_argument("--rate", type=float, default=0.5, help="--serve: Poisson arrival rate, requests per second")
@@ -531,6 +570,10 @@ def main(argv=None) -> int:
                            "cells": w["cells"]},
                 "mixed_value": w["mixed"]["value"], "detail": w}
         print(json.dumps(line), flush=True)
+        return 0
+    if args.polar_forces:
+        print(json.dumps({"metric": "mixed vs FP64 per case, batched polar", **polar_forces(args.polar_forces)}),
+              flush=True)
         return 0
     if args.serve:
         line = {"metric": "solves/min + p50/p95/p99 latency, served stream", "n_gpus": 1,
