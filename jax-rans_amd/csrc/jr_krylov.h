@@ -363,7 +363,11 @@ __global__ void __launch_bounds__(BLOCK, (NCH == 3 && PW == 1) ? 3 : 1) k_update
                                                        double* __restrict__ W, int j, double* __restrict__ hv,
                                                        Cg cg) {
   constexpr int NK = Cgs<NCH>::NK, NP = Cgs<NCH>::NP;
-  const int cs = blockIdx.y;
+  // Cases in reverse order: workgroups are dispatched in the order of their
+  // index, so pass 1 ends, and pass 3 starts, on the cases whose basis this
+  // pass reads first / last, while it is still in the 256 MB Infinity
+  // Cache.  Each case's arithmetic is unchanged.
+  const int cs = gridDim.y - 1 - blockIdx.y;
   if (!ctl[cs].active) return;
   __shared__ double tot[NP];
   get_totals<NP>(cg.p1 + (size_t)cs * MMAX * cg.nblk, cg.nblk, j, tot);
