@@ -184,14 +184,16 @@ int jr_newton_init(jr_ctx* ctx, const double* cfl0);
 /* The same for one case (a slot refilled between blocks, sweep.py:229-264). */
 int jr_newton_init_case(jr_ctx* ctx, int case_index, double cfl0);
 
-/* Park (parked = 1) or release (0) case k of a batched context: a parked
- * case's GMRES never starts (its Newton steps leave its state as it is and
- * skip every Krylov launch for it), so an empty serving slot costs only its
- * share of the residual, wave-speed and factor launches.  A released case
- * continues from its state; refill it first with jr_set_geometry,
- * jr_set_state_case and jr_newton_init_case.  The slot mask of the
- * reference's slot scheduler (masked, src/jaxrans/sweep.py:203-207), which
- * keeps an idle slot's state; here its Krylov launches are skipped too. */
+/* Park (parked = 1) or release (0) case k of a batched context: every pass
+ * of a Newton step (residual, wave speeds, chain blocks and factor, GMRES,
+ * relaxation, acceptance test, update) skips a parked case, whose state and
+ * control stay as they are and whose history entries repeat its last norm;
+ * jr_wave_speeds and jr_precond_factor skip it too.  An empty serving slot
+ * then costs only its workgroups' exit.  A released case continues from its
+ * state; refill it first with jr_set_geometry, jr_set_state_case and
+ * jr_newton_init_case.  The slot mask of the reference's slot scheduler
+ * (masked, src/jaxrans/sweep.py:203-207), which keeps an idle slot's state
+ * and whose batch shrink (:313-320) stops paying for idle slots. */
 int jr_park_case(jr_ctx* ctx, int case_index, int parked);
 
 /* n_steps pseudo-transient Newton-Krylov steps on the device state

@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(BLOCK) k_fill3(Plan3 P, double gamma, const Ct
                                                  double* __restrict__ Q, double* __restrict__ dQ, int check_active) {
   int t = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   typename Free<PT>::type qfree[6];
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(BLOCK) k_nodes3(Plan3 P, double gamma, const C
   constexpr int NPHI = NV > 5 ? 5 : 4;
   const int n = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
   const int nn = (NI + 1) * (NJ + 1) * (NK + 1);
   if (n >= nn) return;
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int fi
                                                     double* __restrict__ F, int check_active) {
   const int f = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   if (f >= P.nft) return;
   if (f < P.nf[0]) face3<T, NV, VISC, 0, PT>(P, ph, first_order, ctl, Q, dQ, Nd, dNd, F, f, f, cs);
   else if (f < P.nf[0] + P.nf[1])
@@ -410,10 +410,11 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces3(Plan3 P, Physics ph, int fi
 constexpr int SALIN = 9;
 
 __global__ void __launch_bounds__(BLOCK) k_sa_lin3(Plan3 P, Physics ph, const Ctl* __restrict__ ctl,
-                                                   const double* __restrict__ Q, double* __restrict__ salin) {
+                                                   const double* __restrict__ Q, double* __restrict__ salin,
+                                                   int check_active) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (c >= P.nc) return;
+  if (c >= P.nc || skip_case(ctl, cs, check_active)) return;
   const int NJ = P.NJ, NK = P.NK, nc = P.nc;
   double* L = salin + (size_t)cs * SALIN * nc + c;
   if (!P.interior[c]) {
@@ -490,7 +491,7 @@ __global__ void __launch_bounds__(BLOCK, 2) k_cells3(Plan3 P, Physics ph, const 
                                                     int check_active) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   if (c >= P.nc) return;
   const int NJ = P.NJ, NK = P.NK, nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
@@ -621,7 +622,7 @@ __global__ void __launch_bounds__(BLOCK) k_face_speeds3(Plan3 P, Physics ph, con
                                                         double* __restrict__ sam, double* __restrict__ sad) {
   const int f = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (f >= P.nft) return;
+  if (f >= P.nft || ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int NI = P.NI, NJ = P.NJ, NK = P.NK, nc = P.nc;
   int a = 0, ff = f;
   if (ff >= P.nf[0]) { ff -= P.nf[0]; a = 1; if (ff >= P.nf[1]) { ff -= P.nf[1]; a = 2; } }
@@ -674,7 +675,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_speeds3(Plan3 P, Physics ph, con
                                                         double* __restrict__ sa_diag) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (c >= P.nc) return;
+  if (c >= P.nc || ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int NJ = P.NJ, NK = P.NK, nc = P.nc;
   const int i = c / (NJ * NK), j = (c / NK) % NJ, k = c % NK;
   const int lo0 = (i * NJ + j) * NK + k, hi0 = lo0 + NJ * NK;
@@ -802,6 +803,7 @@ __global__ void __launch_bounds__(BLOCK) k_blocks3(Plan3 P, Physics ph, const Ct
   // block components are written (and read by k_thomas3) chain-minor
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nl = P.n_lines;
   if (t >= P.max_len * nl) return;
   const int k = t / nl, ch = t - k * nl;
@@ -919,10 +921,10 @@ __device__ __forceinline__ void fac3_values(const R* __restrict__ step_base, R* 
 }
 
 template <class R, int NV>
-__global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const R* __restrict__ blk, R* __restrict__ fac,
-                                                R* __restrict__ fac_sa) {
+__global__ void __launch_bounds__(64) k_thomas3(Plan3 P, const Ctl* __restrict__ ctl, const R* __restrict__ blk,
+                                                R* __restrict__ fac, R* __restrict__ fac_sa) {
   const int ch = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
-  if (ch >= P.n_lines) return;
+  if (ch >= P.n_lines || ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int s0 = P.l_start[ch], n = P.l_start[ch + 1] - s0, nl = P.n_lines;
   const size_t cstride = (size_t)P.max_len * nl;
   const R* B = blk + (size_t)cs * BLKW3 * cstride + ch;   // component e of step k at B[e cstride + k nl]
@@ -1102,7 +1104,7 @@ __global__ void __launch_bounds__(64) k_apply3(Plan3 P, const Ctl* __restrict__ 
                                                double* __restrict__ z, double* __restrict__ tmp, int check_active,
                                                double* __restrict__ vout, const double* __restrict__ p3, int nblk) {
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int nc = P.nc, nl = P.n_lines;
   const size_t base = (size_t)cs * NV * nc;
   const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
@@ -1196,7 +1198,7 @@ __global__ void __launch_bounds__(64) k_apply3_s3(Plan3 P, const Ctl* __restrict
                                                   int nblk) {
   extern __shared__ int cl[];          // cl[k * 64 + lane]: cell of step k of this lane's chain
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int nc = P.nc, nl = P.n_lines;
   const size_t base = (size_t)cs * NV * nc;
   const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);
@@ -1289,6 +1291,7 @@ __global__ void __launch_bounds__(BLOCK) k_relax3(Plan3 P, double gamma, Ctl* __
   __shared__ double sh[1][4];
   __shared__ int is_last;
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   double worst = 0.0;
@@ -1340,6 +1343,7 @@ __global__ void __launch_bounds__(BLOCK) k_check3(Plan3 P, double gamma, Ctl* __
                                                   Red red, Ramp ramp) {
   __shared__ double sh[2][4];
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   const double omega = ramp.relaxed ? ctl[cs].omega : 1.0;

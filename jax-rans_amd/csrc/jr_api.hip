@@ -196,24 +196,26 @@ static int probe_slot(jr_ctx* c, int id) {
 
 // --- residual and tangent launch sequences -----------------------------------
 
+// check: which cases the kernels skip (skip_case; 2 in a Newton step: the
+// parked slots).
 template <int NV, bool VISC>
-static void launch_residual(jr_ctx* c, const double* U, double* out, int first_order) {
+static void launch_residual(jr_ctx* c, const double* U, double* out, int first_order, int check = 0) {
   const Plan& P = c->P;
   if (P.vortex)   // the far-field reference moves with the current lift
     hipLaunchKernelGGL((k_circulation<double, NV>), dim3(c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph.gamma,
-                       c->ctl, U, nullptr, c->circ, 0);
+                       c->ctl, U, nullptr, c->circ, check);
   hipLaunchKernelGGL((k_fill<double, NV>), grid1((size_t)P.nc + P.n_task[0] + P.n_task[1], BLOCK, c->n_cases),
-                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
+                     dim3(BLOCK), 0, c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, check);
   if (VISC)
     hipLaunchKernelGGL((k_nodes<double, NV>), grid1((size_t)(P.NI + 1) * (P.NJ + 1), BLOCK, c->n_cases), dim3(BLOCK),
-                       0, c->stream, P, c->ph.gamma, c->ctl, c->Q, nullptr, c->Nd, nullptr, 0);
+                       0, c->stream, P, c->ph.gamma, c->ctl, c->Q, nullptr, c->Nd, nullptr, check);
   hipLaunchKernelGGL((k_faces<double, NV, VISC>), grid1(P.nfi + P.nfj, BLOCK, c->n_cases), dim3(BLOCK), 0,
-                     c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->Fi, c->Fj, 0);
+                     c->stream, P, c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->Fi, c->Fj, check);
   hipLaunchKernelGGL((k_cells<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     c->ctl, U, nullptr, c->Q, nullptr, c->Fi, c->Fj, nullptr, nullptr, out, 0);
+                     c->ctl, U, nullptr, c->Q, nullptr, c->Fi, c->Fj, nullptr, nullptr, out, check);
   if (NV > 4)   // the SA source's partials at this state, for the tangents that follow
     hipLaunchKernelGGL(k_sa_lin, grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q,
-                       c->salin);
+                       c->salin, check);
 }
 
 // Requires c->Q (primitives with ghosts) for the same U.
@@ -267,10 +269,10 @@ static void launch_factor(jr_ctx* c, const double* U) {
                      c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, blk);
   if (c->thomas)
     JR_LAUNCH(PR_FACTOR, (k_thomas<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
-              (const int*)c->lorder, (const R*)blk, (R*)c->fac, (R*)c->fac_sa);
+              (const Ctl*)c->ctl, (const int*)c->lorder, (const R*)blk, (R*)c->fac, (R*)c->fac_sa);
   else
     JR_LAUNCH(PR_FACTOR, (k_cr_factor<R, NV>), dim3(P.n_lines, c->n_cases), dim3(c->cr_threads), 0, P,
-              (const R*)blk, (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
+              (const Ctl*)c->ctl, (const R*)blk, (R*)c->fac, (R*)c->fac_sa, (R*)c->ws, (R*)c->ws_sa);
 }
 
 template <class R, int NV>
@@ -410,7 +412,7 @@ template <int NV, bool VISC, int MM, class R>
 static void newton_step(jr_ctx* c, double* hist_row) {
   const Plan& P = c->P;
   const dim3 vg(c->nblk, c->n_cases);
-  launch_residual<NV, VISC>(c, c->U, c->R, 0);
+  launch_residual<NV, VISC>(c, c->U, c->R, 0, 2);
   if (c->dq_fused)
     hipLaunchKernelGGL((k_ghost_jac<NV>), grid1(P.n_final, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
                        c->ph.gamma, c->ctl, c->Q, c->gjac);
@@ -449,21 +451,21 @@ static void rk_step(jr_ctx* c, double* hist_row, const double* alphas, int n) {
 // --- the 3D path ---------------------------------------------------------------
 
 template <int NV, bool VISC>
-static void launch_residual3(jr_ctx* c, const double* U, double* out, int first_order) {
+static void launch_residual3(jr_ctx* c, const double* U, double* out, int first_order, int check = 0) {
   const Plan3& P = c->P3;
   hipLaunchKernelGGL((k_fill3<double, NV>), grid1((size_t)P.nc + P.n_ghost, BLOCK, c->n_cases), dim3(BLOCK), 0,
-                     c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, 0);
+                     c->stream, P, c->ph.gamma, c->ctl, U, nullptr, c->Q, nullptr, check);
   const size_t nn = (size_t)(P.NI + 1) * (P.NJ + 1) * (P.NK + 1);
   if (VISC)
     hipLaunchKernelGGL((k_nodes3<double, NV>), grid1(nn, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
-                       c->ph.gamma, c->ctl, c->Q, nullptr, c->Nd, nullptr, 0);
+                       c->ph.gamma, c->ctl, c->Q, nullptr, c->Nd, nullptr, check);
   hipLaunchKernelGGL((k_faces3<double, NV, VISC>), grid1(P.nft, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P,
-                     c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->F3, 0);
+                     c->ph, first_order, c->ctl, c->Q, nullptr, c->Nd, nullptr, c->F3, check);
   hipLaunchKernelGGL((k_cells3<double, NV>), grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph,
-                     c->ctl, U, nullptr, c->Q, nullptr, c->F3, nullptr, nullptr, out, 0);
+                     c->ctl, U, nullptr, c->Q, nullptr, c->F3, nullptr, nullptr, out, check);
   if (NV > 5)   // the SA source's partials at this state, for the tangents that follow
     hipLaunchKernelGGL(k_sa_lin3, grid1(P.nc, BLOCK, c->n_cases), dim3(BLOCK), 0, c->stream, P, c->ph, c->ctl, c->Q,
-                       c->salin);
+                       c->salin, check);
 }
 
 template <int NV, bool VISC>
@@ -501,7 +503,7 @@ static void launch_factor3(jr_ctx* c, const double* U) {
   hipLaunchKernelGGL((k_blocks3<R, NV>), grid1((size_t)P.max_len * P.n_lines, BLOCK, c->n_cases), dim3(BLOCK), 0,
                      c->stream, P, c->ph, c->ctl, U, c->lam, c->lsum, c->sam, c->sad, c->sa_diag, blk);
   JR_LAUNCH(PR_FACTOR, (k_thomas3<R, NV>), dim3((P.n_lines + 63) / 64, c->n_cases), dim3(64), 0, P,
-            (const R*)blk, (R*)c->fac, (R*)c->fac_sa);
+            (const Ctl*)c->ctl, (const R*)blk, (R*)c->fac, (R*)c->fac_sa);
 }
 
 template <class R, int NV>
@@ -571,7 +573,7 @@ template <int NV, bool VISC, int MM, class R>
 static void newton_step3(jr_ctx* c, double* hist_row) {
   const Plan& P = c->P;
   const dim3 vg(c->nblk, c->n_cases);
-  launch_residual3<NV, VISC>(c, c->U, c->R, 0);
+  launch_residual3<NV, VISC>(c, c->U, c->R, 0, 2);
   if (c->solve_b)   // the GMRES start r = -R is then b
     hipLaunchKernelGGL(k_negate, grid1((size_t)c->n_cases * c->N, BLOCK, 1), dim3(BLOCK), 0, c->stream,
                        (size_t)c->n_cases * c->N, c->solve_b, c->R);

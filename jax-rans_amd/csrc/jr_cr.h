@@ -103,10 +103,12 @@ __device__ __forceinline__ void inv4(const R* A, R* X) {
 // levels (diag, lower, upper) live in the workspace, two halves of [48][L]
 // (SA [3][L]) at 96 s0 (SA 6 s0), alternating with the level parity.
 template <class R, int NV>
-__global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const R* __restrict__ blk,
-                                                          R* __restrict__ fac, R* __restrict__ fac_sa,
-                                                          R* __restrict__ ws, R* __restrict__ ws_sa) {
+__global__ void __launch_bounds__(CR_MAX_LEN) k_cr_factor(Plan P, const Ctl* __restrict__ ctl,
+                                                          const R* __restrict__ blk, R* __restrict__ fac,
+                                                          R* __restrict__ fac_sa, R* __restrict__ ws,
+                                                          R* __restrict__ ws_sa) {
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x;
+  if (ctl[cs].parked) return;   // a parked serving slot (the whole workgroup): nothing to do this step
   const int s0 = P.l_start[chain], s1 = P.l_start[chain + 1];
   const int L = s1 - s0;
   __shared__ CrLevels c;   // level table in LDS: read with runtime level indices
@@ -296,7 +298,7 @@ __global__ void __launch_bounds__(CR_MAX_LEN) k_cr_apply(Plan P, const Ctl* __re
   extern __shared__ unsigned char dyn[];
   R* xb = reinterpret_cast<R*>(dyn);
   const int chain = blockIdx.x, cs = blockIdx.y, T = blockDim.x, t = threadIdx.x;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   if (chain >= P.n_lines) {   // cells on no chain: the identity

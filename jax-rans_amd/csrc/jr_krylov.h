@@ -33,6 +33,10 @@ __global__ void __launch_bounds__(BLOCK) k_norms(Plan P, Ctl* __restrict__ ctl, 
   bind_case(P, blockIdx.y);
   __shared__ double sh[3][4];
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) {   // a parked serving slot keeps its state and control; its history repeats its norm
+    if (hist && blockIdx.x == 0 && threadIdx.x == 0) hist[cs] = ctl[cs].norm;
+    return;
+  }
   const int nc = P.nc;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   double v[3] = {0.0, 0.0, 0.0};
@@ -585,6 +589,7 @@ __global__ void __launch_bounds__(BLOCK) k_relax(Plan P, double gamma, Ctl* __re
   __shared__ double sh[1][4];
   __shared__ int is_last;
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   double worst = 0.0;
@@ -643,6 +648,7 @@ __global__ void __launch_bounds__(BLOCK) k_check(Plan P, double gamma, Ctl* __re
                                                  Red red, Ramp ramp) {
   __shared__ double sh[2][4];
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   const double omega = ramp.relaxed ? ctl[cs].omega : 1.0;
@@ -691,7 +697,7 @@ template <int NV>
 __global__ void __launch_bounds__(BLOCK) k_update(int nc, const Ctl* __restrict__ ctl, double* __restrict__ U,
                                                   const double* __restrict__ X, int relaxed) {
   const int cs = blockIdx.y;
-  if (!ctl[cs].accept) return;
+  if (!ctl[cs].accept || ctl[cs].parked) return;   // a parked serving slot keeps its state
   const double omega = relaxed ? ctl[cs].omega : 1.0;
   const size_t N = (size_t)NV * nc, base = (size_t)cs * N;
   JR_VEC_LOOP(N) U[base + e] = U[base + e] + omega * X[base + e];

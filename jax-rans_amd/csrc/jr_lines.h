@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(64) k_line_apply(Plan P, const Ctl* __restrict
                                                    double gamma, const double* __restrict__ gjac) {
   extern __shared__ double xs[];
   const int cs = blockIdx.y, lane = threadIdx.x;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   const double den = apply_den(p3 + (size_t)cs * nblk, nblk, vout);   // every thread: a barrier inside

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(BLOCK) k_fill(Plan P, double gamma, const Ctl*
   bind_case(P, blockIdx.y);
   int t = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
   typename Free<PT>::type qfree[6];
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(BLOCK) k_ghost_jac(Plan P, double gamma, const
   bind_case(P, blockIdx.y);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (q >= P.n_final) return;
+  if (q >= P.n_final || ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nc = P.nc, n0 = P.n_task[0];
   const size_t base = (size_t)cs * NV * nc;
   const double* q_inf = ctl[cs].q_inf;
@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(BLOCK) k_nodes(Plan P, double gamma, const Ctl
   constexpr int NPHI = NV > 4 ? 4 : 3;
   const int n = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int NI = P.NI, NJ = P.NJ, nc = P.nc;
   const int nn = (NI + 1) * (NJ + 1);
   if (n >= nn) return;
@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
   bind_case(P, blockIdx.y);
   int f = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   if (f >= P.nfi + P.nfj) return;
   const int NJ = P.NJ, nc = P.nc;
   const bool ifam = f < P.nfi;
@@ -421,11 +421,12 @@ __global__ void __launch_bounds__(BLOCK, 2) k_faces(Plan P, Physics ph, int firs
 constexpr int SALIN2 = 6;
 
 __global__ void __launch_bounds__(BLOCK) k_sa_lin(Plan P, Physics ph, const Ctl* __restrict__ ctl,
-                                                  const double* __restrict__ Q, double* __restrict__ salin) {
+                                                  const double* __restrict__ Q, double* __restrict__ salin,
+                                                  int check_active) {
   bind_case(P, blockIdx.y);
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
-  if (c >= P.nc) return;
+  if (c >= P.nc || skip_case(ctl, cs, check_active)) return;
   const int NJ = P.NJ, nc = P.nc;
   double* L = salin + (size_t)cs * SALIN2 * nc + c;
   if (!P.interior[c]) {
@@ -495,7 +496,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cells(Plan P, Physics ph, const Ct
   bind_case(P, blockIdx.y);
   int c = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   if (c >= P.nc) return;
   const int NJ = P.NJ, nc = P.nc;
   const size_t base = (size_t)cs * NV * nc;
@@ -624,7 +625,7 @@ __global__ void __launch_bounds__(TT_THREADS, 2) k_tangent_tile(Plan P, Physics 
   __shared__ double sfi[NV][NI1][TT_J], sfj[NV][TT_I][NJ1];
   bind_case(P, blockIdx.y);
   const int cs = blockIdx.y;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const int NI = P.NI, NJ = P.NJ, nc = P.nc;
   const int tiles_j = (NJ + TT_J - 1) / TT_J;
   const int i0 = (int)(blockIdx.x / tiles_j) * TT_I, j0 = (int)(blockIdx.x % tiles_j) * TT_J;

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(BLOCK) k_circulation(Plan P, double gamma, con
   bind_case(P, blockIdx.x);
   __shared__ double sh[4][4];
   const int cs = blockIdx.x;
-  if (check_active && !ctl[cs].active) return;
+  if (skip_case(ctl, cs, check_active)) return;
   const size_t base = (size_t)cs * NV * P.nc;
   typename Free<PT>::type qi[4];
   for (int k = 0; k < 4; ++k) qi[k] = free_val<PT>(ctl[cs].q_inf[k], P.dfree, cs, k);
@@ -151,6 +151,7 @@ __global__ void __launch_bounds__(BLOCK) k_face_speeds(Plan P, Physics ph, const
   bind_case(P, blockIdx.y);
   int f = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int nf = P.nfi + P.nfj;
   if (f >= nf) return;
   const int NI = P.NI, NJ = P.NJ, nc = P.nc;
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_speeds(Plan P, Physics ph, const
   bind_case(P, blockIdx.y);
   int c = blockIdx.x * BLOCK + threadIdx.x;
   int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   if (c >= P.nc) return;
   const int NJ = P.NJ, nf = P.nfi + P.nfj;
   int i = c / NJ, j = c - i * NJ;
@@ -394,6 +396,7 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
   bind_case(P, blockIdx.y);
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   const int cs = blockIdx.y;
+  if (ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   if (s >= P.n_slots) return;
   const int nc = P.nc, nf = P.nfi + P.nfj;
   const size_t base = (size_t)cs * NV * nc;
@@ -450,10 +453,10 @@ __global__ void __launch_bounds__(BLOCK) k_blocks(Plan P, Physics ph, const Ctl*
 //     the chain's top (slot L - 1, L - 3, ...), C[r][q] on the odd ones;
 //   fac_sa[slot][3] = l, 1/p, c  (the scalar SA chain).
 template <class R, int NV>
-__global__ void __launch_bounds__(64) k_thomas(Plan P, const int* __restrict__ order, const R* __restrict__ blk,
-                                               R* __restrict__ fac, R* __restrict__ fac_sa) {
+__global__ void __launch_bounds__(64) k_thomas(Plan P, const Ctl* __restrict__ ctl, const int* __restrict__ order,
+                                               const R* __restrict__ blk, R* __restrict__ fac, R* __restrict__ fac_sa) {
   const int t = blockIdx.x * 64 + threadIdx.x, cs = blockIdx.y;
-  if (t >= P.n_lines) return;
+  if (t >= P.n_lines || ctl[cs].parked) return;   // a parked serving slot: nothing to do this step
   const int chain = order[t];
   const int s0 = P.l_start[chain], n = P.l_start[chain + 1] - s0;
   const R* B = blk + ((size_t)cs * P.n_slots + s0) * BLKW;

@@ -121,4 +121,12 @@ struct Ctl {
   int parked;          // an empty serving slot: its GMRES never starts, so its state stays put
 };
 
+// Whether a kernel skips case cs, by its check_active argument: 0 none, 1 a
+// case whose GMRES cycle is not running, 2 a parked serving slot (the passes
+// of a Newton step, which leave a parked slot's state and control as they
+// are).
+__device__ __forceinline__ bool skip_case(const Ctl* ctl, int cs, int check) {
+  return check == 1 ? !ctl[cs].active : check == 2 && ctl[cs].parked;
+}
+
 }  // namespace jr
