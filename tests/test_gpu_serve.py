@@ -129,8 +129,9 @@ def test_batched_geometries_equal_each_grid_alone():
 
 def test_parked_slot_keeps_its_state_and_the_others_are_unchanged():
     """A parked case of a batch (jr_park_case) keeps its state bit for bit
-    through a block of Newton steps, and the other cases step exactly as in
-    the same batch without parking; released and refilled, it steps again."""
+    through a block of Newton steps (every pass skips it; its history repeats
+    its last norm), and the other cases step exactly as in the same batch
+    without parking; released and refilled, it steps again."""
     print("[gpu] parked slot", flush=True)
     from jaxrans_amd.core.state import q_infinity, uniform_canvas
     reqs = serve.make_requests(3, 0.0, seed=21, sizes=((96, 24),))
@@ -156,12 +157,13 @@ def test_parked_slot_keeps_its_state_and_the_others_are_unchanged():
 
     plain, parked = batch(), batch()
     h_plain = plain.newton_block(8)
-    parked.newton_block(4)
+    h_first = parked.newton_block(4)
     before = parked.get_state()[1].copy()
     parked.park_case(1, True)
     h_parked = parked.newton_block(4)
     after = parked.get_state()
     assert np.array_equal(after[1], before)
+    assert np.all(h_parked[:, 1] == h_first[-1, 1])   # a parked slot's history repeats its last norm
     for k in (0, 2):
         assert np.array_equal(h_parked[:, k], h_plain[4:, k]), k
         assert np.array_equal(after[k], plain.get_state()[k]), k
