@@ -118,7 +118,7 @@ struct Ctl {
   int steps;           // Newton steps taken
   int rejected;
   int cycles_total;    // GMRES cycles run since the context was made (measurement)
-  int parked;          // an empty serving slot: its GMRES never starts, so its state stays put
+  int parked;          // an empty serving slot: every pass of a Newton step skips it (skip_case, parked checks)
 };
 
 // Whether a kernel skips case cs, by its check_active argument: 0 none, 1 a
