@@ -409,13 +409,18 @@ __global__ void __launch_bounds__(BLOCK, (NCH == 3 && PW == 1) ? 3 : 1) k_update
 // CGS2 pass 3: h2 = the pass-2 totals; w <- w - sum h2_k V_k and |w|^2.
 // Workgroup 0 records h2.  The column is finished (H, Givens) by the next
 // column's pass 1, or by the cycle's least squares for the last column.
-// Each thread's first grid-stride step is loaded before the totals are added.
+// Each thread's first grid-stride step is loaded before the totals are added
+// and every next step during the current one, except for the third and
+// fourth chunks of columns at one entry per step: there the step loads its
+// own batch, which leaves 136-150 VGPRs and three waves per SIMD (faster
+// than two waves with the loads ahead).
 template <int NV, int NCH, int PW>
-__global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __restrict__ ctl,
+__global__ void __launch_bounds__(BLOCK, ((NCH == 3 || NCH == 4) && PW == 1) ? 3 : 1) k_update_norm(int nc, const Ctl* __restrict__ ctl,
                                                        const double* __restrict__ V, size_t vstride, size_t vcase,
                                                        double* __restrict__ W, int j, double* __restrict__ h2v,
                                                        Cg cg) {
   constexpr int NKG = Cgs<NCH>::NKG, GS = Cgs<NCH>::GS, NP = Cgs<NCH>::NP;
+  constexpr bool PRE = !((NCH == 3 || NCH == 4) && PW == 1);   // next step loaded ahead
   __shared__ double sh[1][4];
   const int cs = blockIdx.y;
   if (!ctl[cs].active) return;
@@ -424,7 +429,7 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __rest
   const size_t step = (size_t)gridDim.x * BLOCK * PW;
   size_t e = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * PW;
   double w[PW], v0[GS][PW];
-  if (e < N) {
+  if (PRE && e < N) {
     load_pw<PW>(W, base + e, w);
     basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
   }
@@ -436,6 +441,10 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __rest
   for (int k = 0; k < NKG; ++k) h2[k] = k <= j ? tot[min(k, NP - 1)] : 0.0;
   double acc[1] = {0.0};
   while (e < N) {
+    if constexpr (!PRE) {
+      load_pw<PW>(W, base + e, w);
+      basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
+    }
     double s[PW];
 #pragma unroll
     for (int p = 0; p < PW; ++p) s[p] = 0.0;
@@ -461,7 +470,7 @@ __global__ void __launch_bounds__(BLOCK) k_update_norm(int nc, const Ctl* __rest
     for (int p = 0; p < PW; ++p) acc[0] += w[p] * w[p];
     e += step;
     __builtin_amdgcn_sched_barrier(0);   // the next step's loads reuse this step's registers
-    if (e < N) {
+    if (PRE && e < N) {
       load_pw<PW>(W, base + e, w);
       basis_batch_pw<GS, PW>(Vc, vstride, e, 0, j, v0);
     }
