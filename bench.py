@@ -128,7 +128,7 @@ def path_bytes_per_cell(ctx) -> dict:
             "precond_factor": 8 * (nv + d + d * d + (2 if sa else 0)) + blocks * fb}
 
 
-def survey_bytes(ctx) -> dict:
+def algorithmic_bytes_per_cell(ctx) -> dict:
     """Algorithmic bytes per interior cell of each hot path, every array
     touched once (nv variables, d dimensions, m x m flow blocks): residual
     ``8 (2 nv + d^2 + (d + 1) viscous + 1 SA)``; tangent ``+ 8 (nv + 1)``;
@@ -156,7 +156,7 @@ def roofline_of(kernels: dict, ctx, m: int, active: float) -> dict:
     every interior cell of every case once; a CGS2 pass launch is one of the
     3 m passes of a cycle; the tangent's kernels together make one tangent
     evaluation (their summed time is the divisor)."""
-    per = survey_bytes(ctx)
+    per = algorithmic_bytes_per_cell(ctx)
     units = ctx.atlas.cells * ctx.n_cases
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     if dom.startswith("tangent"):
